@@ -1,0 +1,270 @@
+/* ORACLE -- test infrastructure only.
+ *
+ * A plain-C restatement of the reference CPU path tracer (deadVertex/
+ * vk_cinematic, the `sp_` tracer) used by tests/, by bench.py's cpu_baseline
+ * leg and by __graft_entry__.smoke() as the checker of the HIP product.  The
+ * product never includes, links or calls anything in this directory.
+ *
+ * Every function names the reference file:line it restates.  Arithmetic is
+ * written operation-for-operation in the reference's association order and
+ * compiled with -ffp-contract=off, so that with SPO_TX_LIBM it reproduces the
+ * reference's FP32 results bit for bit (checked against the reference's own
+ * sources compiled into oracle/_ref and against golden vectors).
+ *
+ * Modes (spo_Config):
+ *   rng   SPO_RNG_XORSHIFT_TILE  one xorshift32 stream through the tile in
+ *                                pixel order (src/simd_path_tracer.cpp:178-345)
+ *         SPO_RNG_XORSHIFT_PIXEL one xorshift32 stream per pixel, seeded by
+ *                                spo_pixel_seed()
+ *         SPO_RNG_PCG            one PCG32 stream per pixel, sample s starting
+ *                                2^20*s draws into it
+ *   tx    SPO_TX_LIBM  sinf/cosf/atan2f/powf from libm (the reference's)
+ *         SPO_TX_CR    the same functions evaluated in double and rounded
+ *                      once to float (what the GPU path computes)
+ *   hit   SPO_HIT_REFERENCE  candidates from the reference's greedy 4-ary BVH,
+ *                            first-visited wins ties
+ *         SPO_HIT_BRUTE      every triangle is a candidate, ties go to the
+ *                            lower triangle index / lower object index
+ *
+ * Parity status of each function is recorded in DESIGN.md (oracle table).
+ */
+#ifndef SPO_H
+#define SPO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SPO_RNG_XORSHIFT_TILE = 0, SPO_RNG_XORSHIFT_PIXEL = 1, SPO_RNG_PCG = 2 };
+enum { SPO_TX_LIBM = 0, SPO_TX_CR = 1 };
+enum { SPO_HIT_REFERENCE = 0, SPO_HIT_BRUTE = 1 };
+enum { SPO_EST_PARITY = 0, SPO_EST_UNBIASED = 1 };
+
+#define SPO_MAX_METRICS 12
+#define SPO_NO_TEXTURE 0xFFFFFFFFu
+
+typedef struct { float x, y, z; } spo_v3;
+typedef struct { float x, y; } spo_v2;
+
+/* VertexPNT, src/mesh.h:10-15 (32 bytes) */
+typedef struct { spo_v3 p, n; spo_v2 uv; } spo_Vertex;
+
+/* bvh_Node, src/bvh.h:5-12, with child indices instead of pointers */
+typedef struct {
+    spo_v3 min, max;
+    int32_t children[4];
+    uint32_t leafIndex;
+} spo_BvhNode;
+
+typedef struct {
+    spo_BvhNode *nodes;
+    uint32_t count, capacity;
+    int32_t root; /* -1 = empty tree */
+} spo_Bvh;
+
+typedef struct {
+    uint32_t errorOccurred, count, aabbTestCount;
+} spo_BvhResult;
+
+/* sp_Mesh, src/sp_scene.h:3-12 (owns copies of its arrays) */
+typedef struct {
+    spo_Vertex *vertices;
+    uint32_t *indices;
+    uint32_t vertexCount, indexCount;
+    int useSmoothShading;
+    spo_Bvh midphase;
+} spo_Mesh;
+
+typedef struct { float m[16]; } spo_m4; /* column-major, src/math_lib.h:93 */
+
+typedef struct {
+    uint32_t mesh, material;
+    spo_m4 model, invModel;
+    spo_v3 aabbMin, aabbMax;
+} spo_Object;
+
+/* sp_Material, src/sp_material_system.h:3-12 */
+typedef struct {
+    spo_v3 albedo;
+    uint32_t albedoTexture;
+    spo_v3 emission;
+    uint32_t emissionTexture;
+    float roughness;
+} spo_Material;
+
+typedef struct {
+    const float *pixels; /* RGBA32F; cube maps hold 6 faces in order */
+    uint32_t width, height;
+    int isCube;
+} spo_Image;
+
+#define SPO_MAX_MATERIALS 64
+#define SPO_MAX_IMAGES 16
+
+typedef struct spo_Scene {
+    spo_Mesh *meshes;
+    uint32_t meshCount, meshCapacity;
+    spo_Object *objects;
+    uint32_t objectCount, objectCapacity;
+    spo_Bvh broadphase;
+    /* sp_MaterialSystem, src/sp_material_system.h:35-46 */
+    uint32_t materialKeys[SPO_MAX_MATERIALS];
+    spo_Material materials[SPO_MAX_MATERIALS];
+    uint32_t materialCount;
+    uint32_t imageKeys[SPO_MAX_IMAGES];
+    spo_Image images[SPO_MAX_IMAGES];
+    uint32_t imageCount;
+    uint32_t backgroundMaterialId;
+    /* limits that the reference hard-codes (src/sp_scene.cpp:139,:246) */
+    uint32_t maxBroadphaseHits, maxMidphaseHits;
+} spo_Scene;
+
+/* sp_Camera, src/simd_path_tracer.h:17-32 */
+typedef struct {
+    spo_v3 right, up, forward;
+    spo_v3 position, filmCenter;
+    uint32_t width, height;
+    float halfPixelWidth, halfPixelHeight;
+    float halfFilmWidth, halfFilmHeight;
+} spo_Camera;
+
+typedef struct {
+    int rng, tx, hit, estimator;
+    uint32_t samplesPerPixel, bounces;
+    float radianceClamp;      /* 0 = no clamp */
+    int jitterYFirst;         /* argument evaluation order of the jitter draws */
+    uint32_t seed;            /* xorshift tile state, or base seed */
+} spo_Config;
+
+/* closest-hit record, sp_RayIntersectSceneResult (src/sp_scene.h:39-53)
+ * plus the object and triangle that produced it */
+typedef struct {
+    float t;
+    uint32_t materialId;
+    spo_v3 normal;
+    spo_v2 uv;
+    uint32_t object, prim;
+} spo_SceneHit;
+
+/* ---- primitives ---- */
+float spo_rt_mt(const float *o, const float *d, const float *a, const float *b,
+                const float *c, float *out6);
+float spo_rt_triangle_slow(const float *o, const float *d, const float *a,
+                           const float *b, const float *c, float *out6);
+float spo_rt_aabb_scalar(const float *mn, const float *mx, const float *o,
+                         const float *d);
+float spo_rt_aabb_simd1(const float *mn, const float *mx, const float *o,
+                        const float *d);
+uint32_t spo_rt_aabb4(const float *mins12, const float *maxs12, const float *o,
+                      const float *invD);
+double spo_rt_mt_f64(const double *o, const double *d, const double *a,
+                     const double *b, const double *c);
+
+uint32_t spo_xorshift32(uint32_t *state);
+float spo_random_unilateral(uint32_t *state);
+float spo_random_bilateral(uint32_t *state);
+uint32_t spo_pixel_seed(uint32_t base, uint32_t pixelIndex);
+void spo_pcg32_seed(uint64_t *state, uint64_t *inc, uint64_t initstate,
+                    uint64_t initseq);
+uint32_t spo_pcg32_next(uint64_t *state, uint64_t inc);
+void spo_pcg32_advance(uint64_t *state, uint64_t inc, uint64_t delta);
+
+void spo_hemisphere(const float *normal, uint32_t *state, int tx, float *out);
+void spo_to_spherical(const float *v, int tx, float *out2);
+void spo_map_to_equirect(const float *sc, int tx, float *out2);
+void spo_spherical_to_cartesian(const float *sc, int tx, float *out3);
+void spo_sample_nearest(const float *pixels, uint32_t w, uint32_t h,
+                        const float *uv, float *out4);
+void spo_sample_bilinear(const float *pixels, uint32_t w, uint32_t h,
+                         const float *uv, float *out4);
+void spo_cubemap_basis(uint32_t face, float *out9);
+void spo_cubemap_lookup(const float *faces, uint32_t w, uint32_t h,
+                        const float *dir, float *out4);
+void spo_create_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
+                        uint32_t w, uint32_t h, float *out);
+uint32_t spo_compute_tiles(uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
+                           uint32_t maxTiles, uint32_t *out4);
+void spo_transform_aabb(const float *mn, const float *mx, const float *pos,
+                        const float *q, const float *scale, float *out6);
+void spo_object_matrices(const float *pos, const float *q, const float *scale,
+                         float *model16, float *inv16);
+void spo_rotate_vector(const float *v, const float *q, float *out3);
+void spo_quat_axis_angle(const float *axis, float angle, int tx, float *out4);
+void spo_quat_mul(const float *p, const float *q, float *out4);
+
+/* ---- meshes ---- */
+void spo_icosphere(uint32_t level, float *verts8, uint32_t *idx, uint32_t *nv,
+                   uint32_t *ni);
+void spo_plane(float *verts8, uint32_t *idx, uint32_t *nv, uint32_t *ni);
+
+/* ---- reference BVH (src/bvh.cpp) ---- */
+spo_Bvh *spo_bvh_create(const float *mins, const float *maxs, uint32_t n);
+void spo_bvh_destroy(spo_Bvh *t);
+uint32_t spo_bvh_intersect(const spo_Bvh *t, const float *o, const float *d,
+                           uint32_t maxHits, uint32_t *leaves,
+                           uint32_t *aabbTests, uint32_t *errorOccurred);
+uint32_t spo_bvh_dump(const spo_Bvh *t, float *boxes, int32_t *children,
+                      uint32_t *leafIndex, uint32_t cap);
+
+/* ---- scene / materials / camera ---- */
+spo_Scene *spo_scene_create(void);
+void spo_scene_destroy(spo_Scene *s);
+int spo_mesh_create(spo_Scene *s, const float *verts8, uint32_t nv,
+                    const uint32_t *idx, uint32_t ni, int smooth,
+                    int buildMidphase);
+int spo_scene_add_object(spo_Scene *s, uint32_t mesh, uint32_t material,
+                         const float *pos, const float *q, const float *scale);
+void spo_scene_build(spo_Scene *s);
+int spo_material_register(spo_Scene *s, const float *albedo,
+                          uint32_t albedoTexture, const float *emission,
+                          uint32_t emissionTexture, float roughness,
+                          uint32_t id);
+int spo_texture_register(spo_Scene *s, const float *rgba, uint32_t w,
+                         uint32_t h, uint32_t id, int isCube);
+void spo_set_background(spo_Scene *s, uint32_t id);
+void spo_evaluate_material(const spo_Scene *s, uint32_t id, const float *uv,
+                           const float *outgoingDir, int tx, float *out7);
+void spo_compute_radiance_for_path(const spo_Scene *s, const float *path15,
+                                   uint32_t len, int tx, float clampValue,
+                                   float *out3);
+
+void spo_camera_configure(spo_Camera *c, uint32_t w, uint32_t h,
+                          const float *pos, const float *q,
+                          float filmDistance);
+void spo_film_position(const spo_Camera *c, float px, float py, float *out3);
+
+int spo_scene_intersect(const spo_Scene *s, const float *o, const float *d,
+                        int hitMode, int tx, spo_SceneHit *out,
+                        uint64_t *metrics12);
+float spo_mesh_intersect(const spo_Scene *s, uint32_t mesh, const float *o,
+                         const float *d, int hitMode, float *out6,
+                         uint32_t *prim, uint64_t *metrics12);
+
+/* batch closest-hit over one mesh in mesh space (brute force, threaded) */
+void spo_closest_hit_brute(const float *tris9, uint32_t ntris,
+                           const float *rays6, uint32_t nrays,
+                           uint32_t threads, uint32_t *primOut, float *tOut);
+/* same rays in FP64 (the t-error oracle) */
+void spo_closest_hit_brute_f64(const float *tris9, uint32_t ntris,
+                               const float *rays6, uint32_t nrays,
+                               uint32_t threads, uint32_t *primOut,
+                               double *tOut);
+
+/* ---- path tracing ---- */
+int spo_path_trace_tile(const spo_Scene *s, const spo_Camera *c,
+                        const spo_Config *cfg, uint32_t minX, uint32_t minY,
+                        uint32_t maxX, uint32_t maxY, uint32_t *rngState,
+                        float *pixels, uint64_t *metrics12);
+/* Whole image with a pool of worker threads pulling tiles from an atomic
+ * counter, as src/main.cpp:731-844 does; returns wall seconds. */
+double spo_render_threaded(const spo_Scene *s, const spo_Camera *c,
+                           const spo_Config *cfg, uint32_t tileW,
+                           uint32_t tileH, uint32_t threads, float *pixels,
+                           uint64_t *metrics12);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,421 @@
+/* ORACLE -- test infrastructure only (see spo.h).
+ * Scene assembly and the two-level closest-hit query (src/sp_scene.cpp),
+ * the material system (src/sp_material_system.cpp) and the camera
+ * (src/simd_path_tracer.cpp:1-63). */
+#include <stdlib.h>
+#include <string.h>
+#include "spo_math.h"
+
+spo_Scene *spo_scene_create(void)
+{
+    spo_Scene *s = (spo_Scene *)calloc(1, sizeof *s);
+    s->broadphase.root = -1;
+    s->maxBroadphaseHits = 32;  /* src/sp_scene.cpp:246 */
+    s->maxMidphaseHits = 128;   /* src/sp_scene.cpp:139 */
+    return s;
+}
+
+void spo_scene_destroy(spo_Scene *s)
+{
+    if (!s) return;
+    for (uint32_t i = 0; i < s->meshCount; ++i)
+    {
+        free(s->meshes[i].vertices);
+        free(s->meshes[i].indices);
+        free(s->meshes[i].midphase.nodes);
+    }
+    free(s->meshes);
+    free(s->objects);
+    free(s->broadphase.nodes);
+    for (uint32_t i = 0; i < s->imageCount; ++i)
+        free((void *)s->images[i].pixels);
+    free(s);
+}
+
+/* sp_CreateMesh + sp_BuildMeshMidphase, src/sp_scene.cpp:8-75 (per-triangle
+ * AABB = Min/Max of its three vertices).  The arrays are copied. */
+int spo_mesh_create(spo_Scene *s, const float *verts8, uint32_t nv,
+                    const uint32_t *idx, uint32_t ni, int smooth,
+                    int buildMidphase)
+{
+    if (ni % 3 != 0) return -1;
+    if (s->meshCount == s->meshCapacity)
+    {
+        s->meshCapacity = s->meshCapacity ? 2 * s->meshCapacity : 8;
+        s->meshes = (spo_Mesh *)realloc(s->meshes,
+                                        sizeof(spo_Mesh) * s->meshCapacity);
+    }
+    spo_Mesh *m = &s->meshes[s->meshCount];
+    memset(m, 0, sizeof *m);
+    m->vertices = (spo_Vertex *)malloc(sizeof(spo_Vertex) * (nv ? nv : 1));
+    memcpy(m->vertices, verts8, sizeof(spo_Vertex) * nv);
+    m->indices = (uint32_t *)malloc(4 * (ni ? ni : 1));
+    memcpy(m->indices, idx, 4 * ni);
+    m->vertexCount = nv;
+    m->indexCount = ni;
+    m->useSmoothShading = smooth;
+    m->midphase.root = -1;
+    if (buildMidphase && ni > 0)
+    {
+        uint32_t tc = ni / 3;
+        float *mn = (float *)malloc(12 * tc), *mx = (float *)malloc(12 * tc);
+        for (uint32_t i = 0; i < tc; ++i)
+        {
+            spo_v3 a = m->vertices[idx[3 * i]].p;
+            spo_v3 b = m->vertices[idx[3 * i + 1]].p;
+            spo_v3 c = m->vertices[idx[3 * i + 2]].p;
+            v3put(mn + 3 * i, v3min(a, v3min(b, c)));
+            v3put(mx + 3 * i, v3max(a, v3max(b, c)));
+        }
+        spo_Bvh *t = spo_bvh_create(mn, mx, tc);
+        m->midphase = *t;
+        free(t);
+        free(mn);
+        free(mx);
+    }
+    return (int)s->meshCount++;
+}
+
+/* sp_AddObjectToScene, src/sp_scene.cpp:77-117 (ComputeAabb over all mesh
+ * vertices, src/sp_scene.cpp:56-74, then TransformAabb). */
+int spo_scene_add_object(spo_Scene *s, uint32_t mesh, uint32_t material,
+                         const float *pos, const float *q, const float *scale)
+{
+    if (mesh >= s->meshCount) return -1;
+    const spo_Mesh *m = &s->meshes[mesh];
+    if (m->vertexCount == 0) return -1;
+    if (s->objectCount == s->objectCapacity)
+    {
+        s->objectCapacity = s->objectCapacity ? 2 * s->objectCapacity : 32;
+        s->objects = (spo_Object *)realloc(
+            s->objects, sizeof(spo_Object) * s->objectCapacity);
+    }
+    spo_v3 lo = m->vertices[0].p, hi = lo;
+    for (uint32_t i = 1; i < m->vertexCount; ++i)
+    {
+        lo = v3min(lo, m->vertices[i].p);
+        hi = v3max(hi, m->vertices[i].p);
+    }
+    float bmin[3], bmax[3], box[6];
+    v3put(bmin, lo);
+    v3put(bmax, hi);
+    spo_transform_aabb(bmin, bmax, pos, q, scale, box);
+    spo_Object *o = &s->objects[s->objectCount];
+    o->mesh = mesh;
+    o->material = material;
+    spo_object_matrices(pos, q, scale, o->model.m, o->invModel.m);
+    o->aabbMin = v3p(box);
+    o->aabbMax = v3p(box + 3);
+    return (int)s->objectCount++;
+}
+
+/* sp_BuildSceneBroadphase, src/sp_scene.cpp:119-125 */
+void spo_scene_build(spo_Scene *s)
+{
+    free(s->broadphase.nodes);
+    memset(&s->broadphase, 0, sizeof s->broadphase);
+    s->broadphase.root = -1;
+    if (s->objectCount == 0) return;
+    float *mn = (float *)malloc(12 * s->objectCount);
+    float *mx = (float *)malloc(12 * s->objectCount);
+    for (uint32_t i = 0; i < s->objectCount; ++i)
+    {
+        v3put(mn + 3 * i, s->objects[i].aabbMin);
+        v3put(mx + 3 * i, s->objects[i].aabbMax);
+    }
+    spo_Bvh *t = spo_bvh_create(mn, mx, s->objectCount);
+    s->broadphase = *t;
+    free(t);
+    free(mn);
+    free(mx);
+}
+
+/* ------------------------------------------------------------ materials */
+/* sp_RegisterMaterial, src/sp_material_system.cpp:1-13 */
+int spo_material_register(spo_Scene *s, const float *albedo,
+                          uint32_t albedoTexture, const float *emission,
+                          uint32_t emissionTexture, float roughness,
+                          uint32_t id)
+{
+    if (s->materialCount >= SPO_MAX_MATERIALS) return 0;
+    uint32_t i = s->materialCount++;
+    s->materialKeys[i] = id;
+    s->materials[i].albedo = v3p(albedo);
+    s->materials[i].albedoTexture = albedoTexture;
+    s->materials[i].emission = v3p(emission);
+    s->materials[i].emissionTexture = emissionTexture;
+    s->materials[i].roughness = roughness;
+    return 1;
+}
+
+/* sp_RegisterTexture, src/sp_material_system.cpp:43-57 (pixels copied) */
+int spo_texture_register(spo_Scene *s, const float *rgba, uint32_t w,
+                         uint32_t h, uint32_t id, int isCube)
+{
+    if (s->imageCount >= SPO_MAX_IMAGES) return 0;
+    uint32_t i = s->imageCount++;
+    uint64_t n = (uint64_t)w * h * 4 * (isCube ? 6 : 1);
+    float *p = (float *)malloc(4 * n);
+    memcpy(p, rgba, 4 * n);
+    s->imageKeys[i] = id;
+    s->images[i].pixels = p;
+    s->images[i].width = w;
+    s->images[i].height = h;
+    s->images[i].isCube = isCube;
+    return 1;
+}
+
+void spo_set_background(spo_Scene *s, uint32_t id)
+{
+    s->backgroundMaterialId = id;
+}
+
+/* sp_FindMaterialById, src/sp_material_system.cpp:15-28 */
+static const spo_Material *find_material(const spo_Scene *s, uint32_t id)
+{
+    for (uint32_t i = 0; i < s->materialCount; ++i)
+        if (s->materialKeys[i] == id) return &s->materials[i];
+    return NULL;
+}
+
+/* sp_FindTexture, src/sp_material_system.cpp:30-41 */
+static const spo_Image *find_texture(const spo_Scene *s, uint32_t id)
+{
+    for (uint32_t i = 0; i < s->imageCount; ++i)
+        if (s->imageKeys[i] == id) return &s->images[i];
+    return NULL;
+}
+
+/* sp_EvaluateMaterial, src/sp_material_system.cpp:59-105, plus the
+ * cube-map emission texture this build adds (sampled by direction).  A
+ * missing material yields emission (1,0,1) and zero albedo/roughness
+ * (src/simd_path_tracer.cpp:125-134).  out7 = albedo, emission, roughness */
+void spo_material_eval_internal(const spo_Scene *s, uint32_t id, spo_v2 uv,
+                                spo_v3 outgoingDir, int tx, float *out7)
+{
+    memset(out7, 0, 7 * sizeof(float));
+    const spo_Material *m = find_material(s, id);
+    if (!m)
+    {
+        out7[3] = 1.0f; out7[4] = 0.0f; out7[5] = 1.0f;
+        return;
+    }
+    const spo_Image *albedoTex = find_texture(s, m->albedoTexture);
+    const spo_Image *emissionTex = find_texture(s, m->emissionTexture);
+    if (albedoTex && !albedoTex->isCube)
+    {
+        float t[4], uvv[2] = {uv.x, uv.y};
+        spo_sample_nearest(albedoTex->pixels, albedoTex->width,
+                           albedoTex->height, uvv, t);
+        out7[0] = t[0]; out7[1] = t[1]; out7[2] = t[2];
+    }
+    else
+        v3put(out7, m->albedo);
+    if (emissionTex)
+    {
+        spo_v3 dir = v3neg(outgoingDir);
+        float dd[3], t[4];
+        v3put(dd, dir);
+        if (emissionTex->isCube)
+            spo_cubemap_lookup(emissionTex->pixels, emissionTex->width,
+                               emissionTex->height, dd, t);
+        else
+        {
+            float sc[2], e[2];
+            spo_to_spherical(dd, tx, sc);
+            spo_map_to_equirect(sc, tx, e);
+            e[1] = 1.0f - e[1];
+            spo_sample_nearest(emissionTex->pixels, emissionTex->width,
+                               emissionTex->height, e, t);
+        }
+        out7[3] = t[0] * 1.0f; out7[4] = t[1] * 1.0f; out7[5] = t[2] * 1.0f;
+    }
+    else
+        v3put(out7 + 3, m->emission);
+    out7[6] = m->roughness;
+}
+
+void spo_evaluate_material(const spo_Scene *s, uint32_t id, const float *uv,
+                           const float *outgoingDir, int tx, float *out7)
+{
+    spo_v2 u = {uv[0], uv[1]};
+    spo_material_eval_internal(s, id, u, v3p(outgoingDir), tx, out7);
+}
+
+/* -------------------------------------------------------------- camera */
+/* sp_ConfigureCamera, src/simd_path_tracer.cpp:1-36 */
+void spo_camera_configure(spo_Camera *c, uint32_t w, uint32_t h,
+                          const float *pos, const float *q,
+                          float filmDistance)
+{
+    spo_q qq = qmake(q);
+    memset(c, 0, sizeof *c);
+    c->width = w;
+    c->height = h;
+    c->position = v3p(pos);
+    c->right = qrotate(v3(1, 0, 0), qq);
+    c->up = qrotate(v3(0, 1, 0), qq);
+    c->forward = qrotate(v3(0, 0, -1), qq);
+    c->filmCenter = v3add(c->position, v3mul(c->forward, filmDistance));
+    c->halfPixelWidth = 0.5f / (float)w;
+    c->halfPixelHeight = 0.5f / (float)h;
+    float filmWidth = 1.0f, filmHeight = 1.0f;
+    if (w > h) filmHeight = (float)h / (float)w;
+    else if (w < h) filmWidth = (float)w / (float)h;
+    c->halfFilmWidth = 0.5f * filmWidth;
+    c->halfFilmHeight = 0.5f * filmHeight;
+}
+
+/* sp_CalculateFilmPositions, src/simd_path_tracer.cpp:38-63 */
+void spo_film_position(const spo_Camera *c, float px, float py, float *out3)
+{
+    float fx = px / (float)c->width;
+    float fy = py / (float)c->height;
+    fy = 1.0f - fy;
+    fx = fx * 2.0f - 1.0f;
+    fy = fy * 2.0f - 1.0f;
+    spo_v3 p = v3mul(c->right, c->halfFilmWidth * fx);
+    p = v3add(p, v3mul(c->up, c->halfFilmHeight * fy));
+    p = v3add(p, c->filmCenter);
+    v3put(out3, p);
+}
+
+/* ------------------------------------------------------------ queries */
+typedef struct { float t; spo_v2 uv; spo_v3 normal; uint32_t prim; } mesh_hit;
+
+/* The per-triangle step of sp_RayIntersectMesh, src/sp_scene.cpp:161-214:
+ * Moller-Trumbore, keep t > 0 and strictly smaller; UV and (optionally)
+ * smooth normal from the barycentrics. */
+static void consider_triangle(const spo_Mesh *m, uint32_t tri, const float *o,
+                              const float *d, mesh_hit *best)
+{
+    const spo_Vertex *v0 = &m->vertices[m->indices[3 * tri]];
+    const spo_Vertex *v1 = &m->vertices[m->indices[3 * tri + 1]];
+    const spo_Vertex *v2 = &m->vertices[m->indices[3 * tri + 2]];
+    float a[3], b[3], c[3], r[6];
+    v3put(a, v0->p); v3put(b, v1->p); v3put(c, v2->p);
+    spo_rt_mt(o, d, a, b, c, r);
+    if (r[0] > 0.0f)
+    {
+        if (r[0] < best->t || best->t < 0.0f)
+        {
+            best->t = r[0];
+            best->normal = v3(r[3], r[4], r[5]);
+            best->prim = tri;
+            float u = r[1], v = r[2];
+            float w = 1.0f - u - v;
+            spo_v2 uv;
+            uv.x = (v0->uv.x * w + v1->uv.x * u) + v2->uv.x * v;
+            uv.y = (v0->uv.y * w + v1->uv.y * u) + v2->uv.y * v;
+            if (m->useSmoothShading)
+            {
+                spo_v3 n = v3add(v3add(v3mul(v0->n, w), v3mul(v1->n, u)),
+                                 v3mul(v2->n, v));
+                best->normal = v3norm(n);
+            }
+            best->uv = uv;
+        }
+    }
+}
+
+/* sp_RayIntersectMesh, src/sp_scene.cpp:127-227.  Returns -1 if the midphase
+ * query overflows (the reference asserts, src/sp_scene.cpp:158). */
+static int mesh_query(const spo_Scene *s, const spo_Mesh *m, const float *o,
+                      const float *d, int hitMode, mesh_hit *best,
+                      uint64_t *metrics12)
+{
+    best->t = -1.0f;
+    best->prim = 0xFFFFFFFFu;
+    best->uv.x = best->uv.y = 0.0f;
+    best->normal = v3(0, 0, 0);
+    if (hitMode == SPO_HIT_BRUTE)
+    {
+        for (uint32_t tri = 0; tri < m->indexCount / 3; ++tri)
+            consider_triangle(m, tri, o, d, best);
+        return 0;
+    }
+    uint32_t leaves[1024];
+    uint32_t cap = s->maxMidphaseHits < 1024 ? s->maxMidphaseHits : 1024;
+    uint32_t tests = 0, err = 0;
+    uint32_t n = spo_bvh_intersect(&m->midphase, o, d, cap, leaves, &tests, &err);
+    if (metrics12) metrics12[10] += tests;
+    if (err) return -1;
+    for (uint32_t i = 0; i < n; ++i) consider_triangle(m, leaves[i], o, d, best);
+    return 0;
+}
+
+float spo_mesh_intersect(const spo_Scene *s, uint32_t mesh, const float *o,
+                         const float *d, int hitMode, float *out6,
+                         uint32_t *prim, uint64_t *metrics12)
+{
+    mesh_hit h;
+    if (mesh_query(s, &s->meshes[mesh], o, d, hitMode, &h, metrics12) < 0)
+        return -2.0f;
+    if (out6)
+    {
+        out6[0] = h.t; out6[1] = h.uv.x; out6[2] = h.uv.y;
+        out6[3] = h.normal.x; out6[4] = h.normal.y; out6[5] = h.normal.z;
+    }
+    if (prim) *prim = h.prim;
+    return h.t;
+}
+
+/* sp_RayIntersectScene, src/sp_scene.cpp:229-339.  Returns -1 on a BVH
+ * overflow (the reference asserts). */
+int spo_scene_intersect(const spo_Scene *s, const float *o, const float *d,
+                        int hitMode, int tx, spo_SceneHit *out,
+                        uint64_t *metrics12)
+{
+    (void)tx;
+    out->t = -1.0f;
+    out->materialId = 0;
+    out->normal = v3(0, 0, 0);
+    out->uv.x = out->uv.y = 0.0f;
+    out->object = out->prim = 0xFFFFFFFFu;
+    uint32_t objs[1024];
+    uint32_t n;
+    if (hitMode == SPO_HIT_BRUTE)
+    {
+        n = s->objectCount < 1024 ? s->objectCount : 1024;
+        for (uint32_t i = 0; i < n; ++i) objs[i] = i;
+    }
+    else
+    {
+        uint32_t err = 0, tests = 0;
+        uint32_t cap = s->maxBroadphaseHits < 1024 ? s->maxBroadphaseHits : 1024;
+        n = spo_bvh_intersect(&s->broadphase, o, d, cap, objs, &tests, &err);
+        if (err) return -1;
+    }
+    spo_v3 ro = v3p(o), rd = v3p(d);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const spo_Object *obj = &s->objects[objs[i]];
+        spo_v3 lo = m4point(obj->invModel, ro);
+        spo_v3 ld = v3norm(m4vector(obj->invModel, rd));
+        float lof[3], ldf[3];
+        v3put(lof, lo);
+        v3put(ldf, ld);
+        mesh_hit h;
+        if (mesh_query(s, &s->meshes[obj->mesh], lof, ldf, hitMode, &h,
+                       metrics12) < 0)
+            return -1;
+        if (metrics12) metrics12[11] += 1;
+        if (h.t >= 0.0f)
+        {
+            spo_v3 localHit = v3add(lo, v3mul(ld, h.t));
+            spo_v3 worldHit = m4point(obj->model, localHit);
+            float t = v3dot(v3sub(worldHit, ro), rd);
+            spo_v3 worldNormal = v3norm(m4vector(obj->model, h.normal));
+            if (t < out->t || out->t < 0.0f)
+            {
+                out->t = t;
+                out->materialId = obj->material;
+                out->normal = worldNormal;
+                out->uv = h.uv;
+                out->object = objs[i];
+                out->prim = h.prim;
+            }
+        }
+    }
+    return 0;
+}

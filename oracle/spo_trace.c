@@ -1,0 +1,520 @@
+/* ORACLE -- test infrastructure only (see spo.h).
+ * sp_PathTraceTile and ComputeRadianceForPath (src/simd_path_tracer.cpp),
+ * the threaded tile scheduler (src/main.cpp:731-844) used as the CPU
+ * baseline, and brute-force closest-hit batches (FP32 and FP64). */
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "spo_math.h"
+
+void spo_material_eval_internal(const spo_Scene *s, uint32_t id, spo_v2 uv,
+                                spo_v3 outgoingDir, int tx, float *out7);
+
+/* ---------------------------------------------------------------- RNG */
+typedef struct { int mode; uint32_t xs; uint64_t state, inc; } spo_Rng;
+
+static uint32_t rng_next(spo_Rng *r)
+{
+    if (r->mode == SPO_RNG_PCG) return spo_pcg32_next(&r->state, r->inc);
+    return spo_xorshift32(&r->xs);
+}
+/* RandomUnilateral / RandomBilateral, src/math_utils.h:200-214, fed by the
+ * active generator. */
+static float rng_uni(spo_Rng *r)
+{
+    uint32_t x = rng_next(r);
+    float numerator = (float)(x >> 1);
+    float denom = (float)(0xFFFFFFFFu >> 1);
+    return numerator / denom;
+}
+static float rng_bi(spo_Rng *r) { return -1.0f + 2.0f * rng_uni(r); }
+/* uniform in [0,1) with 24 random bits (unbiased estimator only) */
+static float rng_u24(spo_Rng *r)
+{
+    return (float)(rng_next(r) >> 8) * (1.0f / 16777216.0f);
+}
+
+/* --------------------------------------------------------------- BRDF */
+/* src/simd_path_tracer.cpp:65-105 */
+static spo_v3 fresnel_schlick(float cosTheta, spo_v3 F0, int tx)
+{
+    float p = spo_pow(1.0f - cosTheta, 5.0f, tx);
+    return v3add(F0, v3mul(v3sub(v3s(1.0f), F0), p));
+}
+static float distribution_ggx(spo_v3 N, spo_v3 H, float roughness)
+{
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float NdotH = spo_max(v3dot(N, H), 0.0f);
+    float NdotH2 = NdotH * NdotH;
+    float num = a2;
+    float denom = (NdotH2 * (a2 - 1.0f) + 1.0f);
+    denom = SPO_PI * denom * denom;
+    return num / denom;
+}
+static float geometry_schlick_ggx(float NdotV, float roughness)
+{
+    float r = (roughness + 1.0f);
+    float k = (r * r) / 8.0f;
+    float num = NdotV;
+    float denom = NdotV * (1.0f - k) + k;
+    return num / denom;
+}
+static float geometry_smith(spo_v3 N, spo_v3 V, spo_v3 L, float roughness)
+{
+    float NdotV = spo_max(v3dot(N, V), 0.0f);
+    float NdotL = spo_max(v3dot(N, L), 0.0f);
+    float ggx2 = geometry_schlick_ggx(NdotV, roughness);
+    float ggx1 = geometry_schlick_ggx(NdotL, roughness);
+    return ggx1 * ggx2;
+}
+
+/* The bracket of ComputeRadianceForPath, src/simd_path_tracer.cpp:136-165:
+ * X = kD*albedo/pi + specular; the caller forms E + (X o Lin) * cosine.
+ * With noDeltaSpecular the GGX term is dropped at roughness 0, where the
+ * reference's D is 0 almost everywhere and 0/0 on the measure-zero set
+ * N.H = 1 (used by the unbiased estimator only). */
+static spo_v3 brdf_bracket(spo_v3 N, spo_v3 V, spo_v3 L, const float *mat,
+                           int tx, int noDeltaSpecular)
+{
+    spo_v3 H = v3norm(v3add(L, V));
+    spo_v3 F0 = v3s(0.04f);
+    spo_v3 F = fresnel_schlick(spo_max(v3dot(H, V), 0.0f), F0, tx);
+    spo_v3 kS = F;
+    spo_v3 kD = v3sub(v3s(1.0f), kS);
+    float oneOverPI = 1.0f / SPO_PI;
+    float roughness = mat[6];
+    spo_v3 specular = v3(0, 0, 0);
+    if (!(noDeltaSpecular && roughness == 0.0f))
+    {
+        float NDF = distribution_ggx(N, H, roughness);
+        float G = geometry_smith(N, V, L, roughness);
+        spo_v3 numerator = v3mul(F, NDF * G);
+        float denominator =
+            4.0f * spo_max(v3dot(N, V), 0.0f) * spo_max(v3dot(N, L), 0.0f) +
+            0.0001f;
+        specular = v3mul(numerator, 1.0f / denominator);
+    }
+    spo_v3 albedo = v3p(mat);
+    return v3add(v3mul(v3had(kD, albedo), oneOverPI), specular);
+}
+
+typedef struct {
+    uint32_t materialId;
+    spo_v3 worldPosition, outgoingDir, incomingDir, normal;
+    spo_v2 uv;
+} spo_PathVertex; /* sp_PathVertex, src/sp_material_system.h:22-30 */
+
+/* ComputeRadianceForPath, src/simd_path_tracer.cpp:107-175 */
+static spo_v3 radiance_for_path(const spo_Scene *s, const spo_PathVertex *path,
+                                uint32_t len, int tx, float clampValue)
+{
+    spo_v3 radiance = v3(0, 0, 0);
+    for (int32_t i = (int32_t)len - 1; i >= 0; i--)
+    {
+        const spo_PathVertex *v = &path[i];
+        float mat[7];
+        spo_material_eval_internal(s, v->materialId, v->uv, v->outgoingDir, tx,
+                                   mat);
+        float cosine = spo_max(0.0f, v3dot(v->normal, v->incomingDir));
+        spo_v3 incoming = radiance;
+        if (clampValue > 0.0f) incoming = v3clamp(incoming, 0.0f, clampValue);
+        spo_v3 X = brdf_bracket(v->normal, v->outgoingDir, v->incomingDir, mat,
+                                tx, 0);
+        radiance = v3add(v3p(mat + 3), v3mul(v3had(X, incoming), cosine));
+    }
+    return radiance;
+}
+
+/* path15: per vertex materialId(bits), worldPosition, outgoingDir,
+ * incomingDir, normal, uv */
+void spo_compute_radiance_for_path(const spo_Scene *s, const float *path15,
+                                   uint32_t len, int tx, float clampValue,
+                                   float *out3)
+{
+    spo_PathVertex v[8];
+    memset(v, 0, sizeof v);
+    for (uint32_t i = 0; i < len && i < 8; ++i)
+    {
+        const float *p = path15 + 15 * i;
+        memcpy(&v[i].materialId, p, 4);
+        v[i].worldPosition = v3p(p + 1);
+        v[i].outgoingDir = v3p(p + 4);
+        v[i].incomingDir = v3p(p + 7);
+        v[i].normal = v3p(p + 10);
+        v[i].uv.x = p[13];
+        v[i].uv.y = p[14];
+    }
+    v3put(out3, radiance_for_path(s, v, len, tx, clampValue));
+}
+
+/* RandomDirectionOnHemisphere, src/math_lib.h:932-946, on the active RNG */
+static spo_v3 hemisphere(spo_v3 normal, spo_Rng *rng, int tx)
+{
+    float theta = SPO_PI * rng_uni(rng);
+    float phi = SPO_PI * rng_bi(rng);
+    spo_v3 dir;
+    dir.x = spo_sin(theta, tx) * spo_cos(phi, tx);
+    dir.z = spo_sin(theta, tx) * spo_sin(phi, tx);
+    dir.y = spo_cos(theta, tx);
+    if (v3dot(dir, normal) < 0.0f) dir = v3neg(dir);
+    return dir;
+}
+
+/* Cosine-weighted direction about n (unbiased estimator; no reference
+ * counterpart): r = sqrt(u1), phi = 2 pi u2, z = sqrt(max(0, 1 - u1)), in the
+ * orthonormal frame of Duff et al. 2017 ("Building an Orthonormal Basis,
+ * Revisited"). */
+static spo_v3 cosine_direction(spo_v3 n, float u1, float u2, int tx)
+{
+    float r = sqrtf(u1);
+    float phi = 6.28318548f * u2;
+    float lx = r * spo_cos(phi, tx);
+    float ly = r * spo_sin(phi, tx);
+    float lz = sqrtf(spo_max(0.0f, 1.0f - u1));
+    float sign = copysignf(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    spo_v3 t1 = v3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    spo_v3 t2 = v3(b, sign + n.y * n.y * a, -n.y);
+    return v3add(v3add(v3mul(t1, lx), v3mul(t2, ly)), v3mul(n, lz));
+}
+
+static void rng_for_pixel(spo_Rng *rng, const spo_Config *cfg, uint32_t width,
+                          uint32_t x, uint32_t y, uint32_t sample,
+                          uint32_t *tileState)
+{
+    uint32_t pixelIndex = y * width + x;
+    rng->mode = cfg->rng;
+    if (cfg->rng == SPO_RNG_PCG)
+    {
+        spo_pcg32_seed(&rng->state, &rng->inc, cfg->seed, pixelIndex);
+        spo_pcg32_advance(&rng->state, rng->inc, (uint64_t)sample << 20);
+    }
+    else if (cfg->rng == SPO_RNG_XORSHIFT_PIXEL)
+    {
+        if (sample == 0) rng->xs = spo_pixel_seed(cfg->seed, pixelIndex);
+    }
+    else
+        rng->xs = *tileState;
+}
+
+/* One sample of one pixel.  Parity estimator: src/simd_path_tracer.cpp:
+ * 216-322.  Unbiased estimator: the same material model and camera, cosine
+ * sampling and the 1/pdf weight. */
+static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
+                           const spo_Config *cfg, uint32_t x, uint32_t y,
+                           spo_Rng *rng, uint64_t *metrics, int *fail)
+{
+    int tx = cfg->tx;
+    float b1 = rng_bi(rng), b2 = rng_bi(rng);
+    float jx, jy;
+    if (cfg->jitterYFirst) { jy = c->halfPixelHeight * b1; jx = c->halfPixelWidth * b2; }
+    else { jx = c->halfPixelWidth * b1; jy = c->halfPixelHeight * b2; }
+    float px = (float)x + 0.5f, py = (float)y + 0.5f;
+    px = px + jx;
+    py = py + jy;
+    float filmP[3];
+    spo_film_position(c, px, py, filmP);
+    spo_v3 o = c->position;
+    spo_v3 d = v3norm(v3sub(v3p(filmP), c->position));
+    if (cfg->estimator == SPO_EST_UNBIASED)
+    {
+        spo_v3 beta = v3s(1.0f), L = v3(0, 0, 0);
+        for (uint32_t bounce = 0; bounce < cfg->bounces; ++bounce)
+        {
+            float of[3], df[3];
+            v3put(of, o); v3put(df, d);
+            spo_SceneHit h;
+            if (spo_scene_intersect(s, of, df, cfg->hit, tx, &h, metrics) < 0)
+            { *fail = 1; return L; }
+            metrics[2]++;
+            spo_v3 V = v3neg(d);
+            if (h.t > 0.0f)
+            {
+                metrics[3]++;
+                float mat[7];
+                spo_material_eval_internal(s, h.materialId, h.uv, V, tx, mat);
+                L = v3add(L, v3had(beta, v3p(mat + 3)));
+                float u1 = rng_u24(rng), u2 = rng_u24(rng);
+                spo_v3 wi = cosine_direction(h.normal, u1, u2, tx);
+                float cosine = v3dot(h.normal, wi);
+                if (!(cosine > 0.0f)) break;
+                spo_v3 X = brdf_bracket(h.normal, V, wi, mat, tx, 1);
+                beta = v3had(beta, v3mul(X, SPO_PI));
+                spo_v3 p = v3add(o, v3mul(d, h.t));
+                o = v3add(p, v3mul(h.normal, 0.0001f));
+                d = wi;
+            }
+            else
+            {
+                metrics[4]++;
+                float mat[7];
+                spo_v2 uv0 = {0, 0};
+                spo_material_eval_internal(s, s->backgroundMaterialId, uv0, V,
+                                           tx, mat);
+                L = v3add(L, v3had(beta, v3p(mat + 3)));
+                break;
+            }
+        }
+        return L;
+    }
+    spo_PathVertex path[16];
+    memset(path, 0, sizeof path);
+    uint32_t len = 0;
+    uint32_t bounces = cfg->bounces < 16 ? cfg->bounces : 16;
+    for (uint32_t bounce = 0; bounce < bounces; ++bounce)
+    {
+        float of[3], df[3];
+        v3put(of, o); v3put(df, d);
+        spo_SceneHit h;
+        if (spo_scene_intersect(s, of, df, cfg->hit, tx, &h, metrics) < 0)
+        { *fail = 1; return v3(0, 0, 0); }
+        metrics[2]++;
+        spo_PathVertex *v = &path[len++];
+        if (h.t > 0.0f)
+        {
+            v->materialId = h.materialId;
+            v->worldPosition = v3add(o, v3mul(d, h.t));
+            v->outgoingDir = v3neg(d);
+            v->normal = h.normal;
+            v->uv = h.uv;
+            spo_v3 dir = hemisphere(h.normal, rng, tx);
+            v->incomingDir = dir;
+            o = v3add(v->worldPosition, v3mul(h.normal, 0.0001f));
+            d = dir;
+            metrics[3]++;
+        }
+        else
+        {
+            v->materialId = s->backgroundMaterialId;
+            v->outgoingDir = v3neg(d);
+            metrics[4]++;
+            break;
+        }
+    }
+    return radiance_for_path(s, path, len, tx, cfg->radianceClamp);
+}
+
+/* sp_PathTraceTile, src/simd_path_tracer.cpp:178-345.  pixels is the whole
+ * RGBA32F image (width*height*4); only the tile's pixels are written. */
+int spo_path_trace_tile(const spo_Scene *s, const spo_Camera *c,
+                        const spo_Config *cfg, uint32_t minX, uint32_t minY,
+                        uint32_t maxX, uint32_t maxY, uint32_t *rngState,
+                        float *pixels, uint64_t *metrics12)
+{
+    uint64_t local[SPO_MAX_METRICS] = {0};
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    maxX = spo_minu(maxX, c->width);
+    maxY = spo_minu(maxY, c->height);
+    uint32_t tileState = rngState ? *rngState : cfg->seed;
+    int fail = 0;
+    for (uint32_t y = minY; y < maxY; y++)
+        for (uint32_t x = minX; x < maxX; x++)
+        {
+            spo_v3 total = v3(0, 0, 0);
+            spo_Rng rng;
+            memset(&rng, 0, sizeof rng);
+            for (uint32_t sample = 0; sample < cfg->samplesPerPixel; sample++)
+            {
+                rng_for_pixel(&rng, cfg, c->width, x, y, sample, &tileState);
+                spo_v3 radiance = trace_sample(s, c, cfg, x, y, &rng, local,
+                                               &fail);
+                if (cfg->rng == SPO_RNG_XORSHIFT_TILE) tileState = rng.xs;
+                if (fail) return -1;
+                total = v3add(total, v3mul(radiance,
+                                           1.0f / (float)cfg->samplesPerPixel));
+                local[1]++;
+            }
+            float *px = pixels + 4 * ((uint64_t)y * c->width + x);
+            px[0] = total.x; px[1] = total.y; px[2] = total.z; px[3] = 1.0f;
+        }
+    if (rngState) *rngState = tileState;
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    local[0] = (uint64_t)((t1.tv_sec - t0.tv_sec) * 1000000000LL +
+                          (t1.tv_nsec - t0.tv_nsec));
+    if (metrics12)
+        for (int i = 0; i < SPO_MAX_METRICS; ++i) metrics12[i] += local[i];
+    return 0;
+}
+
+/* ------------------------------------------------- threaded CPU baseline */
+typedef struct {
+    const spo_Scene *s;
+    const spo_Camera *c;
+    const spo_Config *cfg;
+    uint32_t *tiles;
+    uint32_t tileCount;
+    atomic_uint next;
+    float *pixels;
+    uint64_t metrics[64][SPO_MAX_METRICS];
+    atomic_int fail;
+} render_job;
+
+typedef struct { render_job *job; uint32_t id; } render_arg;
+
+/* WorkerThread, src/main.cpp:731-759: pop a tile, reset the RNG to the
+ * per-tile constant, trace it. */
+static void *render_worker(void *p)
+{
+    render_arg *a = (render_arg *)p;
+    render_job *j = a->job;
+    for (;;)
+    {
+        uint32_t i = atomic_fetch_add(&j->next, 1u);
+        if (i >= j->tileCount) break;
+        uint32_t *t = j->tiles + 4 * i;
+        uint32_t state = j->cfg->seed;
+        if (spo_path_trace_tile(j->s, j->c, j->cfg, t[0], t[1], t[2], t[3],
+                                &state, j->pixels, j->metrics[a->id]) < 0)
+            atomic_store(&j->fail, 1);
+    }
+    return NULL;
+}
+
+double spo_render_threaded(const spo_Scene *s, const spo_Camera *c,
+                           const spo_Config *cfg, uint32_t tileW,
+                           uint32_t tileH, uint32_t threads, float *pixels,
+                           uint64_t *metrics12)
+{
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64;
+    uint32_t maxTiles = ((c->width + tileW - 1) / tileW) *
+                        ((c->height + tileH - 1) / tileH);
+    render_job *j = (render_job *)calloc(1, sizeof *j);
+    j->s = s; j->c = c; j->cfg = cfg; j->pixels = pixels;
+    j->tiles = (uint32_t *)malloc(16 * (maxTiles ? maxTiles : 1));
+    j->tileCount = spo_compute_tiles(c->width, c->height, tileW, tileH,
+                                     maxTiles, j->tiles);
+    atomic_init(&j->next, 0u);
+    atomic_init(&j->fail, 0);
+    pthread_t th[64];
+    render_arg args[64];
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (uint32_t i = 0; i < threads; ++i)
+    {
+        args[i].job = j;
+        args[i].id = i;
+        pthread_create(&th[i], NULL, render_worker, &args[i]);
+    }
+    for (uint32_t i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    if (metrics12)
+        for (uint32_t i = 0; i < threads; ++i)
+            for (int k = 0; k < SPO_MAX_METRICS; ++k)
+                metrics12[k] += j->metrics[i][k];
+    int fail = atomic_load(&j->fail);
+    free(j->tiles);
+    free(j);
+    double secs = (double)(t1.tv_sec - t0.tv_sec) +
+                  1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    return fail ? -secs : secs;
+}
+
+/* -------------------------------------------- brute-force closest hit */
+typedef struct {
+    const float *tris; uint32_t ntris;
+    const float *rays; uint32_t nrays;
+    uint32_t *prim; float *t; double *t64;
+    atomic_uint next;
+} brute_job;
+
+static void *brute_worker(void *p)
+{
+    brute_job *j = (brute_job *)p;
+    for (;;)
+    {
+        uint32_t r0 = atomic_fetch_add(&j->next, 64u);
+        if (r0 >= j->nrays) break;
+        uint32_t r1 = r0 + 64 < j->nrays ? r0 + 64 : j->nrays;
+        for (uint32_t r = r0; r < r1; ++r)
+        {
+            const float *o = j->rays + 6 * r, *d = o + 3;
+            uint32_t best = 0xFFFFFFFFu;
+            if (j->t64)
+            {
+                double od[3] = {o[0], o[1], o[2]}, dd[3] = {d[0], d[1], d[2]};
+                double bt = -1.0;
+                for (uint32_t k = 0; k < j->ntris; ++k)
+                {
+                    const float *tr = j->tris + 9 * k;
+                    double a[3] = {tr[0], tr[1], tr[2]}, b[3] = {tr[3], tr[4], tr[5]},
+                           c[3] = {tr[6], tr[7], tr[8]};
+                    double t = spo_rt_mt_f64(od, dd, a, b, c);
+                    if (t > 0.0 && (t < bt || bt < 0.0)) { bt = t; best = k; }
+                }
+                j->t64[r] = bt;
+            }
+            else
+            {
+                float bt = -1.0f;
+                for (uint32_t k = 0; k < j->ntris; ++k)
+                {
+                    const float *tr = j->tris + 9 * k;
+                    float t = spo_rt_mt(o, d, tr, tr + 3, tr + 6, NULL);
+                    if (t > 0.0f && (t < bt || bt < 0.0f)) { bt = t; best = k; }
+                }
+                j->t[r] = bt;
+            }
+            j->prim[r] = best;
+        }
+    }
+    return NULL;
+}
+
+static void brute_run(brute_job *j, uint32_t threads)
+{
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64;
+    pthread_t th[64];
+    atomic_init(&j->next, 0u);
+    for (uint32_t i = 0; i < threads; ++i)
+        pthread_create(&th[i], NULL, brute_worker, j);
+    for (uint32_t i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+}
+
+/* The closest-hit definition the GPU kernel is held to: among all triangles
+ * with a Moller-Trumbore hit at t > 0 (src/ray_intersection.cpp:156-190,
+ * acceptance as src/sp_scene.cpp:189-193), the smallest t; equal t goes to
+ * the lower triangle index. */
+void spo_closest_hit_brute(const float *tris9, uint32_t ntris,
+                           const float *rays6, uint32_t nrays,
+                           uint32_t threads, uint32_t *primOut, float *tOut)
+{
+    brute_job j;
+    memset(&j, 0, sizeof j);
+    j.tris = tris9; j.ntris = ntris; j.rays = rays6; j.nrays = nrays;
+    j.prim = primOut; j.t = tOut;
+    brute_run(&j, threads);
+}
+
+void spo_closest_hit_brute_f64(const float *tris9, uint32_t ntris,
+                               const float *rays6, uint32_t nrays,
+                               uint32_t threads, uint32_t *primOut,
+                               double *tOut)
+{
+    brute_job j;
+    memset(&j, 0, sizeof j);
+    j.tris = tris9; j.ntris = ntris; j.rays = rays6; j.nrays = nrays;
+    j.prim = primOut; j.t64 = tOut;
+    brute_run(&j, threads);
+}
+
+/* FP64 Moller-Trumbore t for given (ray, triangle) pairs. */
+void spo_mt_f64_pairs(const float *tris9, const float *rays6,
+                      const uint32_t *prim, uint32_t n, double *tOut)
+{
+    for (uint32_t r = 0; r < n; ++r)
+    {
+        if (prim[r] == 0xFFFFFFFFu) { tOut[r] = -1.0; continue; }
+        const float *o = rays6 + 6 * r, *d = o + 3, *tr = tris9 + 9 * (uint64_t)prim[r];
+        double od[3] = {o[0], o[1], o[2]}, dd[3] = {d[0], d[1], d[2]};
+        double a[3] = {tr[0], tr[1], tr[2]}, b[3] = {tr[3], tr[4], tr[5]},
+               c[3] = {tr[6], tr[7], tr[8]};
+        tOut[r] = spo_rt_mt_f64(od, dd, a, b, c);
+    }
+}
