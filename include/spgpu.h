@@ -1,0 +1,290 @@
+/*
+ * spgpu -- MI355X (gfx950) wavefront path tracer behind a C ABI.
+ *
+ * A drop-in for the reference's tile-based CPU tracer (deadVertex/
+ * vk_cinematic, `sp_` prefix).  Every entry point below names the reference
+ * interface it stands in for.  Conventions:
+ *
+ *   - Every call returns SPG_OK (0) or a negative SPG_E_* code; nothing
+ *     aborts.  spg_last_error() returns the calling thread's last message.
+ *   - Plain pointers and sizes only.  Host pointers unless the name ends in
+ *     _device; device pointers belong to the device passed in and are used on
+ *     that device's stream.
+ *   - Layouts match the reference: VertexPNT is 8 floats (position, normal,
+ *     uv; src/mesh.h:10-15), quaternions are xyzw (src/math_lib.h:99),
+ *     pixels are RGBA32F row-major x + y*width (src/simd_path_tracer.h:3-8).
+ *   - Data passed in is copied to HBM (the reference borrows the caller's
+ *     arrays, src/sp_scene.cpp:8-19).
+ *   - Calls on one device are serialized by the library; handles may be
+ *     shared between host threads.
+ */
+#ifndef SPGPU_H
+#define SPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPG_OK 0
+#define SPG_E_INVALID -1    /* bad argument */
+#define SPG_E_HIP -2        /* a HIP runtime call failed */
+#define SPG_E_CAPACITY -3   /* a fixed limit was exceeded */
+#define SPG_E_STATE -4      /* call order (e.g. trace before build) */
+#define SPG_E_NOT_FOUND -5  /* unknown id */
+
+#define SPG_NO_TEXTURE 0xFFFFFFFFu
+#define SPG_MISS 0xFFFFFFFFu
+
+/* ---- enums ---- */
+enum spg_RngMode {
+    SPG_RNG_XORSHIFT_PIXEL = 1, /* xorshift32 per pixel, seeded by
+                                   spg_pixel_seed(seed, y*width+x); samples of
+                                   a pixel draw from one stream in order */
+    SPG_RNG_PCG = 2             /* PCG32, stream = pixel index, sample s starts
+                                   2^20*s draws into the stream */
+};
+enum spg_Estimator {
+    SPG_EST_PARITY = 0,   /* the reference estimator (ComputeRadianceForPath,
+                             src/simd_path_tracer.cpp:107-175) */
+    SPG_EST_UNBIASED = 1, /* same materials, cosine sampling with 1/pdf */
+    SPG_EST_GUIDED = 2    /* unbiased + learned guiding, one-sample MIS */
+};
+enum spg_Transcendentals {
+    SPG_TX_DOUBLE = 1,    /* sin/cos/atan2/pow evaluated in double and
+                             rounded once to float (bit-reproducible) */
+    SPG_TX_FAST = 2       /* device float intrinsics (not bit-reproducible) */
+};
+
+/* sp_Metrics counters (src/sp_metrics.h:3-41).  Index 0 holds nanoseconds
+ * instead of rdtsc cycles; the other per-stage cycle counters stay 0. */
+enum {
+    SPG_METRIC_ELAPSED_NS = 0,
+    SPG_METRIC_PATHS_TRACED = 1,
+    SPG_METRIC_RAYS_TRACED = 2,
+    SPG_METRIC_RAY_HIT_COUNT = 3,
+    SPG_METRIC_RAY_MISS_COUNT = 4,
+    SPG_METRIC_MIDPHASE_AABB_TESTS = 10,
+    SPG_METRIC_MESH_TESTS = 11,
+    SPG_MAX_METRICS = 12
+};
+
+/* ---- POD types ---- */
+typedef struct spg_Device spg_Device;
+typedef struct spg_Scene spg_Scene;
+
+/* Tile, src/tile.h:3-9 */
+typedef struct { uint32_t minX, minY, maxX, maxY; } spg_Tile;
+
+/* sp_Metrics, src/sp_metrics.h:45-48 */
+typedef struct { uint64_t values[SPG_MAX_METRICS]; } spg_Metrics;
+
+/* sp_Material, src/sp_material_system.h:3-12 */
+typedef struct {
+    float albedo[3];
+    uint32_t albedoTexture;
+    float emission[3];
+    uint32_t emissionTexture;
+    float roughness;
+} spg_Material;
+
+/* sp_Camera, src/simd_path_tracer.h:17-32 (the image plane is implied by
+ * width/height; pixels are passed to the trace calls). */
+typedef struct {
+    float right[3], up[3], forward[3];
+    float position[3], filmCenter[3];
+    uint32_t width, height;
+    float halfPixelWidth, halfPixelHeight;
+    float halfFilmWidth, halfFilmHeight;
+} spg_Camera;
+
+/* Per-call tracing options (the reference's compile-time switches,
+ * src/config.h:25-37, and its hard-coded bounce count,
+ * src/simd_path_tracer.cpp:195). */
+typedef struct {
+    uint32_t samplesPerPixel;  /* SAMPLES_PER_PIXEL */
+    uint32_t bounces;          /* rays per path, 3 in the reference */
+    float radianceClamp;       /* RADIANCE_CLAMP, 0 = off */
+    uint32_t estimator;        /* spg_Estimator */
+    uint32_t rngMode;          /* spg_RngMode */
+    uint32_t seed;             /* base seed of the per-pixel streams */
+    uint32_t transcendentals;  /* spg_Transcendentals */
+    uint32_t sampleOffset;     /* first sample index (progressive passes) */
+    uint32_t collectMetrics;   /* count AABB/triangle/mesh tests */
+    uint32_t guideHandle;      /* 0, or a spg_guide handle for SPG_EST_GUIDED */
+} spg_TraceConfig;
+
+/* sp_Context, src/simd_path_tracer.h:34-46 */
+typedef struct {
+    const spg_Camera *camera;
+    spg_Scene *scene;
+    spg_TraceConfig config;
+} spg_Context;
+
+/* Scene build options (replaces sp_BuildMeshMidphase's greedy 4-ary tree,
+ * src/bvh.cpp:51-200, with a binned SAH BVH2). */
+typedef struct {
+    uint32_t maxLeafSize;   /* triangles per leaf, 1..8 (default 4) */
+    uint32_t sahBins;       /* SAH bins per axis (default 16) */
+    uint32_t threads;       /* host build threads, 0 = all cores */
+} spg_BuildOpts;
+
+/* Counters of one closest-hit launch (no reference counterpart beyond
+ * bvh_IntersectRayResult.aabbTestCount, src/bvh.h:20-25). */
+typedef struct {
+    uint64_t rays, boxTests, triTests, nodeFetches, hits;
+} spg_TraversalStats;
+
+/* ---- errors ---- */
+const char *spg_last_error(void);
+
+/* ---- lifecycle ---- */
+int spg_device_create(int ordinal, spg_Device **out);
+int spg_device_destroy(spg_Device *dev);
+int spg_device_synchronize(spg_Device *dev);
+/* Device memory helpers for callers without their own allocator. */
+int spg_device_alloc(spg_Device *dev, size_t bytes, void **out);
+int spg_device_free(spg_Device *dev, void *ptr);
+int spg_copy_to_device(spg_Device *dev, void *dst, const void *src, size_t bytes);
+int spg_copy_to_host(spg_Device *dev, void *dst, const void *src, size_t bytes);
+
+/* ---- scene: sp_InitializeScene / sp_CreateMesh / sp_BuildMeshMidphase /
+ *      sp_AddObjectToScene / sp_BuildSceneBroadphase (src/sp_scene.cpp:1-125) */
+int spg_scene_create(spg_Device *dev, spg_Scene **out);
+int spg_scene_destroy(spg_Scene *scene);
+/* sp_CreateMesh(VertexPNT*, u32, u32*, u32, b32) + sp_BuildMeshMidphase:
+ * verts8 is nv*8 floats, idx is ni u32 (ni % 3 == 0). */
+int spg_mesh_create(spg_Scene *scene, const float *verts8, uint32_t nv,
+                    const uint32_t *idx, uint32_t ni, int smoothShading,
+                    uint32_t *meshId);
+/* sp_AddObjectToScene(scene, mesh, material, position, orientation, scale) */
+int spg_scene_add_object(spg_Scene *scene, uint32_t meshId,
+                         uint32_t materialId, const float pos[3],
+                         const float quat[4], const float scale[3],
+                         uint32_t *objectId);
+/* sp_BuildSceneBroadphase + the per-mesh midphase builds; uploads to HBM. */
+int spg_scene_build(spg_Scene *scene, const spg_BuildOpts *opts);
+/* Host-only SAH build of a triangle list (tris9 = a,b,c per triangle), no
+ * device involved: nodes16 receives the 64-byte nodes (layout as
+ * spg_mesh_bvh_export), order the leaf primitive order. */
+int spg_bvh_build_host(const float *tris9, uint32_t n, const spg_BuildOpts *opts,
+                       float *nodes16, uint32_t nodeCap, uint32_t *order,
+                       uint32_t *nodeCount, uint32_t *maxDepth);
+/* Node/triangle counts and depth of a built mesh BVH. */
+int spg_mesh_bvh_info(spg_Scene *scene, uint32_t meshId, uint32_t *nodes,
+                      uint32_t *leaves, uint32_t *maxDepth, double *sahCost);
+/* Host copy of a built mesh BVH (for checkers): nodes as 16 floats each
+ * (child0 lo.xyz, child0 index bits, child0 hi.xyz, child1 index bits,
+ *  child1 lo.xyz, pad, child1 hi.xyz, pad), primitive order, root. */
+int spg_mesh_bvh_export(spg_Scene *scene, uint32_t meshId, float *nodes16,
+                        uint32_t nodeCap, uint32_t *primOrder,
+                        uint32_t primCap, int32_t *root);
+
+/* ---- materials: sp_RegisterMaterial / sp_RegisterTexture
+ *      (src/sp_material_system.cpp:1-57) and backgroundMaterialId
+ *      (src/sp_material_system.h:45) ---- */
+int spg_material_register(spg_Scene *scene, const spg_Material *material,
+                          uint32_t id);
+/* 2D RGBA32F texture (albedo by uv; as emission it is an equirect map). */
+int spg_texture_register(spg_Scene *scene, const float *rgba, uint32_t width,
+                         uint32_t height, uint32_t id);
+/* Cube-map RGBA32F texture, 6 faces of width*height in the face order and
+ * basis of src/cubemap.cpp:55-103; as emission it is looked up by direction. */
+int spg_cubemap_register(spg_Scene *scene, const float *rgba6, uint32_t width,
+                         uint32_t height, uint32_t id);
+int spg_set_background_material(spg_Scene *scene, uint32_t id);
+
+/* ---- camera: sp_ConfigureCamera (src/simd_path_tracer.cpp:1-36) ---- */
+int spg_camera_configure(spg_Camera *camera, uint32_t width, uint32_t height,
+                         const float pos[3], const float quat[4],
+                         float filmDistance);
+
+/* ---- tiles: ComputeTiles (src/tile.h:11-42) ---- */
+int spg_compute_tiles(uint32_t width, uint32_t height, uint32_t tileWidth,
+                      uint32_t tileHeight, spg_Tile *tiles, uint32_t maxTiles,
+                      uint32_t *count);
+
+/* ---- tracing ---- */
+/* sp_PathTraceTile(ctx, tile, rng, metrics), src/simd_path_tracer.cpp:178:
+ * overwrites the tile's pixels of the host image (width*height RGBA32F). */
+int spg_path_trace_tile(const spg_Context *ctx, spg_Tile tile,
+                        spg_Metrics *metrics, float *hostPixels);
+/* The work queue of src/main.cpp:819-844 in one call: all tiles are traced
+ * in one wavefront batch. */
+int spg_path_trace_tiles(const spg_Context *ctx, const spg_Tile *tiles,
+                         uint32_t count, spg_Metrics *metrics,
+                         float *hostPixels);
+/* Same, image in HBM (width*height*4 floats), no host copies. */
+int spg_path_trace_tiles_device(const spg_Context *ctx, const spg_Tile *tiles,
+                                uint32_t count, spg_Metrics *metrics,
+                                float *devicePixels);
+
+/* Closest hit in mesh space for a batch of rays: sp_RayIntersectMesh
+ * (src/sp_scene.cpp:127-227) over a SAH BVH.  Among triangles with a
+ * Moller-Trumbore hit at t > 0 (src/ray_intersection.cpp:156-190) the
+ * smallest t wins, equal t goes to the lower triangle index.  primId is
+ * SPG_MISS and t is -1 on a miss.  u, v may be NULL. */
+int spg_intersect_closest(spg_Scene *scene, uint32_t meshId,
+                          const float *ox, const float *oy, const float *oz,
+                          const float *dx, const float *dy, const float *dz,
+                          uint32_t n, uint32_t *primId, float *t, float *u,
+                          float *v);
+int spg_intersect_closest_device(spg_Scene *scene, uint32_t meshId,
+                                 const float *ox, const float *oy,
+                                 const float *oz, const float *dx,
+                                 const float *dy, const float *dz, uint32_t n,
+                                 uint32_t *primId, float *t, float *u,
+                                 float *v, spg_TraversalStats *stats);
+/* Scene-level closest hit in world space: sp_RayIntersectScene
+ * (src/sp_scene.cpp:229-339).  out7 per ray: t, materialId bits,
+ * normal.xyz, uv (7 floats); object/prim may be NULL. */
+int spg_intersect_scene(spg_Scene *scene, const float *rays6, uint32_t n,
+                        float *out7, uint32_t *object, uint32_t *prim);
+
+/* Times `iterations` launches of the closest-hit kernel on device-resident
+ * rays with HIP events on the launch stream; returns the mean kernel ms. */
+int spg_time_intersect_closest(spg_Scene *scene, uint32_t meshId,
+                               const float *ox, const float *oy,
+                               const float *oz, const float *dx,
+                               const float *dy, const float *dz, uint32_t n,
+                               uint32_t *primId, float *t, uint32_t iterations,
+                               double *meanKernelMs);
+/* Kernel timings of the last trace call on this device, in ms, summed over
+ * its launches: [0] raygen [1] extend [2] shade [3] finalize [4] guide
+ * [5] total wall; launches[i] counts launches. */
+int spg_last_trace_timings(spg_Device *dev, double *ms6, uint32_t *launches6);
+/* Raw counters of the last trace call on this device: rays, hits, misses,
+ * slab tests, mesh (instance) tests, triangle tests.  The test counts are
+ * collected only when the call's config set collectMetrics. */
+int spg_last_trace_counters(spg_Device *dev, uint64_t *counters6);
+
+/* ---- generators used by the benchmark scenes (no reference counterpart
+ *      except the icosphere, CreateIcosahedronMesh,
+ *      src/mesh_generation.cpp:220-337, and the plane, :1-21) ---- */
+int spg_gen_icosphere(uint32_t level, float *verts8, uint32_t vertCap,
+                      uint32_t *idx, uint32_t idxCap, uint32_t *nv,
+                      uint32_t *ni);
+int spg_gen_plane(float *verts8, uint32_t *idx);
+/* Triangle soup: centroids uniform in [0,1]^3, equilateral triangles of
+ * edge exp(N(mu, sigma)) in a random orientation; tris9 = a,b,c. */
+int spg_gen_triangle_soup(uint64_t seed, uint32_t n, float mu, float sigma,
+                          float *tris9);
+/* Sky gradient + sun disk environment baked into a cube map
+ * (6*size*size RGBA32F); sunDir need not be normalized. */
+int spg_gen_sky_cubemap(uint32_t size, const float sunDir[3],
+                        float sunAngularRadius, float sunRadiance,
+                        float *rgba6);
+/* The same environment as an equirect image for the reference's lookup. */
+int spg_gen_sky_equirect(uint32_t width, uint32_t height,
+                         const float sunDir[3], float sunAngularRadius,
+                         float sunRadiance, float *rgba);
+
+/* xorshift32 seed of pixel `index` in SPG_RNG_XORSHIFT_PIXEL mode. */
+uint32_t spg_pixel_seed(uint32_t base, uint32_t index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPGPU_H */

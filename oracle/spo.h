@@ -252,6 +252,18 @@ void spo_closest_hit_brute_f64(const float *tris9, uint32_t ntris,
                                uint32_t threads, uint32_t *primOut,
                                double *tOut);
 
+/* FP64 Moller-Trumbore t for given (ray, triangle) pairs (-1 = no hit) */
+void spo_mt_f64_pairs(const float *tris9, const float *rays6,
+                      const uint32_t *prim, uint32_t n, double *tOut);
+
+/* ---- the GPU's BVH2 traversal restated (spo_bvh2.c) ---- */
+void spo_bvh2_closest(const float *nodes16, const uint32_t *order,
+                      const float *tris9, const float *rays6, uint32_t n,
+                      uint32_t threads, uint32_t *primOut, float *tOut,
+                      uint64_t *boxTests, uint64_t *triTests);
+int spo_bvh2_check(const float *nodes16, uint32_t nNodes, const uint32_t *order,
+                   const float *tris9, uint32_t nTris);
+
 /* ---- path tracing ---- */
 int spo_path_trace_tile(const spo_Scene *s, const spo_Camera *c,
                         const spo_Config *cfg, uint32_t minX, uint32_t minY,

@@ -136,6 +136,9 @@ def oracle():
     _sig(L, "spo_closest_hit_brute_f64", None, f32p, C.c_uint32, f32p, C.c_uint32,
          C.c_uint32, u32p, f64p)
     _sig(L, "spo_mt_f64_pairs", None, f32p, f32p, u32p, C.c_uint32, f64p)
+    _sig(L, "spo_bvh2_closest", None, f32p, u32p, f32p, f32p, C.c_uint32,
+         C.c_uint32, u32p, f32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
+    _sig(L, "spo_bvh2_check", C.c_int, f32p, C.c_uint32, u32p, f32p, C.c_uint32)
     _sig(L, "spo_path_trace_tile", C.c_int, vp, C.POINTER(SpoCamera),
          C.POINTER(SpoConfig), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
          u32ref, f32p, u64p)

@@ -1,0 +1,62 @@
+"""Loads a scene description (vk_cinematic_amd.scenes) into the oracle and
+into the reference build, so checker and product trace identical inputs."""
+import ctypes as C
+
+import numpy as np
+
+from _libs import SpoCamera, f32, oracle, ref
+
+
+def load_oracle(desc, build_midphase=True):
+    """Returns (oracle scene handle, SpoCamera)."""
+    O = oracle()
+    s = O.spo_scene_create()
+    for v, i, smooth in desc["meshes"]:
+        v = np.ascontiguousarray(np.asarray(v, np.float32).ravel())
+        i = np.ascontiguousarray(np.asarray(i, np.uint32).ravel())
+        O.spo_mesh_create(s, v, v.size // 8, i, i.size, int(smooth),
+                          int(build_midphase))
+    for mesh, mat, pos, q, sc in desc["objects"]:
+        O.spo_scene_add_object(s, mesh, mat, f32(*pos), f32(*q), f32(*sc))
+    O.spo_scene_build(s)
+    for tid, rgba, w, h in desc.get("textures", []):
+        O.spo_texture_register(s, np.ascontiguousarray(rgba, np.float32), w, h,
+                               tid, 0)
+    for tid, rgba, size in desc.get("cubemaps", []):
+        O.spo_texture_register(s, np.ascontiguousarray(rgba, np.float32), size,
+                               size, tid, 1)
+    for mid, albedo, emission, rough, atex, etex in desc["materials"]:
+        O.spo_material_register(s, f32(*albedo), atex, f32(*emission), etex,
+                                rough, mid)
+    O.spo_set_background(s, desc["background"])
+    w, h, pos, q, fd = desc["camera"]
+    cam = SpoCamera()
+    O.spo_camera_configure(C.byref(cam), w, h, f32(*pos), f32(*q), fd)
+    return s, cam
+
+
+def load_reference(desc):
+    """Loads the description into the reference build (equirect textures
+    only); returns the library or None when it is not built."""
+    R = ref()
+    if R is None:
+        return None
+    R.ref_reset(512 << 20)
+    meshes = []
+    for v, i, smooth in desc["meshes"]:
+        v = np.ascontiguousarray(np.asarray(v, np.float32).ravel())
+        i = np.ascontiguousarray(np.asarray(i, np.uint32).ravel())
+        meshes.append(R.ref_mesh_create(v, v.size // 8, i, i.size, int(smooth)))
+    for mesh, mat, pos, q, sc in desc["objects"]:
+        R.ref_scene_add_object(meshes[mesh], mat, f32(*pos), f32(*q), f32(*sc))
+    R.ref_scene_build()
+    for tid, rgba, w, h in desc.get("textures", []):
+        R.ref_texture_register(np.ascontiguousarray(rgba, np.float32), w, h, tid)
+    assert not desc.get("cubemaps"), "the reference samples equirect maps only"
+    for mid, albedo, emission, rough, atex, etex in desc["materials"]:
+        R.ref_material_register(f32(*albedo), atex, f32(*emission), etex, rough,
+                                mid)
+    R.ref_set_background(desc["background"])
+    w, h, pos, q, fd = desc["camera"]
+    R.ref_camera_configure(w, h, f32(*pos), f32(*q), fd)
+    return R

@@ -1,0 +1,170 @@
+"""Closest-hit kernel on the GPU against the oracle.
+
+The kernel's definition (include/spgpu.h spg_intersect_closest): among the
+triangles with a Moller-Trumbore hit at t > 0 (src/ray_intersection.cpp:
+156-190) the smallest t wins, equal t goes to the lower triangle index.
+Bar: triangle index and t bit-exact against the oracle's brute force and its
+restatement of the GPU traversal; t error against FP64 stated.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import vk_cinematic_amd as V
+from vk_cinematic_amd import scenes
+from _libs import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(device, tris):
+    s = V.Scene(device)
+    v, i = scenes.soup_mesh(np.asarray(tris, np.float32).reshape(-1, 9))
+    m = s.create_mesh(v, i, smooth=False)
+    s.build()
+    return s, m
+
+
+def _brute(tris, o, d):
+    O = oracle()
+    rays = np.ascontiguousarray(np.concatenate([o, d], 1).astype(np.float32).ravel())
+    p = np.zeros(len(o), np.uint32)
+    t = np.zeros(len(o), np.float32)
+    O.spo_closest_hit_brute(np.ascontiguousarray(tris, np.float32).ravel(), len(tris),
+                            rays, len(o), 16, p, t)
+    return p, t
+
+
+def test_known_answer_triangles(device):
+    """TestRayIntersectTriangleHit/Miss/HitUV and the barycentric issue
+    (unit_tests/unit_tests.cpp:348-426) through the GPU kernel."""
+    tris = np.array([[-.5, -.5, -5, .5, -.5, -5, 0, .5, -5],
+                     [4.5, -.5, -5, 5.5, -.5, -5, 5, .5, -5],
+                     [0, 0, -5, 1, 0, -5, .5, 1, -5]], np.float32)
+    s, m = _scene(device, tris)
+    o = np.array([[0, 0, 1], [0, 0, 1], [.5, .5, 1], [10, 10, 1]], np.float32)
+    d = np.array([[0, 0, -1]] * 4, np.float32)
+    prim, t, u, v = s.intersect_closest(m, o, d, want_uv=True)
+    # ray 0 hits triangle 0 at t=6 (and triangle 2 at its vertex a, also t=6:
+    # equal t goes to the lower index)
+    assert prim[0] == 0 and abs(t[0] - 6) <= 1.2e-7
+    assert prim[2] == 2 and abs(t[2] - 6) < 1e-6
+    assert prim[3] == V.SPG_MISS and t[3] == -1
+    # the hit UV case: ray at triangle 2's first vertex
+    p2, t2, u2, v2 = s.intersect_closest(m, [[0, 0, 1]], [[0, 0, -1]], want_uv=True)
+    s1, m1 = _scene(device, tris[2:3])
+    p3, t3, u3, v3 = s1.intersect_closest(m1, [[0, 0, 1]], [[0, 0, -1]], want_uv=True)
+    assert p3[0] == 0 and abs(t3[0] - 6) <= 1.2e-7 and u3[0] == 0 and v3[0] == 0
+    # the barycentric-issue ray misses triangle 0
+    s2, m2 = _scene(device, tris[0:1])
+    p4, t4 = s2.intersect_closest(m2, [[.5, .5, 1]], [[0, 0, -1]])
+    assert p4[0] == V.SPG_MISS and t4[0] == -1
+
+
+@pytest.mark.parametrize("mu", [-2.5, -5.0])
+def test_closest_hit_matches_brute_force(device, mu):
+    tris = V.gen_triangle_soup(200_000, seed=11, mu=mu, sigma=0.5)
+    _, o, d = scenes.soup(16, 16384, seed=12)
+    s, m = _scene(device, tris)
+    prim, t = s.intersect_closest(m, o, d)
+    bp, bt = _brute(tris, o, d)
+    assert np.array_equal(prim, bp)
+    assert np.array_equal(t.view(np.uint32), bt.view(np.uint32))
+    assert (bp != V.SPG_MISS).sum() > (1000 if mu > -3 else 100)
+
+
+def test_edge_case_rays(device):
+    """Axis-parallel rays (infinite inverse direction components), rays
+    starting inside the hierarchy, a NaN direction, and rays through shared
+    edges and vertices of a fan, where equal t must go to the lower index."""
+    rng = np.random.default_rng(3)
+    tris = V.gen_triangle_soup(50_000, seed=4, mu=-2.0, sigma=0.3)
+    # a fan of triangles sharing the vertex (0.5, 0.5, 0.5) in the plane z=0.5
+    fan = []
+    for k in range(12):
+        a0, a1 = 2 * np.pi * k / 12, 2 * np.pi * (k + 1) / 12
+        fan.append([.5, .5, .5, .5 + .1 * np.cos(a0), .5 + .1 * np.sin(a0), .5,
+                    .5 + .1 * np.cos(a1), .5 + .1 * np.sin(a1), .5])
+    tris = np.concatenate([tris, np.array(fan, np.float32)]).astype(np.float32)
+    n = 4096
+    o = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    d[: n // 4, rng.integers(0, 3, n // 4)] = 0.0
+    d[n // 4: n // 4 + 64] = 0.0
+    d[n // 4: n // 4 + 64, rng.integers(0, 3, 64)] = 1.0
+    d[n // 2] = np.nan
+    o[n // 2 + 1: n // 2 + 17] = [.5, .5, 2.0]
+    d[n // 2 + 1: n // 2 + 17] = [0, 0, -1]
+    s, m = _scene(device, tris)
+    prim, t = s.intersect_closest(m, o, d)
+    bp, bt = _brute(tris, o, d)
+    assert np.array_equal(prim, bp)
+    assert np.array_equal(t.view(np.uint32), bt.view(np.uint32))
+    assert prim[n // 2] == V.SPG_MISS
+
+
+def test_empty_and_single_ray(device):
+    tris = V.gen_triangle_soup(100, seed=1, mu=-1.0)
+    s, m = _scene(device, tris)
+    prim, t = s.intersect_closest(m, np.zeros((0, 3)), np.zeros((0, 3)))
+    assert prim.size == 0
+    prim, t = s.intersect_closest(m, [[0.5, 0.5, -3]], [[0, 0, 1]])
+    bp, bt = _brute(tris, np.array([[0.5, 0.5, -3]], np.float32),
+                    np.array([[0, 0, 1]], np.float32))
+    assert prim[0] == bp[0] and t.view(np.uint32)[0] == bt.view(np.uint32)[0]
+
+
+def test_full_size_workload(device):
+    """2^22 rays against the 1M-triangle soup: every ray against the CPU
+    restatement of the traversal (identical triangle, t bits and test counts),
+    2048 rays against brute force, t error against FP64 stated."""
+    tris, o, d = scenes.soup(1_000_000, 1 << 22, seed=7)
+    s, m = _scene(device, tris)
+    n = len(o)
+    bufs = [device.upload(np.ascontiguousarray(x)) for x in
+            (o[:, 0], o[:, 1], o[:, 2], d[:, 0], d[:, 1], d[:, 2])]
+    outp, outt = device.alloc(4 * n), device.alloc(4 * n)
+    st = V.TraversalStats()
+    V.check(V.lib().spg_intersect_closest_device(
+        s.handle, m, *[b.ptr for b in bufs], n, outp.ptr, outt.ptr, None, None,
+        C.byref(st)))
+    prim = outp.download(np.uint32, n)
+    t = outt.download(np.float32, n)
+    # second launch: the result is reproducible
+    V.check(V.lib().spg_intersect_closest_device(
+        s.handle, m, *[b.ptr for b in bufs], n, outp.ptr, outt.ptr, None, None, None))
+    assert np.array_equal(prim, outp.download(np.uint32, n))
+
+    info = s.bvh_info(m)
+    nodes, order = s.bvh_export(m, info["nodes"], len(tris))
+    O = oracle()
+    rays = np.ascontiguousarray(np.concatenate([o, d], 1).ravel())
+    tp = np.zeros(n, np.uint32)
+    tt = np.zeros(n, np.float32)
+    box, tri = C.c_uint64(), C.c_uint64()
+    O.spo_bvh2_closest(np.ascontiguousarray(nodes.ravel()), order,
+                       np.ascontiguousarray(tris.ravel()), rays, n, 16, tp, tt,
+                       C.byref(box), C.byref(tri))
+    assert np.array_equal(prim, tp)
+    assert np.array_equal(t.view(np.uint32), tt.view(np.uint32))
+    assert st.boxTests == box.value and st.triTests == tri.value
+    assert st.rays == n and st.hits == (prim != V.SPG_MISS).sum()
+
+    sub = np.random.default_rng(0).choice(n, 2048, replace=False)
+    bp, bt = _brute(tris, o[sub], d[sub])
+    assert np.array_equal(prim[sub], bp)
+    assert np.array_equal(t[sub].view(np.uint32), bt.view(np.uint32))
+
+    hit = np.nonzero(prim != V.SPG_MISS)[0]
+    t64 = np.zeros(len(hit), np.float64)
+    O.spo_mt_f64_pairs(np.ascontiguousarray(tris.ravel()),
+                       np.ascontiguousarray(np.concatenate([o[hit], d[hit]], 1).ravel()),
+                       np.ascontiguousarray(prim[hit]), len(hit), t64)
+    ok = t64 > 0
+    rel = np.abs(t[hit][ok] - t64[ok]) / t64[ok]
+    print(f"hits {len(hit)}; FP32 t vs FP64: median rel err {np.median(rel):.2e}, "
+          f"99.9th pct {np.quantile(rel, 0.999):.2e}, max {rel.max():.2e}")
+    assert np.median(rel) < 1e-6 and np.quantile(rel, 0.999) < 1e-4

@@ -1,0 +1,1177 @@
+// C-ABI implementation: device and scene lifecycle, SAH builds, HBM upload
+// and the wavefront trace driver.  Kernels live in spg_trace.hip.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "spg_internal.h"
+
+namespace spg {
+
+static thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return SPG_OK;
+    return fail(SPG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- host mat4 arithmetic in the reference's order (src/math_lib.h) ----
+struct M4 { float m[16]; };
+static M4 m4zero() { M4 r; memset(r.m, 0, 64); return r; }
+static M4 m4identity()
+{
+    M4 r = m4zero();
+    r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.0f;
+    return r;
+}
+static M4 m4scale(v3 s)
+{
+    M4 r = m4zero();
+    r.m[0] = s.x; r.m[5] = s.y; r.m[10] = s.z; r.m[15] = 1.0f;
+    return r;
+}
+static M4 m4translate(v3 t)
+{
+    M4 r = m4identity();
+    r.m[12] = t.x; r.m[13] = t.y; r.m[14] = t.z;
+    return r;
+}
+// operator*(mat4, mat4), src/math_lib.h:250-270
+static M4 m4mul(const M4 &a, const M4 &b)
+{
+    M4 r = m4zero();
+    for (int i = 0; i < 4; ++i)
+    {
+        float row[4] = {a.m[i], a.m[4 + i], a.m[8 + i], a.m[12 + i]};
+        for (int c = 0; c < 4; ++c)
+            r.m[4 * c + i] = row[0] * b.m[4 * c] + row[1] * b.m[4 * c + 1] +
+                             row[2] * b.m[4 * c + 2] + row[3] * b.m[4 * c + 3];
+    }
+    return r;
+}
+struct Q { float x, y, z, w; };
+// quat operator*, src/math_lib.h:565-571
+static Q qmul(Q p, Q q)
+{
+    v3 pv{p.x, p.y, p.z}, qv{q.x, q.y, q.z};
+    v3 v = (qv * p.w + pv * q.w) + cross(pv, qv);
+    return Q{v.x, v.y, v.z, p.w * q.w - dot(pv, qv)};
+}
+static Q qconj(Q p) { return Q{-p.x, -p.y, -p.z, p.w}; }
+// RotateVector, src/math_lib.h:588-593
+static v3 qrotate(v3 v, Q p)
+{
+    Q q{v.x, v.y, v.z, 0.0f};
+    Q r = qmul(qmul(p, q), qconj(p));
+    return v3{r.x, r.y, r.z};
+}
+// Rotate(quat), src/math_lib.h:602-623
+static M4 m4rotate(Q q)
+{
+    M4 r = m4identity();
+    float x = q.x, y = q.y, z = q.z, w = q.w;
+    r.m[0] = 1.0f - 2.0f * y * y - 2.0f * z * z;
+    r.m[1] = 2.0f * x * y + 2.0f * z * w;
+    r.m[2] = 2.0f * x * z - 2.0f * y * w;
+    r.m[4] = 2.0f * x * y - 2.0f * z * w;
+    r.m[5] = 1.0f - 2.0f * x * x - 2.0f * z * z;
+    r.m[6] = 2.0f * y * z + 2.0f * x * w;
+    r.m[8] = 2.0f * x * z + 2.0f * y * w;
+    r.m[9] = 2.0f * y * z - 2.0f * x * w;
+    r.m[10] = 1.0f - 2.0f * x * x - 2.0f * y * y;
+    return r;
+}
+static v3 m4point(const M4 &a, v3 p)
+{
+    float r[3];
+    for (int i = 0; i < 3; ++i)
+        r[i] = a.m[i] * p.x + a.m[4 + i] * p.y + a.m[8 + i] * p.z +
+               a.m[12 + i] * 1.0f;
+    return v3{r[0], r[1], r[2]};
+}
+static m34 rows34(const float *m)
+{
+    m34 r;
+    for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 4; ++c) r.r[i][c] = m[4 * c + i];
+    return r;
+}
+
+template <typename T>
+static int upload(void **dst, const T *src, size_t count)
+{
+    if (*dst) hipFree(*dst);
+    *dst = nullptr;
+    size_t bytes = std::max<size_t>(sizeof(T) * count, 16);
+    int r = hip_check(hipMalloc(dst, bytes), "hipMalloc");
+    if (r) return r;
+    if (count) return hip_check(hipMemcpy(*dst, src, sizeof(T) * count,
+                                          hipMemcpyHostToDevice),
+                                "hipMemcpy");
+    return SPG_OK;
+}
+
+static void free_ptr(void *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace spg
+
+using namespace spg;
+
+#define CHK(x)                  \
+    do                          \
+    {                           \
+        int _r = (x);           \
+        if (_r) return _r;      \
+    } while (0)
+#define HIPCHK(x) CHK(spg::hip_check((x), #x))
+
+extern "C" {
+
+const char *spg_last_error(void) { return g_err.c_str(); }
+
+uint32_t spg_pixel_seed(uint32_t base, uint32_t index)
+{
+    return pixel_seed(base, index);
+}
+
+// ------------------------------------------------------------ lifecycle
+int spg_device_create(int ordinal, spg_Device **out)
+{
+    if (!out) return fail(SPG_E_INVALID, "spg_device_create: out is NULL");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (ordinal < 0 || ordinal >= n)
+        return fail(SPG_E_INVALID, "spg_device_create: no device " +
+                                       std::to_string(ordinal));
+    HIPCHK(hipSetDevice(ordinal));
+    spg_Device *d = new spg_Device();
+    d->ordinal = ordinal;
+    hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+    {
+        delete d;
+        return hip_check(e, "hipStreamCreate");
+    }
+    for (int i = 0; i < 64; ++i) hipEventCreate(&d->ev[i]);
+    d->eventsMade = true;
+    *out = d;
+    return SPG_OK;
+}
+
+static void free_paths(spg_Device *d)
+{
+    free_ptr(d->pathsBlock);
+    d->pathsBytes = 0;
+    memset(&d->paths, 0, sizeof d->paths);
+}
+
+static void delete_device(spg_Device *d)
+{
+    hipSetDevice(d->ordinal);
+    hipStreamSynchronize(d->stream);
+    free_paths(d);
+    free_ptr(d->image);
+    free_ptr(d->slotPixel);
+    free_ptr(d->accum);
+    if (d->eventsMade)
+        for (int i = 0; i < 64; ++i) hipEventDestroy(d->ev[i]);
+    hipStreamDestroy(d->stream);
+    delete d;
+}
+
+// A device outlives its scenes: destroying it while scenes are alive defers
+// the release to the last spg_scene_destroy.
+int spg_device_destroy(spg_Device *d)
+{
+    if (!d) return SPG_OK;
+    {
+        std::lock_guard<std::mutex> g(d->lock);
+        if (d->liveScenes > 0)
+        {
+            d->destroyRequested = true;
+            return SPG_OK;
+        }
+    }
+    delete_device(d);
+    return SPG_OK;
+}
+
+int spg_device_synchronize(spg_Device *d)
+{
+    if (!d) return fail(SPG_E_INVALID, "device is NULL");
+    HIPCHK(hipSetDevice(d->ordinal));
+    return hip_check(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+}
+
+int spg_device_alloc(spg_Device *d, size_t bytes, void **out)
+{
+    if (!d || !out) return fail(SPG_E_INVALID, "spg_device_alloc: NULL");
+    HIPCHK(hipSetDevice(d->ordinal));
+    return hip_check(hipMalloc(out, bytes ? bytes : 16), "hipMalloc");
+}
+
+int spg_device_free(spg_Device *d, void *p)
+{
+    if (!d) return fail(SPG_E_INVALID, "spg_device_free: NULL device");
+    HIPCHK(hipSetDevice(d->ordinal));
+    return hip_check(hipFree(p), "hipFree");
+}
+
+int spg_copy_to_device(spg_Device *d, void *dst, const void *src, size_t bytes)
+{
+    if (!d) return fail(SPG_E_INVALID, "NULL device");
+    HIPCHK(hipSetDevice(d->ordinal));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, d->stream));
+    return hip_check(hipStreamSynchronize(d->stream), "sync");
+}
+
+int spg_copy_to_host(spg_Device *d, void *dst, const void *src, size_t bytes)
+{
+    if (!d) return fail(SPG_E_INVALID, "NULL device");
+    HIPCHK(hipSetDevice(d->ordinal));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, d->stream));
+    return hip_check(hipStreamSynchronize(d->stream), "sync");
+}
+
+// ---------------------------------------------------------------- scene
+int spg_scene_create(spg_Device *dev, spg_Scene **out)
+{
+    if (!dev || !out) return fail(SPG_E_INVALID, "spg_scene_create: NULL");
+    std::lock_guard<std::mutex> g(dev->lock);
+    if (dev->destroyRequested)
+        return fail(SPG_E_STATE, "spg_scene_create: device is being destroyed");
+    spg_Scene *s = new spg_Scene();
+    s->dev = dev;
+    dev->liveScenes++;
+    *out = s;
+    return SPG_OK;
+}
+
+static void free_scene_device(spg_Scene *s)
+{
+    for (auto &m : s->meshes)
+    {
+        free_ptr(m.dNodes);
+        free_ptr(m.dTris);
+        free_ptr(m.dVerts);
+        free_ptr(m.dIdx);
+    }
+    free_ptr(s->dTlasNodes);
+    free_ptr(s->dTlasInst);
+    free_ptr(s->dInst);
+    free_ptr(s->dMeshes);
+    free_ptr(s->dMats);
+    free_ptr(s->dTexs);
+    free_ptr(s->dTexels);
+}
+
+int spg_scene_destroy(spg_Scene *s)
+{
+    if (!s) return SPG_OK;
+    spg_Device *d = s->dev;
+    bool last;
+    {
+        std::lock_guard<std::mutex> g(d->lock);
+        hipSetDevice(d->ordinal);
+        hipStreamSynchronize(d->stream);
+        free_scene_device(s);
+        delete s;
+        d->liveScenes--;
+        last = d->destroyRequested && d->liveScenes == 0;
+    }
+    if (last) delete_device(d);
+    return SPG_OK;
+}
+
+int spg_mesh_create(spg_Scene *s, const float *verts8, uint32_t nv,
+                    const uint32_t *idx, uint32_t ni, int smooth,
+                    uint32_t *meshId)
+{
+    if (!s || (!verts8 && nv) || (!idx && ni))
+        return fail(SPG_E_INVALID, "spg_mesh_create: NULL argument");
+    if (ni % 3 != 0 || nv == 0 || ni == 0)
+        return fail(SPG_E_INVALID,
+                    "spg_mesh_create: need nv > 0 and a positive multiple of 3 indices");
+    for (uint32_t i = 0; i < ni; ++i)
+        if (idx[i] >= nv)
+            return fail(SPG_E_INVALID, "spg_mesh_create: index out of range");
+    std::lock_guard<std::mutex> g(s->dev->lock);
+    s->meshes.emplace_back();
+    HostMesh &m = s->meshes.back();
+    m.verts.assign(verts8, verts8 + (size_t)nv * 8);
+    m.idx.assign(idx, idx + ni);
+    m.smooth = smooth ? 1 : 0;
+    // ComputeAabb over all vertices (src/sp_scene.cpp:56-74)
+    v3 lo{verts8[0], verts8[1], verts8[2]}, hi = lo;
+    for (uint32_t i = 1; i < nv; ++i)
+    {
+        v3 p{verts8[8 * i], verts8[8 * i + 1], verts8[8 * i + 2]};
+        lo = vmin(lo, p);
+        hi = vmax(hi, p);
+    }
+    memcpy(m.boundsLo, &lo, 12);
+    memcpy(m.boundsHi, &hi, 12);
+    s->built = false;
+    if (meshId) *meshId = (uint32_t)(s->meshes.size() - 1);
+    return SPG_OK;
+}
+
+int spg_scene_add_object(spg_Scene *s, uint32_t meshId, uint32_t materialId,
+                         const float pos[3], const float quat[4],
+                         const float scale[3], uint32_t *objectId)
+{
+    if (!s || !pos || !quat || !scale)
+        return fail(SPG_E_INVALID, "spg_scene_add_object: NULL argument");
+    std::lock_guard<std::mutex> g(s->dev->lock);
+    if (meshId >= s->meshes.size())
+        return fail(SPG_E_NOT_FOUND, "spg_scene_add_object: unknown mesh");
+    HostObject o;
+    o.mesh = meshId;
+    o.materialId = materialId;
+    memcpy(o.pos, pos, 12);
+    memcpy(o.quat, quat, 16);
+    memcpy(o.scale, scale, 12);
+    v3 P{pos[0], pos[1], pos[2]}, S{scale[0], scale[1], scale[2]};
+    Q qq{quat[0], quat[1], quat[2], quat[3]};
+    // sp_AddObjectToScene, src/sp_scene.cpp:77-117
+    M4 model = m4mul(m4mul(m4translate(P), m4rotate(qq)), m4scale(S));
+    M4 inv = m4mul(m4mul(m4scale(inverse(S)), m4rotate(qconj(qq))),
+                   m4translate(-P));
+    memcpy(o.model, model.m, 64);
+    memcpy(o.inv, inv.m, 64);
+    // TransformAabb, src/aabb.h:29-57
+    const HostMesh &m = s->meshes[meshId];
+    v3 lo, hi;
+    memcpy(&lo, m.boundsLo, 12);
+    memcpy(&hi, m.boundsHi, 12);
+    v3 c[8] = {{lo.x, lo.y, lo.z}, {hi.x, lo.y, lo.z}, {hi.x, lo.y, hi.z},
+               {lo.x, lo.y, hi.z}, {lo.x, hi.y, lo.z}, {hi.x, hi.y, lo.z},
+               {hi.x, hi.y, hi.z}, {lo.x, hi.y, hi.z}};
+    v3 rmin = m4point(model, c[0]), rmax = rmin;
+    for (int i = 1; i < 8; ++i)
+    {
+        v3 p = m4point(model, c[i]);
+        rmin = vmin(rmin, p);
+        rmax = vmax(rmax, p);
+    }
+    memcpy(o.aabbLo, &rmin, 12);
+    memcpy(o.aabbHi, &rmax, 12);
+    s->objects.push_back(o);
+    s->built = false;
+    if (objectId) *objectId = (uint32_t)(s->objects.size() - 1);
+    return SPG_OK;
+}
+
+static int build_mesh(HostMesh &m, const spg_BuildOpts &opt)
+{
+    uint32_t tc = (uint32_t)(m.idx.size() / 3);
+    std::vector<float> lo(3 * (size_t)tc), hi(3 * (size_t)tc);
+    for (uint32_t i = 0; i < tc; ++i)
+    {
+        const float *a = &m.verts[8 * (size_t)m.idx[3 * i]];
+        const float *b = &m.verts[8 * (size_t)m.idx[3 * i + 1]];
+        const float *c = &m.verts[8 * (size_t)m.idx[3 * i + 2]];
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[3 * i + k] = fmin_sel(a[k], fmin_sel(b[k], c[k]));
+            hi[3 * i + k] = fmax_sel(a[k], fmax_sel(b[k], c[k]));
+        }
+    }
+    BvhBuildInput in{lo.data(), hi.data(), tc, opt.maxLeafSize, opt.sahBins,
+                     opt.threads, kMaxBlasDepth};
+    if (!build_bvh(in, m.bvh))
+        return fail(SPG_E_INVALID, "mesh BVH build failed");
+    if (m.bvh.maxDepth > kMaxBlasDepth)
+        return fail(SPG_E_CAPACITY, "mesh BVH deeper than the traversal stack");
+    // triangles in leaf order: (a, e1 = b - a, e2 = c - a, prim)
+    std::vector<float4> tris(3 * (size_t)tc);
+    for (uint32_t k = 0; k < tc; ++k)
+    {
+        uint32_t prim = m.bvh.prims[k];
+        const float *a = &m.verts[8 * (size_t)m.idx[3 * prim]];
+        const float *b = &m.verts[8 * (size_t)m.idx[3 * prim + 1]];
+        const float *c = &m.verts[8 * (size_t)m.idx[3 * prim + 2]];
+        float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        float primBits;
+        memcpy(&primBits, &prim, 4);
+        tris[3 * k] = make_float4(a[0], a[1], a[2], e1[0]);
+        tris[3 * k + 1] = make_float4(e1[1], e1[2], e2[0], e2[1]);
+        tris[3 * k + 2] = make_float4(e2[2], primBits, 0.0f, 0.0f);
+    }
+    CHK(upload(&m.dNodes, m.bvh.nodes.data(), m.bvh.nodes.size()));
+    CHK(upload(&m.dTris, tris.data(), tris.size()));
+    CHK(upload(&m.dVerts, m.verts.data(), m.verts.size()));
+    CHK(upload(&m.dIdx, m.idx.data(), m.idx.size()));
+    m.built = true;
+    return SPG_OK;
+}
+
+static int find_index(const std::vector<uint32_t> &keys, uint32_t id)
+{
+    for (size_t i = 0; i < keys.size(); ++i)
+        if (keys[i] == id) return (int)i;
+    return -1;
+}
+
+// Uploads the material and texture tables and the instance records (which
+// hold resolved material slots), as sp_FindMaterialById/sp_FindTexture would
+// resolve them (first registration of an id wins).
+static int upload_tables(spg_Scene *s)
+{
+    std::vector<uint32_t> texKeys;
+    for (auto &t : s->textures) texKeys.push_back(t.id);
+    std::vector<DevTexture> texs;
+    std::vector<float> texels;
+    for (auto &t : s->textures)
+    {
+        DevTexture d{};
+        d.offset = texels.size() / 4;
+        d.width = t.width;
+        d.height = t.height;
+        d.isCube = t.isCube;
+        texels.insert(texels.end(), t.rgba.begin(), t.rgba.end());
+        texs.push_back(d);
+    }
+    std::vector<DevMaterial> mats;
+    for (auto &m : s->materials)
+    {
+        DevMaterial d{};
+        memcpy(d.albedo, m.albedo, 12);
+        memcpy(d.emission, m.emission, 12);
+        d.roughness = m.roughness;
+        d.albedoTex = find_index(texKeys, m.albedoTexture);
+        d.emissionTex = find_index(texKeys, m.emissionTexture);
+        mats.push_back(d);
+    }
+    std::vector<DevInstance> inst;
+    for (auto &o : s->objects)
+    {
+        DevInstance d{};
+        d.model = rows34(o.model);
+        d.inv = rows34(o.inv);
+        float smin = std::min(fabsf(o.scale[0]),
+                              std::min(fabsf(o.scale[1]), fabsf(o.scale[2])));
+        d.cullScale = smin > 0.0f ? 1.0f / smin : INFINITY;
+        d.mesh = o.mesh;
+        d.material = find_index(s->materialKeys, o.materialId);
+        d.materialId = o.materialId;
+        inst.push_back(d);
+    }
+    CHK(upload(&s->dTexs, texs.data(), texs.size()));
+    CHK(upload(&s->dTexels, reinterpret_cast<const float4 *>(texels.data()),
+               texels.size() / 4));
+    CHK(upload(&s->dMats, mats.data(), mats.size()));
+    CHK(upload(&s->dInst, inst.data(), inst.size()));
+    DevScene &ds = s->dev_scene;
+    ds.texs = (const DevTexture *)s->dTexs;
+    ds.texels = (const float4 *)s->dTexels;
+    ds.mats = (const DevMaterial *)s->dMats;
+    ds.inst = (const DevInstance *)s->dInst;
+    ds.background = find_index(s->materialKeys, s->backgroundId);
+    s->dirty = false;
+    return SPG_OK;
+}
+
+int spg_scene_build(spg_Scene *s, const spg_BuildOpts *opts)
+{
+    if (!s) return fail(SPG_E_INVALID, "spg_scene_build: NULL scene");
+    spg_BuildOpts opt{4, 16, 0};
+    if (opts) opt = *opts;
+    if (opt.maxLeafSize == 0) opt.maxLeafSize = 4;
+    if (opt.maxLeafSize > 8)
+        return fail(SPG_E_INVALID, "spg_scene_build: maxLeafSize > 8");
+    if (opt.sahBins == 0) opt.sahBins = 16;
+    std::lock_guard<std::mutex> g(s->dev->lock);
+    HIPCHK(hipSetDevice(s->dev->ordinal));
+    for (auto &m : s->meshes)
+        if (!m.built) CHK(build_mesh(m, opt));
+    std::vector<DevMesh> dm;
+    for (auto &m : s->meshes)
+    {
+        DevMesh d{};
+        d.nodes = (const float4 *)m.dNodes;
+        d.tris = (const float4 *)m.dTris;
+        d.verts = (const float *)m.dVerts;
+        d.idx = (const uint32_t *)m.dIdx;
+        d.smooth = m.smooth;
+        d.triCount = (uint32_t)(m.idx.size() / 3);
+        dm.push_back(d);
+    }
+    CHK(upload(&s->dMeshes, dm.data(), dm.size()));
+    // broadphase: sp_BuildSceneBroadphase over object boxes
+    // (src/sp_scene.cpp:119-125), here a SAH BVH2 with one object per leaf
+    s->tlas = BvhBuildResult();
+    uint32_t no = (uint32_t)s->objects.size();
+    if (no > 0)
+    {
+        std::vector<float> lo(3 * (size_t)no), hi(3 * (size_t)no);
+        for (uint32_t i = 0; i < no; ++i)
+        {
+            memcpy(&lo[3 * i], s->objects[i].aabbLo, 12);
+            memcpy(&hi[3 * i], s->objects[i].aabbHi, 12);
+        }
+        BvhBuildInput in{lo.data(), hi.data(), no, 1, opt.sahBins, opt.threads,
+                         kMaxTlasDepth};
+        build_bvh(in, s->tlas);
+        if (s->tlas.maxDepth > kMaxTlasDepth)
+            return fail(SPG_E_CAPACITY, "instance BVH too deep");
+        CHK(upload(&s->dTlasNodes, s->tlas.nodes.data(), s->tlas.nodes.size()));
+        CHK(upload(&s->dTlasInst, s->tlas.prims.data(), s->tlas.prims.size()));
+    }
+    DevScene &ds = s->dev_scene;
+    ds.tlasNodes = (const float4 *)s->dTlasNodes;
+    ds.tlasInst = (const uint32_t *)s->dTlasInst;
+    ds.meshes = (const DevMesh *)s->dMeshes;
+    ds.instCount = no;
+    CHK(upload_tables(s));
+    s->built = true;
+    return SPG_OK;
+}
+
+int spg_bvh_build_host(const float *tris9, uint32_t n, const spg_BuildOpts *opts,
+                       float *nodes16, uint32_t nodeCap, uint32_t *order,
+                       uint32_t *nodeCount, uint32_t *maxDepth)
+{
+    if (!tris9 || n == 0 || !nodes16 || !order || !nodeCount)
+        return fail(SPG_E_INVALID, "spg_bvh_build_host: bad argument");
+    spg_BuildOpts opt{4, 16, 0};
+    if (opts) opt = *opts;
+    if (opt.maxLeafSize == 0) opt.maxLeafSize = 4;
+    if (opt.maxLeafSize > 8)
+        return fail(SPG_E_INVALID, "spg_bvh_build_host: maxLeafSize > 8");
+    std::vector<float> lo(3 * (size_t)n), hi(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k)
+        {
+            const float *t = tris9 + 9 * (size_t)i;
+            lo[3 * i + k] = fmin_sel(t[k], fmin_sel(t[3 + k], t[6 + k]));
+            hi[3 * i + k] = fmax_sel(t[k], fmax_sel(t[3 + k], t[6 + k]));
+        }
+    BvhBuildResult b;
+    BvhBuildInput in{lo.data(), hi.data(), n, opt.maxLeafSize,
+                     opt.sahBins ? opt.sahBins : 16, opt.threads, kMaxBlasDepth};
+    build_bvh(in, b);
+    if (b.nodes.size() > nodeCap)
+        return fail(SPG_E_CAPACITY, "spg_bvh_build_host: nodeCap too small");
+    memcpy(nodes16, b.nodes.data(), 64 * b.nodes.size());
+    memcpy(order, b.prims.data(), 4 * b.prims.size());
+    *nodeCount = (uint32_t)b.nodes.size();
+    if (maxDepth) *maxDepth = b.maxDepth;
+    return SPG_OK;
+}
+
+int spg_mesh_bvh_info(spg_Scene *s, uint32_t meshId, uint32_t *nodes,
+                      uint32_t *leaves, uint32_t *maxDepth, double *sahCost)
+{
+    if (!s || meshId >= s->meshes.size() || !s->meshes[meshId].built)
+        return fail(SPG_E_STATE, "spg_mesh_bvh_info: mesh not built");
+    const BvhBuildResult &b = s->meshes[meshId].bvh;
+    if (nodes) *nodes = (uint32_t)b.nodes.size();
+    if (leaves) *leaves = b.leafCount;
+    if (maxDepth) *maxDepth = b.maxDepth;
+    if (sahCost) *sahCost = b.sahCost;
+    return SPG_OK;
+}
+
+int spg_mesh_bvh_export(spg_Scene *s, uint32_t meshId, float *nodes16,
+                        uint32_t nodeCap, uint32_t *primOrder,
+                        uint32_t primCap, int32_t *root)
+{
+    if (!s || meshId >= s->meshes.size() || !s->meshes[meshId].built)
+        return fail(SPG_E_STATE, "spg_mesh_bvh_export: mesh not built");
+    const BvhBuildResult &b = s->meshes[meshId].bvh;
+    if (nodeCap < b.nodes.size() || primCap < b.prims.size())
+        return fail(SPG_E_CAPACITY, "spg_mesh_bvh_export: buffers too small");
+    memcpy(nodes16, b.nodes.data(), 64 * b.nodes.size());
+    memcpy(primOrder, b.prims.data(), 4 * b.prims.size());
+    if (root) *root = 0;
+    return SPG_OK;
+}
+
+// ------------------------------------------------------------ materials
+int spg_material_register(spg_Scene *s, const spg_Material *m, uint32_t id)
+{
+    if (!s || !m) return fail(SPG_E_INVALID, "spg_material_register: NULL");
+    std::lock_guard<std::mutex> g(s->dev->lock);
+    if (s->materials.size() >= 4096)
+        return fail(SPG_E_CAPACITY, "spg_material_register: table full");
+    s->materialKeys.push_back(id);
+    s->materials.push_back(*m);
+    s->dirty = true;
+    return SPG_OK;
+}
+
+static int add_texture(spg_Scene *s, const float *rgba, uint32_t w, uint32_t h,
+                       uint32_t id, uint32_t cube)
+{
+    if (!s || !rgba || w == 0 || h == 0)
+        return fail(SPG_E_INVALID, "texture: NULL pixels or zero size");
+    std::lock_guard<std::mutex> g(s->dev->lock);
+    if (s->textures.size() >= 256)
+        return fail(SPG_E_CAPACITY, "texture table full");
+    HostTexture t;
+    t.id = id;
+    t.width = w;
+    t.height = h;
+    t.isCube = cube;
+    size_t n = (size_t)w * h * 4 * (cube ? 6 : 1);
+    t.rgba.assign(rgba, rgba + n);
+    s->textures.push_back(std::move(t));
+    s->dirty = true;
+    return SPG_OK;
+}
+
+int spg_texture_register(spg_Scene *s, const float *rgba, uint32_t w,
+                         uint32_t h, uint32_t id)
+{
+    return add_texture(s, rgba, w, h, id, 0);
+}
+
+int spg_cubemap_register(spg_Scene *s, const float *rgba6, uint32_t w,
+                         uint32_t h, uint32_t id)
+{
+    return add_texture(s, rgba6, w, h, id, 1);
+}
+
+int spg_set_background_material(spg_Scene *s, uint32_t id)
+{
+    if (!s) return fail(SPG_E_INVALID, "NULL scene");
+    std::lock_guard<std::mutex> g(s->dev->lock);
+    s->backgroundId = id;
+    s->dirty = true;
+    return SPG_OK;
+}
+
+// ---------------------------------------------------------------- camera
+int spg_camera_configure(spg_Camera *c, uint32_t w, uint32_t h,
+                         const float pos[3], const float quat[4],
+                         float filmDistance)
+{
+    if (!c || !pos || !quat || w == 0 || h == 0)
+        return fail(SPG_E_INVALID, "spg_camera_configure: bad argument");
+    // sp_ConfigureCamera, src/simd_path_tracer.cpp:1-36
+    Q q{quat[0], quat[1], quat[2], quat[3]};
+    v3 P{pos[0], pos[1], pos[2]};
+    v3 right = qrotate(v3{1, 0, 0}, q);
+    v3 up = qrotate(v3{0, 1, 0}, q);
+    v3 fwd = qrotate(v3{0, 0, -1}, q);
+    v3 fc = P + fwd * filmDistance;
+    memcpy(c->right, &right, 12);
+    memcpy(c->up, &up, 12);
+    memcpy(c->forward, &fwd, 12);
+    memcpy(c->position, &P, 12);
+    memcpy(c->filmCenter, &fc, 12);
+    c->width = w;
+    c->height = h;
+    c->halfPixelWidth = 0.5f / (float)w;
+    c->halfPixelHeight = 0.5f / (float)h;
+    float filmWidth = 1.0f, filmHeight = 1.0f;
+    if (w > h) filmHeight = (float)h / (float)w;
+    else if (w < h) filmWidth = (float)w / (float)h;
+    c->halfFilmWidth = 0.5f * filmWidth;
+    c->halfFilmHeight = 0.5f * filmHeight;
+    return SPG_OK;
+}
+
+// ComputeTiles, src/tile.h:11-42
+int spg_compute_tiles(uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
+                      spg_Tile *tiles, uint32_t maxTiles, uint32_t *count)
+{
+    if (tw == 0 || th == 0 || (!tiles && maxTiles) || !count)
+        return fail(SPG_E_INVALID, "spg_compute_tiles: bad argument");
+    uint32_t ny = (uint32_t)ceilf((float)H / (float)th);
+    uint32_t nx = (uint32_t)ceilf((float)W / (float)tw);
+    for (uint32_t ty = 0; ty < ny; ++ty)
+        for (uint32_t tx = 0; tx < nx; ++tx)
+        {
+            uint32_t index = tx + ty * nx;
+            if (index >= maxTiles) break;
+            spg_Tile t;
+            t.minX = tx * tw;
+            t.minY = ty * th;
+            t.maxX = std::min(t.minX + tw, W);
+            t.maxY = std::min(t.minY + th, H);
+            tiles[index] = t;
+        }
+    *count = std::min(ny * nx, maxTiles);
+    return SPG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ tracing
+namespace spg {
+
+constexpr uint32_t kMaxPathsInFlight = 1u << 21;
+constexpr uint32_t kMaxBounces = 16;
+
+static int ensure_paths(spg_Device *d, uint32_t capacity, uint32_t bounces)
+{
+    if (d->pathsBlock && d->paths.capacity >= capacity &&
+        d->paths.bounces >= bounces)
+        return SPG_OK;
+    free_paths(d);
+    capacity = std::max(capacity, 1024u);
+    size_t P = capacity;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    size_t oSlot = take(4 * P), oSample = take(4 * P), oXs = take(4 * P),
+           oPcg = take(8 * P), oSlotRng = take(4 * P),
+           oVtx = take(4 * P * 7 * bounces), oVc = take(4 * P),
+           oBeta = take(12 * P), oRad = take(12 * P), oQo0 = take(12 * P),
+           oQo1 = take(12 * P), oQd0 = take(12 * P), oQd1 = take(12 * P),
+           oQp0 = take(4 * P), oQp1 = take(4 * P), oCnt = take(64),
+           oMet = take(128), oHt = take(4 * P), oHu = take(4 * P),
+           oHv = take(4 * P), oHi = take(4 * P), oHp = take(4 * P);
+    HIPCHK(hipMalloc(&d->pathsBlock, off));
+    char *b = (char *)d->pathsBlock;
+    DevPaths &p = d->paths;
+    p.capacity = (uint32_t)P;
+    p.bounces = bounces;
+    p.pathSlot = (uint32_t *)(b + oSlot);
+    p.pathSample = (uint32_t *)(b + oSample);
+    p.rngXs = (uint32_t *)(b + oXs);
+    p.rngPcg = (uint64_t *)(b + oPcg);
+    p.slotRng = (uint32_t *)(b + oSlotRng);
+    p.vtx = (float *)(b + oVtx);
+    p.vcount = (uint32_t *)(b + oVc);
+    p.beta = (float *)(b + oBeta);
+    p.radiance = (float *)(b + oRad);
+    p.qo[0] = (float *)(b + oQo0);
+    p.qo[1] = (float *)(b + oQo1);
+    p.qd[0] = (float *)(b + oQd0);
+    p.qd[1] = (float *)(b + oQd1);
+    p.qpath[0] = (uint32_t *)(b + oQp0);
+    p.qpath[1] = (uint32_t *)(b + oQp1);
+    p.counters = (uint32_t *)(b + oCnt);
+    p.metrics = (unsigned long long *)(b + oMet);
+    p.ht = (float *)(b + oHt);
+    p.hu = (float *)(b + oHu);
+    p.hv = (float *)(b + oHv);
+    p.hinst = (uint32_t *)(b + oHi);
+    p.hprim = (uint32_t *)(b + oHp);
+    d->pathsBytes = off;
+    return SPG_OK;
+}
+
+template <typename T>
+static int ensure_buffer(void *&ptr, size_t &have, size_t count)
+{
+    if (ptr && have >= count) return SPG_OK;
+    free_ptr(ptr);
+    have = 0;
+    HIPCHK(hipMalloc(&ptr, std::max<size_t>(count, 16) * sizeof(T)));
+    have = count;
+    return SPG_OK;
+}
+
+struct LaunchTimer
+{
+    spg_Device *d;
+    int next = 0;
+    int cls[32];
+    bool on = true;
+    void begin(int c)
+    {
+        if (!on || next >= 32) return;
+        cls[next / 2] = c;
+        hipEventRecord(d->ev[next], d->stream);
+    }
+    void end()
+    {
+        if (!on || next >= 32) return;
+        hipEventRecord(d->ev[next + 1], d->stream);
+        next += 2;
+    }
+};
+
+static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
+                 spg_Metrics *metrics, float *devImage, float *hostImage)
+{
+    if (!ctx || !ctx->camera || !ctx->scene || (!tiles && count))
+        return fail(SPG_E_INVALID, "trace: NULL context, camera, scene or tiles");
+    spg_Scene *s = ctx->scene;
+    spg_Device *d = s->dev;
+    const spg_Camera &c = *ctx->camera;
+    const spg_TraceConfig &cfg = ctx->config;
+    if (!s->built) return fail(SPG_E_STATE, "trace: scene not built");
+    if (cfg.samplesPerPixel == 0 || cfg.bounces == 0 || cfg.bounces > kMaxBounces)
+        return fail(SPG_E_INVALID, "trace: samplesPerPixel must be > 0 and bounces in 1..16");
+    if (cfg.rngMode != SPG_RNG_XORSHIFT_PIXEL && cfg.rngMode != SPG_RNG_PCG)
+        return fail(SPG_E_INVALID, "trace: unknown rngMode");
+    if (cfg.estimator != SPG_EST_PARITY && cfg.estimator != SPG_EST_UNBIASED)
+        return fail(SPG_E_INVALID, "trace: estimator not available in this call");
+    if (cfg.transcendentals != SPG_TX_DOUBLE && cfg.transcendentals != SPG_TX_FAST)
+        return fail(SPG_E_INVALID, "trace: unknown transcendentals mode");
+    if (c.width == 0 || c.height == 0)
+        return fail(SPG_E_INVALID, "trace: camera not configured");
+    auto t0 = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    if (s->dirty) CHK(upload_tables(s));
+
+    // pixel slots in tile order (clamped to the image as
+    // sp_PathTraceTile clamps, src/simd_path_tracer.cpp:186-189)
+    std::vector<uint32_t> slots;
+    for (uint32_t k = 0; k < count; ++k)
+    {
+        uint32_t x1 = std::min(tiles[k].maxX, c.width);
+        uint32_t y1 = std::min(tiles[k].maxY, c.height);
+        for (uint32_t y = tiles[k].minY; y < y1; ++y)
+            for (uint32_t x = tiles[k].minX; x < x1; ++x)
+                slots.push_back(y * c.width + x);
+    }
+    const uint32_t spp = cfg.samplesPerPixel;
+    const bool pcg = cfg.rngMode == SPG_RNG_PCG;
+    uint64_t want = (uint64_t)slots.size() * (pcg ? spp : 1);
+    uint32_t cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1),
+                                                kMaxPathsInFlight);
+    CHK(ensure_paths(d, cap, cfg.bounces));
+    cap = d->paths.capacity;
+    CHK(ensure_buffer<uint32_t>(d->slotPixel, d->slotPixelCount, cap));
+    CHK(ensure_buffer<float>(d->accum, d->accumCount, 3 * (size_t)cap));
+    HIPCHK(hipMemsetAsync(d->paths.metrics, 0, 128, d->stream));
+
+    TraceParams p{};
+    p.scene = s->dev_scene;
+    memcpy(&p.cam.right, c.right, 12);
+    memcpy(&p.cam.up, c.up, 12);
+    memcpy(&p.cam.forward, c.forward, 12);
+    memcpy(&p.cam.position, c.position, 12);
+    memcpy(&p.cam.filmCenter, c.filmCenter, 12);
+    p.cam.width = c.width;
+    p.cam.height = c.height;
+    p.cam.halfPixelWidth = c.halfPixelWidth;
+    p.cam.halfPixelHeight = c.halfPixelHeight;
+    p.cam.halfFilmWidth = c.halfFilmWidth;
+    p.cam.halfFilmHeight = c.halfFilmHeight;
+    p.paths = d->paths;
+    p.slotPixel = (const uint32_t *)d->slotPixel;
+    p.accum = (float *)d->accum;
+    p.spp = spp;
+    p.bounces = cfg.bounces;
+    p.clampValue = cfg.radianceClamp;
+    p.estimator = cfg.estimator;
+    p.rngMode = cfg.rngMode;
+    p.seed = cfg.seed;
+    p.tx = cfg.transcendentals;
+    p.collect = cfg.collectMetrics;
+
+    LaunchTimer tm{d};
+    uint32_t launches[6] = {0};
+    hipEventRecord(d->ev[62], d->stream);
+    for (size_t s0 = 0; s0 < slots.size(); s0 += cap)
+    {
+        uint32_t nslots = (uint32_t)std::min<size_t>(cap, slots.size() - s0);
+        HIPCHK(hipMemcpyAsync(d->slotPixel, slots.data() + s0, 4 * (size_t)nslots,
+                              hipMemcpyHostToDevice, d->stream));
+        HIPCHK(launch_clear_floats(d->stream, p.accum, 3 * (size_t)nslots));
+        p.slots = nslots;
+        uint32_t sppPass = pcg ? std::max(1u, std::min(spp, cap / nslots)) : 1u;
+        for (uint32_t done = 0; done < spp;)
+        {
+            uint32_t sp = std::min(sppPass, spp - done);
+            p.sppPass = sp;
+            p.sampleBase = cfg.sampleOffset + done;
+            // accum and the paths are laid out with the slot count of this
+            // chunk; accum rows are 3 x nslots
+            HIPCHK(launch_set_counter(d->stream, &p.paths.counters[0], nslots * sp));
+            tm.begin(kRaygen);
+            HIPCHK(launch_raygen(d->stream, p));
+            tm.end();
+            launches[kRaygen]++;
+            int q = 0;
+            for (uint32_t b = 0; b < cfg.bounces; ++b)
+            {
+                tm.begin(kExtend);
+                HIPCHK(launch_extend(d->stream, p, q));
+                tm.end();
+                HIPCHK(launch_set_counter(d->stream, &p.paths.counters[q ^ 1], 0));
+                tm.begin(kShade);
+                HIPCHK(launch_shade(d->stream, p, q));
+                tm.end();
+                launches[kExtend]++;
+                launches[kShade]++;
+                q ^= 1;
+            }
+            tm.begin(kFinal);
+            HIPCHK(launch_finalize(d->stream, p));
+            tm.end();
+            launches[kFinal]++;
+            done += sp;
+        }
+        HIPCHK(launch_write_pixels(d->stream, p, devImage));
+    }
+    hipEventRecord(d->ev[63], d->stream);
+    unsigned long long m[16] = {0};
+    HIPCHK(hipMemcpyAsync(m, d->paths.metrics, 128, hipMemcpyDeviceToHost,
+                          d->stream));
+    if (hostImage)
+    {
+        for (uint32_t k = 0; k < count; ++k)
+        {
+            uint32_t x1 = std::min(tiles[k].maxX, c.width);
+            uint32_t y1 = std::min(tiles[k].maxY, c.height);
+            if (x1 <= tiles[k].minX || y1 <= tiles[k].minY) continue;
+            size_t pitch = (size_t)c.width * 16;
+            size_t off = ((size_t)tiles[k].minY * c.width + tiles[k].minX) * 16;
+            HIPCHK(hipMemcpy2DAsync((char *)hostImage + off, pitch,
+                                    (const char *)devImage + off, pitch,
+                                    (size_t)(x1 - tiles[k].minX) * 16,
+                                    y1 - tiles[k].minY, hipMemcpyDeviceToHost,
+                                    d->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(d->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    for (int k = 0; k < 6; ++k) d->lastMs[k] = 0.0;
+    for (int k = 0; k < tm.next; k += 2)
+    {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, d->ev[k], d->ev[k + 1]);
+        d->lastMs[tm.cls[k / 2]] += ms;
+    }
+    float total = 0.0f;
+    hipEventElapsedTime(&total, d->ev[62], d->ev[63]);
+    d->lastMs[5] = total;
+    for (int k = 0; k < 6; ++k) d->lastLaunches[k] = launches[k];
+    for (int k = 0; k < 6; ++k) d->lastCounters[k] = m[k];
+    if (metrics)
+    {
+        metrics->values[SPG_METRIC_ELAPSED_NS] += (uint64_t)
+            std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+        metrics->values[SPG_METRIC_PATHS_TRACED] += (uint64_t)slots.size() * spp;
+        metrics->values[SPG_METRIC_RAYS_TRACED] += m[0];
+        metrics->values[SPG_METRIC_RAY_HIT_COUNT] += m[1];
+        metrics->values[SPG_METRIC_RAY_MISS_COUNT] += m[2];
+        metrics->values[SPG_METRIC_MIDPHASE_AABB_TESTS] += m[3];
+        metrics->values[SPG_METRIC_MESH_TESTS] += m[4];
+    }
+    return SPG_OK;
+}
+
+static int host_trace(const spg_Context *ctx, const spg_Tile *tiles,
+                      uint32_t count, spg_Metrics *metrics, float *hostPixels)
+{
+    if (!ctx || !ctx->camera || !ctx->scene || !hostPixels)
+        return fail(SPG_E_INVALID, "trace: NULL argument");
+    spg_Device *d = ctx->scene->dev;
+    size_t n = (size_t)ctx->camera->width * ctx->camera->height * 4;
+    {
+        std::lock_guard<std::mutex> g(d->lock);
+        HIPCHK(hipSetDevice(d->ordinal));
+        CHK(ensure_buffer<float>(d->image, d->imageCount, n));
+    }
+    return trace(ctx, tiles, count, metrics, (float *)d->image, hostPixels);
+}
+
+}  // namespace spg
+
+extern "C" {
+
+int spg_path_trace_tile(const spg_Context *ctx, spg_Tile tile,
+                        spg_Metrics *metrics, float *hostPixels)
+{
+    return host_trace(ctx, &tile, 1, metrics, hostPixels);
+}
+
+int spg_path_trace_tiles(const spg_Context *ctx, const spg_Tile *tiles,
+                         uint32_t count, spg_Metrics *metrics,
+                         float *hostPixels)
+{
+    return host_trace(ctx, tiles, count, metrics, hostPixels);
+}
+
+int spg_path_trace_tiles_device(const spg_Context *ctx, const spg_Tile *tiles,
+                                uint32_t count, spg_Metrics *metrics,
+                                float *devicePixels)
+{
+    if (!devicePixels) return fail(SPG_E_INVALID, "trace: NULL device image");
+    return trace(ctx, tiles, count, metrics, devicePixels, nullptr);
+}
+
+int spg_last_trace_timings(spg_Device *d, double *ms6, uint32_t *launches6)
+{
+    if (!d) return fail(SPG_E_INVALID, "NULL device");
+    std::lock_guard<std::mutex> g(d->lock);
+    for (int k = 0; k < 6; ++k)
+    {
+        if (ms6) ms6[k] = d->lastMs[k];
+        if (launches6) launches6[k] = d->lastLaunches[k];
+    }
+    return SPG_OK;
+}
+
+int spg_last_trace_counters(spg_Device *d, uint64_t *counters6)
+{
+    if (!d || !counters6) return fail(SPG_E_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> g(d->lock);
+    for (int k = 0; k < 6; ++k) counters6[k] = d->lastCounters[k];
+    return SPG_OK;
+}
+
+// ------------------------------------------------------------ closest hit
+static int check_mesh(spg_Scene *s, uint32_t meshId)
+{
+    if (!s) return fail(SPG_E_INVALID, "NULL scene");
+    if (!s->built) return fail(SPG_E_STATE, "scene not built");
+    if (meshId >= s->meshes.size())
+        return fail(SPG_E_NOT_FOUND, "unknown mesh id");
+    return SPG_OK;
+}
+
+static DevMesh dev_mesh(const HostMesh &m)
+{
+    DevMesh d{};
+    d.nodes = (const float4 *)m.dNodes;
+    d.tris = (const float4 *)m.dTris;
+    d.verts = (const float *)m.dVerts;
+    d.idx = (const uint32_t *)m.dIdx;
+    d.smooth = m.smooth;
+    d.triCount = (uint32_t)(m.idx.size() / 3);
+    return d;
+}
+
+int spg_intersect_closest_device(spg_Scene *s, uint32_t meshId,
+                                 const float *ox, const float *oy,
+                                 const float *oz, const float *dx,
+                                 const float *dy, const float *dz, uint32_t n,
+                                 uint32_t *primId, float *t, float *u, float *v,
+                                 spg_TraversalStats *stats)
+{
+    CHK(check_mesh(s, meshId));
+    if (n && (!ox || !oy || !oz || !dx || !dy || !dz || !primId || !t))
+        return fail(SPG_E_INVALID, "spg_intersect_closest: NULL array");
+    spg_Device *d = s->dev;
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    DevMesh m = dev_mesh(s->meshes[meshId]);
+    unsigned long long *st = nullptr;
+    if (stats)
+    {
+        CHK(ensure_paths(d, 1024, 1));
+        st = d->paths.metrics;
+        HIPCHK(hipMemsetAsync(st, 0, 64, d->stream));
+    }
+    HIPCHK(launch_closest_mesh(d->stream, &m, ox, oy, oz, dx, dy, dz, n, primId,
+                               t, u, v, st));
+    if (stats)
+    {
+        unsigned long long h[8];
+        HIPCHK(hipMemcpyAsync(h, st, 64, hipMemcpyDeviceToHost, d->stream));
+        HIPCHK(hipStreamSynchronize(d->stream));
+        stats->rays = h[0];
+        stats->boxTests = h[1];
+        stats->triTests = h[2];
+        stats->nodeFetches = h[1] / 2;
+        stats->hits = h[4];
+    }
+    return hip_check(hipStreamSynchronize(d->stream), "closest hit");
+}
+
+int spg_intersect_closest(spg_Scene *s, uint32_t meshId, const float *ox,
+                          const float *oy, const float *oz, const float *dx,
+                          const float *dy, const float *dz, uint32_t n,
+                          uint32_t *primId, float *t, float *u, float *v)
+{
+    CHK(check_mesh(s, meshId));
+    if (n == 0) return SPG_OK;
+    spg_Device *d = s->dev;
+    void *buf = nullptr;
+    {
+        std::lock_guard<std::mutex> g(d->lock);
+        HIPCHK(hipSetDevice(d->ordinal));
+        HIPCHK(hipMalloc(&buf, (size_t)n * 4 * 10));
+    }
+    float *f = (float *)buf;
+    const float *src[6] = {ox, oy, oz, dx, dy, dz};
+    int r = SPG_OK;
+    for (int k = 0; k < 6 && !r; ++k)
+        r = hip_check(hipMemcpy(f + (size_t)k * n, src[k], 4 * (size_t)n,
+                                hipMemcpyHostToDevice), "upload rays");
+    uint32_t *dp = (uint32_t *)(f + 6 * (size_t)n);
+    float *dt = f + 7 * (size_t)n, *du = f + 8 * (size_t)n, *dv = f + 9 * (size_t)n;
+    if (!r)
+        r = spg_intersect_closest_device(s, meshId, f, f + n, f + 2 * (size_t)n,
+                                         f + 3 * (size_t)n, f + 4 * (size_t)n,
+                                         f + 5 * (size_t)n, n, dp, dt, du, dv,
+                                         nullptr);
+    if (!r) r = hip_check(hipMemcpy(primId, dp, 4 * (size_t)n, hipMemcpyDeviceToHost), "d2h");
+    if (!r) r = hip_check(hipMemcpy(t, dt, 4 * (size_t)n, hipMemcpyDeviceToHost), "d2h");
+    if (!r && u) r = hip_check(hipMemcpy(u, du, 4 * (size_t)n, hipMemcpyDeviceToHost), "d2h");
+    if (!r && v) r = hip_check(hipMemcpy(v, dv, 4 * (size_t)n, hipMemcpyDeviceToHost), "d2h");
+    hipFree(buf);
+    return r;
+}
+
+int spg_time_intersect_closest(spg_Scene *s, uint32_t meshId, const float *ox,
+                               const float *oy, const float *oz,
+                               const float *dx, const float *dy,
+                               const float *dz, uint32_t n, uint32_t *primId,
+                               float *t, uint32_t iterations,
+                               double *meanKernelMs)
+{
+    CHK(check_mesh(s, meshId));
+    if (iterations == 0 || !meanKernelMs)
+        return fail(SPG_E_INVALID, "spg_time_intersect_closest: bad argument");
+    spg_Device *d = s->dev;
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    DevMesh m = dev_mesh(s->meshes[meshId]);
+    HIPCHK(hipEventRecord(d->ev[60], d->stream));
+    for (uint32_t i = 0; i < iterations; ++i)
+        HIPCHK(launch_closest_mesh(d->stream, &m, ox, oy, oz, dx, dy, dz, n,
+                                   primId, t, nullptr, nullptr, nullptr));
+    HIPCHK(hipEventRecord(d->ev[61], d->stream));
+    HIPCHK(hipEventSynchronize(d->ev[61]));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, d->ev[60], d->ev[61]));
+    *meanKernelMs = ms / iterations;
+    return SPG_OK;
+}
+
+int spg_intersect_scene(spg_Scene *s, const float *rays6, uint32_t n,
+                        float *out7, uint32_t *object, uint32_t *prim)
+{
+    if (!s || (n && (!rays6 || !out7)))
+        return fail(SPG_E_INVALID, "spg_intersect_scene: NULL argument");
+    if (!s->built) return fail(SPG_E_STATE, "scene not built");
+    if (n == 0) return SPG_OK;
+    spg_Device *d = s->dev;
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    if (s->dirty) CHK(upload_tables(s));
+    void *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, (size_t)n * 4 * 15));
+    float *dr = (float *)buf, *dout = dr + 6 * (size_t)n;
+    uint32_t *dobj = (uint32_t *)(dout + 7 * (size_t)n), *dprim = dobj + n;
+    int r = hip_check(hipMemcpy(dr, rays6, 24 * (size_t)n, hipMemcpyHostToDevice), "h2d");
+    if (!r) r = hip_check(launch_intersect_scene(d->stream, s->dev_scene, dr, n,
+                                                 dout, dobj, dprim), "launch");
+    if (!r) r = hip_check(hipStreamSynchronize(d->stream), "sync");
+    if (!r) r = hip_check(hipMemcpy(out7, dout, 28 * (size_t)n, hipMemcpyDeviceToHost), "d2h");
+    if (!r && object) r = hip_check(hipMemcpy(object, dobj, 4 * (size_t)n, hipMemcpyDeviceToHost), "d2h");
+    if (!r && prim) r = hip_check(hipMemcpy(prim, dprim, 4 * (size_t)n, hipMemcpyDeviceToHost), "d2h");
+    hipFree(buf);
+    return r;
+}
+
+}  // extern "C"

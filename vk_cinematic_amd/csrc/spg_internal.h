@@ -1,0 +1,230 @@
+// Internal structures shared by the kernels (spg_trace.hip) and the C-ABI
+// implementation (spg_api.cpp).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/spgpu.h"
+#include "spg_bvh.h"
+#include "spg_math.h"
+
+namespace spg {
+
+// Traversal stack depth per lane, in LDS.  The builder bounds the tree depth
+// so a TLAS path plus a BLAS path always fits (see kMaxBlasDepth).
+constexpr int kStackDepth = 64;
+constexpr uint32_t kMaxTlasDepth = 20;
+constexpr uint32_t kMaxBlasDepth = kStackDepth - kMaxTlasDepth - 2;
+constexpr int kTraceBlock = 64;  // one wave per block: 16 KB of LDS stack
+
+// Relative slack on the t-max cull: a box is skipped only if its entry
+// exceeds the best t by more than 2^-12 of it, so rounding differences
+// between the slab entry and the Moller-Trumbore t never cull a closer hit.
+constexpr float kCullSlack = 1.0f + 1.0f / 4096.0f;
+
+struct DevMesh
+{
+    const float4 *nodes;    // GpuNode, 4 float4 each
+    const float4 *tris;     // 3 float4 per triangle in leaf order:
+                            // (a.xyz, e1.x) (e1.yz, e2.xy) (e2.z, prim, 0, 0)
+    const float *verts;     // VertexPNT, 8 floats
+    const uint32_t *idx;    // 3 per triangle
+    uint32_t smooth;
+    uint32_t triCount;
+};
+
+struct DevInstance
+{
+    m34 model;        // rows 0..2 of M = T*R*S       (src/sp_scene.cpp:91)
+    m34 inv;          // rows 0..2 of S^-1*R(q*)*T(-p) (src/sp_scene.cpp:94-96)
+    float cullScale;  // 1 / min |scale|: world t bound -> local t bound
+    uint32_t mesh;
+    int32_t material; // dense material slot, -1 = not registered
+    uint32_t materialId;
+};
+
+struct DevMaterial
+{
+    float albedo[3];
+    int32_t albedoTex;   // dense texture slot or -1
+    float emission[3];
+    int32_t emissionTex; // dense texture slot or -1
+    float roughness;
+    float pad[3];
+};
+
+struct DevTexture
+{
+    uint64_t offset;  // in float4 texels
+    uint32_t width, height;
+    uint32_t isCube;
+    uint32_t pad;
+};
+
+struct DevScene
+{
+    const float4 *tlasNodes;
+    const uint32_t *tlasInst;   // TLAS leaf order -> instance index
+    const DevInstance *inst;
+    const DevMesh *meshes;
+    const DevMaterial *mats;
+    const DevTexture *texs;
+    const float4 *texels;
+    int32_t background;         // dense material slot, -1 = none
+    uint32_t instCount;
+};
+
+struct DevCamera
+{
+    v3 right, up, forward, position, filmCenter;
+    uint32_t width, height;
+    float halfPixelWidth, halfPixelHeight, halfFilmWidth, halfFilmHeight;
+};
+
+// Wavefront state for `capacity` paths in flight (structure of arrays).
+struct DevPaths
+{
+    uint32_t capacity, bounces;
+    uint32_t *pathSlot;          // pixel slot of each path
+    uint32_t *pathSample;        // sample index of each path
+    uint32_t *rngXs;             // xorshift state per path
+    uint64_t *rngPcg;            // PCG state per path
+    uint32_t *slotRng;           // xorshift state carried per pixel slot
+    float *vtx;                  // [bounces*7][capacity]: E.xyz X.xyz cos
+    uint32_t *vcount;            // vertices per path
+    float *beta;                 // [3][capacity]
+    float *radiance;             // [3][capacity]
+    float *qo[2], *qd[2];        // [3][capacity] ray origins / directions
+    uint32_t *qpath[2];          // path index of each queued ray
+    uint32_t *counters;          // [0..1] queue lengths
+    unsigned long long *metrics; // rays, hits, misses, box, mesh, tri tests
+    float *ht, *hu, *hv;         // hit records per queued ray
+    uint32_t *hinst, *hprim;
+};
+
+enum KernelClass
+{
+    kRaygen = 0,
+    kExtend = 1,
+    kShade = 2,
+    kFinal = 3,
+    kGuide = 4,
+    kTimingClasses = 5
+};
+
+struct TraceParams
+{
+    DevScene scene;
+    DevCamera cam;
+    DevPaths paths;
+    const uint32_t *slotPixel;   // pixel index of each slot
+    float *accum;                // [3][slots] running sums (sample order)
+    uint32_t slots;
+    uint32_t sppPass;            // samples of each slot in this pass
+    uint32_t sampleBase;         // first sample index of this pass
+    uint32_t spp;                // samples per pixel of the whole render
+    uint32_t bounces;
+    float clampValue;
+    uint32_t estimator, rngMode, seed, tx, collect;
+};
+
+}  // namespace spg
+
+struct spg_Device
+{
+    int ordinal = 0;
+    int liveScenes = 0;          // guarded by lock
+    bool destroyRequested = false;
+    hipStream_t stream = nullptr;
+    std::mutex lock;
+    spg::DevPaths paths{};
+    size_t pathsBytes = 0;
+    void *pathsBlock = nullptr;
+    void *image = nullptr;       // staging image for host-pixel calls
+    size_t imageCount = 0;
+    void *slotPixel = nullptr;   // pixel index per slot
+    size_t slotPixelCount = 0;
+    void *accum = nullptr;       // per-slot radiance sums
+    size_t accumCount = 0;
+    hipEvent_t ev[64];
+    bool eventsMade = false;
+    double lastMs[6] = {0};
+    uint32_t lastLaunches[6] = {0};
+    uint64_t lastCounters[6] = {0};
+};
+
+struct HostMesh
+{
+    std::vector<float> verts;     // 8 per vertex
+    std::vector<uint32_t> idx;
+    int smooth = 0;
+    spg::BvhBuildResult bvh;
+    bool built = false;
+    // device copies
+    void *dNodes = nullptr, *dTris = nullptr, *dVerts = nullptr, *dIdx = nullptr;
+    float boundsLo[3], boundsHi[3];
+};
+
+struct HostObject
+{
+    uint32_t mesh, materialId;
+    float pos[3], quat[4], scale[3];
+    float model[16], inv[16];
+    float aabbLo[3], aabbHi[3];
+};
+
+struct HostTexture
+{
+    uint32_t id, width, height, isCube;
+    std::vector<float> rgba;
+};
+
+struct spg_Scene
+{
+    spg_Device *dev = nullptr;
+    std::vector<HostMesh> meshes;
+    std::vector<HostObject> objects;
+    std::vector<uint32_t> materialKeys;
+    std::vector<spg_Material> materials;
+    std::vector<HostTexture> textures;
+    uint32_t backgroundId = 0;
+    bool built = false;
+    bool dirty = true;
+    spg::BvhBuildResult tlas;
+    // device-side tables
+    void *dTlasNodes = nullptr, *dTlasInst = nullptr, *dInst = nullptr,
+         *dMeshes = nullptr, *dMats = nullptr, *dTexs = nullptr,
+         *dTexels = nullptr;
+    spg::DevScene dev_scene{};
+};
+
+namespace spg {
+// error reporting (thread-local message)
+int fail(int code, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+
+// kernel launchers (spg_trace.hip)
+hipError_t launch_closest_mesh(hipStream_t s, const DevMesh *mesh,
+                               const float *ox, const float *oy,
+                               const float *oz, const float *dx,
+                               const float *dy, const float *dz, uint32_t n,
+                               uint32_t *prim, float *t, float *u, float *v,
+                               unsigned long long *stats);
+hipError_t launch_intersect_scene(hipStream_t s, const DevScene &scene,
+                                  const float *rays6, uint32_t n, float *out7,
+                                  uint32_t *object, uint32_t *prim);
+hipError_t launch_raygen(hipStream_t s, const TraceParams &p);
+hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q);
+hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q);
+hipError_t launch_finalize(hipStream_t s, const TraceParams &p);
+hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,
+                               float *image);
+hipError_t launch_clear_counters(hipStream_t s, uint32_t *counters, int n);
+hipError_t launch_set_counter(hipStream_t s, uint32_t *counter, uint32_t v);
+hipError_t launch_clear_floats(hipStream_t s, float *p, size_t n);
+}  // namespace spg

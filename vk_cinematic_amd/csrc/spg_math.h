@@ -1,0 +1,206 @@
+// Shared host/device FP32 arithmetic for the tracer.
+//
+// Every expression keeps the association order of the reference's math
+// library (src/math_lib.h, src/math_utils.h) and everything is compiled with
+// -ffp-contract=off and correctly rounded division/sqrt, so results are
+// bit-identical between the host build, the gfx950 build and the reference's
+// own x86 build.  Min/Max are compare-and-select like the reference's
+// (src/math_utils.h:9-19) -- not v_min/v_max, whose NaN and signed-zero
+// handling differ.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#define SPG_HD __host__ __device__ __forceinline__
+
+namespace spg {
+
+constexpr float kPi = 3.14159265359f;          // src/math_utils.h:6
+constexpr float kEpsilon = 1.19209290e-07f;    // FLT_EPSILON
+constexpr float kF32Max = 3.40282347e+38f;
+
+struct v3 { float x, y, z; };
+struct v2 { float x, y; };
+
+SPG_HD v3 mk(float x, float y, float z) { return v3{x, y, z}; }
+SPG_HD v3 splat(float s) { return v3{s, s, s}; }
+SPG_HD float fmin_sel(float a, float b) { return a < b ? a : b; }
+SPG_HD float fmax_sel(float a, float b) { return a > b ? a : b; }
+SPG_HD float clamp_sel(float x, float lo, float hi)
+{
+    return fmin_sel(fmax_sel(x, lo), hi);
+}
+SPG_HD v3 operator+(v3 a, v3 b) { return v3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+SPG_HD v3 operator-(v3 a, v3 b) { return v3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+SPG_HD v3 operator-(v3 a) { return v3{-a.x, -a.y, -a.z}; }
+SPG_HD v3 operator*(v3 a, float s) { return v3{a.x * s, a.y * s, a.z * s}; }
+SPG_HD v3 had(v3 a, v3 b) { return v3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+// Dot: (x*x + y*y) + z*z, src/math_lib.h:461-465
+SPG_HD float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// Cross, src/math_lib.h:467-477
+SPG_HD v3 cross(v3 a, v3 b)
+{
+    return v3{(a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z),
+              (a.x * b.y) - (a.y * b.x)};
+}
+SPG_HD v3 vmin(v3 a, v3 b)
+{
+    return v3{fmin_sel(a.x, b.x), fmin_sel(a.y, b.y), fmin_sel(a.z, b.z)};
+}
+SPG_HD v3 vmax(v3 a, v3 b)
+{
+    return v3{fmax_sel(a.x, b.x), fmax_sel(a.y, b.y), fmax_sel(a.z, b.z)};
+}
+SPG_HD v3 vclamp(v3 v, float lo, float hi)
+{
+    return v3{clamp_sel(v.x, lo, hi), clamp_sel(v.y, lo, hi),
+              clamp_sel(v.z, lo, hi)};
+}
+// Normalize with the EPSILON guard, src/math_lib.h:503-512
+SPG_HD v3 normalize(v3 v)
+{
+    float length = sqrtf(dot(v, v));
+    v3 r{0.0f, 0.0f, 0.0f};
+    if (length > kEpsilon) r = v * (1.0f / length);
+    return r;
+}
+// Inverse(vec3), src/math_lib.h:911-915
+SPG_HD v3 inverse(v3 a) { return v3{1.0f / a.x, 1.0f / a.y, 1.0f / a.z}; }
+
+// Rows 0..2 of a column-major mat4 (src/math_lib.h:93-97), i.e. what
+// TransformPoint/TransformVector read (src/math_lib.h:748-758).
+struct m34 { float r[3][4]; };
+
+// mat4 * vec4 row dot, src/math_lib.h:238-248 and :381-398
+SPG_HD float row_dot(const float *row, float x, float y, float z, float w)
+{
+    return row[0] * x + row[1] * y + row[2] * z + row[3] * w;
+}
+SPG_HD v3 xform(const m34 &m, v3 p, float w)
+{
+    return v3{row_dot(m.r[0], p.x, p.y, p.z, w), row_dot(m.r[1], p.x, p.y, p.z, w),
+              row_dot(m.r[2], p.x, p.y, p.z, w)};
+}
+SPG_HD v3 xform_point(const m34 &m, v3 p) { return xform(m, p, 1.0f); }
+SPG_HD v3 xform_vector(const m34 &m, v3 d) { return xform(m, d, 0.0f); }
+
+// ---- the reference's 4-wide slab test, one box (src/simd.h:198-271):
+// t0=(min-o)*invD, t1=(max-o)*invD, per-axis MINPS/MAXPS (second operand
+// unless the compare holds), entry = Max(0, MaxComponent(tmin)),
+// exit = MinComponent(tmax); hit <=> entry <= exit.
+SPG_HD bool slab_hit(float lox, float loy, float loz, float hix, float hiy,
+                     float hiz, v3 o, v3 invD, float *entryOut)
+{
+    float t0x = (lox - o.x) * invD.x, t1x = (hix - o.x) * invD.x;
+    float t0y = (loy - o.y) * invD.y, t1y = (hiy - o.y) * invD.y;
+    float t0z = (loz - o.z) * invD.z, t1z = (hiz - o.z) * invD.z;
+    float mnx = t0x < t1x ? t0x : t1x, mxx = t0x > t1x ? t0x : t1x;
+    float mny = t0y < t1y ? t0y : t1y, mxy = t0y > t1y ? t0y : t1y;
+    float mnz = t0z < t1z ? t0z : t1z, mxz = t0z > t1z ? t0z : t1z;
+    float entry = fmax_sel(0.0f, fmax_sel(mnx, fmax_sel(mny, mnz)));
+    float exit_ = fmin_sel(mxx, fmin_sel(mxy, mxz));
+    *entryOut = entry;
+    return entry <= exit_;
+}
+
+// ---- Moller-Trumbore, src/ray_intersection.cpp:156-190, with e1 = b - a
+// and e2 = c - a supplied (precomputed with the same subtraction).
+// Returns t (or -1 on a miss) and u, v.
+SPG_HD float mt_test(v3 o, v3 d, v3 a, v3 e1, v3 e2, float *uo, float *vo)
+{
+    v3 T = o - a;
+    v3 p = cross(d, e2);
+    v3 q = cross(T, e1);
+    v3 n = cross(e1, e2);
+    float mx = dot(q, e2), my = dot(p, T), mz = dot(q, d);
+    float det = 1.0f / dot(p, e1);
+    float t = det * mx;
+    float u = det * my;
+    float v = det * mz;
+    float w = 1.0f - u - v;
+    float f = dot(d, n);
+    if (u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f && w >= 0.0f &&
+        w <= 1.0f && f < 0.0f)
+    {
+        *uo = u;
+        *vo = v;
+        return t;
+    }
+    return -1.0f;
+}
+
+// ---- RNG --------------------------------------------------------------
+// XorShift32, src/math_utils.h:190-198
+SPG_HD uint32_t xorshift32(uint32_t &x)
+{
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+// RandomUnilateral's conversion, src/math_utils.h:200-208
+SPG_HD float unilateral(uint32_t x)
+{
+    float numerator = (float)(x >> 1);
+    float denom = (float)(0xFFFFFFFFu >> 1);
+    return numerator / denom;
+}
+// PCG32 XSH-RR (O'Neill 2014), seeding as pcg32_srandom_r
+constexpr uint64_t kPcgMult = 6364136223846793005ULL;
+SPG_HD uint32_t pcg32_next(uint64_t &state, uint64_t inc)
+{
+    uint64_t old = state;
+    state = old * kPcgMult + inc;
+    uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xorshifted >> rot) | (xorshifted << ((-rot) & 31u));
+}
+SPG_HD void pcg32_seed(uint64_t &state, uint64_t &inc, uint64_t initstate,
+                       uint64_t initseq)
+{
+    state = 0u;
+    inc = (initseq << 1u) | 1u;
+    pcg32_next(state, inc);
+    state += initstate;
+    pcg32_next(state, inc);
+}
+// Jump ahead delta LCG steps in O(log delta) (Brown 1994)
+SPG_HD void pcg32_advance(uint64_t &state, uint64_t inc, uint64_t delta)
+{
+    uint64_t cur_mult = kPcgMult, cur_plus = inc;
+    uint64_t acc_mult = 1u, acc_plus = 0u;
+    while (delta > 0)
+    {
+        if (delta & 1)
+        {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1) * cur_plus;
+        cur_mult *= cur_mult;
+        delta /= 2;
+    }
+    state = acc_mult * state + acc_plus;
+}
+// Per-pixel xorshift32 seed: a 32-bit integer finalizer of
+// base + golden-ratio * index, zero remapped (xorshift32 stays at 0).
+SPG_HD uint32_t pixel_seed(uint32_t base, uint32_t index)
+{
+    uint32_t x = base + index * 0x9E3779B9u;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x ? x : 0x6D2B79F5u;
+}
+
+// ---- transcendentals: evaluated in double, rounded once to float -------
+SPG_HD float tsin(float x) { return (float)sin((double)x); }
+SPG_HD float tcos(float x) { return (float)cos((double)x); }
+SPG_HD float tatan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+SPG_HD float tpow(float x, float y) { return (float)pow((double)x, (double)y); }
+
+}  // namespace spg

@@ -1,0 +1,1057 @@
+// gfx950 kernels of the wavefront path tracer.
+//
+//   closest hit   k_closest_mesh   one ray per lane, BVH2 with both child
+//                                  boxes per 64-byte node, LDS stack, t-max
+//                                  culling, strict-FP slab and MT tests
+//   scene query   k_intersect_scene  TLAS over instances -> per-instance
+//                                  local ray -> BLAS (sp_RayIntersectScene)
+//   wavefront     k_raygen -> (k_extend -> k_shade)* -> k_finalize
+//                 rays live in SoA queues; k_shade appends continuing rays
+//                 to the next queue with a wave ballot + one atomic per wave
+//
+// All arithmetic that the reference defines is written in its operation
+// order (spg_math.h) and built with -ffp-contract=off and correctly rounded
+// division/sqrt, so a path computes the same FP32 bits as the reference CPU
+// tracer given the same random numbers and transcendental results.
+#include "spg_internal.h"
+
+namespace spg {
+
+// ------------------------------------------------------------- helpers
+__device__ __forceinline__ bool d_is_leaf(int32_t r) { return r < 0; }
+__device__ __forceinline__ uint32_t d_leaf_first(int32_t r)
+{
+    return ((uint32_t)~r) >> 3;
+}
+__device__ __forceinline__ uint32_t d_leaf_count(int32_t r)
+{
+    return (((uint32_t)~r) & 7u) + 1u;
+}
+
+__device__ __forceinline__ uint32_t lane_id()
+{
+    return __lane_id();
+}
+
+// Sum over the active lanes of the wave, result valid in every lane.
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    return x;
+}
+
+__device__ __forceinline__ void wave_add(unsigned long long *dst,
+                                         unsigned long long x)
+{
+    // all 64 lanes must be active here
+    unsigned long long s = wave_sum(x);
+    if (lane_id() == 0 && s) atomicAdd(dst, s);
+}
+
+// Transcendentals: kDouble evaluates in double and rounds once (bit-
+// reproducible across host and device); otherwise device float intrinsics.
+template <bool kDouble> struct Tx;
+template <> struct Tx<true>
+{
+    static __device__ __forceinline__ float sin_(float x) { return tsin(x); }
+    static __device__ __forceinline__ float cos_(float x) { return tcos(x); }
+    static __device__ __forceinline__ float atan2_(float y, float x)
+    {
+        return tatan2(y, x);
+    }
+    static __device__ __forceinline__ float pow_(float x, float y)
+    {
+        return tpow(x, y);
+    }
+};
+template <> struct Tx<false>
+{
+    static __device__ __forceinline__ float sin_(float x) { return __sinf(x); }
+    static __device__ __forceinline__ float cos_(float x) { return __cosf(x); }
+    static __device__ __forceinline__ float atan2_(float y, float x)
+    {
+        return atan2f(y, x);
+    }
+    static __device__ __forceinline__ float pow_(float x, float y)
+    {
+        return __powf(x, y);
+    }
+};
+
+struct Hit
+{
+    float t, u, v;
+    uint32_t prim;
+};
+
+struct Counts
+{
+    uint32_t box, tri, mesh;
+};
+
+// ------------------------------------------------------------ traversal
+// Closest hit in one BVH2.  The stack column of this lane starts at
+// stack[sp0*kTraceBlock]; entries are inner-node indices.  best.t = +inf on
+// entry means "no hit yet"; tcullLimit bounds the search from outside.
+template <bool kCount>
+__device__ __forceinline__ void blas_traverse(const DevMesh &m, v3 o, v3 d,
+                                              v3 invD, float tcullLimit,
+                                              Hit &best, int *stack, int sp0,
+                                              Counts &cnt)
+{
+    int sp = sp0;
+    int node = 0;
+    const int lane = threadIdx.x;
+    for (;;)
+    {
+        const float4 *N = m.nodes + 4 * node;
+        float4 A = N[0], B = N[1], C = N[2], D = N[3];
+        int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
+        float tc = best.t * kCullSlack;
+        tc = tc < tcullLimit ? tc : tcullLimit;
+        float e0, e1;
+        bool h0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, invD, &e0) &&
+                  e0 <= tc;
+        bool h1 = r1 != kEmptyChild &&
+                  slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, invD, &e1) &&
+                  e1 <= tc;
+        if (kCount) cnt.box += r1 != kEmptyChild ? 2u : 1u;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+        {
+            int32_t r = c == 0 ? r0 : r1;
+            bool h = c == 0 ? h0 : h1;
+            if (h && d_is_leaf(r))
+            {
+                uint32_t first = d_leaf_first(r), count = d_leaf_count(r);
+                for (uint32_t k = 0; k < count; ++k)
+                {
+                    const float4 *T = m.tris + 3 * (first + k);
+                    float4 t0 = T[0], t1 = T[1], t2 = T[2];
+                    v3 a{t0.x, t0.y, t0.z}, e1v{t0.w, t1.x, t1.y},
+                        e2v{t1.z, t1.w, t2.x};
+                    uint32_t prim = __float_as_uint(t2.y);
+                    float u, v;
+                    float t = mt_test(o, d, a, e1v, e2v, &u, &v);
+                    if (t > 0.0f &&
+                        (t < best.t || (t == best.t && prim < best.prim)))
+                    {
+                        best.t = t;
+                        best.u = u;
+                        best.v = v;
+                        best.prim = prim;
+                    }
+                }
+                if (kCount) cnt.tri += count;
+                if (c == 0) h0 = false;
+                else h1 = false;
+            }
+        }
+        if (h0 && h1)
+        {
+            int32_t nearN = r0, farN = r1;
+            if (e1 < e0)
+            {
+                nearN = r1;
+                farN = r0;
+            }
+            stack[sp * kTraceBlock + lane] = farN;
+            ++sp;
+            node = nearN;
+        }
+        else if (h0)
+            node = r0;
+        else if (h1)
+            node = r1;
+        else
+        {
+            if (sp == sp0) break;
+            --sp;
+            node = stack[sp * kTraceBlock + lane];
+        }
+    }
+}
+
+struct SceneHit
+{
+    float t, u, v;
+    uint32_t inst, prim;
+};
+
+// sp_RayIntersectScene (src/sp_scene.cpp:229-339): TLAS over instance boxes,
+// per instance the local ray o' = M^-1 o, d' = norm(M^-1 d), the mesh query,
+// then world t = dot(M (o' + t' d') - o, d).  Smallest world t wins; equal t
+// goes to the lower instance index.  out.t = -1 and inst = miss if nothing.
+template <bool kCount>
+__device__ __forceinline__ void scene_traverse(const DevScene &S, v3 o, v3 d,
+                                               SceneHit &out, int *stack,
+                                               Counts &cnt)
+{
+    out.t = -1.0f;
+    out.inst = SPG_MISS;
+    out.prim = SPG_MISS;
+    out.u = out.v = 0.0f;
+    if (S.instCount == 0) return;
+    v3 invD = inverse(d);
+    const int lane = threadIdx.x;
+    int sp = 0;
+    int node = 0;
+    for (;;)
+    {
+        const float4 *N = S.tlasNodes + 4 * node;
+        float4 A = N[0], B = N[1], C = N[2], D = N[3];
+        int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
+        float tc = out.t > 0.0f ? out.t * kCullSlack : INFINITY;
+        float e0, e1;
+        bool h0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, invD, &e0) &&
+                  e0 <= tc;
+        bool h1 = r1 != kEmptyChild &&
+                  slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, invD, &e1) &&
+                  e1 <= tc;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+        {
+            int32_t r = c == 0 ? r0 : r1;
+            bool h = c == 0 ? h0 : h1;
+            if (h && d_is_leaf(r))
+            {
+                uint32_t first = d_leaf_first(r), count = d_leaf_count(r);
+                for (uint32_t k = 0; k < count; ++k)
+                {
+                    uint32_t ii = S.tlasInst[first + k];
+                    const DevInstance &I = S.inst[ii];
+                    v3 lo = xform_point(I.inv, o);
+                    v3 ld = normalize(xform_vector(I.inv, d));
+                    v3 linv = inverse(ld);
+                    float lim = out.t > 0.0f
+                                    ? out.t * I.cullScale * kCullSlack
+                                    : INFINITY;
+                    Hit h{INFINITY, 0.0f, 0.0f, SPG_MISS};
+                    blas_traverse<kCount>(S.meshes[I.mesh], lo, ld, linv, lim,
+                                          h, stack, sp, cnt);
+                    if (kCount) cnt.mesh += 1;
+                    if (h.prim != SPG_MISS)
+                    {
+                        v3 lh = lo + ld * h.t;
+                        v3 wh = xform_point(I.model, lh);
+                        float t = dot(wh - o, d);
+                        if (t < out.t || out.t < 0.0f ||
+                            (t == out.t && ii < out.inst))
+                        {
+                            out.t = t;
+                            out.u = h.u;
+                            out.v = h.v;
+                            out.inst = ii;
+                            out.prim = h.prim;
+                        }
+                    }
+                }
+                if (c == 0) h0 = false;
+                else h1 = false;
+            }
+        }
+        if (h0 && h1)
+        {
+            int32_t nearN = r0, farN = r1;
+            if (e1 < e0)
+            {
+                nearN = r1;
+                farN = r0;
+            }
+            stack[sp * kTraceBlock + lane] = farN;
+            ++sp;
+            node = nearN;
+        }
+        else if (h0)
+            node = r0;
+        else if (h1)
+            node = r1;
+        else
+        {
+            if (sp == 0) break;
+            --sp;
+            node = stack[sp * kTraceBlock + lane];
+        }
+    }
+}
+
+// Surface at a hit (src/sp_scene.cpp:196-213 and :302-312): uv from the
+// barycentrics, smooth normal normalize(w n0 + u n1 + v n2) or the MT normal
+// cross(e1, e2), then world normal normalize(M n).
+__device__ __forceinline__ void surface_at(const DevScene &S, uint32_t inst,
+                                           uint32_t prim, float u, float v,
+                                           v3 &worldN, v2 &uv)
+{
+    const DevInstance &I = S.inst[inst];
+    const DevMesh &M = S.meshes[I.mesh];
+    uint32_t i0 = M.idx[3 * prim], i1 = M.idx[3 * prim + 1],
+             i2 = M.idx[3 * prim + 2];
+    const float *p0 = M.verts + 8 * i0, *p1 = M.verts + 8 * i1,
+                *p2 = M.verts + 8 * i2;
+    float w = 1.0f - u - v;
+    uv.x = (p0[6] * w + p1[6] * u) + p2[6] * v;
+    uv.y = (p0[7] * w + p1[7] * u) + p2[7] * v;
+    v3 ln;
+    if (M.smooth)
+    {
+        v3 n0{p0[3], p0[4], p0[5]}, n1{p1[3], p1[4], p1[5]},
+            n2{p2[3], p2[4], p2[5]};
+        ln = normalize((n0 * w + n1 * u) + n2 * v);
+    }
+    else
+    {
+        v3 a{p0[0], p0[1], p0[2]}, b{p1[0], p1[1], p1[2]}, c{p2[0], p2[1], p2[2]};
+        ln = cross(b - a, c - a);
+    }
+    worldN = normalize(xform_vector(I.model, ln));
+}
+
+// ------------------------------------------------------------- materials
+__device__ __forceinline__ float4 texel(const DevScene &S, const DevTexture &T,
+                                        uint64_t i)
+{
+    return S.texels[T.offset + i];
+}
+
+// SampleImageNearest, src/image.h:3-17, with (u32)Floor taking NaN and
+// negatives to 0 and the flat index clamped to the last texel.
+__device__ __forceinline__ float4 sample_nearest(const DevScene &S,
+                                                 const DevTexture &T, float u,
+                                                 float v)
+{
+    float fx = floorf(u * (float)T.width);
+    float fy = floorf(v * (float)T.height);
+    uint32_t x = fx >= 0.0f ? (fx < 4294967296.0f ? (uint32_t)fx : 0xFFFFFFFFu) : 0u;
+    uint32_t y = fy >= 0.0f ? (fy < 4294967296.0f ? (uint32_t)fy : 0xFFFFFFFFu) : 0u;
+    uint64_t idx = (uint64_t)y * T.width + x;
+    uint64_t last = (uint64_t)T.width * T.height - 1;
+    if (idx > last) idx = last;
+    return texel(S, T, idx);
+}
+
+// Nearest cube-map texel by direction: the inverse of CreateCubeMap's
+// texel-to-direction map (src/cubemap.cpp:237-291) with the face bases of
+// src/cubemap.cpp:55-103.
+__device__ __forceinline__ float4 sample_cube(const DevScene &S,
+                                              const DevTexture &T, v3 d)
+{
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    uint32_t face;
+    float ma, sc, tc;
+    if (ax >= ay && ax >= az)
+    {
+        face = d.x >= 0.0f ? 0 : 1;
+        ma = ax;
+        sc = d.x >= 0.0f ? -d.z : d.z;
+        tc = d.y;
+    }
+    else if (ay >= az)
+    {
+        face = d.y >= 0.0f ? 2 : 3;
+        ma = ay;
+        sc = d.x;
+        tc = d.y >= 0.0f ? -d.z : d.z;
+    }
+    else
+    {
+        face = d.z >= 0.0f ? 4 : 5;
+        ma = az;
+        sc = d.z >= 0.0f ? d.x : -d.x;
+        tc = d.y;
+    }
+    float fx = sc / ma;
+    float fy = tc / ma;
+    float px = (fx + 1.0f) * (0.5f * (float)T.width) + 0.5f;
+    float py = (1.0f - fy) * (0.5f * (float)T.height) + 0.5f;
+    px = px >= 0.0f ? px : 0.0f;
+    py = py >= 0.0f ? py : 0.0f;
+    px = px < (float)(T.width - 1) ? px : (float)(T.width - 1);
+    py = py < (float)(T.height - 1) ? py : (float)(T.height - 1);
+    uint32_t ix = (uint32_t)floorf(px), iy = (uint32_t)floorf(py);
+    return texel(S, T, ((uint64_t)face * T.height + iy) * T.width + ix);
+}
+
+struct MatOut
+{
+    v3 albedo, emission;
+    float roughness;
+};
+
+// sp_EvaluateMaterial, src/sp_material_system.cpp:59-105 (emission texture:
+// equirect by ToSphericalCoordinates/MapToEquirectangular, or a cube map by
+// direction).  A missing material gives emission (1,0,1),
+// src/simd_path_tracer.cpp:125-134.
+template <bool kDouble>
+__device__ __forceinline__ MatOut eval_material(const DevScene &S, int32_t slot,
+                                                v2 uv, v3 outgoingDir)
+{
+    MatOut r;
+    r.albedo = v3{0.0f, 0.0f, 0.0f};
+    r.emission = v3{1.0f, 0.0f, 1.0f};
+    r.roughness = 0.0f;
+    if (slot < 0) return r;
+    const DevMaterial &M = S.mats[slot];
+    if (M.albedoTex >= 0 && !S.texs[M.albedoTex].isCube)
+    {
+        float4 t = sample_nearest(S, S.texs[M.albedoTex], uv.x, uv.y);
+        r.albedo = v3{t.x, t.y, t.z};
+    }
+    else
+        r.albedo = v3{M.albedo[0], M.albedo[1], M.albedo[2]};
+    if (M.emissionTex >= 0)
+    {
+        const DevTexture &T = S.texs[M.emissionTex];
+        v3 dir = -outgoingDir;
+        float4 t;
+        if (T.isCube)
+            t = sample_cube(S, T, dir);
+        else
+        {
+            // ToSphericalCoordinates, src/math_lib.h:849-860
+            float inc = Tx<kDouble>::atan2_(sqrtf(dir.x * dir.x + dir.z * dir.z),
+                                            dir.y);
+            float az = Tx<kDouble>::atan2_(dir.z, dir.x);
+            // MapToEquirectangular, src/math_lib.h:873-884
+            if (az < 0.0f) az += 2.0f * kPi;
+            float eu = az / (2.0f * kPi);
+            float ev = Tx<kDouble>::cos_(inc) * 0.5f + 0.5f;
+            ev = 1.0f - ev;
+            t = sample_nearest(S, T, eu, ev);
+        }
+        r.emission = v3{t.x * 1.0f, t.y * 1.0f, t.z * 1.0f};
+    }
+    else
+        r.emission = v3{M.emission[0], M.emission[1], M.emission[2]};
+    r.roughness = M.roughness;
+    return r;
+}
+
+// The bracket of ComputeRadianceForPath (src/simd_path_tracer.cpp:136-165):
+// X = kD o albedo / pi + D G F / (4 (N.V)(N.L) + 1e-4); the vertex radiance
+// is E + (X o Lin) * max(0, N.L).  noDeltaSpecular drops the GGX term at
+// roughness 0 (unbiased estimator only; there D is 0 except on a measure-
+// zero set where the reference computes 0/0).
+template <bool kDouble>
+__device__ __forceinline__ v3 brdf_bracket(v3 N, v3 V, v3 L, const MatOut &m,
+                                           bool noDeltaSpecular)
+{
+    v3 H = normalize(L + V);
+    float cosTheta = fmax_sel(dot(H, V), 0.0f);
+    float p5 = Tx<kDouble>::pow_(1.0f - cosTheta, 5.0f);
+    v3 F0 = splat(0.04f);
+    v3 F = F0 + (splat(1.0f) - F0) * p5;
+    v3 kD = splat(1.0f) - F;
+    float oneOverPI = 1.0f / kPi;
+    float roughness = m.roughness;
+    v3 specular{0.0f, 0.0f, 0.0f};
+    if (!(noDeltaSpecular && roughness == 0.0f))
+    {
+        // DistributionGGX, src/simd_path_tracer.cpp:72-84
+        float a = roughness * roughness;
+        float a2 = a * a;
+        float NdotH = fmax_sel(dot(N, H), 0.0f);
+        float NdotH2 = NdotH * NdotH;
+        float denom = (NdotH2 * (a2 - 1.0f) + 1.0f);
+        denom = kPi * denom * denom;
+        float NDF = a2 / denom;
+        // GeometrySmith / GeometrySchlickGGX, src/simd_path_tracer.cpp:86-105
+        float NdotV = fmax_sel(dot(N, V), 0.0f);
+        float NdotL = fmax_sel(dot(N, L), 0.0f);
+        float r = (roughness + 1.0f);
+        float k = (r * r) / 8.0f;
+        float ggx2 = NdotV / (NdotV * (1.0f - k) + k);
+        float ggx1 = NdotL / (NdotL * (1.0f - k) + k);
+        float G = ggx1 * ggx2;
+        v3 numerator = F * (NDF * G);
+        float denominator =
+            4.0f * fmax_sel(dot(N, V), 0.0f) * fmax_sel(dot(N, L), 0.0f) +
+            0.0001f;
+        specular = numerator * (1.0f / denominator);
+    }
+    return had(kD, m.albedo) * oneOverPI + specular;
+}
+
+// --------------------------------------------------------------- RNG
+struct Rng
+{
+    uint32_t xs;
+    uint64_t state, inc;
+    bool pcg;
+    __device__ __forceinline__ uint32_t next()
+    {
+        if (pcg) return pcg32_next(state, inc);
+        return xorshift32(xs);
+    }
+    __device__ __forceinline__ float uni() { return unilateral(next()); }
+    __device__ __forceinline__ float bi() { return -1.0f + 2.0f * uni(); }
+    __device__ __forceinline__ float u24()
+    {
+        return (float)(next() >> 8) * (1.0f / 16777216.0f);
+    }
+};
+
+__device__ __forceinline__ Rng load_rng(const TraceParams &p, uint32_t path,
+                                        uint32_t pixel)
+{
+    Rng r;
+    r.pcg = p.rngMode == SPG_RNG_PCG;
+    if (r.pcg)
+    {
+        r.state = p.paths.rngPcg[path];
+        r.inc = ((uint64_t)pixel << 1u) | 1u;
+        r.xs = 0;
+    }
+    else
+    {
+        r.xs = p.paths.rngXs[path];
+        r.state = r.inc = 0;
+    }
+    return r;
+}
+
+__device__ __forceinline__ void store_rng(const TraceParams &p, uint32_t path,
+                                          const Rng &r)
+{
+    if (r.pcg) p.paths.rngPcg[path] = r.state;
+    else p.paths.rngXs[path] = r.xs;
+}
+
+// ------------------------------------------------------------- kernels
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_closest_mesh(
+    DevMesh m, const float *__restrict__ ox, const float *__restrict__ oy,
+    const float *__restrict__ oz, const float *__restrict__ dx,
+    const float *__restrict__ dy, const float *__restrict__ dz, uint32_t n,
+    uint32_t *__restrict__ primOut, float *__restrict__ tOut,
+    float *__restrict__ uOut, float *__restrict__ vOut,
+    unsigned long long *__restrict__ stats)
+{
+    __shared__ int stack[kStackDepth * kTraceBlock];
+    uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    bool active = i < n;
+    Counts cnt{0, 0, 0};
+    Hit best{INFINITY, 0.0f, 0.0f, SPG_MISS};
+    if (active)
+    {
+        v3 o{ox[i], oy[i], oz[i]}, d{dx[i], dy[i], dz[i]};
+        v3 invD = inverse(d);
+        blas_traverse<kCount>(m, o, d, invD, INFINITY, best, stack, 0, cnt);
+        primOut[i] = best.prim;
+        tOut[i] = best.prim == SPG_MISS ? -1.0f : best.t;
+        if (uOut) uOut[i] = best.u;
+        if (vOut) vOut[i] = best.v;
+    }
+    if (kCount)
+    {
+        wave_add(&stats[0], active ? 1ull : 0ull);
+        wave_add(&stats[1], cnt.box);
+        wave_add(&stats[2], cnt.tri);
+        wave_add(&stats[4], active && best.prim != SPG_MISS ? 1ull : 0ull);
+    }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_intersect_scene(
+    DevScene S, const float *__restrict__ rays6, uint32_t n,
+    float *__restrict__ out7, uint32_t *__restrict__ objOut,
+    uint32_t *__restrict__ primOut)
+{
+    __shared__ int stack[kStackDepth * kTraceBlock];
+    uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    v3 o{rays6[6 * i], rays6[6 * i + 1], rays6[6 * i + 2]};
+    v3 d{rays6[6 * i + 3], rays6[6 * i + 4], rays6[6 * i + 5]};
+    Counts cnt{0, 0, 0};
+    SceneHit h;
+    scene_traverse<false>(S, o, d, h, stack, cnt);
+    float *r = out7 + 7 * (size_t)i;
+    if (h.inst == SPG_MISS)
+    {
+        r[0] = -1.0f;
+        for (int k = 1; k < 7; ++k) r[k] = 0.0f;
+    }
+    else
+    {
+        v3 N;
+        v2 uv;
+        surface_at(S, h.inst, h.prim, h.u, h.v, N, uv);
+        const DevInstance &I = S.inst[h.inst];
+        r[0] = h.t;
+        r[1] = __uint_as_float(I.materialId);
+        r[2] = N.x;
+        r[3] = N.y;
+        r[4] = N.z;
+        r[5] = uv.x;
+        r[6] = uv.y;
+    }
+    if (objOut) objOut[i] = h.inst;
+    if (primOut) primOut[i] = h.prim;
+}
+
+// Logical block index spread so consecutive ranges of work stay on one XCD
+// (blocks are dispatched to the 8 XCDs round-robin).
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    uint32_t g = gridDim.x, b = blockIdx.x;
+    if (g % 8u) return b;
+    return (b % 8u) * (g / 8u) + b / 8u;
+}
+
+// Camera ray of one path: the jitter draws (y drawn first: the reference's
+// two RandomBilateral arguments, src/simd_path_tracer.cpp:219-222, evaluate
+// right to left as g++ on x86-64 builds them), sp_CalculateFilmPositions
+// (src/simd_path_tracer.cpp:38-63), dir = Normalize(filmP - position).
+__global__ __launch_bounds__(256) void k_raygen(TraceParams p)
+{
+    const uint32_t nPaths = p.slots * p.sppPass;
+    for (uint32_t path = xcd_block() * 256 + threadIdx.x; path < nPaths;
+         path += gridDim.x * 256)
+    {
+        uint32_t slot = path / p.sppPass;
+        uint32_t sample = p.sampleBase + path % p.sppPass;
+        uint32_t pixel = p.slotPixel[slot];
+        uint32_t x = pixel % p.cam.width, y = pixel / p.cam.width;
+        Rng rng;
+        rng.pcg = p.rngMode == SPG_RNG_PCG;
+        if (rng.pcg)
+        {
+            pcg32_seed(rng.state, rng.inc, p.seed, pixel);
+            pcg32_advance(rng.state, rng.inc, (uint64_t)sample << 20);
+            rng.xs = 0;
+        }
+        else
+        {
+            rng.xs = sample == 0 ? pixel_seed(p.seed, pixel)
+                                 : p.paths.slotRng[slot];
+            rng.state = rng.inc = 0;
+        }
+        float b1 = rng.bi();
+        float b2 = rng.bi();
+        float jy = p.cam.halfPixelHeight * b1;
+        float jx = p.cam.halfPixelWidth * b2;
+        float px = (float)x + 0.5f, py = (float)y + 0.5f;
+        px = px + jx;
+        py = py + jy;
+        float fx = px / (float)p.cam.width;
+        float fy = py / (float)p.cam.height;
+        fy = 1.0f - fy;
+        fx = fx * 2.0f - 1.0f;
+        fy = fy * 2.0f - 1.0f;
+        v3 film = p.cam.right * (p.cam.halfFilmWidth * fx);
+        film = film + p.cam.up * (p.cam.halfFilmHeight * fy);
+        film = film + p.cam.filmCenter;
+        v3 o = p.cam.position;
+        v3 d = normalize(film - p.cam.position);
+        const uint32_t cap = p.paths.capacity;
+        p.paths.qo[0][path] = o.x;
+        p.paths.qo[0][cap + path] = o.y;
+        p.paths.qo[0][2 * cap + path] = o.z;
+        p.paths.qd[0][path] = d.x;
+        p.paths.qd[0][cap + path] = d.y;
+        p.paths.qd[0][2 * cap + path] = d.z;
+        p.paths.qpath[0][path] = path;
+        p.paths.pathSlot[path] = slot;
+        p.paths.pathSample[path] = sample;
+        p.paths.vcount[path] = 0;
+        p.paths.beta[path] = 1.0f;
+        p.paths.beta[cap + path] = 1.0f;
+        p.paths.beta[2 * cap + path] = 1.0f;
+        p.paths.radiance[path] = 0.0f;
+        p.paths.radiance[cap + path] = 0.0f;
+        p.paths.radiance[2 * cap + path] = 0.0f;
+        store_rng(p, path, rng);
+    }
+}
+
+// Closest hit of every queued ray.
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q)
+{
+    __shared__ int stack[kStackDepth * kTraceBlock];
+    const uint32_t n = p.paths.counters[q];
+    const uint32_t cap = p.paths.capacity;
+    const float *qo = p.paths.qo[q], *qd = p.paths.qd[q];
+    Counts cnt{0, 0, 0};
+    uint32_t rays = 0;
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    uint32_t base = xcd_block() * kTraceBlock;
+    for (uint32_t i0 = base; i0 < n; i0 += stride)
+    {
+        uint32_t i = i0 + threadIdx.x;
+        if (i < n)
+        {
+            v3 o{qo[i], qo[cap + i], qo[2 * cap + i]};
+            v3 d{qd[i], qd[cap + i], qd[2 * cap + i]};
+            SceneHit h;
+            scene_traverse<kCount>(p.scene, o, d, h, stack, cnt);
+            p.paths.ht[i] = h.t;
+            p.paths.hu[i] = h.u;
+            p.paths.hv[i] = h.v;
+            p.paths.hinst[i] = h.inst;
+            p.paths.hprim[i] = h.prim;
+            rays++;
+        }
+    }
+    if (kCount)
+    {
+        wave_add(&p.paths.metrics[3], cnt.box);
+        wave_add(&p.paths.metrics[4], cnt.mesh);
+        wave_add(&p.paths.metrics[5], cnt.tri);
+    }
+    wave_add(&p.paths.metrics[0], rays);
+}
+
+// Cosine-weighted direction about n: r = sqrt(u1), phi = 2 pi u2,
+// z = sqrt(max(0, 1-u1)) in the orthonormal basis of Duff et al. 2017.
+template <bool kDouble>
+__device__ __forceinline__ v3 cosine_direction(v3 n, float u1, float u2)
+{
+    float r = sqrtf(u1);
+    float phi = 6.28318548f * u2;
+    float lx = r * Tx<kDouble>::cos_(phi);
+    float ly = r * Tx<kDouble>::sin_(phi);
+    float lz = sqrtf(fmax_sel(0.0f, 1.0f - u1));
+    float sign = copysignf(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    v3 t1{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    v3 t2{b, sign + n.y * n.y * a, -n.y};
+    return (t1 * lx + t2 * ly) + n * lz;
+}
+
+// One bounce of every queued path: surface, material, scatter, vertex record
+// (parity) or throughput update (unbiased), append continuing rays to queue
+// q^1 with a ballot + one atomic per wave.
+template <bool kDouble>
+__global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
+{
+    const uint32_t n = p.paths.counters[q];
+    const uint32_t cap = p.paths.capacity;
+    const int nq = q ^ 1;
+    uint32_t hits = 0, misses = 0;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t i0 = xcd_block() * 256; i0 < n; i0 += stride)
+    {
+        uint32_t i = i0 + threadIdx.x;
+        bool push = false;
+        v3 no{0, 0, 0}, nd{0, 0, 0};
+        uint32_t path = 0;
+        if (i < n)
+        {
+            path = p.paths.qpath[q][i];
+            uint32_t slot = p.paths.pathSlot[path];
+            uint32_t pixel = p.slotPixel[slot];
+            Rng rng = load_rng(p, path, pixel);
+            v3 o{p.paths.qo[q][i], p.paths.qo[q][cap + i], p.paths.qo[q][2 * cap + i]};
+            v3 d{p.paths.qd[q][i], p.paths.qd[q][cap + i], p.paths.qd[q][2 * cap + i]};
+            float t = p.paths.ht[i];
+            uint32_t inst = p.paths.hinst[i];
+            uint32_t vc = p.paths.vcount[path];
+            v3 V = -d;
+            bool hit = inst != SPG_MISS && t > 0.0f;
+            if (p.estimator == SPG_EST_PARITY)
+            {
+                v3 E, X;
+                float cosine;
+                if (hit)
+                {
+                    hits++;
+                    v3 N;
+                    v2 uv;
+                    surface_at(p.scene, inst, p.paths.hprim[i], p.paths.hu[i],
+                               p.paths.hv[i], N, uv);
+                    v3 P = o + d * t;
+                    // RandomDirectionOnHemisphere, src/math_lib.h:932-946
+                    float theta = kPi * rng.uni();
+                    float phi = kPi * rng.bi();
+                    v3 dir;
+                    dir.x = Tx<kDouble>::sin_(theta) * Tx<kDouble>::cos_(phi);
+                    dir.z = Tx<kDouble>::sin_(theta) * Tx<kDouble>::sin_(phi);
+                    dir.y = Tx<kDouble>::cos_(theta);
+                    if (dot(dir, N) < 0.0f) dir = -dir;
+                    MatOut m = eval_material<kDouble>(
+                        p.scene, p.scene.inst[inst].material, uv, V);
+                    E = m.emission;
+                    X = brdf_bracket<kDouble>(N, V, dir, m, false);
+                    cosine = fmax_sel(0.0f, dot(N, dir));
+                    if (vc + 1 < p.bounces)
+                    {
+                        push = true;
+                        no = P + N * 0.0001f;
+                        nd = dir;
+                    }
+                }
+                else
+                {
+                    misses++;
+                    v2 uv0{0.0f, 0.0f};
+                    MatOut m = eval_material<kDouble>(p.scene, p.scene.background,
+                                                      uv0, V);
+                    v3 Z{0.0f, 0.0f, 0.0f};
+                    E = m.emission;
+                    X = brdf_bracket<kDouble>(Z, V, Z, m, false);
+                    cosine = fmax_sel(0.0f, dot(Z, Z));
+                }
+                float *vt = p.paths.vtx + (size_t)vc * 7 * cap + path;
+                vt[0] = E.x;
+                vt[cap] = E.y;
+                vt[2 * cap] = E.z;
+                vt[3 * cap] = X.x;
+                vt[4 * cap] = X.y;
+                vt[5 * cap] = X.z;
+                vt[6 * cap] = cosine;
+                p.paths.vcount[path] = vc + 1;
+            }
+            else
+            {
+                v3 beta{p.paths.beta[path], p.paths.beta[cap + path],
+                        p.paths.beta[2 * cap + path]};
+                v3 L{p.paths.radiance[path], p.paths.radiance[cap + path],
+                     p.paths.radiance[2 * cap + path]};
+                if (hit)
+                {
+                    hits++;
+                    v3 N;
+                    v2 uv;
+                    surface_at(p.scene, inst, p.paths.hprim[i], p.paths.hu[i],
+                               p.paths.hv[i], N, uv);
+                    MatOut m = eval_material<kDouble>(
+                        p.scene, p.scene.inst[inst].material, uv, V);
+                    L = L + had(beta, m.emission);
+                    float u1 = rng.u24(), u2 = rng.u24();
+                    v3 wi = cosine_direction<kDouble>(N, u1, u2);
+                    float cosine = dot(N, wi);
+                    if (cosine > 0.0f && vc + 1 < p.bounces)
+                    {
+                        v3 X = brdf_bracket<kDouble>(N, V, wi, m, true);
+                        beta = had(beta, X * kPi);
+                        v3 P = o + d * t;
+                        push = true;
+                        no = P + N * 0.0001f;
+                        nd = wi;
+                    }
+                }
+                else
+                {
+                    misses++;
+                    v2 uv0{0.0f, 0.0f};
+                    MatOut m = eval_material<kDouble>(p.scene, p.scene.background,
+                                                      uv0, V);
+                    L = L + had(beta, m.emission);
+                }
+                p.paths.beta[path] = beta.x;
+                p.paths.beta[cap + path] = beta.y;
+                p.paths.beta[2 * cap + path] = beta.z;
+                p.paths.radiance[path] = L.x;
+                p.paths.radiance[cap + path] = L.y;
+                p.paths.radiance[2 * cap + path] = L.z;
+                p.paths.vcount[path] = vc + 1;
+            }
+            store_rng(p, path, rng);
+        }
+        // compaction of continuing rays into queue nq
+        unsigned long long mask = __ballot(push);
+        uint32_t total = (uint32_t)__popcll(mask);
+        uint32_t base = 0;
+        uint32_t lane = lane_id();
+        if (total)
+        {
+            int leader = __ffsll((long long)mask) - 1;
+            if ((int)lane == leader)
+                base = atomicAdd(&p.paths.counters[nq], total);
+            base = __shfl(base, leader);
+        }
+        if (push)
+        {
+            uint32_t slotq = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            p.paths.qo[nq][slotq] = no.x;
+            p.paths.qo[nq][cap + slotq] = no.y;
+            p.paths.qo[nq][2 * cap + slotq] = no.z;
+            p.paths.qd[nq][slotq] = nd.x;
+            p.paths.qd[nq][cap + slotq] = nd.y;
+            p.paths.qd[nq][2 * cap + slotq] = nd.z;
+            p.paths.qpath[nq][slotq] = path;
+        }
+    }
+    wave_add(&p.paths.metrics[1], hits);
+    wave_add(&p.paths.metrics[2], misses);
+}
+
+// Per pixel slot: the radiance of each of its paths in sample order
+// (parity: ComputeRadianceForPath's reverse recurrence with the clamp on the
+// incoming radiance, src/simd_path_tracer.cpp:113-172), then
+// total += radiance * (1/spp) (src/simd_path_tracer.cpp:323).
+__global__ __launch_bounds__(256) void k_finalize(TraceParams p)
+{
+    const uint32_t cap = p.paths.capacity;
+    const float invSpp = 1.0f / (float)p.spp;
+    for (uint32_t slot = xcd_block() * 256 + threadIdx.x; slot < p.slots;
+         slot += gridDim.x * 256)
+    {
+        v3 total{p.accum[slot], p.accum[p.slots + slot],
+                 p.accum[2 * p.slots + slot]};
+        uint32_t lastXs = 0;
+        for (uint32_t k = 0; k < p.sppPass; ++k)
+        {
+            uint32_t path = slot * p.sppPass + k;
+            v3 radiance{0.0f, 0.0f, 0.0f};
+            if (p.estimator == SPG_EST_PARITY)
+            {
+                int32_t vc = (int32_t)p.paths.vcount[path];
+                for (int32_t i = vc - 1; i >= 0; --i)
+                {
+                    const float *vt = p.paths.vtx + (size_t)i * 7 * cap + path;
+                    v3 E{vt[0], vt[cap], vt[2 * cap]};
+                    v3 X{vt[3 * cap], vt[4 * cap], vt[5 * cap]};
+                    float cosine = vt[6 * cap];
+                    v3 incoming = radiance;
+                    if (p.clampValue > 0.0f)
+                        incoming = vclamp(incoming, 0.0f, p.clampValue);
+                    radiance = E + had(X, incoming) * cosine;
+                }
+            }
+            else
+                radiance = v3{p.paths.radiance[path], p.paths.radiance[cap + path],
+                              p.paths.radiance[2 * cap + path]};
+            total = total + radiance * invSpp;
+            lastXs = p.paths.rngXs[path];
+        }
+        p.accum[slot] = total.x;
+        p.accum[p.slots + slot] = total.y;
+        p.accum[2 * p.slots + slot] = total.z;
+        if (p.rngMode != SPG_RNG_PCG) p.paths.slotRng[slot] = lastXs;
+    }
+}
+
+__global__ void k_write_pixels(TraceParams p, float *image)
+{
+    for (uint32_t slot = blockIdx.x * 256 + threadIdx.x; slot < p.slots;
+         slot += gridDim.x * 256)
+    {
+        uint32_t pixel = p.slotPixel[slot];
+        float4 c = make_float4(p.accum[slot], p.accum[p.slots + slot],
+                               p.accum[2 * p.slots + slot], 1.0f);
+        reinterpret_cast<float4 *>(image)[pixel] = c;
+    }
+}
+
+__global__ void k_clear_u32(uint32_t *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
+
+__global__ void k_clear_f32(float *p, size_t n)
+{
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256)
+        p[i] = 0.0f;
+}
+
+// ------------------------------------------------------------ launchers
+static uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap)
+{
+    uint64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+hipError_t launch_closest_mesh(hipStream_t s, const DevMesh *mesh,
+                               const float *ox, const float *oy,
+                               const float *oz, const float *dx,
+                               const float *dy, const float *dz, uint32_t n,
+                               uint32_t *prim, float *t, float *u, float *v,
+                               unsigned long long *stats)
+{
+    if (n == 0) return hipSuccess;
+    dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
+    if (stats)
+        hipLaunchKernelGGL(k_closest_mesh<true>, grid, dim3(kTraceBlock), 0, s,
+                           *mesh, ox, oy, oz, dx, dy, dz, n, prim, t, u, v,
+                           stats);
+    else
+        hipLaunchKernelGGL(k_closest_mesh<false>, grid, dim3(kTraceBlock), 0, s,
+                           *mesh, ox, oy, oz, dx, dy, dz, n, prim, t, u, v,
+                           stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_intersect_scene(hipStream_t s, const DevScene &scene,
+                                  const float *rays6, uint32_t n, float *out7,
+                                  uint32_t *object, uint32_t *prim)
+{
+    if (n == 0) return hipSuccess;
+    dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
+    hipLaunchKernelGGL(k_intersect_scene, grid, dim3(kTraceBlock), 0, s, scene,
+                       rays6, n, out7, object, prim);
+    return hipGetLastError();
+}
+
+hipError_t launch_raygen(hipStream_t s, const TraceParams &p)
+{
+    uint64_t n = (uint64_t)p.slots * p.sppPass;
+    hipLaunchKernelGGL(k_raygen, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s,
+                       p);
+    return hipGetLastError();
+}
+
+hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q)
+{
+    uint32_t g = grid_for((uint64_t)p.slots * p.sppPass, kTraceBlock, 8192);
+    g = (g + 7) / 8 * 8;
+    if (p.collect)
+        hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(kTraceBlock), 0, s, p, q);
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(kTraceBlock), 0, s, p, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
+{
+    uint32_t g = grid_for((uint64_t)p.slots * p.sppPass, 256, 4096);
+    g = (g + 7) / 8 * 8;
+    if (p.tx == SPG_TX_DOUBLE)
+        hipLaunchKernelGGL(k_shade<true>, dim3(g), dim3(256), 0, s, p, q);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(g), dim3(256), 0, s, p, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(hipStream_t s, const TraceParams &p)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(grid_for(p.slots, 256, 4096)), dim3(256),
+                       0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,
+                               float *image)
+{
+    hipLaunchKernelGGL(k_write_pixels, dim3(grid_for(p.slots, 256, 4096)),
+                       dim3(256), 0, s, p, image);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_counters(hipStream_t s, uint32_t *counters, int n)
+{
+    hipLaunchKernelGGL(k_clear_u32, dim3((n + 255) / 256), dim3(256), 0, s,
+                       counters, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_counter(hipStream_t s, uint32_t *counter, uint32_t v)
+{
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, counter, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_floats(hipStream_t s, float *p, size_t n)
+{
+    hipLaunchKernelGGL(k_clear_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
+                       s, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace spg
