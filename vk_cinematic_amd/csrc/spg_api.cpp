@@ -167,6 +167,18 @@ int spg_device_create(int ordinal, spg_Device **out)
     }
     for (int i = 0; i < 64; ++i) hipEventCreate(&d->ev[i]);
     d->eventsMade = true;
+    // persistent-launch work counters (256 B) then the traversal stack spill
+    size_t spillBytes = spill_ints() * sizeof(int);
+    e = hipMalloc(&d->persistBlock, 256 + spillBytes);
+    if (e != hipSuccess)
+    {
+        for (int i = 0; i < 64; ++i) hipEventDestroy(d->ev[i]);
+        hipStreamDestroy(d->stream);
+        delete d;
+        return hip_check(e, "hipMalloc(traversal spill)");
+    }
+    d->persist.counter = (uint32_t *)d->persistBlock;
+    d->persist.spill = (int *)((char *)d->persistBlock + 256);
     *out = d;
     return SPG_OK;
 }
@@ -186,6 +198,7 @@ static void delete_device(spg_Device *d)
     free_ptr(d->image);
     free_ptr(d->slotPixel);
     free_ptr(d->accum);
+    free_ptr(d->persistBlock);
     if (d->eventsMade)
         for (int i = 0; i < 64; ++i) hipEventDestroy(d->ev[i]);
     hipStreamDestroy(d->stream);
@@ -902,7 +915,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
             for (uint32_t b = 0; b < cfg.bounces; ++b)
             {
                 tm.begin(kExtend);
-                HIPCHK(launch_extend(d->stream, p, q));
+                HIPCHK(launch_extend(d->stream, p, q, d->persist));
                 tm.end();
                 HIPCHK(launch_set_counter(d->stream, &p.paths.counters[q ^ 1], 0));
                 tm.begin(kShade);
@@ -1072,7 +1085,7 @@ int spg_intersect_closest_device(spg_Scene *s, uint32_t meshId,
         HIPCHK(hipMemsetAsync(st, 0, 64, d->stream));
     }
     HIPCHK(launch_closest_mesh(d->stream, &m, ox, oy, oz, dx, dy, dz, n, primId,
-                               t, u, v, st));
+                               t, u, v, st, d->persist));
     if (stats)
     {
         unsigned long long h[8];
@@ -1139,7 +1152,8 @@ int spg_time_intersect_closest(spg_Scene *s, uint32_t meshId, const float *ox,
     HIPCHK(hipEventRecord(d->ev[60], d->stream));
     for (uint32_t i = 0; i < iterations; ++i)
         HIPCHK(launch_closest_mesh(d->stream, &m, ox, oy, oz, dx, dy, dz, n,
-                                   primId, t, nullptr, nullptr, nullptr));
+                                   primId, t, nullptr, nullptr, nullptr,
+                                   d->persist));
     HIPCHK(hipEventRecord(d->ev[61], d->stream));
     HIPCHK(hipEventSynchronize(d->ev[61]));
     float ms = 0.0f;
@@ -1165,7 +1179,8 @@ int spg_intersect_scene(spg_Scene *s, const float *rays6, uint32_t n,
     uint32_t *dobj = (uint32_t *)(dout + 7 * (size_t)n), *dprim = dobj + n;
     int r = hip_check(hipMemcpy(dr, rays6, 24 * (size_t)n, hipMemcpyHostToDevice), "h2d");
     if (!r) r = hip_check(launch_intersect_scene(d->stream, s->dev_scene, dr, n,
-                                                 dout, dobj, dprim), "launch");
+                                                 dout, dobj, dprim, d->persist),
+                                  "launch");
     if (!r) r = hip_check(hipStreamSynchronize(d->stream), "sync");
     if (!r) r = hip_check(hipMemcpy(out7, dout, 28 * (size_t)n, hipMemcpyDeviceToHost), "d2h");
     if (!r && object) r = hip_check(hipMemcpy(object, dobj, 4 * (size_t)n, hipMemcpyDeviceToHost), "d2h");

@@ -20,7 +20,17 @@ namespace spg {
 constexpr int kStackDepth = 64;
 constexpr uint32_t kMaxTlasDepth = 20;
 constexpr uint32_t kMaxBlasDepth = kStackDepth - kMaxTlasDepth - 2;
-constexpr int kTraceBlock = 64;  // one wave per block: 16 KB of LDS stack
+constexpr int kTraceBlock = 64;  // traversal blocks are one wave
+// Resident waves per CU a persistent traversal launch may use (sizes the
+// global spill area of the traversal stacks).
+constexpr uint32_t kMaxPersistWavesPerCU = 32;
+
+// Per-launch work counter and stack spill area of persistent launches.
+struct Persist
+{
+    uint32_t *counter;
+    int *spill;
+};
 
 // Relative slack on the t-max cull: a box is skipped only if its entry
 // exceeds the best t by more than 2^-12 of it, so rounding differences
@@ -151,6 +161,8 @@ struct spg_Device
     size_t slotPixelCount = 0;
     void *accum = nullptr;       // per-slot radiance sums
     size_t accumCount = 0;
+    void *persistBlock = nullptr; // launch counters + traversal stack spill
+    spg::Persist persist{};
     hipEvent_t ev[64];
     bool eventsMade = false;
     double lastMs[6] = {0};
@@ -214,12 +226,15 @@ hipError_t launch_closest_mesh(hipStream_t s, const DevMesh *mesh,
                                const float *oz, const float *dx,
                                const float *dy, const float *dz, uint32_t n,
                                uint32_t *prim, float *t, float *u, float *v,
-                               unsigned long long *stats);
+                               unsigned long long *stats, const Persist &ps);
 hipError_t launch_intersect_scene(hipStream_t s, const DevScene &scene,
                                   const float *rays6, uint32_t n, float *out7,
-                                  uint32_t *object, uint32_t *prim);
+                                  uint32_t *object, uint32_t *prim,
+                                  const Persist &ps);
+size_t spill_ints();
 hipError_t launch_raygen(hipStream_t s, const TraceParams &p);
-hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q);
+hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
+                         const Persist &ps);
 hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q);
 hipError_t launch_finalize(hipStream_t s, const TraceParams &p);
 hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,

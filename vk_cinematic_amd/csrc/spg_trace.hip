@@ -13,21 +13,11 @@
 // order (spg_math.h) and built with -ffp-contract=off and correctly rounded
 // division/sqrt, so a path computes the same FP32 bits as the reference CPU
 // tracer given the same random numbers and transcendental results.
-#include "spg_internal.h"
+#include "spg_traverse.h"
 
 namespace spg {
 
 // ------------------------------------------------------------- helpers
-__device__ __forceinline__ bool d_is_leaf(int32_t r) { return r < 0; }
-__device__ __forceinline__ uint32_t d_leaf_first(int32_t r)
-{
-    return ((uint32_t)~r) >> 3;
-}
-__device__ __forceinline__ uint32_t d_leaf_count(int32_t r)
-{
-    return (((uint32_t)~r) & 7u) + 1u;
-}
-
 __device__ __forceinline__ uint32_t lane_id()
 {
     return __lane_id();
@@ -77,203 +67,6 @@ template <> struct Tx<false>
         return __powf(x, y);
     }
 };
-
-struct Hit
-{
-    float t, u, v;
-    uint32_t prim;
-};
-
-struct Counts
-{
-    uint32_t box, tri, mesh;
-};
-
-// ------------------------------------------------------------ traversal
-// Closest hit in one BVH2.  The stack column of this lane starts at
-// stack[sp0*kTraceBlock]; entries are inner-node indices.  best.t = +inf on
-// entry means "no hit yet"; tcullLimit bounds the search from outside.
-template <bool kCount>
-__device__ __forceinline__ void blas_traverse(const DevMesh &m, v3 o, v3 d,
-                                              v3 invD, float tcullLimit,
-                                              Hit &best, int *stack, int sp0,
-                                              Counts &cnt)
-{
-    int sp = sp0;
-    int node = 0;
-    const int lane = threadIdx.x;
-    for (;;)
-    {
-        const float4 *N = m.nodes + 4 * node;
-        float4 A = N[0], B = N[1], C = N[2], D = N[3];
-        int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
-        float tc = best.t * kCullSlack;
-        tc = tc < tcullLimit ? tc : tcullLimit;
-        float e0, e1;
-        bool h0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, invD, &e0) &&
-                  e0 <= tc;
-        bool h1 = r1 != kEmptyChild &&
-                  slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, invD, &e1) &&
-                  e1 <= tc;
-        if (kCount) cnt.box += r1 != kEmptyChild ? 2u : 1u;
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-        {
-            int32_t r = c == 0 ? r0 : r1;
-            bool h = c == 0 ? h0 : h1;
-            if (h && d_is_leaf(r))
-            {
-                uint32_t first = d_leaf_first(r), count = d_leaf_count(r);
-                for (uint32_t k = 0; k < count; ++k)
-                {
-                    const float4 *T = m.tris + 3 * (first + k);
-                    float4 t0 = T[0], t1 = T[1], t2 = T[2];
-                    v3 a{t0.x, t0.y, t0.z}, e1v{t0.w, t1.x, t1.y},
-                        e2v{t1.z, t1.w, t2.x};
-                    uint32_t prim = __float_as_uint(t2.y);
-                    float u, v;
-                    float t = mt_test(o, d, a, e1v, e2v, &u, &v);
-                    if (t > 0.0f &&
-                        (t < best.t || (t == best.t && prim < best.prim)))
-                    {
-                        best.t = t;
-                        best.u = u;
-                        best.v = v;
-                        best.prim = prim;
-                    }
-                }
-                if (kCount) cnt.tri += count;
-                if (c == 0) h0 = false;
-                else h1 = false;
-            }
-        }
-        if (h0 && h1)
-        {
-            int32_t nearN = r0, farN = r1;
-            if (e1 < e0)
-            {
-                nearN = r1;
-                farN = r0;
-            }
-            stack[sp * kTraceBlock + lane] = farN;
-            ++sp;
-            node = nearN;
-        }
-        else if (h0)
-            node = r0;
-        else if (h1)
-            node = r1;
-        else
-        {
-            if (sp == sp0) break;
-            --sp;
-            node = stack[sp * kTraceBlock + lane];
-        }
-    }
-}
-
-struct SceneHit
-{
-    float t, u, v;
-    uint32_t inst, prim;
-};
-
-// sp_RayIntersectScene (src/sp_scene.cpp:229-339): TLAS over instance boxes,
-// per instance the local ray o' = M^-1 o, d' = norm(M^-1 d), the mesh query,
-// then world t = dot(M (o' + t' d') - o, d).  Smallest world t wins; equal t
-// goes to the lower instance index.  out.t = -1 and inst = miss if nothing.
-template <bool kCount>
-__device__ __forceinline__ void scene_traverse(const DevScene &S, v3 o, v3 d,
-                                               SceneHit &out, int *stack,
-                                               Counts &cnt)
-{
-    out.t = -1.0f;
-    out.inst = SPG_MISS;
-    out.prim = SPG_MISS;
-    out.u = out.v = 0.0f;
-    if (S.instCount == 0) return;
-    v3 invD = inverse(d);
-    const int lane = threadIdx.x;
-    int sp = 0;
-    int node = 0;
-    for (;;)
-    {
-        const float4 *N = S.tlasNodes + 4 * node;
-        float4 A = N[0], B = N[1], C = N[2], D = N[3];
-        int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
-        float tc = out.t > 0.0f ? out.t * kCullSlack : INFINITY;
-        float e0, e1;
-        bool h0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, invD, &e0) &&
-                  e0 <= tc;
-        bool h1 = r1 != kEmptyChild &&
-                  slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, invD, &e1) &&
-                  e1 <= tc;
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-        {
-            int32_t r = c == 0 ? r0 : r1;
-            bool h = c == 0 ? h0 : h1;
-            if (h && d_is_leaf(r))
-            {
-                uint32_t first = d_leaf_first(r), count = d_leaf_count(r);
-                for (uint32_t k = 0; k < count; ++k)
-                {
-                    uint32_t ii = S.tlasInst[first + k];
-                    const DevInstance &I = S.inst[ii];
-                    v3 lo = xform_point(I.inv, o);
-                    v3 ld = normalize(xform_vector(I.inv, d));
-                    v3 linv = inverse(ld);
-                    float lim = out.t > 0.0f
-                                    ? out.t * I.cullScale * kCullSlack
-                                    : INFINITY;
-                    Hit h{INFINITY, 0.0f, 0.0f, SPG_MISS};
-                    blas_traverse<kCount>(S.meshes[I.mesh], lo, ld, linv, lim,
-                                          h, stack, sp, cnt);
-                    if (kCount) cnt.mesh += 1;
-                    if (h.prim != SPG_MISS)
-                    {
-                        v3 lh = lo + ld * h.t;
-                        v3 wh = xform_point(I.model, lh);
-                        float t = dot(wh - o, d);
-                        if (t < out.t || out.t < 0.0f ||
-                            (t == out.t && ii < out.inst))
-                        {
-                            out.t = t;
-                            out.u = h.u;
-                            out.v = h.v;
-                            out.inst = ii;
-                            out.prim = h.prim;
-                        }
-                    }
-                }
-                if (c == 0) h0 = false;
-                else h1 = false;
-            }
-        }
-        if (h0 && h1)
-        {
-            int32_t nearN = r0, farN = r1;
-            if (e1 < e0)
-            {
-                nearN = r1;
-                farN = r0;
-            }
-            stack[sp * kTraceBlock + lane] = farN;
-            ++sp;
-            node = nearN;
-        }
-        else if (h0)
-            node = r0;
-        else if (h1)
-            node = r1;
-        else
-        {
-            if (sp == 0) break;
-            --sp;
-            node = stack[sp * kTraceBlock + lane];
-        }
-    }
-}
 
 // Surface at a hit (src/sp_scene.cpp:196-213 and :302-312): uv from the
 // barycentrics, smooth normal normalize(w n0 + u n1 + v n2) or the MT normal
@@ -517,6 +310,7 @@ __device__ __forceinline__ void store_rng(const TraceParams &p, uint32_t path,
 }
 
 // ------------------------------------------------------------- kernels
+// Closest hit of a ray list against one mesh (mesh space).
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_closest_mesh(
     DevMesh m, const float *__restrict__ ox, const float *__restrict__ oy,
@@ -524,67 +318,71 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_mesh(
     const float *__restrict__ dy, const float *__restrict__ dz, uint32_t n,
     uint32_t *__restrict__ primOut, float *__restrict__ tOut,
     float *__restrict__ uOut, float *__restrict__ vOut,
-    unsigned long long *__restrict__ stats)
+    unsigned long long *__restrict__ stats, uint32_t *counter, int *spill)
 {
-    __shared__ int stack[kStackDepth * kTraceBlock];
-    uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-    bool active = i < n;
-    Counts cnt{0, 0, 0};
-    Hit best{INFINITY, 0.0f, 0.0f, SPG_MISS};
-    if (active)
-    {
-        v3 o{ox[i], oy[i], oz[i]}, d{dx[i], dy[i], dz[i]};
-        v3 invD = inverse(d);
-        blas_traverse<kCount>(m, o, d, invD, INFINITY, best, stack, 0, cnt);
-        primOut[i] = best.prim;
-        tOut[i] = best.prim == SPG_MISS ? -1.0f : best.t;
-        if (uOut) uOut[i] = best.u;
-        if (vOut) vOut[i] = best.v;
-    }
+    Counts cnt{0, 0, 0, 0};
+    uint32_t rays = 0, hits = 0;
+    DevScene none{};
+    auto begin = [&](uint32_t i, Trav &s) {
+        trav_begin_mesh(m, s, v3{ox[i], oy[i], oz[i]}, v3{dx[i], dy[i], dz[i]});
+        return true;
+    };
+    auto store = [&](uint32_t i, const Trav &s) {
+        primOut[i] = s.h.prim;
+        tOut[i] = s.h.prim == SPG_MISS ? -1.0f : s.h.t;
+        if (uOut) uOut[i] = s.h.u;
+        if (vOut) vOut[i] = s.h.v;
+        if (kCount && s.h.prim != SPG_MISS) hits++;
+    };
+    persistent_walk<kCount, false>(none, n, counter, spill, begin, store, cnt,
+                                   rays);
     if (kCount)
     {
-        wave_add(&stats[0], active ? 1ull : 0ull);
+        wave_add(&stats[0], rays);
         wave_add(&stats[1], cnt.box);
         wave_add(&stats[2], cnt.tri);
-        wave_add(&stats[4], active && best.prim != SPG_MISS ? 1ull : 0ull);
+        wave_add(&stats[4], hits);
     }
 }
 
+// sp_RayIntersectScene for a ray list, with the surface at the hit.
 __global__ __launch_bounds__(kTraceBlock) void k_intersect_scene(
     DevScene S, const float *__restrict__ rays6, uint32_t n,
     float *__restrict__ out7, uint32_t *__restrict__ objOut,
-    uint32_t *__restrict__ primOut)
+    uint32_t *__restrict__ primOut, uint32_t *counter, int *spill)
 {
-    __shared__ int stack[kStackDepth * kTraceBlock];
-    uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= n) return;
-    v3 o{rays6[6 * i], rays6[6 * i + 1], rays6[6 * i + 2]};
-    v3 d{rays6[6 * i + 3], rays6[6 * i + 4], rays6[6 * i + 5]};
-    Counts cnt{0, 0, 0};
-    SceneHit h;
-    scene_traverse<false>(S, o, d, h, stack, cnt);
-    float *r = out7 + 7 * (size_t)i;
-    if (h.inst == SPG_MISS)
-    {
-        r[0] = -1.0f;
-        for (int k = 1; k < 7; ++k) r[k] = 0.0f;
-    }
-    else
-    {
-        v3 N;
-        v2 uv;
-        surface_at(S, h.inst, h.prim, h.u, h.v, N, uv);
-        const DevInstance &I = S.inst[h.inst];
-        r[0] = h.t;
-        r[1] = __uint_as_float(I.materialId);
-        r[2] = N.x;
-        r[3] = N.y;
-        r[4] = N.z;
-        r[5] = uv.x;
-        r[6] = uv.y;
-    }
-    if (objOut) objOut[i] = h.inst;
-    if (primOut) primOut[i] = h.prim;
+    Counts cnt{0, 0, 0, 0};
+    uint32_t rays = 0;
+    auto begin = [&](uint32_t i, Trav &s) {
+        trav_begin_scene(S, s, v3{rays6[6 * i], rays6[6 * i + 1], rays6[6 * i + 2]},
+                         v3{rays6[6 * i + 3], rays6[6 * i + 4], rays6[6 * i + 5]});
+        return S.instCount > 0;
+    };
+    auto store = [&](uint32_t i, const Trav &s) {
+        const SceneHit &h = s.out;
+        float *r = out7 + 7 * (size_t)i;
+        if (h.inst == SPG_MISS)
+        {
+            r[0] = -1.0f;
+            for (int k = 1; k < 7; ++k) r[k] = 0.0f;
+        }
+        else
+        {
+            v3 N;
+            v2 uv;
+            surface_at(S, h.inst, h.prim, h.u, h.v, N, uv);
+            r[0] = h.t;
+            r[1] = __uint_as_float(S.inst[h.inst].materialId);
+            r[2] = N.x;
+            r[3] = N.y;
+            r[4] = N.z;
+            r[5] = uv.x;
+            r[6] = uv.y;
+        }
+        if (objOut) objOut[i] = h.inst;
+        if (primOut) primOut[i] = h.prim;
+    };
+    persistent_walk<false, true>(S, n, counter, spill, begin, store, cnt, rays);
 }
 
 // Logical block index spread so consecutive ranges of work stay on one XCD
@@ -664,38 +462,35 @@ __global__ __launch_bounds__(256) void k_raygen(TraceParams p)
 
 // Closest hit of every queued ray.
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q)
+__global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
+                                                        uint32_t *counter,
+                                                        int *spill)
 {
-    __shared__ int stack[kStackDepth * kTraceBlock];
     const uint32_t n = p.paths.counters[q];
     const uint32_t cap = p.paths.capacity;
     const float *qo = p.paths.qo[q], *qd = p.paths.qd[q];
-    Counts cnt{0, 0, 0};
+    Counts cnt{0, 0, 0, 0};
     uint32_t rays = 0;
-    const uint32_t stride = gridDim.x * kTraceBlock;
-    uint32_t base = xcd_block() * kTraceBlock;
-    for (uint32_t i0 = base; i0 < n; i0 += stride)
-    {
-        uint32_t i = i0 + threadIdx.x;
-        if (i < n)
-        {
-            v3 o{qo[i], qo[cap + i], qo[2 * cap + i]};
-            v3 d{qd[i], qd[cap + i], qd[2 * cap + i]};
-            SceneHit h;
-            scene_traverse<kCount>(p.scene, o, d, h, stack, cnt);
-            p.paths.ht[i] = h.t;
-            p.paths.hu[i] = h.u;
-            p.paths.hv[i] = h.v;
-            p.paths.hinst[i] = h.inst;
-            p.paths.hprim[i] = h.prim;
-            rays++;
-        }
-    }
+    auto begin = [&](uint32_t i, Trav &s) {
+        trav_begin_scene(p.scene, s, v3{qo[i], qo[cap + i], qo[2 * cap + i]},
+                         v3{qd[i], qd[cap + i], qd[2 * cap + i]});
+        return p.scene.instCount > 0;
+    };
+    auto store = [&](uint32_t i, const Trav &s) {
+        p.paths.ht[i] = s.out.t;
+        p.paths.hu[i] = s.out.u;
+        p.paths.hv[i] = s.out.v;
+        p.paths.hinst[i] = s.out.inst;
+        p.paths.hprim[i] = s.out.prim;
+    };
+    persistent_walk<kCount, true>(p.scene, n, counter, spill, begin, store, cnt,
+                                  rays);
     if (kCount)
     {
         wave_add(&p.paths.metrics[3], cnt.box);
         wave_add(&p.paths.metrics[4], cnt.mesh);
         wave_add(&p.paths.metrics[5], cnt.tri);
+        wave_add(&p.paths.metrics[6], cnt.tlasBox);
     }
     wave_add(&p.paths.metrics[0], rays);
 }
@@ -940,8 +735,6 @@ __global__ void k_clear_u32(uint32_t *p, int n)
     if (i < n) p[i] = 0u;
 }
 
-__global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
-
 __global__ void k_clear_f32(float *p, size_t n)
 {
     for (size_t i = blockIdx.x * 256 + threadIdx.x; i < n;
@@ -958,34 +751,78 @@ static uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap)
     return (uint32_t)g;
 }
 
+static uint32_t device_cus()
+{
+    static uint32_t cus = 0;
+    if (!cus)
+    {
+        int dev = 0, n = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        cus = n > 0 ? (uint32_t)n : 1u;
+    }
+    return cus;
+}
+
+// Waves a persistent launch keeps resident: CUs x resident one-wave blocks
+// per CU, capped at kMaxPersistWavesPerCU per CU (the spill buffer size).
+template <class K> static uint32_t persistent_grid(K kernel)
+{
+    int blocks = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kTraceBlock, 0);
+    if (blocks < 1) blocks = 1;
+    if (blocks > (int)kMaxPersistWavesPerCU) blocks = kMaxPersistWavesPerCU;
+    return device_cus() * (uint32_t)blocks;
+}
+
+size_t spill_ints()
+{
+    return (size_t)device_cus() * kMaxPersistWavesPerCU * kTraceBlock *
+           (size_t)(kStackDepth - kShortStack);
+}
+
+static uint32_t persist_blocks(uint32_t grid, uint64_t n)
+{
+    uint64_t need = (n + kRayChunk - 1) / kRayChunk;
+    if (need < 1) need = 1;
+    return need < grid ? (uint32_t)need : grid;
+}
+
+__global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
+
 hipError_t launch_closest_mesh(hipStream_t s, const DevMesh *mesh,
                                const float *ox, const float *oy,
                                const float *oz, const float *dx,
                                const float *dy, const float *dz, uint32_t n,
                                uint32_t *prim, float *t, float *u, float *v,
-                               unsigned long long *stats)
+                               unsigned long long *stats, const Persist &ps)
 {
     if (n == 0) return hipSuccess;
-    dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
+    static uint32_t gCount = persistent_grid(k_closest_mesh<true>);
+    static uint32_t gPlain = persistent_grid(k_closest_mesh<false>);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, ps.counter, 0u);
     if (stats)
-        hipLaunchKernelGGL(k_closest_mesh<true>, grid, dim3(kTraceBlock), 0, s,
-                           *mesh, ox, oy, oz, dx, dy, dz, n, prim, t, u, v,
-                           stats);
+        hipLaunchKernelGGL(k_closest_mesh<true>, dim3(persist_blocks(gCount, n)),
+                           dim3(kTraceBlock), 0, s, *mesh, ox, oy, oz, dx, dy, dz,
+                           n, prim, t, u, v, stats, ps.counter, ps.spill);
     else
-        hipLaunchKernelGGL(k_closest_mesh<false>, grid, dim3(kTraceBlock), 0, s,
-                           *mesh, ox, oy, oz, dx, dy, dz, n, prim, t, u, v,
-                           stats);
+        hipLaunchKernelGGL(k_closest_mesh<false>, dim3(persist_blocks(gPlain, n)),
+                           dim3(kTraceBlock), 0, s, *mesh, ox, oy, oz, dx, dy, dz,
+                           n, prim, t, u, v, stats, ps.counter, ps.spill);
     return hipGetLastError();
 }
 
 hipError_t launch_intersect_scene(hipStream_t s, const DevScene &scene,
                                   const float *rays6, uint32_t n, float *out7,
-                                  uint32_t *object, uint32_t *prim)
+                                  uint32_t *object, uint32_t *prim,
+                                  const Persist &ps)
 {
     if (n == 0) return hipSuccess;
-    dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
-    hipLaunchKernelGGL(k_intersect_scene, grid, dim3(kTraceBlock), 0, s, scene,
-                       rays6, n, out7, object, prim);
+    static uint32_t g = persistent_grid(k_intersect_scene);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, ps.counter, 0u);
+    hipLaunchKernelGGL(k_intersect_scene, dim3(persist_blocks(g, n)),
+                       dim3(kTraceBlock), 0, s, scene, rays6, n, out7, object,
+                       prim, ps.counter, ps.spill);
     return hipGetLastError();
 }
 
@@ -997,14 +834,19 @@ hipError_t launch_raygen(hipStream_t s, const TraceParams &p)
     return hipGetLastError();
 }
 
-hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q)
+hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
+                         const Persist &ps)
 {
-    uint32_t g = grid_for((uint64_t)p.slots * p.sppPass, kTraceBlock, 8192);
-    g = (g + 7) / 8 * 8;
+    static uint32_t gCount = persistent_grid(k_extend<true>);
+    static uint32_t gPlain = persistent_grid(k_extend<false>);
+    uint64_t n = (uint64_t)p.slots * p.sppPass;  // bound on the queue length
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, ps.counter, 0u);
     if (p.collect)
-        hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(kTraceBlock), 0, s, p, q);
+        hipLaunchKernelGGL(k_extend<true>, dim3(persist_blocks(gCount, n)),
+                           dim3(kTraceBlock), 0, s, p, q, ps.counter, ps.spill);
     else
-        hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(kTraceBlock), 0, s, p, q);
+        hipLaunchKernelGGL(k_extend<false>, dim3(persist_blocks(gPlain, n)),
+                           dim3(kTraceBlock), 0, s, p, q, ps.counter, ps.spill);
     return hipGetLastError();
 }
 
