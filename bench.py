@@ -45,8 +45,8 @@ def cpu_baseline_config1(threads, spp=16):
     counter as src/main.cpp:731-844 does) on config 1 with the reference's
     own BVH and estimator, `spp` samples per pixel."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from _libs import (SPO_HIT_REFERENCE, SPO_RNG_PCG, SPO_TX_LIBM, SpoConfig,
-                       oracle)
+    from _libs import (SPO_EST_UNBIASED, SPO_HIT_REFERENCE, SPO_RNG_PCG,
+                       SPO_TX_LIBM, SpoConfig, oracle)
     from _scenes import load_oracle
     from vk_cinematic_amd import scenes
 
@@ -160,6 +160,119 @@ def bench_config1(args, V, scenes):
     return res
 
 
+def cpu_baseline_box(desc, threads, spp=4, bounces=6):
+    """The oracle on the same scene and estimator (plain unbiased tracer,
+    per-pixel PCG) over the same SAH hierarchy (the reference's greedy
+    builder is O(n^2 log n) and its 32/128 overlap buffers overflow at this
+    scale), 64x64 tiles from an atomic counter, one full frame at `spp`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _libs import (SPO_EST_UNBIASED, SPO_HIT_REFERENCE, SPO_RNG_PCG,
+                       SPO_TX_LIBM, SpoConfig, oracle)
+    from _scenes import load_oracle
+
+    O = oracle()
+    s, cam = load_oracle(desc, sah=True)
+    cfg = SpoConfig(rng=SPO_RNG_PCG, tx=SPO_TX_LIBM, hit=SPO_HIT_REFERENCE,
+                    estimator=SPO_EST_UNBIASED, samplesPerPixel=spp,
+                    bounces=bounces, radianceClamp=0.0, jitterYFirst=1,
+                    seed=0x5EED)
+    w, h = desc["camera"][0], desc["camera"][1]
+    px = np.zeros(w * h * 4, np.float32)
+    sec = O.spo_render_threaded(s, C.byref(cam), C.byref(cfg), 64, 64, threads, px,
+                                np.zeros(12, np.uint64))
+    O.spo_scene_destroy(s)
+    return {"value": w * h * spp / sec / 1e6, "unit": "Msamples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"one {w}x{h} frame at {spp} spp, {bounces} bounces, plain "
+                      f"unbiased tracer, libm transcendentals, same SAH BVH "
+                      f"({sec:.2f} s)"}
+
+
+def bench_box(args, V, scenes, name):
+    """config5 (or config4): the plain (unbiased) GPU tracer, one sample per
+    pixel per step over the whole image."""
+    dev = V.Device(0)
+    desc = scenes.config5() if name == "config5" else scenes.config4()
+    w, h = desc["camera"][0], desc["camera"][1]
+    bounces = 6
+    t0 = time.perf_counter()
+    s, cam = scenes.load_gpu(dev, desc)
+    build_s = time.perf_counter() - t0
+    tiles = V.ComputeTiles(w, h, 64, 64)
+    img = dev.alloc(w * h * 16)
+    tx = V.TX_FAST if args.fast else V.TX_DOUBLE
+
+    def ctx_for(step, collect=False):
+        return V.make_context(cam, s, spp=1, bounces=bounces, clamp=0.0,
+                              estimator=V.EST_UNBIASED, rng=V.RNG_PCG, seed=0x5EED,
+                              transcendentals=tx, sample_offset=step,
+                              collect=collect)
+
+    for k in range(args.warmup):
+        V.path_trace_tiles_device(ctx_for(k), tiles, img)
+    dev.synchronize()
+    ext_ms, ext_n, sh_ms, tot_ms = 0.0, 0, 0.0, 0.0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        V.path_trace_tiles_device(ctx_for(args.warmup + k), tiles, img)
+        tim = dev.last_timings()
+        ext_ms += tim["extend"][0]
+        ext_n += tim["extend"][1]
+        sh_ms += tim["shade"][0]
+        tot_ms += tim["total"][0]
+    dev.synchronize()
+    wall = time.perf_counter() - t0
+    m = V.path_trace_tiles_device(ctx_for(args.warmup, collect=True), tiles, img)
+    c = dev.last_counters()
+    rays, box, tri = c["rays"], c["slab_tests"], c["tri_tests"]
+    tbox, ment = c["tlas_slab_tests"], c["mesh_tests"]
+    # per instance entry: instance index, inverse and forward 3x4 rows,
+    # cull scale and mesh id, the mesh's two pointers = 128 bytes
+    bytes_step = 44 * rays + 24 * (box + tbox) + 36 * tri + 128 * ment
+    launches = ext_n / args.steps
+    per_launch = bytes_step / launches
+    mean_ms = ext_ms / ext_n
+    achieved = per_launch / (mean_ms * 1e-3) / 1e9
+    samples = w * h
+    res = {
+        "metric": METRIC,
+        "value": samples * args.steps / wall / 1e6,
+        "unit": "Msamples/s",
+        "ms_per_step": wall * 1e3 / args.steps,
+        "dtype": "f32",
+        "config": {"workload": f"{name}: {desc['triangles']} triangles ("
+                               f"{len(desc['objects']) - 1} displaced icospheres of "
+                               f"20480 in a closed box with a 1x1 opening), sky cube "
+                               f"6x64x64 (sun 50), {w}x{h}, 1 spp per step, "
+                               f"{bounces} bounces, plain unbiased tracer, per-pixel "
+                               f"PCG, {'fast' if args.fast else 'double-evaluated'} "
+                               f"transcendentals",
+                   "image": [w, h], "spp_per_step": 1, "bounces": bounces,
+                   "triangles": desc["triangles"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_extend (closest-hit traversal)",
+                     "launch_ms": mean_ms, "algorithmic_bytes_per_launch": per_launch,
+                     "rays_per_step": rays, "slab_tests_per_ray": box / max(rays, 1),
+                     "tri_tests_per_ray": tri / max(rays, 1),
+                     "tlas_slab_tests_per_ray": tbox / max(rays, 1),
+                     "mesh_entries_per_ray": ment / max(rays, 1)},
+        "extra": {"Mrays_per_s": rays * args.steps / wall / 1e6,
+                  "extend_ms_per_step": ext_ms / args.steps,
+                  "shade_ms_per_step": sh_ms / args.steps,
+                  "gpu_ms_per_step": tot_ms / args.steps,
+                  "scene_build_s": build_s,
+                  "paths_per_step": m.values[V.METRIC_PATHS]},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_box(desc, min(16, os.cpu_count() or 1),
+                                               bounces=bounces)
+    img.free()
+    s.close()
+    dev.close()
+    return res
+
+
 def bench_soup(args, V, scenes):
     dev = V.Device(0)
     tris, o, d = scenes.soup(1_000_000, 1 << 22, seed=7)
@@ -216,8 +329,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="config1", choices=["config1", "soup"])
+    ap.add_argument("--workload", default="config5",
+                    choices=["config5", "config4", "config1", "soup"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--fast", action="store_true",
+                    help="device float transcendentals instead of double-evaluated")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -226,8 +342,10 @@ def main():
     import vk_cinematic_amd as V
     from vk_cinematic_amd import scenes
 
-    fn = {"config1": bench_config1, "soup": bench_soup}[args.workload]
-    res = fn(args, V, scenes)
+    if args.workload in ("config5", "config4"):
+        res = bench_box(args, V, scenes, args.workload)
+    else:
+        res = {"config1": bench_config1, "soup": bench_soup}[args.workload](args, V, scenes)
     out = {"metric": res.pop("metric"), "value": res.pop("value"),
            "unit": res.pop("unit"), "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),

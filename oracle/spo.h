@@ -68,13 +68,19 @@ typedef struct {
     uint32_t errorOccurred, count, aabbTestCount;
 } spo_BvhResult;
 
-/* sp_Mesh, src/sp_scene.h:3-12 (owns copies of its arrays) */
+/* sp_Mesh, src/sp_scene.h:3-12 (owns copies of its arrays).  A mesh may
+ * instead carry an externally built BVH2 (the product's SAH hierarchy, 16
+ * floats per node, see spo_bvh2.c); its query is then the closest-hit walk
+ * of that hierarchy with equal t going to the lower triangle index. */
 typedef struct {
     spo_Vertex *vertices;
     uint32_t *indices;
     uint32_t vertexCount, indexCount;
     int useSmoothShading;
     spo_Bvh midphase;
+    float *bvh2Nodes;
+    uint32_t *bvh2Order;
+    uint32_t bvh2Count;
 } spo_Mesh;
 
 typedef struct { float m[16]; } spo_m4; /* column-major, src/math_lib.h:93 */
@@ -216,6 +222,13 @@ int spo_mesh_create(spo_Scene *s, const float *verts8, uint32_t nv,
                     int buildMidphase);
 int spo_scene_add_object(spo_Scene *s, uint32_t mesh, uint32_t material,
                          const float *pos, const float *q, const float *scale);
+/* Attach an external BVH2 to a mesh (copied). */
+int spo_mesh_set_bvh2(spo_Scene *s, uint32_t mesh, const float *nodes16,
+                      uint32_t nNodes, const uint32_t *order);
+/* The reference's fixed query limits (32 objects, 128 leaves) can be
+ * raised for scenes beyond the reference's scale. */
+void spo_scene_set_limits(spo_Scene *s, uint32_t maxBroadphase,
+                          uint32_t maxMidphase);
 void spo_scene_build(spo_Scene *s);
 int spo_material_register(spo_Scene *s, const float *albedo,
                           uint32_t albedoTexture, const float *emission,

@@ -23,6 +23,8 @@ void spo_scene_destroy(spo_Scene *s)
         free(s->meshes[i].vertices);
         free(s->meshes[i].indices);
         free(s->meshes[i].midphase.nodes);
+        free(s->meshes[i].bvh2Nodes);
+        free(s->meshes[i].bvh2Order);
     }
     free(s->meshes);
     free(s->objects);
@@ -74,6 +76,29 @@ int spo_mesh_create(spo_Scene *s, const float *verts8, uint32_t nv,
         free(mx);
     }
     return (int)s->meshCount++;
+}
+
+int spo_mesh_set_bvh2(spo_Scene *s, uint32_t mesh, const float *nodes16,
+                      uint32_t nNodes, const uint32_t *order)
+{
+    if (mesh >= s->meshCount || nNodes == 0) return -1;
+    spo_Mesh *m = &s->meshes[mesh];
+    free(m->bvh2Nodes);
+    free(m->bvh2Order);
+    m->bvh2Nodes = (float *)malloc(64 * (size_t)nNodes);
+    memcpy(m->bvh2Nodes, nodes16, 64 * (size_t)nNodes);
+    uint32_t nt = m->indexCount / 3;
+    m->bvh2Order = (uint32_t *)malloc(4 * (size_t)(nt ? nt : 1));
+    memcpy(m->bvh2Order, order, 4 * (size_t)nt);
+    m->bvh2Count = nNodes;
+    return 0;
+}
+
+void spo_scene_set_limits(spo_Scene *s, uint32_t maxBroadphase,
+                          uint32_t maxMidphase)
+{
+    s->maxBroadphaseHits = maxBroadphase;
+    s->maxMidphaseHits = maxMidphase;
 }
 
 /* sp_AddObjectToScene, src/sp_scene.cpp:77-117 (ComputeAabb over all mesh
@@ -297,7 +322,10 @@ static void consider_triangle(const spo_Mesh *m, uint32_t tri, const float *o,
     spo_rt_mt(o, d, a, b, c, r);
     if (r[0] > 0.0f)
     {
-        if (r[0] < best->t || best->t < 0.0f)
+        /* the reference keeps the first of equal t; walks in an order other
+         * than ascending index keep the lower index instead */
+        if (r[0] < best->t || best->t < 0.0f ||
+            (m->bvh2Nodes && r[0] == best->t && tri < best->prim))
         {
             best->t = r[0];
             best->normal = v3(r[3], r[4], r[5]);
@@ -318,6 +346,77 @@ static void consider_triangle(const spo_Mesh *m, uint32_t tri, const float *o,
     }
 }
 
+static int32_t node_ref(const float *f)
+{
+    int32_t r;
+    memcpy(&r, f, 4);
+    return r;
+}
+
+static int bvh2_slab(const float *lo, const float *hi, spo_v3 o, spo_v3 inv,
+                     float *entry)
+{
+    float t0x = (lo[0] - o.x) * inv.x, t1x = (hi[0] - o.x) * inv.x;
+    float t0y = (lo[1] - o.y) * inv.y, t1y = (hi[1] - o.y) * inv.y;
+    float t0z = (lo[2] - o.z) * inv.z, t1z = (hi[2] - o.z) * inv.z;
+    float mnx = t0x < t1x ? t0x : t1x, mxx = t0x > t1x ? t0x : t1x;
+    float mny = t0y < t1y ? t0y : t1y, mxy = t0y > t1y ? t0y : t1y;
+    float mnz = t0z < t1z ? t0z : t1z, mxz = t0z > t1z ? t0z : t1z;
+    float e = spo_max(0.0f, spo_max(mnx, spo_max(mny, mnz)));
+    *entry = e;
+    return e <= spo_min(mxx, spo_min(mxy, mxz));
+}
+
+/* Closest-hit walk of an attached BVH2 (the GPU's traversal, spo_bvh2.c),
+ * candidates scored by consider_triangle. */
+static void mesh_query_bvh2(const spo_Mesh *m, const float *o, const float *d,
+                            mesh_hit *best, uint64_t *metrics12)
+{
+    spo_v3 ro = v3p(o), inv = v3inv(v3p(d));
+    int32_t stack[128];
+    int sp = 0;
+    int32_t node = 0;
+    uint64_t tests = 0;
+    for (;;)
+    {
+        const float *N = m->bvh2Nodes + 16 * (size_t)node;
+        int32_t r0 = node_ref(N + 3), r1 = node_ref(N + 7);
+        float tc = best->t < 0.0f ? INFINITY : best->t * (1.0f + 1.0f / 4096.0f);
+        float e0 = 0, e1 = 0;
+        int h0 = bvh2_slab(N, N + 4, ro, inv, &e0) && e0 <= tc;
+        int h1 = r1 != 0x7FFFFFFF && bvh2_slab(N + 8, N + 12, ro, inv, &e1) && e1 <= tc;
+        tests += r1 != 0x7FFFFFFF ? 2 : 1;
+        for (int c = 0; c < 2; ++c)
+        {
+            int32_t rr = c == 0 ? r0 : r1;
+            int h = c == 0 ? h0 : h1;
+            if (h && rr < 0)
+            {
+                uint32_t v = (uint32_t)~rr, first = v >> 3, count = (v & 7u) + 1u;
+                for (uint32_t k = 0; k < count; ++k)
+                    consider_triangle(m, m->bvh2Order[first + k], o, d, best);
+                if (c == 0) h0 = 0;
+                else h1 = 0;
+            }
+        }
+        if (h0 && h1)
+        {
+            int32_t nearN = r0, farN = r1;
+            if (e1 < e0) { nearN = r1; farN = r0; }
+            stack[sp++] = farN;
+            node = nearN;
+        }
+        else if (h0) node = r0;
+        else if (h1) node = r1;
+        else
+        {
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    if (metrics12) metrics12[10] += tests;
+}
+
 /* sp_RayIntersectMesh, src/sp_scene.cpp:127-227.  Returns -1 if the midphase
  * query overflows (the reference asserts, src/sp_scene.cpp:158). */
 static int mesh_query(const spo_Scene *s, const spo_Mesh *m, const float *o,
@@ -332,6 +431,11 @@ static int mesh_query(const spo_Scene *s, const spo_Mesh *m, const float *o,
     {
         for (uint32_t tri = 0; tri < m->indexCount / 3; ++tri)
             consider_triangle(m, tri, o, d, best);
+        return 0;
+    }
+    if (m->bvh2Nodes)
+    {
+        mesh_query_bvh2(m, o, d, best, metrics12);
         return 0;
     }
     uint32_t leaves[1024];

@@ -7,15 +7,27 @@ import numpy as np
 from _libs import SpoCamera, f32, oracle, ref
 
 
-def load_oracle(desc, build_midphase=True):
-    """Returns (oracle scene handle, SpoCamera)."""
+def load_oracle(desc, build_midphase=True, sah=False):
+    """Returns (oracle scene handle, SpoCamera).  With sah=True the meshes
+    walk the product's SAH hierarchy (built on the host) instead of the
+    reference's greedy tree, whose O(n^2 log n) build does not scale to
+    the large workloads, and the reference's 32/128 overlap limits are
+    lifted."""
     O = oracle()
     s = O.spo_scene_create()
-    for v, i, smooth in desc["meshes"]:
+    if sah:
+        import vk_cinematic_amd as V
+        O.spo_scene_set_limits(s, 1 << 20, 1 << 20)
+    for k, (v, i, smooth) in enumerate(desc["meshes"]):
         v = np.ascontiguousarray(np.asarray(v, np.float32).ravel())
         i = np.ascontiguousarray(np.asarray(i, np.uint32).ravel())
         O.spo_mesh_create(s, v, v.size // 8, i, i.size, int(smooth),
-                          int(build_midphase))
+                          int(build_midphase and not sah))
+        if sah:
+            tris = v.reshape(-1, 8)[i.reshape(-1, 3), :3].reshape(-1, 9)
+            nodes, order, _ = V.bvh_build_host(np.ascontiguousarray(tris))
+            O.spo_mesh_set_bvh2(s, k, np.ascontiguousarray(nodes.ravel()),
+                                len(nodes), order)
     for mesh, mat, pos, q, sc in desc["objects"]:
         O.spo_scene_add_object(s, mesh, mat, f32(*pos), f32(*q), f32(*sc))
     O.spo_scene_build(s)

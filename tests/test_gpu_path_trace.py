@@ -14,8 +14,8 @@ import pytest
 
 import vk_cinematic_amd as V
 from vk_cinematic_amd import scenes
-from _libs import (SPO_HIT_BRUTE, SPO_RNG_PCG, SPO_RNG_XORSHIFT_PIXEL, SPO_TX_CR,
-                   SpoConfig, oracle)
+from _libs import (SPO_HIT_BRUTE, SPO_HIT_REFERENCE, SPO_RNG_PCG,
+                   SPO_RNG_XORSHIFT_PIXEL, SPO_TX_CR, SpoConfig, oracle)
 from _scenes import load_oracle
 
 pytestmark = pytest.mark.gpu
@@ -222,3 +222,46 @@ def test_unbiased_mean_is_stable_across_seeds(device):
         means.append(img.reshape(-1, 4)[:, :3].mean())
     means = np.array(means)
     assert means.std() / means.mean() < 0.05
+
+
+def _oracle_tile_sah(desc, tile, spp, bounces, est, seed=0x5EED, clamp=0.0):
+    """The oracle over the product's SAH hierarchy, one tile of the image."""
+    O = oracle()
+    s, cam = load_oracle(desc, sah=True)
+    w, h = desc["camera"][0], desc["camera"][1]
+    cfg = SpoConfig(rng=SPO_RNG_PCG, tx=SPO_TX_CR, hit=SPO_HIT_REFERENCE,
+                    estimator=est, samplesPerPixel=spp, bounces=bounces,
+                    radianceClamp=clamp, jitterYFirst=1, seed=seed)
+    px = np.zeros(w * h * 4, np.float32)
+    met = np.zeros(12, np.uint64)
+    assert O.spo_path_trace_tile(s, C.byref(cam), C.byref(cfg), tile.minX, tile.minY,
+                                 tile.maxX, tile.maxY, None, px, met) == 0
+    O.spo_scene_destroy(s)
+    return px, met
+
+
+@pytest.mark.parametrize("name,tiles", [
+    ("config4", [(224, 224, 288, 288), (0, 0, 64, 64), (448, 448, 512, 512)]),
+    ("config5", [(928, 508, 992, 572), (928, 0, 992, 48), (0, 1040, 64, 1080)]),
+])
+def test_box_scene_tiles_match_oracle_bit_exact(device, name, tiles):
+    """The million-triangle workloads (many rotated, scaled instances, a
+    two-level hierarchy, 6 bounces, the plain unbiased tracer): tiles of the
+    full-size image equal the oracle's bit for bit, as do the ray, hit and
+    miss counts."""
+    desc = getattr(scenes, name)()
+    w, h = desc["camera"][0], desc["camera"][1]
+    s, cam = scenes.load_gpu(device, desc)
+    ctx = V.make_context(cam, s, spp=2, bounces=6, clamp=0.0, estimator=V.EST_UNBIASED,
+                         rng=V.RNG_PCG, seed=0x5EED, collect=True)
+    for t in tiles:
+        tile = V.Tile(*t)
+        img = np.zeros(w * h * 4, np.float32)
+        m = V.sp_PathTraceTile(ctx, tile, img)
+        px, met = _oracle_tile_sah(desc, tile, 2, 6, V.EST_UNBIASED)
+        a = img.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
+        b = px.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (name, t)
+        assert [m.values[k] for k in (2, 3, 4)] == [int(met[k]) for k in (2, 3, 4)]
+        assert m.values[V.METRIC_RAYS] > (t[2] - t[0]) * (t[3] - t[1]) * 2
+    s.close()

@@ -861,7 +861,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     cap = d->paths.capacity;
     CHK(ensure_buffer<uint32_t>(d->slotPixel, d->slotPixelCount, cap));
     CHK(ensure_buffer<float>(d->accum, d->accumCount, 3 * (size_t)cap));
-    HIPCHK(hipMemsetAsync(d->paths.metrics, 0, 128, d->stream));
+    HIPCHK(launch_clear_counters(d->stream, (uint32_t *)d->paths.metrics, 32));
 
     TraceParams p{};
     p.scene = s->dev_scene;
@@ -966,7 +966,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     hipEventElapsedTime(&total, d->ev[62], d->ev[63]);
     d->lastMs[5] = total;
     for (int k = 0; k < 6; ++k) d->lastLaunches[k] = launches[k];
-    for (int k = 0; k < 6; ++k) d->lastCounters[k] = m[k];
+    for (int k = 0; k < 7; ++k) d->lastCounters[k] = m[k];
     if (metrics)
     {
         metrics->values[SPG_METRIC_ELAPSED_NS] += (uint64_t)
@@ -1033,11 +1033,11 @@ int spg_last_trace_timings(spg_Device *d, double *ms6, uint32_t *launches6)
     return SPG_OK;
 }
 
-int spg_last_trace_counters(spg_Device *d, uint64_t *counters6)
+int spg_last_trace_counters(spg_Device *d, uint64_t *counters8)
 {
-    if (!d || !counters6) return fail(SPG_E_INVALID, "NULL argument");
+    if (!d || !counters8) return fail(SPG_E_INVALID, "NULL argument");
     std::lock_guard<std::mutex> g(d->lock);
-    for (int k = 0; k < 6; ++k) counters6[k] = d->lastCounters[k];
+    for (int k = 0; k < 8; ++k) counters8[k] = d->lastCounters[k];
     return SPG_OK;
 }
 
@@ -1082,7 +1082,7 @@ int spg_intersect_closest_device(spg_Scene *s, uint32_t meshId,
     {
         CHK(ensure_paths(d, 1024, 1));
         st = d->paths.metrics;
-        HIPCHK(hipMemsetAsync(st, 0, 64, d->stream));
+        HIPCHK(launch_clear_counters(d->stream, (uint32_t *)st, 16));
     }
     HIPCHK(launch_closest_mesh(d->stream, &m, ox, oy, oz, dx, dy, dz, n, primId,
                                t, u, v, st, d->persist));

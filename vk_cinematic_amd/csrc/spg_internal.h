@@ -167,7 +167,7 @@ struct spg_Device
     bool eventsMade = false;
     double lastMs[6] = {0};
     uint32_t lastLaunches[6] = {0};
-    uint64_t lastCounters[6] = {0};
+    uint64_t lastCounters[8] = {0};
 };
 
 struct HostMesh

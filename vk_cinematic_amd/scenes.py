@@ -91,6 +91,157 @@ def load_gpu(device, desc, max_leaf=4):
     return s, cam
 
 
+def displaced_icosphere(level, seed, amplitude=0.12, waves=6):
+    """An icosphere (the reference's topology, vertices welded) displaced
+    along the radius by a seeded sum of plane waves; smooth normals from
+    area-weighted face normals; uv = spherical angles."""
+    v, idx = gen_icosphere(level)
+    pos = v[:, :3].astype(np.float64)
+    key = np.round(pos * 1e6).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    inv = inv.ravel()
+    p = pos[first]
+    idx = inv[idx.astype(np.int64)]
+    rng = np.random.default_rng(seed)
+    freq = rng.normal(size=(waves, 3)) * rng.uniform(2.0, 6.0, size=(waves, 1))
+    phase = rng.uniform(0, 2 * np.pi, size=waves)
+    amp = rng.uniform(0.3, 1.0, size=waves)
+    amp = amp / amp.sum() * amplitude
+    r = 1.0 + (np.sin(p @ freq.T + phase) * amp).sum(axis=1)
+    p = p * r[:, None]
+    tri = idx.reshape(-1, 3)
+    fn = np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, tri[:, k], fn)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    out = np.zeros((len(p), 8), np.float32)
+    out[:, 0:3] = p
+    out[:, 3:6] = n
+    out[:, 6] = (np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi) + 0.5)
+    out[:, 7] = np.arccos(np.clip(p[:, 1] / np.linalg.norm(p, axis=1), -1, 1)) / np.pi
+    return out, idx.astype(np.uint32).ravel()
+
+
+def box_with_opening(half=8.0, height=6.0, hole=1.0):
+    """A closed box [-half,half] x [0,height] x [-half,half] whose faces
+    point inward (the Moller-Trumbore test culls back faces), with a
+    hole x hole opening in the middle of the ceiling."""
+    quads = []
+    h, H, e = half, height, hole / 2
+
+    def quad(a, b, c, d):
+        quads.append((a, b, c, d))
+
+    quad((-h, 0, -h), (-h, 0, h), (h, 0, h), (h, 0, -h))          # floor, +y
+    quad((-h, 0, -h), (h, 0, -h), (h, H, -h), (-h, H, -h))        # back, +z
+    quad((-h, 0, h), (-h, H, h), (h, H, h), (h, 0, h))            # front, -z
+    quad((-h, 0, -h), (-h, H, -h), (-h, H, h), (-h, 0, h))        # left, +x
+    quad((h, 0, -h), (h, 0, h), (h, H, h), (h, H, -h))            # right, -x
+    # ceiling (normal -y) around the opening
+    for x0, x1, z0, z1 in ((-h, h, -h, -e), (-h, h, e, h), (-h, -e, -e, e),
+                           (e, h, -e, e)):
+        quad((x0, H, z0), (x1, H, z0), (x1, H, z1), (x0, H, z1))
+    verts, idx = [], []
+    for a, b, c, d in quads:
+        a, b, c, d = (np.array(x, np.float64) for x in (a, b, c, d))
+        n = np.cross(b - a, c - a)
+        n /= np.linalg.norm(n)
+        base = len(verts)
+        for p, uv in ((a, (0, 0)), (b, (1, 0)), (c, (1, 1)), (d, (0, 1))):
+            verts.append([*p, *n, *uv])
+        idx += [base, base + 1, base + 2, base, base + 2, base + 3]
+    return np.array(verts, np.float32), np.array(idx, np.uint32)
+
+
+# Material types of the reference app (src/main.cpp:1318-1336): diffuse red,
+# blue, white with roughness 0.6, black, whites with roughness
+# 0.1/0.3/0.6/0.8, and three emitters.
+REFERENCE_MATERIALS = [
+    ("red", (0.18, 0.1, 0.1), (0, 0, 0), 0.0),
+    ("blue", (0.1, 0.1, 0.18), (0, 0, 0), 0.0),
+    ("white", (1, 1, 1), (0, 0, 0), 0.6),
+    ("black", (0, 0, 0), (0, 0, 0), 0.0),
+    ("white_r10", (1, 1, 1), (0, 0, 0), 0.1),
+    ("white_r30", (1, 1, 1), (0, 0, 0), 0.3),
+    ("white_r60", (1, 1, 1), (0, 0, 0), 0.6),
+    ("white_r80", (1, 1, 1), (0, 0, 0), 0.8),
+    ("blue_light", (0, 0, 0), (1.6, 2.4, 4.0), 0.0),
+    ("white_light", (0, 0, 0), (10, 10, 10), 0.0),
+    ("orange_light", (0, 0, 0), (10, 8, 7.5), 0.0),
+]
+
+
+def box_scene(grid, level, width, height, seed=0, unique_meshes=4,
+              light_fraction=0.03, half=8.0, room_height=6.0):
+    """Displaced-icosphere instances on a grid x grid layout inside the box
+    with a 1x1 ceiling opening, lit by the sky through the opening.  Each
+    instance draws a reference material type; non-emitters get albedo
+    U[0.1, 0.9] (the type's colour normalized, times the draw)."""
+    rng = np.random.default_rng(seed)
+    meshes = [displaced_icosphere(level, seed * 101 + k) + (True,)
+              for k in range(unique_meshes)]
+    bv, bi = box_with_opening(half, room_height, 1.0)
+    meshes.append((bv, bi, False))
+    box_mesh = len(meshes) - 1
+    materials = [(1000, (0.7, 0.7, 0.7), (0, 0, 0), 0.0, SPG_NO_TEXTURE,
+                  SPG_NO_TEXTURE),
+                 (BACKGROUND_ID, (0, 0, 0), (0, 0, 0), 0.0, SPG_NO_TEXTURE,
+                  ENV_TEXTURE_ID)]
+    objects = [(box_mesh, 1000, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0),
+                (1.0, 1.0, 1.0))]
+    spacing = 2.0 * (half - 0.8) / grid
+    radius = 0.42 * spacing
+    mid = 2000
+    for gz in range(grid):
+        for gx in range(grid):
+            x = -half + 0.8 + spacing * (gx + 0.5) + rng.uniform(-0.05, 0.05) * spacing
+            z = -half + 0.8 + spacing * (gz + 0.5) + rng.uniform(-0.05, 0.05) * spacing
+            s = radius * rng.uniform(0.8, 1.0)
+            q = quat_axis_angle(rng.normal(size=3) / np.sqrt(3), rng.uniform(0, np.pi))
+            q = (q / np.linalg.norm(q.astype(np.float64))).astype(np.float32)
+            if rng.uniform() < light_fraction:
+                name, alb, em, rough = REFERENCE_MATERIALS[8 + rng.integers(0, 3)]
+                albedo = (0.0, 0.0, 0.0)
+            else:
+                name, alb, em, rough = REFERENCE_MATERIALS[rng.integers(0, 8)]
+                c = np.array(alb, np.float64)
+                c = c / c.max() if c.max() > 0 else np.ones(3)
+                albedo = tuple(float(v) for v in c * rng.uniform(0.1, 0.9))
+            materials.append((mid, albedo, tuple(float(v) for v in em), float(rough),
+                              SPG_NO_TEXTURE, SPG_NO_TEXTURE))
+            objects.append((int(rng.integers(0, unique_meshes)), mid,
+                            (float(x), float(s * 1.05), float(z)), tuple(q),
+                            (float(s), float(s), float(s))))
+            mid += 1
+    cam_q = quat_axis_angle((1.0, 0.0, 0.0), -0.25)
+    desc = {
+        "meshes": meshes,
+        "objects": objects,
+        "materials": materials,
+        "background": BACKGROUND_ID,
+        "camera": (width, height, (0.0, 3.2, half - 0.6), tuple(cam_q), 0.8),
+        "cubemaps": [(ENV_TEXTURE_ID, gen_sky_cubemap(64, SUN_DIR, SUN_RADIUS,
+                                                      SUN_RADIANCE), 64)],
+        "textures": [],
+    }
+    tris = sum(len(meshes[o[0]][1]) // 3 for o in objects)
+    desc["triangles"] = tris
+    return desc
+
+
+def config4(width=512, height=512, seed=4):
+    """About 1M triangles: 7x7 displaced icospheres of 20480 triangles in
+    the box with the opening, 512x512."""
+    return box_scene(7, 5, width, height, seed=seed)
+
+
+def config5(width=1920, height=1080, seed=5):
+    """About 4M triangles: 14x14 displaced icospheres of 20480 triangles in
+    the box with the opening, 1920x1080."""
+    return box_scene(14, 5, width, height, seed=seed)
+
+
 def soup(n_tris=1_000_000, n_rays=1 << 22, seed=7):
     """Closest-hit workload: origins uniform in [-2,2]^3, directions uniform
     on the sphere, against n_tris triangles with centroids uniform in the unit
