@@ -224,10 +224,9 @@ def test_unbiased_mean_is_stable_across_seeds(device):
     assert means.std() / means.mean() < 0.05
 
 
-def _oracle_tile_sah(desc, tile, spp, bounces, est, seed=0x5EED, clamp=0.0):
+def _oracle_tile_sah(s, cam, desc, tile, spp, bounces, est, seed=0x5EED, clamp=0.0):
     """The oracle over the product's SAH hierarchy, one tile of the image."""
     O = oracle()
-    s, cam = load_oracle(desc, sah=True)
     w, h = desc["camera"][0], desc["camera"][1]
     cfg = SpoConfig(rng=SPO_RNG_PCG, tx=SPO_TX_CR, hit=SPO_HIT_REFERENCE,
                     estimator=est, samplesPerPixel=spp, bounces=bounces,
@@ -236,7 +235,6 @@ def _oracle_tile_sah(desc, tile, spp, bounces, est, seed=0x5EED, clamp=0.0):
     met = np.zeros(12, np.uint64)
     assert O.spo_path_trace_tile(s, C.byref(cam), C.byref(cfg), tile.minX, tile.minY,
                                  tile.maxX, tile.maxY, None, px, met) == 0
-    O.spo_scene_destroy(s)
     return px, met
 
 
@@ -254,14 +252,16 @@ def test_box_scene_tiles_match_oracle_bit_exact(device, name, tiles):
     s, cam = scenes.load_gpu(device, desc)
     ctx = V.make_context(cam, s, spp=2, bounces=6, clamp=0.0, estimator=V.EST_UNBIASED,
                          rng=V.RNG_PCG, seed=0x5EED, collect=True)
+    os_, ocam = load_oracle(desc, sah=True)
     for t in tiles:
         tile = V.Tile(*t)
         img = np.zeros(w * h * 4, np.float32)
         m = V.sp_PathTraceTile(ctx, tile, img)
-        px, met = _oracle_tile_sah(desc, tile, 2, 6, V.EST_UNBIASED)
+        px, met = _oracle_tile_sah(os_, ocam, desc, tile, 2, 6, V.EST_UNBIASED)
         a = img.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
         b = px.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (name, t)
         assert [m.values[k] for k in (2, 3, 4)] == [int(met[k]) for k in (2, 3, 4)]
         assert m.values[V.METRIC_RAYS] > (t[2] - t[0]) * (t[3] - t[1]) * 2
+    oracle().spo_scene_destroy(os_)
     s.close()

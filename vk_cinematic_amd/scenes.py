@@ -11,7 +11,12 @@ the same inputs.
   sun disk of radiance 50 as a 6x64x64 RGBA32F cube map, 256x256 pixels,
   4 samples, 3 bounces.
 * ``soup``: the closest-hit workload, 2^22 rays against a 1M-triangle soup.
+* ``config4`` / ``config5``: 7x7 / 14x14 displaced icospheres of 20480
+  triangles each (all distinct geometry, about 1M / 4M triangles) inside a
+  closed box with a 1x1 ceiling opening lit by the sky, materials drawn from
+  the reference app's material types, 512x512 / 1920x1080, 6 bounces.
 """
+import functools
 import math
 
 import numpy as np
@@ -91,17 +96,21 @@ def load_gpu(device, desc, max_leaf=4):
     return s, cam
 
 
-def displaced_icosphere(level, seed, amplitude=0.12, waves=6):
-    """An icosphere (the reference's topology, vertices welded) displaced
-    along the radius by a seeded sum of plane waves; smooth normals from
-    area-weighted face normals; uv = spherical angles."""
+@functools.lru_cache(maxsize=8)
+def _welded_icosphere(level):
     v, idx = gen_icosphere(level)
     pos = v[:, :3].astype(np.float64)
     key = np.round(pos * 1e6).astype(np.int64)
     _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
     inv = inv.ravel()
-    p = pos[first]
-    idx = inv[idx.astype(np.int64)]
+    return pos[first], inv[idx.astype(np.int64)]
+
+
+def displaced_icosphere(level, seed, amplitude=0.12, waves=6):
+    """An icosphere (the reference's topology, vertices welded) displaced
+    along the radius by a seeded sum of plane waves; smooth normals from
+    area-weighted face normals; uv = spherical angles."""
+    p, idx = _welded_icosphere(level)
     rng = np.random.default_rng(seed)
     freq = rng.normal(size=(waves, 3)) * rng.uniform(2.0, 6.0, size=(waves, 1))
     phase = rng.uniform(0, 2 * np.pi, size=waves)
@@ -111,9 +120,8 @@ def displaced_icosphere(level, seed, amplitude=0.12, waves=6):
     p = p * r[:, None]
     tri = idx.reshape(-1, 3)
     fn = np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]])
-    n = np.zeros_like(p)
-    for k in range(3):
-        np.add.at(n, tri[:, k], fn)
+    n = np.stack([sum(np.bincount(tri[:, k], fn[:, c], len(p)) for k in range(3))
+                  for c in range(3)], axis=1)
     n /= np.linalg.norm(n, axis=1, keepdims=True)
     out = np.zeros((len(p), 8), np.float32)
     out[:, 0:3] = p
@@ -172,14 +180,17 @@ REFERENCE_MATERIALS = [
 ]
 
 
-def box_scene(grid, level, width, height, seed=0, unique_meshes=4,
+def box_scene(grid, level, width, height, seed=0, unique_meshes=None,
               light_fraction=0.03, half=8.0, room_height=6.0):
     """Displaced-icosphere instances on a grid x grid layout inside the box
     with a 1x1 ceiling opening, lit by the sky through the opening.  Each
     instance draws a reference material type; non-emitters get albedo
-    U[0.1, 0.9] (the type's colour normalized, times the draw)."""
+    U[0.1, 0.9] (the type's colour normalized, times the draw).  By default
+    every instance has its own displaced mesh (unique_meshes = grid^2), so
+    the triangle count is all distinct geometry."""
     rng = np.random.default_rng(seed)
-    meshes = [displaced_icosphere(level, seed * 101 + k) + (True,)
+    unique_meshes = unique_meshes or grid * grid
+    meshes = [displaced_icosphere(level, seed * 1009 + k) + (True,)
               for k in range(unique_meshes)]
     bv, bi = box_with_opening(half, room_height, 1.0)
     meshes.append((bv, bi, False))
@@ -210,7 +221,9 @@ def box_scene(grid, level, width, height, seed=0, unique_meshes=4,
                 albedo = tuple(float(v) for v in c * rng.uniform(0.1, 0.9))
             materials.append((mid, albedo, tuple(float(v) for v in em), float(rough),
                               SPG_NO_TEXTURE, SPG_NO_TEXTURE))
-            objects.append((int(rng.integers(0, unique_meshes)), mid,
+            mesh = gz * grid + gx if unique_meshes == grid * grid else \
+                int(rng.integers(0, unique_meshes))
+            objects.append((mesh, mid,
                             (float(x), float(s * 1.05), float(z)), tuple(q),
                             (float(s), float(s), float(s))))
             mid += 1
