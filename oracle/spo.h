@@ -106,7 +106,9 @@ typedef struct {
     int isCube;
 } spo_Image;
 
-#define SPO_MAX_MATERIALS 64
+/* storage; the registry accepts materialCapacity entries, 32 by default
+ * (SP_MAX_MATERIALS, src/sp_material_system.h:32) */
+#define SPO_MAX_MATERIALS 4096
 #define SPO_MAX_IMAGES 16
 
 typedef struct spo_Scene {
@@ -118,7 +120,7 @@ typedef struct spo_Scene {
     /* sp_MaterialSystem, src/sp_material_system.h:35-46 */
     uint32_t materialKeys[SPO_MAX_MATERIALS];
     spo_Material materials[SPO_MAX_MATERIALS];
-    uint32_t materialCount;
+    uint32_t materialCount, materialCapacity;
     uint32_t imageKeys[SPO_MAX_IMAGES];
     spo_Image images[SPO_MAX_IMAGES];
     uint32_t imageCount;
@@ -225,10 +227,11 @@ int spo_scene_add_object(spo_Scene *s, uint32_t mesh, uint32_t material,
 /* Attach an external BVH2 to a mesh (copied). */
 int spo_mesh_set_bvh2(spo_Scene *s, uint32_t mesh, const float *nodes16,
                       uint32_t nNodes, const uint32_t *order);
-/* The reference's fixed query limits (32 objects, 128 leaves) can be
- * raised for scenes beyond the reference's scale. */
+/* The reference's fixed limits (32 objects and 128 leaves per query, 32
+ * materials) can be raised for scenes beyond the reference's scale
+ * (materials up to SPO_MAX_MATERIALS). */
 void spo_scene_set_limits(spo_Scene *s, uint32_t maxBroadphase,
-                          uint32_t maxMidphase);
+                          uint32_t maxMidphase, uint32_t maxMaterials);
 void spo_scene_build(spo_Scene *s);
 int spo_material_register(spo_Scene *s, const float *albedo,
                           uint32_t albedoTexture, const float *emission,

@@ -12,6 +12,7 @@ spo_Scene *spo_scene_create(void)
     s->broadphase.root = -1;
     s->maxBroadphaseHits = 32;  /* src/sp_scene.cpp:246 */
     s->maxMidphaseHits = 128;   /* src/sp_scene.cpp:139 */
+    s->materialCapacity = 32;   /* src/sp_material_system.h:32 */
     return s;
 }
 
@@ -95,10 +96,12 @@ int spo_mesh_set_bvh2(spo_Scene *s, uint32_t mesh, const float *nodes16,
 }
 
 void spo_scene_set_limits(spo_Scene *s, uint32_t maxBroadphase,
-                          uint32_t maxMidphase)
+                          uint32_t maxMidphase, uint32_t maxMaterials)
 {
     s->maxBroadphaseHits = maxBroadphase;
     s->maxMidphaseHits = maxMidphase;
+    s->materialCapacity = maxMaterials < SPO_MAX_MATERIALS ? maxMaterials
+                                                           : SPO_MAX_MATERIALS;
 }
 
 /* sp_AddObjectToScene, src/sp_scene.cpp:77-117 (ComputeAabb over all mesh
@@ -162,7 +165,7 @@ int spo_material_register(spo_Scene *s, const float *albedo,
                           uint32_t emissionTexture, float roughness,
                           uint32_t id)
 {
-    if (s->materialCount >= SPO_MAX_MATERIALS) return 0;
+    if (s->materialCount >= s->materialCapacity) return 0;
     uint32_t i = s->materialCount++;
     s->materialKeys[i] = id;
     s->materials[i].albedo = v3p(albedo);

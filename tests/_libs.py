@@ -115,7 +115,7 @@ def oracle():
     _sig(L, "spo_scene_add_object", C.c_int, vp, C.c_uint32, C.c_uint32, f32p,
          f32p, f32p)
     _sig(L, "spo_mesh_set_bvh2", C.c_int, vp, C.c_uint32, f32p, C.c_uint32, u32p)
-    _sig(L, "spo_scene_set_limits", None, vp, C.c_uint32, C.c_uint32)
+    _sig(L, "spo_scene_set_limits", None, vp, C.c_uint32, C.c_uint32, C.c_uint32)
     _sig(L, "spo_scene_build", None, vp)
     _sig(L, "spo_material_register", C.c_int, vp, f32p, C.c_uint32, f32p,
          C.c_uint32, C.c_float, C.c_uint32)

@@ -11,13 +11,13 @@ def load_oracle(desc, build_midphase=True, sah=False):
     """Returns (oracle scene handle, SpoCamera).  With sah=True the meshes
     walk the product's SAH hierarchy (built on the host) instead of the
     reference's greedy tree, whose O(n^2 log n) build does not scale to
-    the large workloads, and the reference's 32/128 overlap limits are
-    lifted."""
+    the large workloads, and the reference's 32/128 overlap limits and
+    32-material registry are lifted."""
     O = oracle()
     s = O.spo_scene_create()
     if sah:
         import vk_cinematic_amd as V
-        O.spo_scene_set_limits(s, 1 << 20, 1 << 20)
+        O.spo_scene_set_limits(s, 1 << 20, 1 << 20, 4096)
     for k, (v, i, smooth) in enumerate(desc["meshes"]):
         v = np.ascontiguousarray(np.asarray(v, np.float32).ravel())
         i = np.ascontiguousarray(np.asarray(i, np.uint32).ravel())
@@ -38,8 +38,8 @@ def load_oracle(desc, build_midphase=True, sah=False):
         O.spo_texture_register(s, np.ascontiguousarray(rgba, np.float32), size,
                                size, tid, 1)
     for mid, albedo, emission, rough, atex, etex in desc["materials"]:
-        O.spo_material_register(s, f32(*albedo), atex, f32(*emission), etex,
-                                rough, mid)
+        assert O.spo_material_register(s, f32(*albedo), atex, f32(*emission), etex,
+                                       rough, mid), "oracle material registry full"
     O.spo_set_background(s, desc["background"])
     w, h, pos, q, fd = desc["camera"]
     cam = SpoCamera()

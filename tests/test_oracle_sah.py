@@ -63,3 +63,17 @@ def test_box_scene_sizes():
     # every object's material is registered
     ids = {m[0] for m in d5["materials"]}
     assert all(o[1] in ids for o in d5["objects"])
+
+
+def test_material_registry_capacity():
+    """sp_RegisterMaterial (src/sp_material_system.cpp:1-13) refuses the
+    33rd material (SP_MAX_MATERIALS 32); the large scenes lift the cap."""
+    from _libs import f32
+    O = oracle()
+    s = O.spo_scene_create()
+    ok = [O.spo_material_register(s, f32(1, 1, 1), 0, f32(0, 0, 0), 0, 0.0, k)
+          for k in range(33)]
+    assert ok == [1] * 32 + [0]
+    O.spo_scene_set_limits(s, 32, 128, 64)
+    assert O.spo_material_register(s, f32(1, 1, 1), 0, f32(0, 0, 0), 0, 0.0, 99) == 1
+    O.spo_scene_destroy(s)
