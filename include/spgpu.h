@@ -74,6 +74,7 @@ enum {
 /* ---- POD types ---- */
 typedef struct spg_Device spg_Device;
 typedef struct spg_Scene spg_Scene;
+typedef struct spg_Guide spg_Guide;
 
 /* Tile, src/tile.h:3-9 */
 typedef struct { uint32_t minX, minY, maxX, maxY; } spg_Tile;
@@ -113,7 +114,10 @@ typedef struct {
     uint32_t transcendentals;  /* spg_Transcendentals */
     uint32_t sampleOffset;     /* first sample index (progressive passes) */
     uint32_t collectMetrics;   /* count AABB/triangle/mesh tests */
-    uint32_t guideHandle;      /* 0, or a spg_guide handle for SPG_EST_GUIDED */
+    uint32_t guideTrain;       /* SPG_EST_GUIDED: collect training samples and
+                                  run the guide's training steps after each
+                                  pass */
+    spg_Guide *guide;          /* SPG_EST_GUIDED: the guide (spg_guide_create) */
 } spg_TraceConfig;
 
 /* sp_Context, src/simd_path_tracer.h:34-46 */
@@ -261,6 +265,66 @@ int spg_last_trace_timings(spg_Device *dev, double *ms6, uint32_t *launches6);
  * The test counts are collected only when the call's config set
  * collectMetrics. */
 int spg_last_trace_counters(spg_Device *dev, uint64_t *counters8);
+
+/* ---- learned path guiding (no reference counterpart: the reference
+ *      samples directions uniformly, src/math_lib.h:932-946) ----
+ * A guide is an MLP (64 -> 4 x 64 ReLU -> 48, FP16 weights, FP32
+ * accumulation on MFMA) mapping a hit (position normalized to the scene
+ * bounds and frequency-encoded, shading normal, view direction) to a
+ * mixture of 8 von Mises-Fisher lobes.  SPG_EST_GUIDED samples each bounce
+ * by one-sample MIS of cosine BSDF sampling (probability bsdfFraction) and
+ * the mixture, so the estimator stays unbiased for any network.  Training
+ * (online, inside the trace call when guideTrain is set) minimizes the
+ * weighted negative log-likelihood of the sampled directions with weights
+ * f cos Li / pdf, normalized to mean 1 per batch, with Adam. */
+#define SPG_GUIDE_PARAMS 19760  /* floats: W0 b0 ... W4 b4, W[out][in] */
+#define SPG_GUIDE_OUTPUTS 48
+#define SPG_GUIDE_LOBE_FLOATS 40 /* pi[8] kappa[8] mu[8][3] */
+#define SPG_GUIDE_SAMPLE_FLOATS 13 /* pn[3] n[3] v[3] wi[3] weight */
+
+typedef struct {
+    uint32_t seed;          /* parameter initialization */
+    float learningRate;     /* Adam step size (default 3e-3) */
+    uint32_t batchSize;     /* samples per training step, multiple of 64
+                               (default 16384) */
+    uint32_t stepsPerPass;  /* training steps after each traced pass
+                               (default 4) */
+    float maxWeight;        /* clamp of batch-normalized sample weights,
+                               0 = off (default 50) */
+    float bsdfFraction;     /* MIS probability of BSDF sampling (0.5) */
+} spg_GuideConfig;
+
+/* Fills the defaults above. */
+void spg_guide_default_config(spg_GuideConfig *cfg);
+/* A guide on dev, initialized from cfg->seed (NULL cfg = defaults). */
+int spg_guide_create(spg_Device *dev, const spg_GuideConfig *cfg,
+                     spg_Guide **out);
+int spg_guide_destroy(spg_Guide *g);
+/* FP32 master parameters (SPG_GUIDE_PARAMS floats); set also resets the
+ * Adam state. */
+int spg_guide_get_params(spg_Guide *g, float *params, uint32_t count);
+int spg_guide_set_params(spg_Guide *g, const float *params, uint32_t count);
+/* The network on n hits given as rows of 9 floats (pn[3] in [0,1], n[3],
+ * v[3]): raw outputs (n x 48) and the mixtures (n x 40), either may be
+ * NULL.  Host arrays. */
+int spg_guide_evaluate(spg_Guide *g, const float *inputs9, uint32_t n,
+                       float *outputs48, float *lobes40);
+/* Gradient of L = (1/n) sum_j w_j (-log q(wi_j | x_j)) over n samples (rows
+ * of SPG_GUIDE_SAMPLE_FLOATS) computed by the training kernels, without a
+ * parameter update.  normalize != 0 scales the weights to mean 1 and
+ * maxWeight > 0 clamps them, as training does.  grad: SPG_GUIDE_PARAMS
+ * floats; loss may be NULL.  n <= 65536. */
+int spg_guide_gradient(spg_Guide *g, const float *samples13, uint32_t n,
+                       int normalize, float maxWeight, float *grad,
+                       float *loss);
+/* `steps` Adam steps on host samples, consecutive batches of batchSize
+ * (wrapping); mean loss of the last step in *loss (may be NULL). */
+int spg_guide_train(spg_Guide *g, const float *samples13, uint32_t n,
+                    uint32_t steps, float *loss);
+/* Training state: samples collected in the last traced pass, Adam steps
+ * taken, mean loss of the last step. */
+int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps,
+                    float *loss);
 
 /* ---- generators used by the benchmark scenes (no reference counterpart
  *      except the icosphere, CreateIcosahedronMesh,

@@ -54,6 +54,15 @@ class SpoSceneHit(C.Structure):
                 ("object", C.c_uint32), ("prim", C.c_uint32)]
 
 
+GUIDE_PARAMS = 4 * (64 * 64 + 64) + 48 * 64 + 48
+GUIDE_LOBES = 8
+
+
+class SpoLobes(C.Structure):
+    _fields_ = [("pi", C.c_double * 8), ("kappa", C.c_double * 8),
+                ("mu", (C.c_double * 3) * 8)]
+
+
 def _sig(lib, name, res, *args):
     fn = getattr(lib, name)
     fn.restype = res
@@ -146,6 +155,19 @@ def oracle():
          u32ref, f32p, u64p)
     _sig(L, "spo_render_threaded", C.c_double, vp, C.POINTER(SpoCamera),
          C.POINTER(SpoConfig), C.c_uint32, C.c_uint32, C.c_uint32, f32p, u64p)
+    # learned guide (oracle/spo_guide.h)
+    _sig(L, "spo_guide_encode", None, f32p, f32p, f32p, f64p)
+    _sig(L, "spo_guide_forward", None, f64p, f64p, f64p, C.c_void_p)
+    _sig(L, "spo_guide_lobes", None, f64p, C.POINTER(SpoLobes))
+    _sig(L, "spo_guide_pdf", C.c_double, C.POINTER(SpoLobes), f64p)
+    _sig(L, "spo_guide_sample", None, C.POINTER(SpoLobes), C.c_double, C.c_double,
+         C.c_double, f64p)
+    _sig(L, "spo_vmf_pdf", C.c_double, C.c_double, f64p, f64p)
+    _sig(L, "spo_vmf_sample", None, C.c_double, f64p, C.c_double, C.c_double, f64p)
+    _sig(L, "spo_guide_loss", C.c_double, f64p, f64p, f64p, f64p, C.c_uint32,
+         C.c_void_p)
+    _sig(L, "spo_guide_adam", None, f32p, f32p, f32p, f64p, C.c_uint32, C.c_double,
+         C.c_double, C.c_double, C.c_double, C.c_uint32)
     _oracle = L
     return L
 

@@ -71,7 +71,15 @@ class TraceConfig(C.Structure):
                 ("radianceClamp", C.c_float), ("estimator", C.c_uint32),
                 ("rngMode", C.c_uint32), ("seed", C.c_uint32),
                 ("transcendentals", C.c_uint32), ("sampleOffset", C.c_uint32),
-                ("collectMetrics", C.c_uint32), ("guideHandle", C.c_uint32)]
+                ("collectMetrics", C.c_uint32), ("guideTrain", C.c_uint32),
+                ("guide", C.c_void_p)]
+
+
+class GuideConfig(C.Structure):
+    """spg_GuideConfig (include/spgpu.h)."""
+    _fields_ = [("seed", C.c_uint32), ("learningRate", C.c_float),
+                ("batchSize", C.c_uint32), ("stepsPerPass", C.c_uint32),
+                ("maxWeight", C.c_float), ("bsdfFraction", C.c_float)]
 
 
 class Context(C.Structure):
@@ -150,6 +158,17 @@ _SIGS = {
     "spg_gen_sky_cubemap": (_int, [_u32, _f32p, C.c_float, C.c_float, _f32p]),
     "spg_gen_sky_equirect": (_int, [_u32, _u32, _f32p, C.c_float, C.c_float, _f32p]),
     "spg_pixel_seed": (_u32, [_u32, _u32]),
+    "spg_guide_default_config": (None, [_vp]),
+    "spg_guide_create": (_int, [_vp, _vp, C.POINTER(_vp)]),
+    "spg_guide_destroy": (_int, [_vp]),
+    "spg_guide_get_params": (_int, [_vp, _f32p, _u32]),
+    "spg_guide_set_params": (_int, [_vp, _f32p, _u32]),
+    "spg_guide_evaluate": (_int, [_vp, _f32p, _u32, _vp, _vp]),
+    "spg_guide_gradient": (_int, [_vp, _f32p, _u32, _int, C.c_float, _f32p,
+                                  C.POINTER(C.c_float)]),
+    "spg_guide_train": (_int, [_vp, _f32p, _u32, _u32, C.POINTER(C.c_float)]),
+    "spg_guide_stats": (_int, [_vp, C.POINTER(_u32), C.POINTER(_u32),
+                               C.POINTER(C.c_float)]),
 }
 
 _lib = None
@@ -159,8 +178,9 @@ _live_devices = weakref.WeakSet()
 
 @atexit.register
 def _release_all():
-    # Release GPU objects while the HIP runtime is still up: scenes first,
-    # then devices (interpreter teardown would otherwise order them freely).
+    # Release GPU objects while the HIP runtime is still up: scenes and
+    # guides first, then devices (interpreter teardown would otherwise order
+    # them freely).
     for s in list(_live_scenes):
         s.close()
     for d in list(_live_devices):
@@ -409,12 +429,88 @@ def ComputeTiles(width, height, tile_width, tile_height, max_tiles=4096):
 
 def make_context(camera, scene, spp=1, bounces=3, clamp=10.0,
                  estimator=EST_PARITY, rng=RNG_PCG, seed=0x5EED,
-                 transcendentals=TX_DOUBLE, sample_offset=0, collect=False):
+                 transcendentals=TX_DOUBLE, sample_offset=0, collect=False,
+                 guide=None, train=False):
     cfg = TraceConfig(spp, bounces, clamp, estimator, rng, seed, transcendentals,
-                      sample_offset, int(bool(collect)), 0)
+                      sample_offset, int(bool(collect)), int(bool(train)),
+                      guide.handle if guide is not None else None)
     ctx = Context(C.pointer(camera), scene.handle, cfg)
-    ctx._keep = (camera, scene)
+    ctx._keep = (camera, scene, guide)
     return ctx
+
+
+GUIDE_PARAMS = 19760
+GUIDE_OUTPUTS = 48
+GUIDE_LOBE_FLOATS = 40
+GUIDE_SAMPLE_FLOATS = 13
+
+
+class Guide:
+    """A learned path guide (spg_guide_*): MLP -> 8-lobe vMF mixture,
+    trained online by the tracer when a context sets train=True."""
+
+    def __init__(self, device, seed=1, learning_rate=3e-3, batch_size=16384,
+                 steps_per_pass=4, max_weight=50.0, bsdf_fraction=0.5):
+        self.device = device
+        cfg = GuideConfig(seed, learning_rate, batch_size, steps_per_pass,
+                          max_weight, bsdf_fraction)
+        h = C.c_void_p()
+        check(lib().spg_guide_create(device.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h
+        _live_scenes.add(self)
+
+    def params(self):
+        p = np.zeros(GUIDE_PARAMS, np.float32)
+        check(lib().spg_guide_get_params(self.handle, p, GUIDE_PARAMS))
+        return p
+
+    def set_params(self, p):
+        check(lib().spg_guide_set_params(self.handle, _f32(p, GUIDE_PARAMS),
+                                         GUIDE_PARAMS))
+
+    def evaluate(self, inputs9):
+        """Raw outputs (n x 48) and mixtures (n x 40) for rows pn, n, v."""
+        x = _f32(inputs9)
+        n = x.size // 9
+        out = np.zeros((n, GUIDE_OUTPUTS), np.float32)
+        lobes = np.zeros((n, GUIDE_LOBE_FLOATS), np.float32)
+        check(lib().spg_guide_evaluate(self.handle, x, n, out.ctypes.data,
+                                       lobes.ctypes.data))
+        return out, lobes
+
+    def gradient(self, samples13, normalize=False, max_weight=0.0):
+        x = _f32(samples13)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        loss = C.c_float()
+        check(lib().spg_guide_gradient(self.handle, x, x.size // GUIDE_SAMPLE_FLOATS,
+                                       int(bool(normalize)), max_weight, g,
+                                       C.byref(loss)))
+        return g, loss.value
+
+    def train(self, samples13, steps=1):
+        x = _f32(samples13)
+        loss = C.c_float()
+        check(lib().spg_guide_train(self.handle, x, x.size // GUIDE_SAMPLE_FLOATS,
+                                    steps, C.byref(loss)))
+        return loss.value
+
+    def stats(self):
+        c, st, loss = C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib().spg_guide_stats(self.handle, C.byref(c), C.byref(st),
+                                    C.byref(loss)))
+        return {"collected": c.value, "steps": st.value, "loss": loss.value}
+
+    def close(self):
+        if self.handle:
+            lib().spg_guide_destroy(self.handle)
+            self.handle = None
+        _live_scenes.discard(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def sp_PathTraceTile(ctx, tile, pixels, metrics=None):

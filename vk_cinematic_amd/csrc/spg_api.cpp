@@ -130,13 +130,6 @@ static void free_ptr(void *&p)
 
 using namespace spg;
 
-#define CHK(x)                  \
-    do                          \
-    {                           \
-        int _r = (x);           \
-        if (_r) return _r;      \
-    } while (0)
-#define HIPCHK(x) CHK(spg::hip_check((x), #x))
 
 extern "C" {
 
@@ -165,14 +158,14 @@ int spg_device_create(int ordinal, spg_Device **out)
         delete d;
         return hip_check(e, "hipStreamCreate");
     }
-    for (int i = 0; i < 64; ++i) hipEventCreate(&d->ev[i]);
+    for (int i = 0; i < kEvents; ++i) hipEventCreate(&d->ev[i]);
     d->eventsMade = true;
     // persistent-launch work counters (256 B) then the traversal stack spill
     size_t spillBytes = spill_ints() * sizeof(int);
     e = hipMalloc(&d->persistBlock, 256 + spillBytes);
     if (e != hipSuccess)
     {
-        for (int i = 0; i < 64; ++i) hipEventDestroy(d->ev[i]);
+        for (int i = 0; i < kEvents; ++i) hipEventDestroy(d->ev[i]);
         hipStreamDestroy(d->stream);
         delete d;
         return hip_check(e, "hipMalloc(traversal spill)");
@@ -200,7 +193,7 @@ static void delete_device(spg_Device *d)
     free_ptr(d->accum);
     free_ptr(d->persistBlock);
     if (d->eventsMade)
-        for (int i = 0; i < 64; ++i) hipEventDestroy(d->ev[i]);
+        for (int i = 0; i < kEvents; ++i) hipEventDestroy(d->ev[i]);
     hipStreamDestroy(d->stream);
     delete d;
 }
@@ -800,17 +793,17 @@ struct LaunchTimer
 {
     spg_Device *d;
     int next = 0;
-    int cls[32];
+    int cls[(kEvents - 4) / 2];
     bool on = true;
     void begin(int c)
     {
-        if (!on || next >= 32) return;
+        if (!on || next >= kEvents - 4) return;
         cls[next / 2] = c;
         hipEventRecord(d->ev[next], d->stream);
     }
     void end()
     {
-        if (!on || next >= 32) return;
+        if (!on || next >= kEvents - 4) return;
         hipEventRecord(d->ev[next + 1], d->stream);
         next += 2;
     }
@@ -830,8 +823,12 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
         return fail(SPG_E_INVALID, "trace: samplesPerPixel must be > 0 and bounces in 1..16");
     if (cfg.rngMode != SPG_RNG_XORSHIFT_PIXEL && cfg.rngMode != SPG_RNG_PCG)
         return fail(SPG_E_INVALID, "trace: unknown rngMode");
-    if (cfg.estimator != SPG_EST_PARITY && cfg.estimator != SPG_EST_UNBIASED)
-        return fail(SPG_E_INVALID, "trace: estimator not available in this call");
+    if (cfg.estimator != SPG_EST_PARITY && cfg.estimator != SPG_EST_UNBIASED &&
+        cfg.estimator != SPG_EST_GUIDED)
+        return fail(SPG_E_INVALID, "trace: unknown estimator");
+    const bool guided = cfg.estimator == SPG_EST_GUIDED;
+    if (guided && (!cfg.guide || cfg.guide->dev != d))
+        return fail(SPG_E_INVALID, "trace: the guided estimator needs a guide on the scene's device");
     if (cfg.transcendentals != SPG_TX_DOUBLE && cfg.transcendentals != SPG_TX_FAST)
         return fail(SPG_E_INVALID, "trace: unknown transcendentals mode");
     if (c.width == 0 || c.height == 0)
@@ -887,10 +884,13 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     p.seed = cfg.seed;
     p.tx = cfg.transcendentals;
     p.collect = cfg.collectMetrics;
+    const bool train = guided && cfg.guideTrain;
+    if (guided)
+        CHK(guide_prepare_trace(cfg.guide, s, cap, cfg.bounces, cap, train, p.guide));
 
     LaunchTimer tm{d};
     uint32_t launches[6] = {0};
-    hipEventRecord(d->ev[62], d->stream);
+    hipEventRecord(d->ev[kEvents - 2], d->stream);
     for (size_t s0 = 0; s0 < slots.size(); s0 += cap)
     {
         uint32_t nslots = (uint32_t)std::min<size_t>(cap, slots.size() - s0);
@@ -907,6 +907,10 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
             // accum and the paths are laid out with the slot count of this
             // chunk; accum rows are 3 x nslots
             HIPCHK(launch_set_counter(d->stream, &p.paths.counters[0], nslots * sp));
+            if (guided)
+                CHK(guide_begin_pass(cfg.guide, d->stream, p.guide,
+                                     (p.sampleBase + 1u) * 0x9E3779B9u ^ cfg.seed ^
+                                         (uint32_t)s0));
             tm.begin(kRaygen);
             HIPCHK(launch_raygen(d->stream, p));
             tm.end();
@@ -918,6 +922,13 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
                 HIPCHK(launch_extend(d->stream, p, q, d->persist));
                 tm.end();
                 HIPCHK(launch_set_counter(d->stream, &p.paths.counters[q ^ 1], 0));
+                if (guided)
+                {
+                    tm.begin(kGuide);
+                    HIPCHK(launch_guide_infer(d->stream, p, q));
+                    tm.end();
+                    launches[kGuide]++;
+                }
                 tm.begin(kShade);
                 HIPCHK(launch_shade(d->stream, p, q));
                 tm.end();
@@ -929,11 +940,18 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
             HIPCHK(launch_finalize(d->stream, p));
             tm.end();
             launches[kFinal]++;
+            if (train)
+            {
+                tm.begin(kGuide);
+                CHK(guide_train_pass(cfg.guide, d->stream));
+                tm.end();
+                launches[kGuide]++;
+            }
             done += sp;
         }
         HIPCHK(launch_write_pixels(d->stream, p, devImage));
     }
-    hipEventRecord(d->ev[63], d->stream);
+    hipEventRecord(d->ev[kEvents - 1], d->stream);
     unsigned long long m[16] = {0};
     HIPCHK(hipMemcpyAsync(m, d->paths.metrics, 128, hipMemcpyDeviceToHost,
                           d->stream));
@@ -963,7 +981,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
         d->lastMs[tm.cls[k / 2]] += ms;
     }
     float total = 0.0f;
-    hipEventElapsedTime(&total, d->ev[62], d->ev[63]);
+    hipEventElapsedTime(&total, d->ev[kEvents - 2], d->ev[kEvents - 1]);
     d->lastMs[5] = total;
     for (int k = 0; k < 6; ++k) d->lastLaunches[k] = launches[k];
     for (int k = 0; k < 7; ++k) d->lastCounters[k] = m[k];
@@ -1149,15 +1167,15 @@ int spg_time_intersect_closest(spg_Scene *s, uint32_t meshId, const float *ox,
     std::lock_guard<std::mutex> g(d->lock);
     HIPCHK(hipSetDevice(d->ordinal));
     DevMesh m = dev_mesh(s->meshes[meshId]);
-    HIPCHK(hipEventRecord(d->ev[60], d->stream));
+    HIPCHK(hipEventRecord(d->ev[kEvents - 4], d->stream));
     for (uint32_t i = 0; i < iterations; ++i)
         HIPCHK(launch_closest_mesh(d->stream, &m, ox, oy, oz, dx, dy, dz, n,
                                    primId, t, nullptr, nullptr, nullptr,
                                    d->persist));
-    HIPCHK(hipEventRecord(d->ev[61], d->stream));
-    HIPCHK(hipEventSynchronize(d->ev[61]));
+    HIPCHK(hipEventRecord(d->ev[kEvents - 3], d->stream));
+    HIPCHK(hipEventSynchronize(d->ev[kEvents - 3]));
     float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, d->ev[60], d->ev[61]));
+    HIPCHK(hipEventElapsedTime(&ms, d->ev[kEvents - 4], d->ev[kEvents - 3]));
     *meanKernelMs = ms / iterations;
     return SPG_OK;
 }

@@ -1,0 +1,471 @@
+// Host side of the learned path guide: the spg_guide_* entry points and the
+// tracer's hooks (buffers per trace, per-pass sample collection, training
+// steps after each pass).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "spg_internal.h"
+
+using namespace spg;
+
+namespace {
+
+constexpr float kBeta1 = 0.9f, kBeta2 = 0.99f, kAdamEps = 1e-8f;
+constexpr uint32_t kMaxGradientSamples = 65536;
+
+size_t take(size_t &off, size_t bytes)
+{
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+}
+
+void free_dev(void *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+float uniform_pm(uint64_t &s, float a)
+{
+    double u = (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
+    return (float)((2.0 * u - 1.0) * a);
+}
+
+// He-uniform hidden layers, a small output layer, lobe means spread on a
+// Fibonacci lattice through the output biases (so |mean| starts near 1).
+void init_params(std::vector<float> &p, uint32_t seed)
+{
+    p.assign(kGParams, 0.0f);
+    uint64_t s = 0x5EEDull * 0x100000001ull + seed;
+    const float he = std::sqrt(6.0f / 64.0f);
+    for (int l = 0; l < kGLayers; ++l)
+    {
+        float a = l < 4 ? he : 0.1f * he;
+        for (int k = 0; k < g_rows(l) * 64; ++k) p[g_w_off(l) + k] = uniform_pm(s, a);
+    }
+    float *b4 = p.data() + g_b_off(4);
+    for (int k = 0; k < kGLobes; ++k)
+    {
+        double z = 1.0 - 2.0 * (k + 0.5) / kGLobes;
+        double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+        double phi = 3.14159265358979 * (1.0 + std::sqrt(5.0)) * (k + 0.5);
+        b4[16 + 3 * k] = (float)(r * std::cos(phi));
+        b4[17 + 3 * k] = (float)(r * std::sin(phi));
+        b4[18 + 3 * k] = (float)z;
+    }
+}
+
+GuideTrainArgs make_args(spg_Guide *g, uint32_t first, uint32_t batch,
+                         bool normalize, float wmax)
+{
+    GuideTrainArgs a{};
+    a.w16 = g->w16;
+    a.wt16 = g->wt16;
+    a.bias = g->bias;
+    a.params = g->params;
+    a.m = g->m;
+    a.v = g->v;
+    a.grad = g->grad;
+    a.samples = g->samples;
+    a.count = g->count;
+    a.sampleCap = g->sampleCap;
+    a.first = first;
+    a.batch = batch;
+    a.wsum = normalize ? g->wsum : nullptr;
+    a.wmax = wmax;
+    a.H = g->H;
+    a.G = g->G;
+    a.loss = g->loss;
+    return a;
+}
+
+int ensure_samples(spg_Guide *g, uint32_t cap)
+{
+    if (g->samplesBlock && g->sampleCap >= cap) return SPG_OK;
+    free_dev(g->samplesBlock);
+    g->sampleCap = 0;
+    HIPCHK(hipMalloc(&g->samplesBlock, (size_t)kGSampleFloats * cap * sizeof(float)));
+    g->samples = (float *)g->samplesBlock;
+    g->sampleCap = cap;
+    return SPG_OK;
+}
+
+int ensure_hg(spg_Guide *g, uint32_t batch)
+{
+    if (g->hgBlock && g->batchCap >= batch) return SPG_OK;
+    free_dev(g->hgBlock);
+    g->batchCap = 0;
+    size_t half = (size_t)kGLayers * 64 * batch;
+    HIPCHK(hipMalloc(&g->hgBlock, 2 * half * sizeof(_Float16)));
+    g->H = (_Float16 *)g->hgBlock;
+    g->G = g->H + half;
+    g->batchCap = batch;
+    return SPG_OK;
+}
+
+int upload_samples(spg_Guide *g, const float *rows, uint32_t n)
+{
+    CHK(ensure_samples(g, n));
+    std::vector<float> soa((size_t)kGSampleFloats * g->sampleCap, 0.0f);
+    for (uint32_t j = 0; j < n; ++j)
+        for (int k = 0; k < kGSampleFloats; ++k)
+            soa[(size_t)k * g->sampleCap + j] = rows[(size_t)j * kGSampleFloats + k];
+    HIPCHK(hipMemcpyAsync(g->samples, soa.data(), soa.size() * sizeof(float),
+                          hipMemcpyHostToDevice, g->dev->stream));
+    HIPCHK(hipMemcpyAsync(g->count, &n, 4, hipMemcpyHostToDevice, g->dev->stream));
+    HIPCHK(hipStreamSynchronize(g->dev->stream));
+    return SPG_OK;
+}
+
+int adam_step(spg_Guide *g, const GuideTrainArgs &a)
+{
+    g->steps++;
+    GuideAdam h{};
+    h.lr = g->cfg.learningRate;
+    h.beta1 = kBeta1;
+    h.beta2 = kBeta2;
+    h.eps = kAdamEps;
+    h.c1 = (float)(1.0 - std::pow((double)kBeta1, (double)g->steps));
+    h.c2 = (float)(1.0 - std::pow((double)kBeta2, (double)g->steps));
+    h.apply = 1;
+    HIPCHK(launch_guide_adam(g->dev->stream, a, h));
+    return SPG_OK;
+}
+
+uint32_t cus_of(spg_Device *d)
+{
+    int n = 0;
+    hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, d->ordinal);
+    return n > 0 ? (uint32_t)n : 1u;
+}
+
+int set_params_locked(spg_Guide *g, const float *p)
+{
+    spg_Device *d = g->dev;
+    HIPCHK(hipMemcpyAsync(g->params, p, sizeof(float) * kGParams,
+                          hipMemcpyHostToDevice, d->stream));
+    HIPCHK(hipMemsetAsync(g->m, 0, sizeof(float) * kGParams, d->stream));
+    HIPCHK(hipMemsetAsync(g->v, 0, sizeof(float) * kGParams, d->stream));
+    HIPCHK(hipMemsetAsync(g->grad, 0, sizeof(float) * kGParams, d->stream));
+    g->steps = 0;
+    HIPCHK(launch_guide_convert(d->stream, make_args(g, 0, 64, false, 0.0f)));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return SPG_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ tracer hooks
+namespace spg {
+
+int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
+                        uint32_t bounces, uint32_t paths, bool train,
+                        DevGuide &out)
+{
+    if (!g->pathBlock || g->pathCap < capacity || g->pathBounces < bounces)
+    {
+        free_dev(g->pathBlock);
+        g->pathCap = g->pathBounces = 0;
+        size_t off = 0;
+        size_t oL = take(off, (size_t)kGLobeFloats * capacity * 4);
+        size_t oV = take(off, (size_t)kGVertexFloats * bounces * capacity * 4);
+        HIPCHK(hipMalloc(&g->pathBlock, off));
+        g->lobes = (float *)((char *)g->pathBlock + oL);
+        g->gvtx = (float *)((char *)g->pathBlock + oV);
+        g->pathCap = capacity;
+        g->pathBounces = bounces;
+    }
+    const uint32_t B = g->cfg.batchSize, steps = g->cfg.stepsPerPass;
+    if (train)
+    {
+        CHK(ensure_samples(g, std::max<uint32_t>(4u * B * steps, 64u)));
+        CHK(ensure_hg(g, B));
+    }
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const HostObject &o : s->objects)
+        for (int a = 0; a < 3; ++a)
+        {
+            lo[a] = std::min(lo[a], o.aabbLo[a]);
+            hi[a] = std::max(hi[a], o.aabbHi[a]);
+        }
+    DevGuide dg{};
+    dg.w16 = g->w16;
+    dg.bias = g->bias;
+    dg.lobes = g->lobes;
+    dg.gvtx = g->gvtx;
+    dg.samples = g->samples;
+    dg.sampleCount = g->count;
+    dg.sampleCap = g->sampleCap;
+    double psel = paths ? std::min(1.0, (double)B * steps / (double)paths) : 1.0;
+    dg.selectBelow = (uint32_t)std::min(4294967295.0, psel * 4294967296.0);
+    dg.train = train ? 1u : 0u;
+    dg.bsdfFraction = g->cfg.bsdfFraction;
+    for (int a = 0; a < 3; ++a)
+    {
+        if (!(lo[a] <= hi[a]))
+        {
+            lo[a] = 0.0f;
+            hi[a] = 1.0f;
+        }
+        dg.lo[a] = lo[a];
+        dg.invExt[a] = 1.0f / std::max(hi[a] - lo[a], 1e-6f);
+    }
+    out = dg;
+    return SPG_OK;
+}
+
+int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt)
+{
+    dg.salt = salt;
+    if (dg.train) HIPCHK(launch_set_counter(st, g->count, 0u));
+    return SPG_OK;
+}
+
+int guide_train_pass(spg_Guide *g, hipStream_t st)
+{
+    const uint32_t B = g->cfg.batchSize;
+    const uint32_t cus = cus_of(g->dev);
+    for (uint32_t k = 0; k < g->cfg.stepsPerPass; ++k)
+    {
+        GuideTrainArgs a = make_args(g, k * B, B, true, g->cfg.maxWeight);
+        HIPCHK(launch_clear_floats(st, g->loss, 1));
+        HIPCHK(launch_guide_gradient(st, a, cus));
+        CHK(adam_step(g, a));
+    }
+    return SPG_OK;
+}
+
+}  // namespace spg
+
+// ------------------------------------------------------------ C ABI
+extern "C" {
+
+void spg_guide_default_config(spg_GuideConfig *c)
+{
+    if (!c) return;
+    c->seed = 1;
+    c->learningRate = 3e-3f;
+    c->batchSize = 16384;
+    c->stepsPerPass = 4;
+    c->maxWeight = 50.0f;
+    c->bsdfFraction = 0.5f;
+}
+
+int spg_guide_create(spg_Device *dev, const spg_GuideConfig *cfg, spg_Guide **out)
+{
+    if (!dev || !out) return fail(SPG_E_INVALID, "spg_guide_create: NULL");
+    spg_GuideConfig c;
+    spg_guide_default_config(&c);
+    if (cfg) c = *cfg;
+    if (c.batchSize == 0 || c.batchSize % 64 || c.batchSize > (1u << 20))
+        return fail(SPG_E_INVALID, "spg_guide_create: batchSize must be a multiple of 64 in 64..2^20");
+    if (!(c.bsdfFraction >= 0.0f && c.bsdfFraction <= 1.0f) || c.learningRate < 0.0f)
+        return fail(SPG_E_INVALID, "spg_guide_create: bsdfFraction in [0,1], learningRate >= 0");
+    std::lock_guard<std::mutex> lk(dev->lock);
+    if (dev->destroyRequested)
+        return fail(SPG_E_STATE, "spg_guide_create: device is being destroyed");
+    HIPCHK(hipSetDevice(dev->ordinal));
+    spg_Guide *g = new spg_Guide();
+    g->dev = dev;
+    g->cfg = c;
+    size_t off = 0;
+    const size_t P = (size_t)kGParams * 4;
+    size_t oP = take(off, P), oM = take(off, P), oV = take(off, P), oG = take(off, P),
+           oB = take(off, (size_t)kGRows * 4), oS = take(off, 16), oL = take(off, 16),
+           oC = take(off, 16), oW = take(off, (size_t)kGRows * 64 * 2),
+           oT = take(off, (size_t)kGLayers * 64 * 64 * 2);
+    hipError_t e = hipMalloc(&g->block, off);
+    if (e != hipSuccess)
+    {
+        delete g;
+        return hip_check(e, "hipMalloc(guide)");
+    }
+    char *b = (char *)g->block;
+    g->params = (float *)(b + oP);
+    g->m = (float *)(b + oM);
+    g->v = (float *)(b + oV);
+    g->grad = (float *)(b + oG);
+    g->bias = (float *)(b + oB);
+    g->wsum = (float *)(b + oS);
+    g->loss = (float *)(b + oL);
+    g->count = (uint32_t *)(b + oC);
+    g->w16 = (_Float16 *)(b + oW);
+    g->wt16 = (_Float16 *)(b + oT);
+    std::vector<float> p;
+    init_params(p, c.seed);
+    int r = hip_check(hipMemsetAsync(g->block, 0, off, dev->stream), "hipMemset(guide)");
+    if (!r) r = set_params_locked(g, p.data());
+    if (r)
+    {
+        free_dev(g->block);
+        delete g;
+        return r;
+    }
+    dev->liveScenes++;
+    *out = g;
+    return SPG_OK;
+}
+
+int spg_guide_destroy(spg_Guide *g)
+{
+    if (!g) return SPG_OK;
+    spg_Device *d = g->dev;
+    bool last;
+    {
+        std::lock_guard<std::mutex> lk(d->lock);
+        hipSetDevice(d->ordinal);
+        hipStreamSynchronize(d->stream);
+        free_dev(g->block);
+        free_dev(g->samplesBlock);
+        free_dev(g->hgBlock);
+        free_dev(g->pathBlock);
+        delete g;
+        d->liveScenes--;
+        last = d->destroyRequested && d->liveScenes == 0;
+    }
+    if (last) spg_device_destroy(d);
+    return SPG_OK;
+}
+
+int spg_guide_get_params(spg_Guide *g, float *params, uint32_t count)
+{
+    if (!g || !params || count != (uint32_t)kGParams)
+        return fail(SPG_E_INVALID, "spg_guide_get_params: need SPG_GUIDE_PARAMS floats");
+    std::lock_guard<std::mutex> lk(g->dev->lock);
+    HIPCHK(hipSetDevice(g->dev->ordinal));
+    HIPCHK(hipMemcpyAsync(params, g->params, sizeof(float) * kGParams,
+                          hipMemcpyDeviceToHost, g->dev->stream));
+    HIPCHK(hipStreamSynchronize(g->dev->stream));
+    return SPG_OK;
+}
+
+int spg_guide_set_params(spg_Guide *g, const float *params, uint32_t count)
+{
+    if (!g || !params || count != (uint32_t)kGParams)
+        return fail(SPG_E_INVALID, "spg_guide_set_params: need SPG_GUIDE_PARAMS floats");
+    std::lock_guard<std::mutex> lk(g->dev->lock);
+    HIPCHK(hipSetDevice(g->dev->ordinal));
+    return set_params_locked(g, params);
+}
+
+int spg_guide_evaluate(spg_Guide *g, const float *in9, uint32_t n, float *out48,
+                       float *lobes40)
+{
+    if (!g || (!in9 && n)) return fail(SPG_E_INVALID, "spg_guide_evaluate: NULL");
+    if (n == 0) return SPG_OK;
+    spg_Device *d = g->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    void *buf = nullptr;
+    size_t bytes = (size_t)n * (9 + kGOut + kGLobeFloats) * 4;
+    HIPCHK(hipMalloc(&buf, bytes));
+    float *dIn = (float *)buf, *dOut = dIn + (size_t)n * 9,
+          *dLobes = dOut + (size_t)n * kGOut;
+    int r = hip_check(hipMemcpyAsync(dIn, in9, (size_t)n * 36, hipMemcpyHostToDevice,
+                                     d->stream), "hipMemcpy");
+    if (!r)
+        r = hip_check(launch_guide_eval(d->stream, g->w16, g->bias, dIn, n, dOut, dLobes),
+                      "k_guide_eval");
+    if (!r && out48)
+        r = hip_check(hipMemcpyAsync(out48, dOut, (size_t)n * kGOut * 4,
+                                     hipMemcpyDeviceToHost, d->stream), "hipMemcpy");
+    if (!r && lobes40)
+        r = hip_check(hipMemcpyAsync(lobes40, dLobes, (size_t)n * kGLobeFloats * 4,
+                                     hipMemcpyDeviceToHost, d->stream), "hipMemcpy");
+    if (!r) r = hip_check(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+    hipFree(buf);
+    return r;
+}
+
+int spg_guide_gradient(spg_Guide *g, const float *samples, uint32_t n, int normalize,
+                       float maxWeight, float *grad, float *loss)
+{
+    if (!g || !samples || !grad) return fail(SPG_E_INVALID, "spg_guide_gradient: NULL");
+    if (n == 0 || n > kMaxGradientSamples)
+        return fail(SPG_E_CAPACITY, "spg_guide_gradient: n must be in 1..65536");
+    spg_Device *d = g->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    CHK(upload_samples(g, samples, n));
+    const uint32_t batch = (n + 63) / 64 * 64;
+    CHK(ensure_hg(g, batch));
+    HIPCHK(launch_clear_floats(d->stream, g->loss, 1));
+    GuideTrainArgs a = make_args(g, 0, batch, normalize != 0, maxWeight);
+    HIPCHK(launch_guide_gradient(d->stream, a, cus_of(d)));
+    float l = 0.0f;
+    HIPCHK(hipMemcpyAsync(grad, g->grad, sizeof(float) * kGParams,
+                          hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipMemcpyAsync(&l, g->loss, 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(launch_clear_floats(d->stream, g->grad, kGParams));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    const float scale = 1.0f / ((float)n * kLossScale);
+    for (int k = 0; k < kGParams; ++k) grad[k] *= scale;
+    if (loss) *loss = l / (float)n;
+    return SPG_OK;
+}
+
+int spg_guide_train(spg_Guide *g, const float *samples, uint32_t n, uint32_t steps,
+                    float *loss)
+{
+    if (!g || !samples || n == 0) return fail(SPG_E_INVALID, "spg_guide_train: no samples");
+    spg_Device *d = g->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    CHK(upload_samples(g, samples, n));
+    const uint32_t B = g->cfg.batchSize;
+    CHK(ensure_hg(g, B));
+    const uint32_t batches = std::max(1u, n / B);
+    const uint32_t cus = cus_of(d);
+    uint32_t lastN = 0;
+    for (uint32_t k = 0; k < steps; ++k)
+    {
+        uint32_t first = (k % batches) * B;
+        lastN = std::min(B, n - first);
+        GuideTrainArgs a = make_args(g, first, B, true, g->cfg.maxWeight);
+        HIPCHK(launch_clear_floats(d->stream, g->loss, 1));
+        HIPCHK(launch_guide_gradient(d->stream, a, cus));
+        CHK(adam_step(g, a));
+    }
+    float l = 0.0f;
+    HIPCHK(hipMemcpyAsync(&l, g->loss, 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    if (loss) *loss = lastN ? l / (float)lastN : 0.0f;
+    return SPG_OK;
+}
+
+int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps, float *loss)
+{
+    if (!g) return fail(SPG_E_INVALID, "spg_guide_stats: NULL");
+    spg_Device *d = g->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    uint32_t c = 0;
+    float l = 0.0f;
+    HIPCHK(hipMemcpyAsync(&c, g->count, 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipMemcpyAsync(&l, g->loss, 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    if (collected) *collected = c;
+    if (steps) *steps = g->steps;
+    if (loss)
+    {
+        uint32_t have = std::min(c, g->sampleCap);
+        uint32_t k = g->cfg.stepsPerPass ? g->cfg.stepsPerPass - 1 : 0;
+        uint32_t first = k * g->cfg.batchSize;
+        uint32_t n = have > first ? std::min(have - first, g->cfg.batchSize) : 0;
+        *loss = n ? l / (float)n : 0.0f;
+    }
+    return SPG_OK;
+}
+
+}  // extern "C"

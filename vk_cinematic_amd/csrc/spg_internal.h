@@ -11,6 +11,7 @@
 
 #include "../../include/spgpu.h"
 #include "spg_bvh.h"
+#include "spg_guide.h"
 #include "spg_math.h"
 
 namespace spg {
@@ -141,9 +142,39 @@ struct TraceParams
     uint32_t bounces;
     float clampValue;
     uint32_t estimator, rngMode, seed, tx, collect;
+    DevGuide guide;              // SPG_EST_GUIDED only
+};
+
+// Loss scale of the FP16 gradients in guide training.
+constexpr float kLossScale = 1.0f / 32.0f;
+
+// One guide training step's buffers (device pointers).
+struct GuideTrainArgs
+{
+    _Float16 *w16, *wt16;        // [kGRows][64], [5][64][64] (in-major)
+    float *bias;                 // [kGRows]
+    float *params, *m, *v, *grad;// [kGParams] master, Adam moments, gradient
+    const float *samples;        // [kGSampleFloats][sampleCap]
+    const uint32_t *count;       // samples available
+    uint32_t sampleCap;
+    uint32_t first, batch;       // this step's samples [first, first+batch)
+    float *wsum;                 // batch weight sum (normalize) or nullptr
+    float wmax;                  // clamp of normalized weights, 0 = off
+    _Float16 *H, *G;             // [5][64][batch] layer inputs / out grads
+    float *loss;                 // += sum_j w_j (-log q_j)
+};
+
+struct GuideAdam
+{
+    float lr, beta1, beta2, eps, c1, c2;
+    uint32_t apply;
 };
 
 }  // namespace spg
+
+// HIP events per device: timed intervals of a trace use [0, kEvents - 4),
+// the trace total the last two, a closest-hit timing the two before.
+constexpr int kEvents = 256;
 
 struct spg_Device
 {
@@ -163,11 +194,36 @@ struct spg_Device
     size_t accumCount = 0;
     void *persistBlock = nullptr; // launch counters + traversal stack spill
     spg::Persist persist{};
-    hipEvent_t ev[64];
+    hipEvent_t ev[kEvents];
     bool eventsMade = false;
     double lastMs[6] = {0};
     uint32_t lastLaunches[6] = {0};
     uint64_t lastCounters[8] = {0};
+};
+
+struct spg_Guide
+{
+    spg_Device *dev = nullptr;
+    spg_GuideConfig cfg{};
+    // parameters and training state (one allocation)
+    void *block = nullptr;
+    float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    float *bias = nullptr, *wsum = nullptr, *loss = nullptr;
+    _Float16 *w16 = nullptr, *wt16 = nullptr;
+    uint32_t *count = nullptr;        // training samples collected
+    // training samples, SoA [13][sampleCap]
+    void *samplesBlock = nullptr;
+    float *samples = nullptr;
+    uint32_t sampleCap = 0;
+    // per-layer activations / gradients of a batch, [5][64][batchCap] each
+    void *hgBlock = nullptr;
+    _Float16 *H = nullptr, *G = nullptr;
+    uint32_t batchCap = 0;
+    // per-path buffers of the tracer (lobes per queued ray, vertex records)
+    void *pathBlock = nullptr;
+    float *lobes = nullptr, *gvtx = nullptr;
+    uint32_t pathCap = 0, pathBounces = 0;
+    uint32_t steps = 0;               // Adam steps taken
 };
 
 struct HostMesh
@@ -215,7 +271,22 @@ struct spg_Scene
     spg::DevScene dev_scene{};
 };
 
+#define CHK(x)                  \
+    do                          \
+    {                           \
+        int _r = (x);           \
+        if (_r) return _r;      \
+    } while (0)
+#define HIPCHK(x) CHK(spg::hip_check((x), #x))
+
 namespace spg {
+// guide hooks of the tracer (spg_guide_api.cpp)
+int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
+                        uint32_t bounces, uint32_t paths, bool train,
+                        DevGuide &out);
+int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt);
+int guide_train_pass(spg_Guide *g, hipStream_t st);
+
 // error reporting (thread-local message)
 int fail(int code, const std::string &msg);
 int hip_check(hipError_t e, const char *what);
@@ -242,4 +313,14 @@ hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,
 hipError_t launch_clear_counters(hipStream_t s, uint32_t *counters, int n);
 hipError_t launch_set_counter(hipStream_t s, uint32_t *counter, uint32_t v);
 hipError_t launch_clear_floats(hipStream_t s, float *p, size_t n);
+hipError_t launch_guide_infer(hipStream_t s, const TraceParams &p, int q);
+// guide network (spg_guide.hip)
+hipError_t launch_guide_eval(hipStream_t s, const _Float16 *w16, const float *bias,
+                             const float *in9, uint32_t n, float *out48,
+                             float *lobes40);
+hipError_t launch_guide_convert(hipStream_t s, const GuideTrainArgs &a);
+hipError_t launch_guide_gradient(hipStream_t s, const GuideTrainArgs &a,
+                                 uint32_t cus);
+hipError_t launch_guide_adam(hipStream_t s, const GuideTrainArgs &a,
+                             const GuideAdam &h);
 }  // namespace spg

@@ -13,6 +13,7 @@
 // order (spg_math.h) and built with -ffp-contract=off and correctly rounded
 // division/sqrt, so a path computes the same FP32 bits as the reference CPU
 // tracer given the same random numbers and transcendental results.
+#include "spg_mlp.h"
 #include "spg_traverse.h"
 
 namespace spg {
@@ -516,7 +517,7 @@ __device__ __forceinline__ v3 cosine_direction(v3 n, float u1, float u2)
 // One bounce of every queued path: surface, material, scatter, vertex record
 // (parity) or throughput update (unbiased), append continuing rays to queue
 // q^1 with a ballot + one atomic per wave.
-template <bool kDouble>
+template <bool kDouble, int kEst>
 __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
 {
     const uint32_t n = p.paths.counters[q];
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
             uint32_t vc = p.paths.vcount[path];
             v3 V = -d;
             bool hit = inst != SPG_MISS && t > 0.0f;
-            if (p.estimator == SPG_EST_PARITY)
+            if (kEst == SPG_EST_PARITY)
             {
                 v3 E, X;
                 float cosine;
@@ -596,7 +597,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                 vt[6 * cap] = cosine;
                 p.paths.vcount[path] = vc + 1;
             }
-            else
+            else if (kEst == SPG_EST_UNBIASED)
             {
                 v3 beta{p.paths.beta[path], p.paths.beta[cap + path],
                         p.paths.beta[2 * cap + path]};
@@ -639,6 +640,82 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                 p.paths.radiance[path] = L.x;
                 p.paths.radiance[cap + path] = L.y;
                 p.paths.radiance[2 * cap + path] = L.z;
+                p.paths.vcount[path] = vc + 1;
+            }
+            else
+            {
+                // Learned guiding: one-sample MIS of cosine sampling
+                // (probability fb) and the guide's vMF mixture, pdf
+                // fb cos/pi + (1 - fb) q(wi); the vertex record is
+                // (E, X, cos/pdf) for the reverse recurrence of k_finalize.
+                v3 E{0.0f, 0.0f, 0.0f}, X{0.0f, 0.0f, 0.0f};
+                float c = 0.0f;
+                if (hit)
+                {
+                    hits++;
+                    v3 N;
+                    v2 uv;
+                    surface_at(p.scene, inst, p.paths.hprim[i], p.paths.hu[i],
+                               p.paths.hv[i], N, uv);
+                    MatOut m = eval_material<kDouble>(
+                        p.scene, p.scene.inst[inst].material, uv, V);
+                    E = m.emission;
+                    if (vc + 1 < p.bounces)
+                    {
+                        const LobeRef Lg{p.guide.lobes, cap, i};
+                        float u0 = rng.u24(), u1 = rng.u24(), u2 = rng.u24();
+                        const float fb = p.guide.bsdfFraction;
+                        v3 wi = u0 < fb ? cosine_direction<kDouble>(N, u1, u2)
+                                        : mixture_sample(Lg, (u0 - fb) / (1.0f - fb),
+                                                         u1, u2);
+                        float cosv = dot(N, wi);
+                        float pdf = fb * cosv * (1.0f / kPi) +
+                                    (1.0f - fb) * mixture_pdf(Lg, wi);
+                        if (cosv > 0.0f && pdf > 0.0f)
+                        {
+                            X = brdf_bracket<kDouble>(N, V, wi, m, true);
+                            c = cosv / pdf;
+                            v3 P = o + d * t;
+                            push = true;
+                            no = P + N * 0.0001f;
+                            nd = wi;
+                            if (p.guide.train)
+                            {
+                                v3 pn = guide_pn(p.guide, P);
+                                float *gv = p.guide.gvtx + (size_t)vc * cap + path;
+                                const size_t st = (size_t)p.bounces * cap;
+                                gv[0] = pn.x;
+                                gv[st] = pn.y;
+                                gv[2 * st] = pn.z;
+                                gv[3 * st] = N.x;
+                                gv[4 * st] = N.y;
+                                gv[5 * st] = N.z;
+                                gv[6 * st] = V.x;
+                                gv[7 * st] = V.y;
+                                gv[8 * st] = V.z;
+                                gv[9 * st] = wi.x;
+                                gv[10 * st] = wi.y;
+                                gv[11 * st] = wi.z;
+                            }
+                        }
+                    }
+                }
+                else
+                {
+                    misses++;
+                    v2 uv0{0.0f, 0.0f};
+                    MatOut m = eval_material<kDouble>(p.scene, p.scene.background,
+                                                      uv0, V);
+                    E = m.emission;
+                }
+                float *vt = p.paths.vtx + (size_t)vc * 7 * cap + path;
+                vt[0] = E.x;
+                vt[cap] = E.y;
+                vt[2 * cap] = E.z;
+                vt[3 * cap] = X.x;
+                vt[4 * cap] = X.y;
+                vt[5 * cap] = X.z;
+                vt[6 * cap] = c;
                 p.paths.vcount[path] = vc + 1;
             }
             store_rng(p, path, rng);
@@ -704,9 +781,46 @@ __global__ __launch_bounds__(256) void k_finalize(TraceParams p)
                     radiance = E + had(X, incoming) * cosine;
                 }
             }
-            else
+            else if (p.estimator == SPG_EST_UNBIASED)
                 radiance = v3{p.paths.radiance[path], p.paths.radiance[cap + path],
                               p.paths.radiance[2 * cap + path]};
+            else
+            {
+                // guided: the same recurrence with cos/pdf in place of the
+                // cosine and no clamp; each scattering vertex j yields a
+                // training sample with weight Y(f cos Li / pdf) = Y(T_j)
+                int32_t vc = (int32_t)p.paths.vcount[path];
+                const bool sel = p.guide.train &&
+                                 guide_hash(path ^ p.guide.salt) < p.guide.selectBelow;
+                for (int32_t j = vc - 1; j >= 0; --j)
+                {
+                    const float *vt = p.paths.vtx + (size_t)j * 7 * cap + path;
+                    v3 E{vt[0], vt[cap], vt[2 * cap]};
+                    v3 X{vt[3 * cap], vt[4 * cap], vt[5 * cap]};
+                    float c = vt[6 * cap];
+                    v3 T = had(X, radiance) * c;
+                    if (sel && c > 0.0f)
+                    {
+                        float w = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+                        if (w > 0.0f)
+                        {
+                            uint32_t at = atomicAdd(p.guide.sampleCount, 1u);
+                            if (at < p.guide.sampleCap)
+                            {
+                                const size_t stride = (size_t)p.bounces * cap;
+                                const float *gv =
+                                    p.guide.gvtx + (size_t)j * cap + path;
+                                float *dst = p.guide.samples + at;
+                                const size_t C = p.guide.sampleCap;
+                                for (int k = 0; k < kGVertexFloats; ++k)
+                                    dst[k * C] = gv[k * stride];
+                                dst[12 * C] = w;
+                            }
+                        }
+                    }
+                    radiance = E + T;
+                }
+            }
             total = total + radiance * invSpp;
             lastXs = p.paths.rngXs[path];
         }
@@ -714,6 +828,72 @@ __global__ __launch_bounds__(256) void k_finalize(TraceParams p)
         p.accum[p.slots + slot] = total.y;
         p.accum[2 * p.slots + slot] = total.z;
         if (p.rngMode != SPG_RNG_PCG) p.paths.slotRng[slot] = lastXs;
+    }
+}
+
+// Guide inference for every queued ray that will scatter: hit surface ->
+// encoding -> MLP (FP16 MFMA, FP32 accumulation) -> vMF mixture, stored
+// per queue entry for k_shade.  Blocks of kInferWaves waves share the
+// weights in LDS; each wave evaluates 64 rays per step.
+constexpr int kInferWaves = 8;
+__global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
+                                                                  int q)
+{
+    __shared__ _Float16 sW[kWLdsHalves];
+    __shared__ float scratch[kInferWaves][kWaveScratchFloats];
+    stage_rows(sW, p.guide.w16, kGRows);
+    __syncthreads();
+    const uint32_t n = p.paths.counters[q];
+    const uint32_t cap = p.paths.capacity;
+    const float *qo = p.paths.qo[q], *qd = p.paths.qd[q];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float *scr = scratch[wave];
+    _Float16 *enc = (_Float16 *)scr;
+    for (uint32_t tile = xcd_block() * kInferWaves + wave; tile * 64 < n;
+         tile += gridDim.x * kInferWaves)
+    {
+        const uint32_t i = tile * 64 + lane;
+        bool valid = false;
+        v3 pn{0, 0, 0}, N{0, 0, 0}, V{0, 0, 0};
+        if (i < n)
+        {
+            uint32_t inst = p.paths.hinst[i];
+            float t = p.paths.ht[i];
+            uint32_t path = p.paths.qpath[q][i];
+            if (inst != SPG_MISS && t > 0.0f && p.paths.vcount[path] + 1 < p.bounces)
+            {
+                valid = true;
+                v3 o{qo[i], qo[cap + i], qo[2 * cap + i]};
+                v3 d{qd[i], qd[cap + i], qd[2 * cap + i]};
+                v2 uv;
+                surface_at(p.scene, inst, p.paths.hprim[i], p.paths.hu[i],
+                           p.paths.hv[i], N, uv);
+                pn = guide_pn(p.guide, o + d * t);
+                V = -d;
+            }
+        }
+        encode_row(enc + lane * kLdsPitch, valid, pn, N, V);
+        wave_lds_sync();
+        h8 B[4][2];
+        load_input(enc, B);
+        wave_lds_sync();
+        f4 acc[4][4];
+        for (int l = 0; l < 4; ++l)
+        {
+            layer64(sW, p.guide.bias, l, B, acc);
+            relu_to_b(acc, B);
+        }
+        f4 out[4][3];
+        layer_out(sW, p.guide.bias, B, out);
+        store_outputs(scr, out);
+        wave_lds_sync();
+        if (valid)
+        {
+            Lobes L;
+            outputs_to_lobes(scr + lane, L);
+            store_lobes(p.guide.lobes, cap, i, L);
+        }
+        wave_lds_sync();
     }
 }
 
@@ -854,10 +1034,37 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
 {
     uint32_t g = grid_for((uint64_t)p.slots * p.sppPass, 256, 4096);
     g = (g + 7) / 8 * 8;
-    if (p.tx == SPG_TX_DOUBLE)
-        hipLaunchKernelGGL(k_shade<true>, dim3(g), dim3(256), 0, s, p, q);
-    else
-        hipLaunchKernelGGL(k_shade<false>, dim3(g), dim3(256), 0, s, p, q);
+    const bool dbl = p.tx == SPG_TX_DOUBLE;
+#define SPG_SHADE(D, E) \
+    hipLaunchKernelGGL((k_shade<D, E>), dim3(g), dim3(256), 0, s, p, q)
+    switch (p.estimator)
+    {
+    case SPG_EST_PARITY:
+        if (dbl) SPG_SHADE(true, SPG_EST_PARITY);
+        else SPG_SHADE(false, SPG_EST_PARITY);
+        break;
+    case SPG_EST_UNBIASED:
+        if (dbl) SPG_SHADE(true, SPG_EST_UNBIASED);
+        else SPG_SHADE(false, SPG_EST_UNBIASED);
+        break;
+    default:
+        if (dbl) SPG_SHADE(true, SPG_EST_GUIDED);
+        else SPG_SHADE(false, SPG_EST_GUIDED);
+        break;
+    }
+#undef SPG_SHADE
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_infer(hipStream_t s, const TraceParams &p, int q)
+{
+    uint64_t n = (uint64_t)p.slots * p.sppPass;  // bound on the queue length
+    uint64_t blocks = (n + 64 * kInferWaves - 1) / (64 * kInferWaves);
+    uint32_t cap = device_cus();
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_guide_infer, dim3((uint32_t)blocks), dim3(64 * kInferWaves),
+                       0, s, p, q);
     return hipGetLastError();
 }
 
