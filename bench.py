@@ -273,6 +273,140 @@ def bench_box(args, V, scenes, name):
     return res
 
 
+# ---------------------------------------------------------------- relMSE
+def relmse(img, ref):
+    """Relative MSE, mean over pixels and channels of (x - r)^2 / (r^2 +
+    0.01) (the convention of the path-guiding literature, e.g. Mueller et
+    al. 2017)."""
+    return float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2)))
+
+
+class Accumulator:
+    """Renders chunks of samples of one scene into a device image and
+    accumulates them on the host; GPU time per chunk from the trace call's
+    own events (host copies excluded)."""
+
+    def __init__(self, V, dev, s, cam, w, h):
+        self.V, self.dev, self.s, self.cam, self.w, self.h = V, dev, s, cam, w, h
+        self.tiles = V.ComputeTiles(w, h, 64, 64)
+        self.img = dev.alloc(w * h * 16)
+        self.sum = np.zeros((h, w, 3))
+        self.spp = 0
+        self.gpu_ms = 0.0
+
+    def add(self, spp, est, seed, offset, guide=None, train=False):
+        V = self.V
+        ctx = V.make_context(self.cam, self.s, spp=spp, bounces=6, clamp=0.0,
+                             estimator=est, rng=V.RNG_PCG, seed=seed,
+                             sample_offset=offset, transcendentals=V.TX_FAST,
+                             guide=guide, train=train)
+        V.path_trace_tiles_device(ctx, self.tiles, self.img)
+        self.gpu_ms += self.dev.last_timings()["total"][0]
+        px = self.img.download(np.float32, self.w * self.h * 4)
+        self.sum += px.reshape(self.h, self.w, 4)[:, :, :3] * spp
+        self.spp += spp
+
+    def mean(self):
+        return self.sum / max(self.spp, 1)
+
+
+def make_reference(args, V, scenes):
+    """Plain unbiased render of the workload, args.ref_spp samples in
+    chunks of 512 (samples numbered from args.ref_offset), saved as float32
+    [H][W][3]."""
+    dev = V.Device(0)
+    desc = scenes.config5() if args.workload == "config5" else scenes.config4()
+    w, h = desc["camera"][0], desc["camera"][1]
+    s, cam = scenes.load_gpu(dev, desc)
+    acc = Accumulator(V, dev, s, cam, w, h)
+    chunk = 512
+    t0 = time.perf_counter()
+    while acc.spp < args.ref_spp:
+        n = min(chunk, args.ref_spp - acc.spp)
+        acc.add(n, V.EST_UNBIASED, args.ref_seed, args.ref_offset + acc.spp)
+        log(f"reference {acc.spp}/{args.ref_spp} spp, {time.perf_counter() - t0:.0f} s")
+    np.save(args.make_reference, acc.mean().astype(np.float32))
+    print(json.dumps({"reference": args.make_reference, "spp": acc.spp,
+                      "gpu_s": acc.gpu_ms / 1e3, "mean": float(acc.mean().mean())}))
+    s.close()
+    dev.close()
+
+
+def time_to_relmse(curve, target):
+    """First crossing of `target` on a (gpu_s, spp, relMSE) curve, log-log
+    interpolated between checkpoints; None if never reached."""
+    prev = None
+    for t, n, e in curve:
+        if e <= target:
+            if prev is None:
+                return t, n
+            t0, n0, e0 = prev
+            f = (np.log(target) - np.log(e0)) / (np.log(e) - np.log(e0))
+            return (float(np.exp(np.log(t0) + f * (np.log(t) - np.log(t0)))),
+                    float(np.exp(np.log(n0) + f * (np.log(n) - np.log(n0)))))
+        prev = (t, n, e)
+    return None
+
+
+def bench_relmse(args, V, scenes):
+    """Time to relMSE 1e-3 against a reference image: the plain tracer and
+    the guided tracer (online training for the first passes, then the frozen
+    guide), both accumulating every sample they render."""
+    ref = np.load(args.relmse).astype(np.float64)
+    dev = V.Device(0)
+    desc = scenes.config5() if args.workload == "config5" else scenes.config4()
+    w, h = desc["camera"][0], desc["camera"][1]
+    assert ref.shape == (h, w, 3), ref.shape
+    s, cam = scenes.load_gpu(dev, desc)
+    target = 1e-3
+    res = {"workload": args.workload, "target": target,
+           "reference": os.path.basename(args.relmse), "max_spp": args.max_spp}
+
+    def run(name, step):
+        acc = Accumulator(V, dev, s, cam, w, h)
+        curve, next_check = [], 16
+        t0 = time.perf_counter()
+        while acc.spp < args.max_spp:
+            step(acc)
+            if acc.spp >= next_check:
+                e = relmse(acc.mean(), ref)
+                curve.append((acc.gpu_ms / 1e3, acc.spp, e))
+                log(f"{name}: {acc.spp} spp gpu {acc.gpu_ms / 1e3:.1f} s relMSE {e:.5f} "
+                    f"(wall {time.perf_counter() - t0:.0f} s)")
+                next_check *= 2
+                if e < target / 2:
+                    break
+        return curve
+
+    plain = run("plain", lambda a: a.add(64 if a.spp >= 64 else 16, V.EST_UNBIASED,
+                                         0x5EED, a.spp))
+    g = V.Guide(dev, seed=args.guide_seed, learning_rate=args.guide_lr,
+                batch_size=args.guide_batch, steps_per_pass=args.guide_steps,
+                max_weight=args.guide_max_weight)
+
+    def guided_step(a):
+        if a.spp < args.train_passes:
+            a.add(1, V.EST_GUIDED, 0x6EED, a.spp, guide=g, train=True)
+        else:
+            a.add(64, V.EST_GUIDED, 0x6EED, a.spp, guide=g)
+
+    guided = run("guided", guided_step)
+    tp = time_to_relmse(plain, target)
+    tg = time_to_relmse(guided, target)
+    res.update({"plain_curve": plain, "guided_curve": guided,
+                "plain_time_s": tp[0] if tp else None, "plain_spp": tp[1] if tp else None,
+                "guided_time_s": tg[0] if tg else None, "guided_spp": tg[1] if tg else None,
+                "speedup": (tp[0] / tg[0]) if tp and tg else None,
+                "train_passes": args.train_passes,
+                "guide": {"lr": args.guide_lr, "batch": args.guide_batch,
+                          "steps_per_pass": args.guide_steps,
+                          "max_weight": args.guide_max_weight}})
+    print(json.dumps(res))
+    g.close()
+    s.close()
+    dev.close()
+
+
 def bench_soup(args, V, scenes):
     dev = V.Device(0)
     tris, o, d = scenes.soup(1_000_000, 1 << 22, seed=7)
@@ -334,6 +468,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fast", action="store_true",
                     help="device float transcendentals instead of double-evaluated")
+    ap.add_argument("--make-reference", metavar="OUT.npy",
+                    help="render the workload's plain reference image instead")
+    ap.add_argument("--ref-spp", type=int, default=32768)
+    ap.add_argument("--ref-seed", type=int, default=0xABCD)
+    ap.add_argument("--ref-offset", type=int, default=0)
+    ap.add_argument("--relmse", metavar="REF.npy",
+                    help="time-to-relMSE 1e-3 of the plain and guided tracers "
+                         "against this reference instead")
+    ap.add_argument("--max-spp", type=int, default=8192)
+    ap.add_argument("--train-passes", type=int, default=64)
+    ap.add_argument("--guide-lr", type=float, default=3e-3)
+    ap.add_argument("--guide-batch", type=int, default=16384)
+    ap.add_argument("--guide-steps", type=int, default=4)
+    ap.add_argument("--guide-max-weight", type=float, default=10.0)
+    ap.add_argument("--guide-seed", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -342,6 +491,10 @@ def main():
     import vk_cinematic_amd as V
     from vk_cinematic_amd import scenes
 
+    if args.make_reference:
+        return make_reference(args, V, scenes)
+    if args.relmse:
+        return bench_relmse(args, V, scenes)
     if args.workload in ("config5", "config4"):
         res = bench_box(args, V, scenes, args.workload)
     else:

@@ -886,7 +886,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     p.collect = cfg.collectMetrics;
     const bool train = guided && cfg.guideTrain;
     if (guided)
-        CHK(guide_prepare_trace(cfg.guide, s, cap, cfg.bounces, cap, train, p.guide));
+        CHK(guide_prepare_trace(cfg.guide, s, cap, cfg.bounces, train, p.guide));
 
     LaunchTimer tm{d};
     uint32_t launches[6] = {0};
@@ -910,7 +910,8 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
             if (guided)
                 CHK(guide_begin_pass(cfg.guide, d->stream, p.guide,
                                      (p.sampleBase + 1u) * 0x9E3779B9u ^ cfg.seed ^
-                                         (uint32_t)s0));
+                                         (uint32_t)s0,
+                                     nslots * sp));
             tm.begin(kRaygen);
             HIPCHK(launch_raygen(d->stream, p));
             tm.end();

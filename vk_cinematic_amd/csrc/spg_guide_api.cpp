@@ -170,8 +170,7 @@ int set_params_locked(spg_Guide *g, const float *p)
 namespace spg {
 
 int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
-                        uint32_t bounces, uint32_t paths, bool train,
-                        DevGuide &out)
+                        uint32_t bounces, bool train, DevGuide &out)
 {
     if (!g->pathBlock || g->pathCap < capacity || g->pathBounces < bounces)
     {
@@ -207,8 +206,7 @@ int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
     dg.samples = g->samples;
     dg.sampleCount = g->count;
     dg.sampleCap = g->sampleCap;
-    double psel = paths ? std::min(1.0, (double)B * steps / (double)paths) : 1.0;
-    dg.selectBelow = (uint32_t)std::min(4294967295.0, psel * 4294967296.0);
+    dg.selectBelow = 0xFFFFFFFFu;
     dg.train = train ? 1u : 0u;
     dg.bsdfFraction = g->cfg.bsdfFraction;
     for (int a = 0; a < 3; ++a)
@@ -225,9 +223,16 @@ int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
     return SPG_OK;
 }
 
-int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt)
+// Each pass feeds training from a hashed subset of its paths sized to
+// about batchSize * stepsPerPass paths (a path yields a sample per
+// scattering vertex that saw light).
+int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt,
+                     uint32_t paths)
 {
     dg.salt = salt;
+    double want = (double)g->cfg.batchSize * g->cfg.stepsPerPass;
+    double psel = paths ? std::min(1.0, want / (double)paths) : 1.0;
+    dg.selectBelow = (uint32_t)std::min(4294967295.0, psel * 4294967296.0);
     if (dg.train) HIPCHK(launch_set_counter(st, g->count, 0u));
     return SPG_OK;
 }

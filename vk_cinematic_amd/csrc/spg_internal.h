@@ -282,9 +282,9 @@ struct spg_Scene
 namespace spg {
 // guide hooks of the tracer (spg_guide_api.cpp)
 int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
-                        uint32_t bounces, uint32_t paths, bool train,
-                        DevGuide &out);
-int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt);
+                        uint32_t bounces, bool train, DevGuide &out);
+int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt,
+                     uint32_t paths);
 int guide_train_pass(spg_Guide *g, hipStream_t st);
 
 // error reporting (thread-local message)

@@ -181,7 +181,7 @@ REFERENCE_MATERIALS = [
 
 
 def box_scene(grid, level, width, height, seed=0, unique_meshes=None,
-              light_fraction=0.03, half=8.0, room_height=6.0):
+              light_fraction=None, half=8.0, room_height=6.0):
     """Displaced-icosphere instances on a grid x grid layout inside the box
     with a 1x1 ceiling opening, lit by the sky through the opening.  Each
     instance draws a reference material type; non-emitters get albedo
@@ -211,11 +211,16 @@ def box_scene(grid, level, width, height, seed=0, unique_meshes=None,
             s = radius * rng.uniform(0.8, 1.0)
             q = quat_axis_angle(rng.normal(size=3) / np.sqrt(3), rng.uniform(0, np.pi))
             q = (q / np.linalg.norm(q.astype(np.float64))).astype(np.float32)
-            if rng.uniform() < light_fraction:
-                name, alb, em, rough = REFERENCE_MATERIALS[8 + rng.integers(0, 3)]
+            if light_fraction is None:
+                kind = int(rng.integers(0, len(REFERENCE_MATERIALS)))
+            elif rng.uniform() < light_fraction:
+                kind = 8 + int(rng.integers(0, 3))
+            else:
+                kind = int(rng.integers(0, 8))
+            name, alb, em, rough = REFERENCE_MATERIALS[kind]
+            if kind >= 8:
                 albedo = (0.0, 0.0, 0.0)
             else:
-                name, alb, em, rough = REFERENCE_MATERIALS[rng.integers(0, 8)]
                 c = np.array(alb, np.float64)
                 c = c / c.max() if c.max() > 0 else np.ones(3)
                 albedo = tuple(float(v) for v in c * rng.uniform(0.1, 0.9))
@@ -245,8 +250,9 @@ def box_scene(grid, level, width, height, seed=0, unique_meshes=None,
 
 def config4(width=512, height=512, seed=4):
     """About 1M triangles: 7x7 displaced icospheres of 20480 triangles in
-    the box with the opening, 512x512."""
-    return box_scene(7, 5, width, height, seed=seed)
+    the box with the opening, 512x512, lit only by the sky through the
+    opening (surface material types only)."""
+    return box_scene(7, 5, width, height, seed=seed, light_fraction=0.0)
 
 
 def config5(width=1920, height=1080, seed=5):
