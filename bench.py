@@ -264,6 +264,43 @@ def bench_box(args, V, scenes, name):
                   "scene_build_s": build_s,
                   "paths_per_step": m.values[V.METRIC_PATHS]},
     }
+    # the guided tracer on the same workload: guide inference every bounce
+    # and online training (stepsPerPass Adam steps) after every step
+    g = V.Guide(dev, seed=1, max_weight=10.0)
+
+    def gctx(step, train=True):
+        return V.make_context(cam, s, spp=1, bounces=bounces, clamp=0.0,
+                              estimator=V.EST_GUIDED, rng=V.RNG_PCG, seed=0x6EED,
+                              transcendentals=tx, sample_offset=step, guide=g,
+                              train=train)
+
+    for k in range(args.warmup):
+        V.path_trace_tiles_device(gctx(k), tiles, img)
+    dev.synchronize()
+    cls = {"extend": 0.0, "shade": 0.0, "guide": 0.0, "total": 0.0}
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        V.path_trace_tiles_device(gctx(args.warmup + k), tiles, img)
+        tim = dev.last_timings()
+        for key in cls:
+            cls[key] += tim[key][0]
+    dev.synchronize()
+    gwall = time.perf_counter() - t0
+    # inference only (frozen guide)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        V.path_trace_tiles_device(gctx(args.warmup + k, train=False), tiles, img)
+    dev.synchronize()
+    fwall = time.perf_counter() - t0
+    res["extra"]["guided"] = {
+        "Msamples_per_s_training": samples * args.steps / gwall / 1e6,
+        "Msamples_per_s_frozen": samples * args.steps / fwall / 1e6,
+        "ms_per_step_training": gwall * 1e3 / args.steps,
+        "ms_per_step_frozen": fwall * 1e3 / args.steps,
+        "kernel_ms_per_step": {k: v / args.steps for k, v in cls.items()},
+        "training_steps_per_pass": 4, "batch": 16384,
+        "collected_last_pass": g.stats()["collected"]}
+    g.close()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_box(desc, min(16, os.cpu_count() or 1),
                                                bounces=bounces)

@@ -266,6 +266,46 @@ int spg_last_trace_timings(spg_Device *dev, double *ms6, uint32_t *launches6);
  * collectMetrics. */
 int spg_last_trace_counters(spg_Device *dev, uint64_t *counters8);
 
+/* ---- batched tracing (no reference counterpart: the reference renders one
+ *      scene at a time, 64x64 tiles on 16 CPU threads, src/main.cpp:731-844)
+ * A batch renders many scenes -- each with its own camera and image size --
+ * as one wavefront: every pixel of every item is a slot, paths of all
+ * items share the ray queues, and each closest-hit launch is one persistent
+ * launch over rays of all scenes.  spg_batch_create builds the per-item
+ * tables, picks the launch shape by timing candidates on the batch itself
+ * (autotune != 0) and captures the whole render in a HIP graph;
+ * spg_batch_run replays it.  Each item's image equals what
+ * spg_path_trace_tiles_device renders for that camera and scene alone
+ * (same per-pixel random streams).  The scenes, cameras' images and the
+ * device must outlive the batch; scenes must not be rebuilt meanwhile. */
+typedef struct spg_Batch spg_Batch;
+
+typedef struct {
+    const spg_Camera *camera;  /* copied at creation */
+    spg_Scene *scene;          /* built, on the batch's device */
+    float *deviceImage;        /* RGBA32F width x height, device memory */
+} spg_BatchItem;
+
+typedef struct {
+    uint32_t pathsInFlight;      /* wavefront capacity (paths) */
+    uint32_t extendBlocks;       /* resident one-wave traversal blocks */
+    uint32_t candidates;         /* launch shapes timed by the autotuner */
+    float tunedMsPerSample;      /* the chosen shape's time for 1 spp */
+    uint64_t pixels;             /* slots over all items */
+    uint32_t graphNodes;         /* kernels in the captured graph */
+} spg_BatchTuning;
+
+/* cfg: samplesPerPixel, bounces, radianceClamp, estimator (parity or
+ * unbiased), rngMode, seed, transcendentals, sampleOffset, collectMetrics. */
+int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
+                     const spg_TraceConfig *cfg, int autotune, spg_Batch **out);
+/* Replays the render; metrics (may be NULL) as sp_PathTraceTile's. */
+int spg_batch_run(spg_Batch *b, spg_Metrics *metrics);
+int spg_batch_tuning(const spg_Batch *b, spg_BatchTuning *out);
+/* GPU time of the last spg_batch_run in ms (HIP events around the graph). */
+int spg_batch_last_ms(const spg_Batch *b, double *ms);
+int spg_batch_destroy(spg_Batch *b);
+
 /* ---- learned path guiding (no reference counterpart: the reference
  *      samples directions uniformly, src/math_lib.h:932-946) ----
  * A guide is an MLP (64 -> 4 x 64 ReLU -> 48, FP16 weights, FP32

@@ -1,0 +1,139 @@
+"""Batched rendering (spg_batch_*): many scenes in one wavefront replayed
+from a HIP graph.  Every item's image equals the single-scene render of the
+same camera and scene bit for bit (same per-pixel streams), across replays
+and with other work on the device in between; the autotuner's choice is one
+of its candidates."""
+import numpy as np
+import pytest
+
+import vk_cinematic_amd as V
+from vk_cinematic_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _load(device, descs):
+    out = []
+    for d in descs:
+        s, cam = scenes.load_gpu(device, d)
+        w, h = d["camera"][0], d["camera"][1]
+        out.append((s, cam, device.alloc(w * h * 16), w, h))
+    return out
+
+
+def _single(device, s, cam, w, h, **kw):
+    img = device.alloc(w * h * 16)
+    ctx = V.make_context(cam, s, **kw)
+    V.path_trace_tiles_device(ctx, V.ComputeTiles(w, h, 64, 64), img)
+    out = img.download(np.float32, w * h * 4)
+    img.free()
+    return out
+
+
+def _images(loaded):
+    return [img.download(np.float32, w * h * 4) for _, _, img, w, h in loaded]
+
+
+def _close(loaded):
+    for s, _, img, _, _ in loaded:
+        img.free()
+        s.close()
+
+
+@pytest.mark.parametrize("est,tx", [(V.EST_UNBIASED, V.TX_DOUBLE),
+                                    (V.EST_PARITY, V.TX_DOUBLE),
+                                    (V.EST_UNBIASED, V.TX_FAST)])
+def test_batch_items_equal_single_scene_renders(device, est, tx):
+    descs = scenes.config3(count=5, seed=11, min_side=40, max_side=150,
+                           min_tris=2e4, max_tris=2e5)
+    loaded = _load(device, descs)
+    kw = dict(spp=3, bounces=5, clamp=0.0 if est == V.EST_UNBIASED else 10.0,
+              estimator=est, rng=V.RNG_PCG, seed=0x5EED, transcendentals=tx)
+    b = V.Batch(device, [(cam, s, img) for s, cam, img, _, _ in loaded],
+                autotune=False, **kw)
+    m = b.run()
+    got = _images(loaded)
+    for (s, cam, _, w, h), g in zip(loaded, got):
+        want = _single(device, s, cam, w, h, **kw)
+        assert np.array_equal(g.view(np.uint32), want.view(np.uint32))
+    assert m.values[V.METRIC_PATHS] == sum(w * h for *_, w, h in loaded) * 3
+    assert m.values[V.METRIC_RAYS] == m.values[V.METRIC_HITS] + m.values[V.METRIC_MISSES]
+    b.close()
+    _close(loaded)
+
+
+def test_graph_replays_are_identical_with_work_between(device):
+    descs = scenes.config3(count=3, seed=5, min_side=64, max_side=128,
+                           min_tris=1e4, max_tris=5e4)
+    loaded = _load(device, descs)
+    b = V.Batch(device, [(cam, s, img) for s, cam, img, _, _ in loaded], spp=2,
+                bounces=4, autotune=False)
+    b.run()
+    first = _images(loaded)
+    for rep in range(3):
+        # eager single-scene work on the same device and stream between replays
+        s, cam, _, w, h = loaded[rep % 3]
+        _single(device, s, cam, w, h, spp=1, bounces=2, estimator=V.EST_UNBIASED,
+                clamp=0.0)
+        for _, _, img, w, h in loaded:  # clear the outputs
+            img.upload(np.zeros(w * h * 4, np.float32))
+        b.run()
+        again = _images(loaded)
+        for a, g in zip(first, again):
+            assert np.array_equal(a.view(np.uint32), g.view(np.uint32))
+    b.close()
+    _close(loaded)
+
+
+def test_batch_larger_than_the_wavefront(device):
+    """More pixels than paths in flight: slot chunks, each with its own
+    sample passes."""
+    descs = [scenes.grid_scene(100 + k, 3e4, 1024) for k in range(3)]
+    loaded = _load(device, descs)
+    kw = dict(spp=2, bounces=3, clamp=0.0, estimator=V.EST_UNBIASED, rng=V.RNG_PCG,
+              transcendentals=V.TX_FAST)
+    b = V.Batch(device, [(cam, s, img) for s, cam, img, _, _ in loaded],
+                autotune=False, **kw)
+    t = b.tuning()
+    assert t["pixels"] == 3 * 1024 * 1024 > t["pathsInFlight"]
+    b.run()
+    got = _images(loaded)
+    s, cam, _, w, h = loaded[2]
+    want = _single(device, s, cam, w, h, **kw)
+    assert np.array_equal(got[2].view(np.uint32), want.view(np.uint32))
+    b.close()
+    _close(loaded)
+
+
+def test_autotuner_choice(device):
+    descs = scenes.config3(count=4, seed=9, min_side=128, max_side=256,
+                           min_tris=1e4, max_tris=1e5)
+    loaded = _load(device, descs)
+    b = V.Batch(device, [(cam, s, img) for s, cam, img, _, _ in loaded], spp=4,
+                bounces=5, autotune=True)
+    t = b.tuning()
+    assert t["candidates"] == 3
+    assert t["pathsInFlight"] == 1 << 20
+    assert t["tunedMsPerSample"] > 0
+    assert t["graphNodes"] > 10
+    b.run()
+    assert b.last_ms() > 0
+    assert all(np.isfinite(g).all() and g.reshape(-1, 4)[:, :3].mean() > 0
+               for g in _images(loaded))
+    b.close()
+    _close(loaded)
+
+
+def test_batch_rejects_bad_items(device):
+    d = scenes.config3(count=1, seed=1, min_side=16, max_side=16, min_tris=1e4,
+                       max_tris=1e4)[0]
+    s, cam = scenes.load_gpu(device, d)
+    img = device.alloc(16 * 16 * 16)
+    with pytest.raises(V.SpgError):
+        V.Batch(device, [(cam, s, img)], estimator=V.EST_GUIDED)
+    unbuilt = V.Scene(device)
+    with pytest.raises(V.SpgError):
+        V.Batch(device, [(cam, unbuilt, img)])
+    unbuilt.close()
+    img.free()
+    s.close()

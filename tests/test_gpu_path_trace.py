@@ -265,3 +265,28 @@ def test_box_scene_tiles_match_oracle_bit_exact(device, name, tiles):
         assert m.values[V.METRIC_RAYS] > (t[2] - t[0]) * (t[3] - t[1]) * 2
     oracle().spo_scene_destroy(os_)
     s.close()
+
+
+@pytest.mark.parametrize("est", [V.EST_PARITY, V.EST_UNBIASED, V.EST_GUIDED])
+def test_wavefront_conserves_rays(device, est):
+    """Queue bookkeeping: every ray traced ends as a hit or a miss, every
+    path starts one camera ray, and no path exceeds the bounce count."""
+    desc = scenes.box_scene(3, 2, 96, 64, seed=3)
+    s, cam = scenes.load_gpu(device, desc)
+    g = V.Guide(device, seed=2, batch_size=4096) if est == V.EST_GUIDED else None
+    spp, bounces = 3, 5
+    ctx = V.make_context(cam, s, spp=spp, bounces=bounces, clamp=0.0, estimator=est,
+                         rng=V.RNG_PCG, guide=g, train=g is not None, collect=True)
+    img = np.zeros(96 * 64 * 4, np.float32)
+    m = V.path_trace_tiles(ctx, V.ComputeTiles(96, 64, 64, 64), img)
+    paths = 96 * 64 * spp
+    rays, hits, misses = (m.values[k] for k in (V.METRIC_RAYS, V.METRIC_HITS,
+                                                 V.METRIC_MISSES))
+    assert m.values[V.METRIC_PATHS] == paths
+    assert rays == hits + misses
+    assert paths <= rays <= paths * bounces
+    assert misses <= paths
+    assert np.all(np.isfinite(img))
+    if g is not None:
+        g.close()
+    s.close()

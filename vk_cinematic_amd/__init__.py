@@ -75,6 +75,19 @@ class TraceConfig(C.Structure):
                 ("guide", C.c_void_p)]
 
 
+class BatchItem(C.Structure):
+    """spg_BatchItem (include/spgpu.h)."""
+    _fields_ = [("camera", C.POINTER(Camera)), ("scene", C.c_void_p),
+                ("deviceImage", C.c_void_p)]
+
+
+class BatchTuning(C.Structure):
+    """spg_BatchTuning (include/spgpu.h)."""
+    _fields_ = [("pathsInFlight", C.c_uint32), ("extendBlocks", C.c_uint32),
+                ("candidates", C.c_uint32), ("tunedMsPerSample", C.c_float),
+                ("pixels", C.c_uint64), ("graphNodes", C.c_uint32)]
+
+
 class GuideConfig(C.Structure):
     """spg_GuideConfig (include/spgpu.h)."""
     _fields_ = [("seed", C.c_uint32), ("learningRate", C.c_float),
@@ -158,6 +171,11 @@ _SIGS = {
     "spg_gen_sky_cubemap": (_int, [_u32, _f32p, C.c_float, C.c_float, _f32p]),
     "spg_gen_sky_equirect": (_int, [_u32, _u32, _f32p, C.c_float, C.c_float, _f32p]),
     "spg_pixel_seed": (_u32, [_u32, _u32]),
+    "spg_batch_create": (_int, [_vp, _vp, _u32, _vp, _int, C.POINTER(_vp)]),
+    "spg_batch_run": (_int, [_vp, _vp]),
+    "spg_batch_tuning": (_int, [_vp, _vp]),
+    "spg_batch_last_ms": (_int, [_vp, C.POINTER(C.c_double)]),
+    "spg_batch_destroy": (_int, [_vp]),
     "spg_guide_default_config": (None, [_vp]),
     "spg_guide_create": (_int, [_vp, _vp, C.POINTER(_vp)]),
     "spg_guide_destroy": (_int, [_vp]),
@@ -276,6 +294,14 @@ class Device:
 class DeviceBuffer:
     def __init__(self, dev, ptr, nbytes):
         self.dev, self.ptr, self.nbytes = dev, ptr, nbytes
+
+    def upload(self, array):
+        """Copy a host array into the start of this buffer."""
+        a = np.ascontiguousarray(array)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        check(lib().spg_copy_to_device(self.dev.handle, self.ptr, a.ctypes.data,
+                                       a.nbytes))
 
     def download(self, dtype, count):
         out = np.empty(count, dtype=dtype)
@@ -437,6 +463,56 @@ def make_context(camera, scene, spp=1, bounces=3, clamp=10.0,
     ctx = Context(C.pointer(camera), scene.handle, cfg)
     ctx._keep = (camera, scene, guide)
     return ctx
+
+
+class Batch:
+    """Many scenes rendered as one wavefront from a captured HIP graph
+    (spg_batch_*).  items: [(camera, scene, DeviceBuffer image)]; the
+    remaining arguments as make_context's."""
+
+    def __init__(self, device, items, spp=16, bounces=5, clamp=0.0,
+                 estimator=EST_UNBIASED, rng=RNG_PCG, seed=0x5EED,
+                 transcendentals=TX_FAST, sample_offset=0, collect=False,
+                 autotune=True):
+        self.device = device
+        arr = (BatchItem * len(items))()
+        for k, (cam, scene, img) in enumerate(items):
+            arr[k] = BatchItem(C.pointer(cam), scene.handle, img.ptr)
+        cfg = TraceConfig(spp, bounces, clamp, estimator, rng, seed, transcendentals,
+                          sample_offset, int(bool(collect)), 0, None)
+        h = C.c_void_p()
+        check(lib().spg_batch_create(device.handle, C.byref(arr), len(items),
+                                     C.byref(cfg), int(bool(autotune)), C.byref(h)))
+        self.handle = h
+        self._keep = items
+        _live_scenes.add(self)
+
+    def run(self, metrics=None):
+        m = metrics if metrics is not None else Metrics()
+        check(lib().spg_batch_run(self.handle, C.byref(m)))
+        return m
+
+    def tuning(self):
+        t = BatchTuning()
+        check(lib().spg_batch_tuning(self.handle, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in BatchTuning._fields_}
+
+    def last_ms(self):
+        ms = C.c_double()
+        check(lib().spg_batch_last_ms(self.handle, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            lib().spg_batch_destroy(self.handle)
+            self.handle = None
+        _live_scenes.discard(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 GUIDE_PARAMS = 19760

@@ -722,21 +722,41 @@ int spg_compute_tiles(uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
 // ------------------------------------------------------------ tracing
 namespace spg {
 
-constexpr uint32_t kMaxPathsInFlight = 1u << 21;
-constexpr uint32_t kMaxBounces = 16;
-
-static int ensure_paths(spg_Device *d, uint32_t capacity, uint32_t bounces)
+DevCamera dev_camera(const spg_Camera &c)
 {
-    if (d->pathsBlock && d->paths.capacity >= capacity &&
-        d->paths.bounces >= bounces)
-        return SPG_OK;
-    free_paths(d);
+    DevCamera r{};
+    memcpy(&r.right, c.right, 12);
+    memcpy(&r.up, c.up, 12);
+    memcpy(&r.forward, c.forward, 12);
+    memcpy(&r.position, c.position, 12);
+    memcpy(&r.filmCenter, c.filmCenter, 12);
+    r.width = c.width;
+    r.height = c.height;
+    r.halfPixelWidth = c.halfPixelWidth;
+    r.halfPixelHeight = c.halfPixelHeight;
+    r.halfFilmWidth = c.halfFilmWidth;
+    r.halfFilmHeight = c.halfFilmHeight;
+    return r;
+}
+
+int scene_sync(spg_Scene *s)
+{
+    if (!s->built) return fail(SPG_E_STATE, "scene not built");
+    if (s->dirty) CHK(upload_tables(s));
+    return SPG_OK;
+}
+
+// Wavefront state for `capacity` paths of up to `bounces` rays, one
+// allocation (hipFree of `block` releases it).
+int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity,
+                     uint32_t bounces)
+{
     capacity = std::max(capacity, 1024u);
     size_t P = capacity;
     size_t off = 0;
-    auto take = [&](size_t bytes) {
+    auto take = [&](size_t b) {
         size_t o = off;
-        off += (bytes + 255) / 256 * 256;
+        off += (b + 255) / 256 * 256;
         return o;
     };
     size_t oSlot = take(4 * P), oSample = take(4 * P), oXs = take(4 * P),
@@ -747,9 +767,8 @@ static int ensure_paths(spg_Device *d, uint32_t capacity, uint32_t bounces)
            oQp0 = take(4 * P), oQp1 = take(4 * P), oCnt = take(64),
            oMet = take(128), oHt = take(4 * P), oHu = take(4 * P),
            oHv = take(4 * P), oHi = take(4 * P), oHp = take(4 * P);
-    HIPCHK(hipMalloc(&d->pathsBlock, off));
-    char *b = (char *)d->pathsBlock;
-    DevPaths &p = d->paths;
+    HIPCHK(hipMalloc(&block, off));
+    char *b = (char *)block;
     p.capacity = (uint32_t)P;
     p.bounces = bounces;
     p.pathSlot = (uint32_t *)(b + oSlot);
@@ -774,8 +793,18 @@ static int ensure_paths(spg_Device *d, uint32_t capacity, uint32_t bounces)
     p.hv = (float *)(b + oHv);
     p.hinst = (uint32_t *)(b + oHi);
     p.hprim = (uint32_t *)(b + oHp);
-    d->pathsBytes = off;
+    bytes = off;
     return SPG_OK;
+}
+
+static int ensure_paths(spg_Device *d, uint32_t capacity, uint32_t bounces)
+{
+    if (d->pathsBlock && d->paths.capacity >= capacity &&
+        d->paths.bounces >= bounces)
+        return SPG_OK;
+    free_paths(d);
+    return alloc_path_state(d->paths, d->pathsBlock, d->pathsBytes, capacity,
+                            bounces);
 }
 
 template <typename T>
@@ -862,17 +891,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
 
     TraceParams p{};
     p.scene = s->dev_scene;
-    memcpy(&p.cam.right, c.right, 12);
-    memcpy(&p.cam.up, c.up, 12);
-    memcpy(&p.cam.forward, c.forward, 12);
-    memcpy(&p.cam.position, c.position, 12);
-    memcpy(&p.cam.filmCenter, c.filmCenter, 12);
-    p.cam.width = c.width;
-    p.cam.height = c.height;
-    p.cam.halfPixelWidth = c.halfPixelWidth;
-    p.cam.halfPixelHeight = c.halfPixelHeight;
-    p.cam.halfFilmWidth = c.halfFilmWidth;
-    p.cam.halfFilmHeight = c.halfFilmHeight;
+    p.cam = dev_camera(c);
     p.paths = d->paths;
     p.slotPixel = (const uint32_t *)d->slotPixel;
     p.accum = (float *)d->accum;

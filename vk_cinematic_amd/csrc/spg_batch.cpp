@@ -1,0 +1,412 @@
+// Batched tracing: many (scene, camera, image) items rendered as one
+// wavefront, captured once in a HIP graph and replayed.  The kernels are the
+// single-scene ones in their batch variants (TraceParams::slotItem set):
+// raygen reads the slot's camera, the persistent closest-hit launch walks
+// each ray through its own scene's hierarchy, shading reads its scene's
+// materials, and the pixel writer scatters into the item's image.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "spg_internal.h"
+
+using namespace spg;
+
+struct spg_Batch
+{
+    spg_Device *dev = nullptr;
+    spg_TraceConfig cfg{};
+    uint32_t n = 0;
+    std::vector<spg_Scene *> scenes;
+    void *tables = nullptr;
+    const DevScene *dScenes = nullptr;
+    const DevCamera *dCams = nullptr;
+    float *const *dImages = nullptr;
+    const uint32_t *dSlotItem = nullptr, *dSlotPixel = nullptr;
+    uint64_t slots = 0;
+    DevPaths paths{};
+    void *pathsBlock = nullptr;
+    size_t pathsBytes = 0;
+    void *accum = nullptr;
+    uint32_t maxBlocks = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    spg_BatchTuning tuning{};
+    double lastMs = 0.0;
+};
+
+namespace {
+
+void free_dev(void *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+int alloc_state(spg_Batch *b, uint32_t capacity)
+{
+    free_dev(b->pathsBlock);
+    free_dev(b->accum);
+    memset(&b->paths, 0, sizeof b->paths);
+    CHK(alloc_path_state(b->paths, b->pathsBlock, b->pathsBytes, capacity,
+                         b->cfg.bounces));
+    HIPCHK(hipMalloc(&b->accum, (size_t)3 * b->paths.capacity * sizeof(float)));
+    return SPG_OK;
+}
+
+// The whole render as stream work: slot chunks of at most the path
+// capacity, sample passes per chunk, bounces per pass.  No host copies or
+// memsets, so the same sequence can be captured in a graph.
+int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
+           uint32_t *kernels)
+{
+    const uint32_t cap = b->paths.capacity;
+    TraceParams p{};
+    p.paths = b->paths;
+    p.accum = (float *)b->accum;
+    p.spp = cfg.samplesPerPixel;
+    p.bounces = cfg.bounces;
+    p.clampValue = cfg.radianceClamp;
+    p.estimator = cfg.estimator;
+    p.rngMode = cfg.rngMode;
+    p.seed = cfg.seed;
+    p.tx = cfg.transcendentals;
+    p.collect = cfg.collectMetrics;
+    p.scenes = b->dScenes;
+    p.cams = b->dCams;
+    p.images = b->dImages;
+    Persist ps = b->dev->persist;
+    ps.maxBlocks = b->maxBlocks;
+    uint32_t k = 0;
+    HIPCHK(launch_clear_counters(st, (uint32_t *)b->paths.metrics, 32));
+    ++k;
+    const bool pcg = cfg.rngMode == SPG_RNG_PCG;
+    const uint32_t spp = cfg.samplesPerPixel;
+    for (uint64_t s0 = 0; s0 < b->slots; s0 += cap)
+    {
+        uint32_t nslots = (uint32_t)std::min<uint64_t>(cap, b->slots - s0);
+        p.slotPixel = b->dSlotPixel + s0;
+        p.slotItem = b->dSlotItem + s0;
+        p.slots = nslots;
+        HIPCHK(launch_clear_floats(st, p.accum, 3 * (size_t)nslots));
+        ++k;
+        uint32_t sppPass = pcg ? std::max(1u, std::min(spp, cap / nslots)) : 1u;
+        for (uint32_t done = 0; done < spp;)
+        {
+            uint32_t sp = std::min(sppPass, spp - done);
+            p.sppPass = sp;
+            p.sampleBase = cfg.sampleOffset + done;
+            HIPCHK(launch_set_counter(st, &p.paths.counters[0], nslots * sp));
+            HIPCHK(launch_raygen(st, p));
+            k += 2;
+            int q = 0;
+            for (uint32_t bb = 0; bb < cfg.bounces; ++bb)
+            {
+                HIPCHK(launch_extend(st, p, q, ps));
+                HIPCHK(launch_set_counter(st, &p.paths.counters[q ^ 1], 0));
+                HIPCHK(launch_shade(st, p, q));
+                k += 4;  // the extend launch includes its counter reset
+                q ^= 1;
+            }
+            HIPCHK(launch_finalize(st, p));
+            ++k;
+            done += sp;
+        }
+        HIPCHK(launch_write_pixels(st, p, nullptr));
+        ++k;
+    }
+    if (kernels) *kernels = k;
+    return SPG_OK;
+}
+
+int timed(spg_Batch *b, const spg_TraceConfig &cfg, float *ms)
+{
+    hipStream_t st = b->dev->stream;
+    HIPCHK(hipEventRecord(b->ev[0], st));
+    CHK(record(b, st, cfg, nullptr));
+    HIPCHK(hipEventRecord(b->ev[1], st));
+    HIPCHK(hipEventSynchronize(b->ev[1]));
+    HIPCHK(hipEventElapsedTime(ms, b->ev[0], b->ev[1]));
+    return SPG_OK;
+}
+
+// Candidates: wavefront capacity x resident traversal blocks, each timed on
+// one sample per pixel of the whole batch; the fastest is kept.
+int autotune(spg_Batch *b, uint32_t cus)
+{
+    spg_TraceConfig one = b->cfg;
+    one.samplesPerPixel = 1;
+    one.collectMetrics = 0;
+    std::vector<uint32_t> caps;
+    for (uint32_t c : {1u << 20, 1u << 21, 1u << 22})
+    {
+        caps.push_back(c);
+        if ((uint64_t)c >= b->slots) break;
+    }
+    const uint32_t blocks[] = {0u, cus * 16u, cus * 8u};
+    float best = 0.0f, ms = 0.0f;
+    uint32_t bestCap = caps[0], bestBlocks = 0, tried = 0;
+    // warm-up: first launches pay for code loading and cold caches
+    CHK(alloc_state(b, caps[0]));
+    b->maxBlocks = 0;
+    CHK(timed(b, one, &ms));
+    for (uint32_t c : caps)
+    {
+        CHK(alloc_state(b, c));
+        for (uint32_t mb : blocks)
+        {
+            b->maxBlocks = mb;
+            CHK(timed(b, one, &ms));
+            ++tried;
+            if (tried == 1 || ms < best)
+            {
+                best = ms;
+                bestCap = c;
+                bestBlocks = mb;
+            }
+        }
+    }
+    b->tuning.candidates = tried;
+    b->tuning.tunedMsPerSample = best;
+    CHK(alloc_state(b, bestCap));
+    b->maxBlocks = bestBlocks;
+    return SPG_OK;
+}
+
+void release(spg_Batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (hipEvent_t &e : b->ev)
+        if (e) hipEventDestroy(e);
+    free_dev(b->tables);
+    free_dev(b->pathsBlock);
+    free_dev(b->accum);
+}
+
+}  // namespace
+
+extern "C" {
+
+int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
+                     const spg_TraceConfig *cfg, int tune, spg_Batch **out)
+{
+    if (!dev || !items || !cfg || !out || n == 0)
+        return fail(SPG_E_INVALID, "spg_batch_create: NULL or empty argument");
+    const spg_TraceConfig &c = *cfg;
+    if (c.samplesPerPixel == 0 || c.bounces == 0 || c.bounces > kMaxBounces)
+        return fail(SPG_E_INVALID, "spg_batch_create: samplesPerPixel > 0, bounces 1..16");
+    if (c.estimator != SPG_EST_PARITY && c.estimator != SPG_EST_UNBIASED)
+        return fail(SPG_E_INVALID, "spg_batch_create: parity or unbiased estimator only");
+    if (c.rngMode != SPG_RNG_XORSHIFT_PIXEL && c.rngMode != SPG_RNG_PCG)
+        return fail(SPG_E_INVALID, "spg_batch_create: unknown rngMode");
+    if (c.transcendentals != SPG_TX_DOUBLE && c.transcendentals != SPG_TX_FAST)
+        return fail(SPG_E_INVALID, "spg_batch_create: unknown transcendentals mode");
+    uint64_t slots = 0;
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        const spg_BatchItem &it = items[k];
+        if (!it.camera || !it.scene || !it.deviceImage)
+            return fail(SPG_E_INVALID, "spg_batch_create: item " + std::to_string(k) +
+                                           " has a NULL camera, scene or image");
+        if (it.scene->dev != dev)
+            return fail(SPG_E_INVALID, "spg_batch_create: item " + std::to_string(k) +
+                                           " is on another device");
+        if (!it.scene->built)
+            return fail(SPG_E_STATE, "spg_batch_create: item " + std::to_string(k) +
+                                         " scene not built");
+        if (it.camera->width == 0 || it.camera->height == 0)
+            return fail(SPG_E_INVALID, "spg_batch_create: camera not configured");
+        slots += (uint64_t)it.camera->width * it.camera->height;
+    }
+    if (slots >= 0xFFFFFFFFull)
+        return fail(SPG_E_CAPACITY, "spg_batch_create: more than 2^32 pixels");
+
+    std::lock_guard<std::mutex> lk(dev->lock);
+    if (dev->destroyRequested)
+        return fail(SPG_E_STATE, "spg_batch_create: device is being destroyed");
+    HIPCHK(hipSetDevice(dev->ordinal));
+    std::vector<DevScene> ds(n);
+    std::vector<DevCamera> dc(n);
+    std::vector<float *> im(n);
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        CHK(scene_sync(items[k].scene));
+        ds[k] = items[k].scene->dev_scene;
+        dc[k] = dev_camera(*items[k].camera);
+        im[k] = items[k].deviceImage;
+    }
+    std::vector<uint32_t> slotItem(slots), slotPixel(slots);
+    uint64_t at = 0;
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        uint32_t px = items[k].camera->width * items[k].camera->height;
+        for (uint32_t i = 0; i < px; ++i, ++at)
+        {
+            slotItem[at] = k;
+            slotPixel[at] = i;
+        }
+    }
+
+    spg_Batch *b = new spg_Batch();
+    b->dev = dev;
+    b->cfg = c;
+    b->n = n;
+    b->slots = slots;
+    for (uint32_t k = 0; k < n; ++k) b->scenes.push_back(items[k].scene);
+    int r = SPG_OK;
+    auto step = [&](hipError_t e, const char *what) {
+        if (!r) r = hip_check(e, what);
+    };
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    size_t oS = take(sizeof(DevScene) * n), oC = take(sizeof(DevCamera) * n),
+           oI = take(sizeof(float *) * n), oSI = take(4 * slots), oSP = take(4 * slots);
+    step(hipMalloc(&b->tables, off), "hipMalloc(batch tables)");
+    if (!r)
+    {
+        char *t = (char *)b->tables;
+        b->dScenes = (const DevScene *)(t + oS);
+        b->dCams = (const DevCamera *)(t + oC);
+        b->dImages = (float *const *)(t + oI);
+        b->dSlotItem = (const uint32_t *)(t + oSI);
+        b->dSlotPixel = (const uint32_t *)(t + oSP);
+        step(hipMemcpy(t + oS, ds.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice),
+             "hipMemcpy");
+        step(hipMemcpy(t + oC, dc.data(), sizeof(DevCamera) * n, hipMemcpyHostToDevice),
+             "hipMemcpy");
+        step(hipMemcpy(t + oI, im.data(), sizeof(float *) * n, hipMemcpyHostToDevice),
+             "hipMemcpy");
+        step(hipMemcpy(t + oSI, slotItem.data(), 4 * slots, hipMemcpyHostToDevice),
+             "hipMemcpy");
+        step(hipMemcpy(t + oSP, slotPixel.data(), 4 * slots, hipMemcpyHostToDevice),
+             "hipMemcpy");
+    }
+    for (hipEvent_t &e : b->ev) step(hipEventCreate(&e), "hipEventCreate");
+    int cus = 0;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev->ordinal);
+    if (!r && tune) r = autotune(b, (uint32_t)std::max(cus, 1));
+    if (!r && !tune)
+    {
+        uint64_t want = slots * (c.rngMode == SPG_RNG_PCG ? c.samplesPerPixel : 1u);
+        r = alloc_state(b, (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1),
+                                                        kMaxPathsInFlight));
+        // one eager sample per pixel: loads the kernels and sizes the
+        // persistent grids outside the capture
+        spg_TraceConfig one = c;
+        one.samplesPerPixel = 1;
+        float ms = 0.0f;
+        if (!r) r = timed(b, one, &ms);
+        b->tuning.candidates = 1;
+        b->tuning.tunedMsPerSample = ms;
+    }
+    uint32_t kernels = 0;
+    if (!r)
+    {
+        // capture the full render
+        hipStream_t st = dev->stream;
+        r = hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
+                      "hipStreamBeginCapture");
+        if (!r)
+        {
+            int rr = record(b, st, c, &kernels);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &g);
+            r = rr ? rr : hip_check(e, "hipStreamEndCapture");
+            if (g) b->graph = g;
+        }
+        if (!r)
+            r = hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0),
+                          "hipGraphInstantiate");
+    }
+    if (r)
+    {
+        release(b);
+        delete b;
+        return r;
+    }
+    b->tuning.pathsInFlight = b->paths.capacity;
+    b->tuning.extendBlocks = b->maxBlocks;
+    b->tuning.pixels = slots;
+    b->tuning.graphNodes = kernels;
+    dev->liveScenes++;
+    *out = b;
+    return SPG_OK;
+}
+
+int spg_batch_run(spg_Batch *b, spg_Metrics *metrics)
+{
+    if (!b) return fail(SPG_E_INVALID, "spg_batch_run: NULL");
+    spg_Device *d = b->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    for (spg_Scene *s : b->scenes)
+        if (s->dirty)
+            return fail(SPG_E_STATE, "spg_batch_run: a scene changed after the batch was made");
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = d->stream;
+    HIPCHK(hipEventRecord(b->ev[0], st));
+    HIPCHK(hipGraphLaunch(b->exec, st));
+    HIPCHK(hipEventRecord(b->ev[1], st));
+    unsigned long long m[16] = {0};
+    HIPCHK(hipMemcpyAsync(m, b->paths.metrics, 128, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    b->lastMs = ms;
+    for (int k = 0; k < 7; ++k) d->lastCounters[k] = m[k];
+    d->lastCounters[7] = 0;
+    if (metrics)
+    {
+        metrics->values[SPG_METRIC_ELAPSED_NS] += (uint64_t)
+            std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+        metrics->values[SPG_METRIC_PATHS_TRACED] += b->slots * b->cfg.samplesPerPixel;
+        metrics->values[SPG_METRIC_RAYS_TRACED] += m[0];
+        metrics->values[SPG_METRIC_RAY_HIT_COUNT] += m[1];
+        metrics->values[SPG_METRIC_RAY_MISS_COUNT] += m[2];
+        metrics->values[SPG_METRIC_MIDPHASE_AABB_TESTS] += m[3];
+        metrics->values[SPG_METRIC_MESH_TESTS] += m[4];
+    }
+    return SPG_OK;
+}
+
+int spg_batch_tuning(const spg_Batch *b, spg_BatchTuning *out)
+{
+    if (!b || !out) return fail(SPG_E_INVALID, "spg_batch_tuning: NULL");
+    *out = b->tuning;
+    return SPG_OK;
+}
+
+int spg_batch_last_ms(const spg_Batch *b, double *ms)
+{
+    if (!b || !ms) return fail(SPG_E_INVALID, "spg_batch_last_ms: NULL");
+    *ms = b->lastMs;
+    return SPG_OK;
+}
+
+int spg_batch_destroy(spg_Batch *b)
+{
+    if (!b) return SPG_OK;
+    spg_Device *d = b->dev;
+    bool last;
+    {
+        std::lock_guard<std::mutex> lk(d->lock);
+        hipSetDevice(d->ordinal);
+        hipStreamSynchronize(d->stream);
+        release(b);
+        delete b;
+        d->liveScenes--;
+        last = d->destroyRequested && d->liveScenes == 0;
+    }
+    if (last) spg_device_destroy(d);
+    return SPG_OK;
+}
+
+}  // extern "C"

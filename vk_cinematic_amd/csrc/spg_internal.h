@@ -31,6 +31,7 @@ struct Persist
 {
     uint32_t *counter;
     int *spill;
+    uint32_t maxBlocks;  // cap on the resident one-wave blocks, 0 = occupancy
 };
 
 // Relative slack on the t-max cull: a box is skipped only if its entry
@@ -143,6 +144,12 @@ struct TraceParams
     float clampValue;
     uint32_t estimator, rngMode, seed, tx, collect;
     DevGuide guide;              // SPG_EST_GUIDED only
+    // batched launches (slotItem != nullptr): each slot belongs to an item
+    // with its own scene, camera and image
+    const uint32_t *slotItem;
+    const DevScene *scenes;
+    const DevCamera *cams;
+    float *const *images;
 };
 
 // Loss scale of the FP16 gradients in guide training.
@@ -280,6 +287,14 @@ struct spg_Scene
 #define HIPCHK(x) CHK(spg::hip_check((x), #x))
 
 namespace spg {
+int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity,
+                     uint32_t bounces);
+DevCamera dev_camera(const spg_Camera &c);
+// Device tables of a built scene up to date (call with the device locked).
+int scene_sync(spg_Scene *s);
+constexpr uint32_t kMaxPathsInFlight = 1u << 21;
+constexpr uint32_t kMaxBounces = 16;
+
 // guide hooks of the tracer (spg_guide_api.cpp)
 int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
                         uint32_t bounces, bool train, DevGuide &out);

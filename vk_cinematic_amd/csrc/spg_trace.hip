@@ -335,8 +335,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_mesh(
         if (vOut) vOut[i] = s.h.v;
         if (kCount && s.h.prim != SPG_MISS) hits++;
     };
-    persistent_walk<kCount, false>(none, n, counter, spill, begin, store, cnt,
-                                   rays);
+    SceneView sv{};
+    persistent_walk<kCount, false>(sv, n, counter, spill, begin, store, cnt, rays);
     if (kCount)
     {
         wave_add(&stats[0], rays);
@@ -354,8 +354,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_scene(
 {
     Counts cnt{0, 0, 0, 0};
     uint32_t rays = 0;
+    SceneView sv = scene_view(S);
     auto begin = [&](uint32_t i, Trav &s) {
-        trav_begin_scene(S, s, v3{rays6[6 * i], rays6[6 * i + 1], rays6[6 * i + 2]},
+        trav_begin_scene(s, v3{rays6[6 * i], rays6[6 * i + 1], rays6[6 * i + 2]},
                          v3{rays6[6 * i + 3], rays6[6 * i + 4], rays6[6 * i + 5]});
         return S.instCount > 0;
     };
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_scene(
         if (objOut) objOut[i] = h.inst;
         if (primOut) primOut[i] = h.prim;
     };
-    persistent_walk<false, true>(S, n, counter, spill, begin, store, cnt, rays);
+    persistent_walk<false, true>(sv, n, counter, spill, begin, store, cnt, rays);
 }
 
 // Logical block index spread so consecutive ranges of work stay on one XCD
@@ -399,6 +400,7 @@ __device__ __forceinline__ uint32_t xcd_block()
 // two RandomBilateral arguments, src/simd_path_tracer.cpp:219-222, evaluate
 // right to left as g++ on x86-64 builds them), sp_CalculateFilmPositions
 // (src/simd_path_tracer.cpp:38-63), dir = Normalize(filmP - position).
+template <bool kBatch>
 __global__ __launch_bounds__(256) void k_raygen(TraceParams p)
 {
     const uint32_t nPaths = p.slots * p.sppPass;
@@ -408,7 +410,8 @@ __global__ __launch_bounds__(256) void k_raygen(TraceParams p)
         uint32_t slot = path / p.sppPass;
         uint32_t sample = p.sampleBase + path % p.sppPass;
         uint32_t pixel = p.slotPixel[slot];
-        uint32_t x = pixel % p.cam.width, y = pixel / p.cam.width;
+        const DevCamera &cam = kBatch ? p.cams[p.slotItem[slot]] : p.cam;
+        uint32_t x = pixel % cam.width, y = pixel / cam.width;
         Rng rng;
         rng.pcg = p.rngMode == SPG_RNG_PCG;
         if (rng.pcg)
@@ -425,21 +428,21 @@ __global__ __launch_bounds__(256) void k_raygen(TraceParams p)
         }
         float b1 = rng.bi();
         float b2 = rng.bi();
-        float jy = p.cam.halfPixelHeight * b1;
-        float jx = p.cam.halfPixelWidth * b2;
+        float jy = cam.halfPixelHeight * b1;
+        float jx = cam.halfPixelWidth * b2;
         float px = (float)x + 0.5f, py = (float)y + 0.5f;
         px = px + jx;
         py = py + jy;
-        float fx = px / (float)p.cam.width;
-        float fy = py / (float)p.cam.height;
+        float fx = px / (float)cam.width;
+        float fy = py / (float)cam.height;
         fy = 1.0f - fy;
         fx = fx * 2.0f - 1.0f;
         fy = fy * 2.0f - 1.0f;
-        v3 film = p.cam.right * (p.cam.halfFilmWidth * fx);
-        film = film + p.cam.up * (p.cam.halfFilmHeight * fy);
-        film = film + p.cam.filmCenter;
-        v3 o = p.cam.position;
-        v3 d = normalize(film - p.cam.position);
+        v3 film = cam.right * (cam.halfFilmWidth * fx);
+        film = film + cam.up * (cam.halfFilmHeight * fy);
+        film = film + cam.filmCenter;
+        v3 o = cam.position;
+        v3 d = normalize(film - cam.position);
         const uint32_t cap = p.paths.capacity;
         p.paths.qo[0][path] = o.x;
         p.paths.qo[0][cap + path] = o.y;
@@ -462,7 +465,7 @@ __global__ __launch_bounds__(256) void k_raygen(TraceParams p)
 }
 
 // Closest hit of every queued ray.
-template <bool kCount>
+template <bool kCount, bool kBatch>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
                                                         uint32_t *counter,
                                                         int *spill)
@@ -472,9 +475,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
     const float *qo = p.paths.qo[q], *qd = p.paths.qd[q];
     Counts cnt{0, 0, 0, 0};
     uint32_t rays = 0;
+    SceneView sv = scene_view(p.scene);
     auto begin = [&](uint32_t i, Trav &s) {
-        trav_begin_scene(p.scene, s, v3{qo[i], qo[cap + i], qo[2 * cap + i]},
+        trav_begin_scene(s, v3{qo[i], qo[cap + i], qo[2 * cap + i]},
                          v3{qd[i], qd[cap + i], qd[2 * cap + i]});
+        if (kBatch)
+        {
+            const DevScene &S =
+                p.scenes[p.slotItem[p.paths.pathSlot[p.paths.qpath[q][i]]]];
+            sv = scene_view(S);
+            return S.instCount > 0;
+        }
         return p.scene.instCount > 0;
     };
     auto store = [&](uint32_t i, const Trav &s) {
@@ -484,8 +495,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
         p.paths.hinst[i] = s.out.inst;
         p.paths.hprim[i] = s.out.prim;
     };
-    persistent_walk<kCount, true>(p.scene, n, counter, spill, begin, store, cnt,
-                                  rays);
+    persistent_walk<kCount, true>(sv, n, counter, spill, begin, store, cnt, rays);
     if (kCount)
     {
         wave_add(&p.paths.metrics[3], cnt.box);
@@ -517,7 +527,7 @@ __device__ __forceinline__ v3 cosine_direction(v3 n, float u1, float u2)
 // One bounce of every queued path: surface, material, scatter, vertex record
 // (parity) or throughput update (unbiased), append continuing rays to queue
 // q^1 with a ballot + one atomic per wave.
-template <bool kDouble, int kEst>
+template <bool kDouble, int kEst, bool kBatch>
 __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
 {
     const uint32_t n = p.paths.counters[q];
@@ -536,6 +546,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
             path = p.paths.qpath[q][i];
             uint32_t slot = p.paths.pathSlot[path];
             uint32_t pixel = p.slotPixel[slot];
+            const DevScene &S = kBatch ? p.scenes[p.slotItem[slot]] : p.scene;
             Rng rng = load_rng(p, path, pixel);
             v3 o{p.paths.qo[q][i], p.paths.qo[q][cap + i], p.paths.qo[q][2 * cap + i]};
             v3 d{p.paths.qd[q][i], p.paths.qd[q][cap + i], p.paths.qd[q][2 * cap + i]};
@@ -553,7 +564,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     hits++;
                     v3 N;
                     v2 uv;
-                    surface_at(p.scene, inst, p.paths.hprim[i], p.paths.hu[i],
+                    surface_at(S, inst, p.paths.hprim[i], p.paths.hu[i],
                                p.paths.hv[i], N, uv);
                     v3 P = o + d * t;
                     // RandomDirectionOnHemisphere, src/math_lib.h:932-946
@@ -565,7 +576,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     dir.y = Tx<kDouble>::cos_(theta);
                     if (dot(dir, N) < 0.0f) dir = -dir;
                     MatOut m = eval_material<kDouble>(
-                        p.scene, p.scene.inst[inst].material, uv, V);
+                        S, S.inst[inst].material, uv, V);
                     E = m.emission;
                     X = brdf_bracket<kDouble>(N, V, dir, m, false);
                     cosine = fmax_sel(0.0f, dot(N, dir));
@@ -580,7 +591,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                 {
                     misses++;
                     v2 uv0{0.0f, 0.0f};
-                    MatOut m = eval_material<kDouble>(p.scene, p.scene.background,
+                    MatOut m = eval_material<kDouble>(S, S.background,
                                                       uv0, V);
                     v3 Z{0.0f, 0.0f, 0.0f};
                     E = m.emission;
@@ -608,10 +619,10 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     hits++;
                     v3 N;
                     v2 uv;
-                    surface_at(p.scene, inst, p.paths.hprim[i], p.paths.hu[i],
+                    surface_at(S, inst, p.paths.hprim[i], p.paths.hu[i],
                                p.paths.hv[i], N, uv);
                     MatOut m = eval_material<kDouble>(
-                        p.scene, p.scene.inst[inst].material, uv, V);
+                        S, S.inst[inst].material, uv, V);
                     L = L + had(beta, m.emission);
                     float u1 = rng.u24(), u2 = rng.u24();
                     v3 wi = cosine_direction<kDouble>(N, u1, u2);
@@ -630,7 +641,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                 {
                     misses++;
                     v2 uv0{0.0f, 0.0f};
-                    MatOut m = eval_material<kDouble>(p.scene, p.scene.background,
+                    MatOut m = eval_material<kDouble>(S, S.background,
                                                       uv0, V);
                     L = L + had(beta, m.emission);
                 }
@@ -655,10 +666,10 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     hits++;
                     v3 N;
                     v2 uv;
-                    surface_at(p.scene, inst, p.paths.hprim[i], p.paths.hu[i],
+                    surface_at(S, inst, p.paths.hprim[i], p.paths.hu[i],
                                p.paths.hv[i], N, uv);
                     MatOut m = eval_material<kDouble>(
-                        p.scene, p.scene.inst[inst].material, uv, V);
+                        S, S.inst[inst].material, uv, V);
                     E = m.emission;
                     if (vc + 1 < p.bounces)
                     {
@@ -704,7 +715,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                 {
                     misses++;
                     v2 uv0{0.0f, 0.0f};
-                    MatOut m = eval_material<kDouble>(p.scene, p.scene.background,
+                    MatOut m = eval_material<kDouble>(S, S.background,
                                                       uv0, V);
                     E = m.emission;
                 }
@@ -897,6 +908,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
     }
 }
 
+template <bool kBatch>
 __global__ void k_write_pixels(TraceParams p, float *image)
 {
     for (uint32_t slot = blockIdx.x * 256 + threadIdx.x; slot < p.slots;
@@ -905,7 +917,8 @@ __global__ void k_write_pixels(TraceParams p, float *image)
         uint32_t pixel = p.slotPixel[slot];
         float4 c = make_float4(p.accum[slot], p.accum[p.slots + slot],
                                p.accum[2 * p.slots + slot], 1.0f);
-        reinterpret_cast<float4 *>(image)[pixel] = c;
+        float *dst = kBatch ? p.images[p.slotItem[slot]] : image;
+        reinterpret_cast<float4 *>(dst)[pixel] = c;
     }
 }
 
@@ -1009,24 +1022,43 @@ hipError_t launch_intersect_scene(hipStream_t s, const DevScene &scene,
 hipError_t launch_raygen(hipStream_t s, const TraceParams &p)
 {
     uint64_t n = (uint64_t)p.slots * p.sppPass;
-    hipLaunchKernelGGL(k_raygen, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s,
-                       p);
+    if (p.slotItem)
+        hipLaunchKernelGGL(k_raygen<true>, dim3(grid_for(n, 256, 8192)), dim3(256), 0,
+                           s, p);
+    else
+        hipLaunchKernelGGL(k_raygen<false>, dim3(grid_for(n, 256, 8192)), dim3(256), 0,
+                           s, p);
     return hipGetLastError();
 }
 
 hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
                          const Persist &ps)
 {
-    static uint32_t gCount = persistent_grid(k_extend<true>);
-    static uint32_t gPlain = persistent_grid(k_extend<false>);
+    static uint32_t gCount = persistent_grid(k_extend<true, false>);
+    static uint32_t gPlain = persistent_grid(k_extend<false, false>);
+    static uint32_t gCountB = persistent_grid(k_extend<true, true>);
+    static uint32_t gPlainB = persistent_grid(k_extend<false, true>);
     uint64_t n = (uint64_t)p.slots * p.sppPass;  // bound on the queue length
+    const bool batch = p.slotItem != nullptr;
+    uint32_t g = batch ? (p.collect ? gCountB : gPlainB) : (p.collect ? gCount : gPlain);
+    if (ps.maxBlocks && g > ps.maxBlocks) g = ps.maxBlocks;
+    dim3 grid(persist_blocks(g, n));
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, ps.counter, 0u);
-    if (p.collect)
-        hipLaunchKernelGGL(k_extend<true>, dim3(persist_blocks(gCount, n)),
-                           dim3(kTraceBlock), 0, s, p, q, ps.counter, ps.spill);
+    if (batch)
+    {
+        if (p.collect)
+            hipLaunchKernelGGL((k_extend<true, true>), grid, dim3(kTraceBlock), 0, s, p,
+                               q, ps.counter, ps.spill);
+        else
+            hipLaunchKernelGGL((k_extend<false, true>), grid, dim3(kTraceBlock), 0, s,
+                               p, q, ps.counter, ps.spill);
+    }
+    else if (p.collect)
+        hipLaunchKernelGGL((k_extend<true, false>), grid, dim3(kTraceBlock), 0, s, p, q,
+                           ps.counter, ps.spill);
     else
-        hipLaunchKernelGGL(k_extend<false>, dim3(persist_blocks(gPlain, n)),
-                           dim3(kTraceBlock), 0, s, p, q, ps.counter, ps.spill);
+        hipLaunchKernelGGL((k_extend<false, false>), grid, dim3(kTraceBlock), 0, s, p,
+                           q, ps.counter, ps.spill);
     return hipGetLastError();
 }
 
@@ -1036,7 +1068,22 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
     g = (g + 7) / 8 * 8;
     const bool dbl = p.tx == SPG_TX_DOUBLE;
 #define SPG_SHADE(D, E) \
-    hipLaunchKernelGGL((k_shade<D, E>), dim3(g), dim3(256), 0, s, p, q)
+    hipLaunchKernelGGL((k_shade<D, E, false>), dim3(g), dim3(256), 0, s, p, q)
+#define SPG_SHADE_B(D, E) \
+    hipLaunchKernelGGL((k_shade<D, E, true>), dim3(g), dim3(256), 0, s, p, q)
+    if (p.slotItem)
+    {
+        if (p.estimator == SPG_EST_PARITY)
+        {
+            if (dbl) SPG_SHADE_B(true, SPG_EST_PARITY);
+            else SPG_SHADE_B(false, SPG_EST_PARITY);
+        }
+        else if (dbl)
+            SPG_SHADE_B(true, SPG_EST_UNBIASED);
+        else
+            SPG_SHADE_B(false, SPG_EST_UNBIASED);
+        return hipGetLastError();
+    }
     switch (p.estimator)
     {
     case SPG_EST_PARITY:
@@ -1053,6 +1100,7 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
         break;
     }
 #undef SPG_SHADE
+#undef SPG_SHADE_B
     return hipGetLastError();
 }
 
@@ -1078,8 +1126,12 @@ hipError_t launch_finalize(hipStream_t s, const TraceParams &p)
 hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,
                                float *image)
 {
-    hipLaunchKernelGGL(k_write_pixels, dim3(grid_for(p.slots, 256, 4096)),
-                       dim3(256), 0, s, p, image);
+    if (p.slotItem)
+        hipLaunchKernelGGL(k_write_pixels<true>, dim3(grid_for(p.slots, 256, 4096)),
+                           dim3(256), 0, s, p, image);
+    else
+        hipLaunchKernelGGL(k_write_pixels<false>, dim3(grid_for(p.slots, 256, 4096)),
+                           dim3(256), 0, s, p, image);
     return hipGetLastError();
 }
 

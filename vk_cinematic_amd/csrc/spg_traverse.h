@@ -72,6 +72,21 @@ struct LaneStack
     }
 };
 
+// The instance level of one scene as a lane sees it (batched launches mix
+// scenes within a wave).
+struct SceneView
+{
+    const float4 *tlasNodes;
+    const uint32_t *tlasInst;
+    const DevInstance *inst;
+    const DevMesh *meshes;
+};
+
+__device__ __forceinline__ SceneView scene_view(const DevScene &S)
+{
+    return SceneView{S.tlasNodes, S.tlasInst, S.inst, S.meshes};
+}
+
 struct Trav
 {
     v3 o, d, inv;              // world ray
@@ -115,16 +130,16 @@ __device__ __forceinline__ void leaf_triangles(const float4 *tris, int32_t r,
 // Enter the instance of a TLAS leaf: o' = M^-1 o, d' = norm(M^-1 d)
 // (src/sp_scene.cpp:272-276); the mesh walk starts above the current stack.
 template <bool kCount>
-__device__ __forceinline__ void enter_instance(const DevScene &S, Trav &s,
+__device__ __forceinline__ void enter_instance(const SceneView &sv, Trav &s,
                                                int32_t ref, Counts &cnt)
 {
-    uint32_t ii = S.tlasInst[d_leaf_first(ref)];
-    const DevInstance &I = S.inst[ii];
+    uint32_t ii = sv.tlasInst[d_leaf_first(ref)];
+    const DevInstance &I = sv.inst[ii];
     s.lo = xform_point(I.inv, s.o);
     s.ld = normalize(xform_vector(I.inv, s.d));
     s.linv = inverse(s.ld);
     s.lim = s.out.t > 0.0f ? s.out.t * I.cullScale * kCullSlack : INFINITY;
-    const DevMesh &M = S.meshes[I.mesh];
+    const DevMesh &M = sv.meshes[I.mesh];
     s.bnodes = M.nodes;
     s.btris = M.tris;
     s.h = Hit{INFINITY, 0.0f, 0.0f, SPG_MISS};
@@ -138,10 +153,10 @@ __device__ __forceinline__ void enter_instance(const DevScene &S, Trav &s,
 // Leave the instance: world t = dot(M (o' + t' d') - o, d) and the closest
 // world hit wins (src/sp_scene.cpp:296-322); equal t goes to the lower
 // instance index.
-__device__ __forceinline__ void finish_instance(const DevScene &S, Trav &s)
+__device__ __forceinline__ void finish_instance(const SceneView &sv, Trav &s)
 {
     if (s.h.prim == SPG_MISS) return;
-    const DevInstance &I = S.inst[s.inst];
+    const DevInstance &I = sv.inst[s.inst];
     v3 lh = s.lo + s.ld * s.h.t;
     v3 wh = xform_point(I.model, lh);
     float t = dot(wh - s.o, s.d);
@@ -158,11 +173,11 @@ __device__ __forceinline__ void finish_instance(const DevScene &S, Trav &s)
 // One node visit.  Returns true when the ray's walk is complete.
 // kTwoLevel = false walks a single mesh (level stays 1, spBase 0).
 template <bool kCount, bool kTwoLevel>
-__device__ __forceinline__ bool trav_step(const DevScene &S, Trav &s,
+__device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
                                          LaneStack &st, Counts &cnt)
 {
     const bool blas = !kTwoLevel || s.level != 0;
-    const float4 *N = (blas ? s.bnodes : S.tlasNodes) + 4 * s.node;
+    const float4 *N = (blas ? s.bnodes : sv.tlasNodes) + 4 * s.node;
     v3 o = blas ? s.lo : s.o;
     v3 inv = blas ? s.linv : s.inv;
     float tc;
@@ -220,7 +235,7 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, Trav &s,
         {
             if (kTwoLevel && s.level == 0 && d_is_leaf(next))
             {
-                enter_instance<kCount>(S, s, next, cnt);
+                enter_instance<kCount>(sv, s, next, cnt);
                 return false;
             }
             s.node = next;
@@ -239,7 +254,7 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, Trav &s,
                 next = st.pop(s.sp);
                 continue;
             }
-            finish_instance(S, s);
+            finish_instance(sv, s);
             s.level = 0;
         }
         if (s.sp == 0) return true;
@@ -247,8 +262,7 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, Trav &s,
     }
 }
 
-__device__ __forceinline__ void trav_begin_scene(const DevScene &S, Trav &s,
-                                                 v3 o, v3 d)
+__device__ __forceinline__ void trav_begin_scene(Trav &s, v3 o, v3 d)
 {
     s.o = o;
     s.d = d;
@@ -283,7 +297,7 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
 // store(ray, s) writes the result.  Must be called by every lane of a
 // one-wave block.
 template <bool kCount, bool kTwoLevel, class Begin, class Store>
-__device__ __forceinline__ void persistent_walk(const DevScene &S, uint32_t n,
+__device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
                                                 uint32_t *counter, int *spill,
                                                 Begin begin, Store store,
                                                 Counts &cnt, uint32_t &rays)
@@ -339,7 +353,7 @@ __device__ __forceinline__ void persistent_walk(const DevScene &S, uint32_t n,
             if (exhausted && chunkNext >= chunkEnd) break;
             continue;
         }
-        if (ray != SPG_MISS && trav_step<kCount, kTwoLevel>(S, s, st, cnt))
+        if (ray != SPG_MISS && trav_step<kCount, kTwoLevel>(sv, s, st, cnt))
         {
             store(ray, s);
             ray = SPG_MISS;

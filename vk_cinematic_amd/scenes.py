@@ -281,3 +281,70 @@ def soup_mesh(tris):
     v = np.zeros((3 * n, 8), np.float32)
     v[:, 0:3] = tris.reshape(-1, 3)
     return v, np.arange(3 * n, dtype=np.uint32)
+
+
+@functools.lru_cache(maxsize=4)
+def _sky_cube(size):
+    return gen_sky_cubemap(size, SUN_DIR, SUN_RADIUS, SUN_RADIANCE)
+
+
+def grid_scene(seed, triangles, side, unique_meshes=4):
+    """One batched-workload scene: noise-displaced icospheres (subdivision
+    3, 4 or 5 by size) instanced on a jittered square grid over a ground
+    quad, about `triangles` triangles in all, Lambert materials (roughness
+    0) with per-instance albedo U[0.1, 0.9], the sky cube map with the sun,
+    a side x side image looking down on the grid."""
+    r = np.random.default_rng(seed)
+    level = 3 if triangles < 1e5 else (4 if triangles < 5e5 else 5)
+    per = 20 * 4 ** level
+    count = max(1, int(round(triangles / per)))
+    g = int(math.ceil(math.sqrt(count)))
+    meshes = [displaced_icosphere(level, seed * 7919 + j, amplitude=0.15) + (True,)
+              for j in range(unique_meshes)]
+    pv, pi = gen_plane()
+    meshes.append((pv, pi, True))
+    ground = len(meshes) - 1
+    spacing = 2.4
+    extent = g * spacing
+    materials = [(1, (0.5, 0.5, 0.5), (0, 0, 0), 0.0, SPG_NO_TEXTURE, SPG_NO_TEXTURE),
+                 (BACKGROUND_ID, (0, 0, 0), (0, 0, 0), 0.0, SPG_NO_TEXTURE,
+                  ENV_TEXTURE_ID)]
+    objects = [(ground, 1, (0.0, 0.0, 0.0),
+                tuple(quat_axis_angle((1.0, 0.0, 0.0), -math.pi / 2)),
+                (extent * 2.0, extent * 2.0, extent * 2.0))]
+    mid = 1000
+    for k in range(count):
+        gx, gz = k % g, k // g
+        x = (gx + 0.5) * spacing - extent / 2 + r.uniform(-0.2, 0.2)
+        z = (gz + 0.5) * spacing - extent / 2 + r.uniform(-0.2, 0.2)
+        s = float(r.uniform(0.7, 1.0))
+        q = quat_axis_angle(r.normal(size=3) / np.sqrt(3), r.uniform(0, np.pi))
+        q = (q / np.linalg.norm(q.astype(np.float64))).astype(np.float32)
+        a = float(r.uniform(0.1, 0.9))
+        materials.append((mid, (a, a, a), (0, 0, 0), 0.0, SPG_NO_TEXTURE,
+                          SPG_NO_TEXTURE))
+        objects.append((int(r.integers(0, unique_meshes)), mid, (float(x), s, float(z)),
+                        tuple(q), (s, s, s)))
+        mid += 1
+    dist = 0.75 * extent + 4.0
+    cam_q = quat_axis_angle((1.0, 0.0, 0.0), -0.6)
+    desc = {"meshes": meshes, "objects": objects, "materials": materials,
+            "background": BACKGROUND_ID,
+            "camera": (side, side, (0.0, 0.68 * dist, 0.82 * dist), tuple(cam_q), 0.8),
+            "cubemaps": [(ENV_TEXTURE_ID, _sky_cube(64), 64)], "textures": []}
+    desc["triangles"] = sum(len(meshes[o[0]][1]) // 3 for o in objects)
+    return desc
+
+
+def config3(count=64, seed=3, min_side=128, max_side=1024, min_tris=1e4,
+            max_tris=2e6):
+    """The batched workload: `count` seeded grid scenes, triangle counts
+    log-uniform in [min_tris, max_tris], square images with sides uniform
+    in [min_side, max_side]; rendered at 16 spp, 5 bounces."""
+    out = []
+    for k in range(count):
+        r = np.random.default_rng([seed, k])
+        tris = float(np.exp(r.uniform(np.log(min_tris), np.log(max_tris))))
+        side = int(r.integers(min_side, max_side + 1))
+        out.append(grid_scene(seed * 1000003 + k, tris, side))
+    return out
