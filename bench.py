@@ -310,6 +310,119 @@ def bench_box(args, V, scenes, name):
     return res
 
 
+def cpu_baseline_config3(descs, threads, budget_s=12.0):
+    """The oracle (same estimator, SAH hierarchy) on the batch's scenes in
+    order, 1 spp each, until about budget_s of rendering."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _libs import (SPO_EST_UNBIASED, SPO_HIT_REFERENCE, SPO_RNG_PCG,
+                       SPO_TX_LIBM, SpoConfig, oracle)
+    from _scenes import load_oracle
+
+    O = oracle()
+    cfg = SpoConfig(rng=SPO_RNG_PCG, tx=SPO_TX_LIBM, hit=SPO_HIT_REFERENCE,
+                    estimator=SPO_EST_UNBIASED, samplesPerPixel=1, bounces=5,
+                    radianceClamp=0.0, jitterYFirst=1, seed=0x5EED)
+    total_s, samples, used = 0.0, 0, 0
+    for d in descs:
+        s, cam = load_oracle(d, sah=True)
+        w, h = d["camera"][0], d["camera"][1]
+        px = np.zeros(w * h * 4, np.float32)
+        total_s += O.spo_render_threaded(s, C.byref(cam), C.byref(cfg), 64, 64, threads,
+                                         px, np.zeros(12, np.uint64))
+        O.spo_scene_destroy(s)
+        samples += w * h
+        used += 1
+        if total_s > budget_s:
+            break
+    return {"value": samples / total_s / 1e6, "unit": "Msamples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"the first {used} scenes of the set at 1 spp, 5 bounces, plain "
+                      f"unbiased tracer over the same SAH BVHs ({samples} samples, "
+                      f"{total_s:.1f} s)"}
+
+
+def bench_config3(args, V, scenes):
+    """The batched workload: 64 scenes rendered as one wavefront from a
+    captured HIP graph, 16 spp, 5 bounces; a step is one replay (every
+    scene's image)."""
+    dev = V.Device(0)
+    descs = scenes.config3()
+    t0 = time.perf_counter()
+    loaded = [scenes.load_gpu(dev, d) for d in descs]
+    build_s = time.perf_counter() - t0
+    imgs = [dev.alloc(d["camera"][0] * d["camera"][1] * 16) for d in descs]
+    items = [(cam, s, img) for (s, cam), img in zip(loaded, imgs)]
+    spp, bounces = 16, 5
+    t0 = time.perf_counter()
+    b = V.Batch(dev, items, spp=spp, bounces=bounces, clamp=0.0,
+                estimator=V.EST_UNBIASED, rng=V.RNG_PCG, seed=0x5EED,
+                transcendentals=V.TX_FAST, autotune=True)
+    create_s = time.perf_counter() - t0
+    tuning = b.tuning()
+    for _ in range(args.warmup):
+        b.run()
+    gpu_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m = b.run()
+        gpu_ms += b.last_ms()
+    wall = time.perf_counter() - t0
+    pixels = tuning["pixels"]
+    samples = pixels * spp
+    # per-kernel times of the same render (eager) and its test counts
+    tim, _ = b.profile(collect=False)
+    _, cnt = b.profile(collect=True)
+    rays = cnt["rays"]
+    bytes_render = (44 * rays + 24 * (cnt["slab_tests"] + cnt["tlas_slab_tests"]) +
+                    36 * cnt["tri_tests"] + 128 * cnt["mesh_tests"])
+    ext_ms, ext_n = tim["extend"]
+    per_launch = bytes_render / ext_n
+    mean_ms = ext_ms / ext_n
+    achieved = per_launch / (mean_ms * 1e-3) / 1e9
+    tris = [d["triangles"] for d in descs]
+    sides = [d["camera"][0] for d in descs]
+    res = {
+        "metric": METRIC,
+        "value": samples * args.steps / wall / 1e6,
+        "unit": "Msamples/s",
+        "ms_per_step": wall * 1e3 / args.steps,
+        "dtype": "f32",
+        "config": {"workload": f"config3: {len(descs)} seeded scenes of noise-displaced "
+                               f"icospheres on jittered instance grids over a ground "
+                               f"quad ({min(tris)}..{max(tris)} triangles), square "
+                               f"images {min(sides)}..{max(sides)} px ({pixels} pixels "
+                               f"in all), {spp} spp, {bounces} bounces, Lambert albedo "
+                               f"U[0.1,0.9], sky cube with sun; one wavefront over all "
+                               f"scenes replayed from a HIP graph, launch shape "
+                               f"autotuned",
+                   "scenes": len(descs), "pixels": pixels, "spp": spp,
+                   "bounces": bounces, "tuning": tuning},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_extend (closest-hit traversal, batch variant)",
+                     "launch_ms": mean_ms, "launches_per_step": ext_n,
+                     "algorithmic_bytes_per_launch": per_launch,
+                     "rays_per_step": rays,
+                     "slab_tests_per_ray": cnt["slab_tests"] / max(rays, 1),
+                     "tlas_slab_tests_per_ray": cnt["tlas_slab_tests"] / max(rays, 1),
+                     "tri_tests_per_ray": cnt["tri_tests"] / max(rays, 1),
+                     "mesh_entries_per_ray": cnt["mesh_tests"] / max(rays, 1)},
+        "extra": {"gpu_ms_per_step": gpu_ms / args.steps,
+                  "Mrays_per_s": m.values[V.METRIC_RAYS] * args.steps / wall / 1e6,
+                  "eager_kernel_ms": {k: v[0] for k, v in tim.items()},
+                  "scene_build_s": build_s, "batch_create_s": create_s},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_config3(descs, min(16, os.cpu_count() or 1))
+    b.close()
+    for img in imgs:
+        img.free()
+    for s, _ in loaded:
+        s.close()
+    dev.close()
+    return res
+
+
 # ---------------------------------------------------------------- relMSE
 def relmse(img, ref):
     """Relative MSE, mean over pixels and channels of (x - r)^2 / (r^2 +
@@ -501,7 +614,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="config5",
-                    choices=["config5", "config4", "config1", "soup"])
+                    choices=["config5", "config4", "config3", "config1", "soup"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fast", action="store_true",
                     help="device float transcendentals instead of double-evaluated")
@@ -535,7 +648,8 @@ def main():
     if args.workload in ("config5", "config4"):
         res = bench_box(args, V, scenes, args.workload)
     else:
-        res = {"config1": bench_config1, "soup": bench_soup}[args.workload](args, V, scenes)
+        res = {"config1": bench_config1, "config3": bench_config3,
+               "soup": bench_soup}[args.workload](args, V, scenes)
     out = {"metric": res.pop("metric"), "value": res.pop("value"),
            "unit": res.pop("unit"), "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),

@@ -304,6 +304,12 @@ int spg_batch_run(spg_Batch *b, spg_Metrics *metrics);
 int spg_batch_tuning(const spg_Batch *b, spg_BatchTuning *out);
 /* GPU time of the last spg_batch_run in ms (HIP events around the graph). */
 int spg_batch_last_ms(const spg_Batch *b, double *ms);
+/* The same render run eagerly (not from the graph) with every launch
+ * timed: ms6/launches6 per kernel class as spg_last_trace_timings ([5] =
+ * host wall time of the call), counters8 as spg_last_trace_counters
+ * (test counts when collect != 0).  Any pointer may be NULL. */
+int spg_batch_profile(spg_Batch *b, int collect, double *ms6, uint32_t *launches6,
+                      uint64_t *counters8);
 int spg_batch_destroy(spg_Batch *b);
 
 /* ---- learned path guiding (no reference counterpart: the reference

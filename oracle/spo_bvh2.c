@@ -51,6 +51,9 @@ typedef struct {
     atomic_ullong box, tri;
 } bvh2_job;
 
+/* The GPU's step order: a step is a node visit (near child next, far child
+ * pushed, whether leaf or inner) or one triangle of the current leaf; leaves
+ * are tested in full once reached, triangles in leaf order. */
 static void trace_one(const bvh2_job *j, uint32_t r, uint64_t *box,
                       uint64_t *tri)
 {
@@ -62,52 +65,55 @@ static void trace_one(const bvh2_job *j, uint32_t r, uint64_t *box,
     int32_t stack[128];
     int sp = 0;
     int32_t node = 0;
+    uint32_t triNext = 0, triEnd = 0;
     for (;;)
     {
-        const float *N = j->nodes + 16 * (size_t)node;
-        int32_t r0 = ref_of(N + 3), r1 = ref_of(N + 7);
-        float tc = bestT * CULL_SLACK;
-        float e0 = 0, e1 = 0;
-        int h0 = slab(N, N + 4, o, inv, &e0) && e0 <= tc;
-        int h1 = r1 != EMPTY_CHILD && slab(N + 8, N + 12, o, inv, &e1) && e1 <= tc;
-        *box += r1 != EMPTY_CHILD ? 2 : 1;
-        for (int c = 0; c < 2; ++c)
+        int32_t next = EMPTY_CHILD;
+        if (triNext < triEnd)
         {
-            int32_t rr = c == 0 ? r0 : r1;
-            int h = c == 0 ? h0 : h1;
-            if (h && rr < 0)
+            uint32_t p = j->order[triNext];
+            const float *tr = j->tris + 9 * (size_t)p;
+            float tt = spo_rt_mt(ro, ro + 3, tr, tr + 3, tr + 6, NULL);
+            if (tt > 0.0f && (tt < bestT || (tt == bestT && p < bestP)))
             {
-                uint32_t v = (uint32_t)~rr, first = v >> 3, count = (v & 7u) + 1u;
-                for (uint32_t k = 0; k < count; ++k)
-                {
-                    uint32_t p = j->order[first + k];
-                    const float *tr = j->tris + 9 * (size_t)p;
-                    float tt = spo_rt_mt(ro, ro + 3, tr, tr + 3, tr + 6, NULL);
-                    if (tt > 0.0f && (tt < bestT || (tt == bestT && p < bestP)))
-                    {
-                        bestT = tt;
-                        bestP = p;
-                    }
-                }
-                *tri += count;
-                if (c == 0) h0 = 0;
-                else h1 = 0;
+                bestT = tt;
+                bestP = p;
             }
+            if (++triNext < triEnd) continue;
         }
-        if (h0 && h1)
-        {
-            int32_t nearN = r0, farN = r1;
-            if (e1 < e0) { nearN = r1; farN = r0; }
-            stack[sp++] = farN;
-            node = nearN;
-        }
-        else if (h0) node = r0;
-        else if (h1) node = r1;
         else
         {
-            if (sp == 0) break;
-            node = stack[--sp];
+            const float *N = j->nodes + 16 * (size_t)node;
+            int32_t r0 = ref_of(N + 3), r1 = ref_of(N + 7);
+            float tc = bestT * CULL_SLACK;
+            float e0 = 0, e1 = 0;
+            int h0 = slab(N, N + 4, o, inv, &e0) && e0 <= tc;
+            int h1 = r1 != EMPTY_CHILD && slab(N + 8, N + 12, o, inv, &e1) && e1 <= tc;
+            *box += r1 != EMPTY_CHILD ? 2 : 1;
+            if (h0 && h1)
+            {
+                int32_t nearN = r0, farN = r1;
+                if (e1 < e0) { nearN = r1; farN = r0; }
+                stack[sp++] = farN;
+                next = nearN;
+            }
+            else if (h0) next = r0;
+            else if (h1) next = r1;
         }
+        if (next == EMPTY_CHILD)
+        {
+            if (sp == 0) break;
+            next = stack[--sp];
+        }
+        if (next < 0)
+        {
+            uint32_t v = (uint32_t)~next;
+            triNext = v >> 3;
+            triEnd = triNext + (v & 7u) + 1u;
+            *tri += (v & 7u) + 1u;
+        }
+        else
+            node = next;
     }
     j->prim[r] = bestP;
     j->t[r] = bestP == 0xFFFFFFFFu ? -1.0f : bestT;

@@ -97,8 +97,13 @@ def test_adam_step_matches_oracle(device):
     v = np.zeros(GUIDE_PARAMS, np.float32)
     oracle().spo_guide_adam(q, m, v, grad.astype(np.float64), GUIDE_PARAMS, lr, 0.9,
                             0.99, 1e-8, 1)
-    # the first step moves every parameter by lr * g / (|g| + eps)
-    assert np.max(np.abs(p1 - q)) < 2e-6
+    # the first step moves every parameter by lr * g / (|g| + eps); where g
+    # is at the level of the FP32 atomics' summation-order noise the two
+    # gradient evaluations may differ in sign
+    sel = np.abs(grad) > 1e-5 * np.abs(grad).max()
+    assert sel.mean() > 0.9
+    assert np.max(np.abs(p1 - q)[sel]) < 2e-6
+    assert np.max(np.abs(p1 - q)) <= 2 * lr * 1.001
     moved = np.abs(p1 - p)
     assert np.median(moved) == pytest.approx(lr, rel=0.05)
     # and against the oracle's own gradient: the same direction

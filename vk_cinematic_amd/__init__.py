@@ -176,6 +176,8 @@ _SIGS = {
     "spg_batch_tuning": (_int, [_vp, _vp]),
     "spg_batch_last_ms": (_int, [_vp, C.POINTER(C.c_double)]),
     "spg_batch_destroy": (_int, [_vp]),
+    "spg_batch_profile": (_int, [_vp, _int, C.POINTER(C.c_double), C.POINTER(_u32),
+                                 C.POINTER(C.c_uint64)]),
     "spg_guide_default_config": (None, [_vp]),
     "spg_guide_create": (_int, [_vp, _vp, C.POINTER(_vp)]),
     "spg_guide_destroy": (_int, [_vp]),
@@ -501,6 +503,17 @@ class Batch:
         ms = C.c_double()
         check(lib().spg_batch_last_ms(self.handle, C.byref(ms)))
         return ms.value
+
+    def profile(self, collect=False):
+        """Eager render with per-class launch timings and counters."""
+        ms = (C.c_double * 6)()
+        n = (C.c_uint32 * 6)()
+        c = (C.c_uint64 * 8)()
+        check(lib().spg_batch_profile(self.handle, int(bool(collect)), ms, n, c))
+        names = ["raygen", "extend", "shade", "finalize", "guide", "total"]
+        return ({k: (ms[i], n[i]) for i, k in enumerate(names)},
+                dict(zip(["rays", "hits", "misses", "slab_tests", "mesh_tests",
+                          "tri_tests", "tlas_slab_tests"], list(c))))
 
     def close(self):
         if self.handle:

@@ -56,12 +56,53 @@ int alloc_state(spg_Batch *b, uint32_t capacity)
     return SPG_OK;
 }
 
+// Per-class kernel timing of an eager (not captured) render: events around
+// each launch from a ring of pairs, drained when the ring fills.
+struct ClassTimer
+{
+    spg_Device *d;
+    double ms[6] = {0};
+    uint32_t launches[6] = {0};
+    int next = 0;
+    int cls[(kEvents - 4) / 2];
+    int drain()
+    {
+        if (next == 0) return SPG_OK;
+        HIPCHK(hipEventSynchronize(d->ev[next - 1]));
+        for (int k = 0; k < next; k += 2)
+        {
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, d->ev[k], d->ev[k + 1]));
+            ms[cls[k / 2]] += t;
+            launches[cls[k / 2]]++;
+        }
+        next = 0;
+        return SPG_OK;
+    }
+    int begin(hipStream_t st, int c)
+    {
+        if (next >= kEvents - 4) CHK(drain());
+        cls[next / 2] = c;
+        HIPCHK(hipEventRecord(d->ev[next], st));
+        return SPG_OK;
+    }
+    int end(hipStream_t st)
+    {
+        HIPCHK(hipEventRecord(d->ev[next + 1], st));
+        next += 2;
+        return SPG_OK;
+    }
+};
+
 // The whole render as stream work: slot chunks of at most the path
 // capacity, sample passes per chunk, bounces per pass.  No host copies or
-// memsets, so the same sequence can be captured in a graph.
+// memsets, so the same sequence can be captured in a graph.  tm (eager runs
+// only) times every launch by kernel class.
 int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
-           uint32_t *kernels)
+           uint32_t *kernels, ClassTimer *tm = nullptr)
 {
+    auto T0 = [&](int c) { return tm ? tm->begin(st, c) : SPG_OK; };
+    auto T1 = [&]() { return tm ? tm->end(st) : SPG_OK; };
     const uint32_t cap = b->paths.capacity;
     TraceParams p{};
     p.paths = b->paths;
@@ -99,18 +140,26 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
             p.sppPass = sp;
             p.sampleBase = cfg.sampleOffset + done;
             HIPCHK(launch_set_counter(st, &p.paths.counters[0], nslots * sp));
+            CHK(T0(kRaygen));
             HIPCHK(launch_raygen(st, p));
+            CHK(T1());
             k += 2;
             int q = 0;
             for (uint32_t bb = 0; bb < cfg.bounces; ++bb)
             {
+                CHK(T0(kExtend));
                 HIPCHK(launch_extend(st, p, q, ps));
+                CHK(T1());
                 HIPCHK(launch_set_counter(st, &p.paths.counters[q ^ 1], 0));
+                CHK(T0(kShade));
                 HIPCHK(launch_shade(st, p, q));
+                CHK(T1());
                 k += 4;  // the extend launch includes its counter reset
                 q ^= 1;
             }
+            CHK(T0(kFinal));
             HIPCHK(launch_finalize(st, p));
+            CHK(T1());
             ++k;
             done += sp;
         }
@@ -388,6 +437,37 @@ int spg_batch_last_ms(const spg_Batch *b, double *ms)
 {
     if (!b || !ms) return fail(SPG_E_INVALID, "spg_batch_last_ms: NULL");
     *ms = b->lastMs;
+    return SPG_OK;
+}
+
+int spg_batch_profile(spg_Batch *b, int collect, double *ms6, uint32_t *launches6,
+                      uint64_t *counters8)
+{
+    if (!b) return fail(SPG_E_INVALID, "spg_batch_profile: NULL");
+    spg_Device *d = b->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    spg_TraceConfig c = b->cfg;
+    c.collectMetrics = collect ? 1u : 0u;
+    ClassTimer tm{d};
+    auto t0 = std::chrono::steady_clock::now();
+    CHK(record(b, d->stream, c, nullptr, &tm));
+    CHK(tm.drain());
+    unsigned long long m[16] = {0};
+    HIPCHK(hipMemcpyAsync(m, b->paths.metrics, 128, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    tm.ms[5] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    for (int k = 0; k < 6; ++k)
+    {
+        if (ms6) ms6[k] = tm.ms[k];
+        if (launches6) launches6[k] = tm.launches[k];
+    }
+    if (counters8)
+    {
+        for (int k = 0; k < 7; ++k) counters8[k] = m[k];
+        counters8[7] = 0;
+    }
     return SPG_OK;
 }
 

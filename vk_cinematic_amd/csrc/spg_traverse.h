@@ -1,8 +1,12 @@
 // Closest-hit traversal as a per-lane step machine, run by persistent waves
 // that refill idle lanes with new rays.
 //
-// One loop iteration of a wave = one BVH2 node visit per active lane (both
-// child slab tests, leaf triangles tested on the spot).  A lane whose ray
+// One loop iteration of a wave = one step per active lane: either a BVH2
+// node visit (both child slab tests) or one triangle of the lane's current
+// leaf.  A hit leaf child becomes the lane's leaf range (the nearer child
+// first, a farther one waits on the stack like an inner node), so an
+// iteration costs one node visit plus one triangle test however the lanes
+// are mixed, instead of a node visit plus a whole leaf.  A lane whose ray
 // finishes takes the next ray from its wave's chunk of the ray list, so the
 // wave's SIMD lanes stay busy instead of idling behind the wave's longest
 // ray.  The two levels of a scene -- instance hierarchy (TLAS) and mesh
@@ -97,34 +101,29 @@ struct Trav
     SceneHit out;              // best hit in the scene
     int node, sp, spBase, level;
     uint32_t inst;
+    uint32_t triNext, triEnd;  // pending triangles of the current leaf
 };
 
-// Triangles of one leaf: Moller-Trumbore (src/ray_intersection.cpp:156-190),
-// keep t > 0 and smaller (src/sp_scene.cpp:189-193); equal t goes to the
-// lower triangle index so the result does not depend on the visit order.
-template <bool kCount>
-__device__ __forceinline__ void leaf_triangles(const float4 *tris, int32_t r,
-                                               v3 o, v3 d, Hit &best,
-                                               Counts &cnt)
+// One triangle (leaf order index k): Moller-Trumbore
+// (src/ray_intersection.cpp:156-190), keep t > 0 and smaller
+// (src/sp_scene.cpp:189-193); equal t goes to the lower triangle index so
+// the result does not depend on the visit order.
+__device__ __forceinline__ void test_triangle(const float4 *tris, uint32_t k, v3 o,
+                                              v3 d, Hit &best)
 {
-    uint32_t first = d_leaf_first(r), count = d_leaf_count(r);
-    for (uint32_t k = 0; k < count; ++k)
+    const float4 *T = tris + 3 * k;
+    float4 t0 = T[0], t1 = T[1], t2 = T[2];
+    v3 a{t0.x, t0.y, t0.z}, e1v{t0.w, t1.x, t1.y}, e2v{t1.z, t1.w, t2.x};
+    uint32_t prim = __float_as_uint(t2.y);
+    float u, v;
+    float t = mt_test(o, d, a, e1v, e2v, &u, &v);
+    if (t > 0.0f && (t < best.t || (t == best.t && prim < best.prim)))
     {
-        const float4 *T = tris + 3 * (first + k);
-        float4 t0 = T[0], t1 = T[1], t2 = T[2];
-        v3 a{t0.x, t0.y, t0.z}, e1v{t0.w, t1.x, t1.y}, e2v{t1.z, t1.w, t2.x};
-        uint32_t prim = __float_as_uint(t2.y);
-        float u, v;
-        float t = mt_test(o, d, a, e1v, e2v, &u, &v);
-        if (t > 0.0f && (t < best.t || (t == best.t && prim < best.prim)))
-        {
-            best.t = t;
-            best.u = u;
-            best.v = v;
-            best.prim = prim;
-        }
+        best.t = t;
+        best.u = u;
+        best.v = v;
+        best.prim = prim;
     }
-    if (kCount) cnt.tri += count;
 }
 
 // Enter the instance of a TLAS leaf: o' = M^-1 o, d' = norm(M^-1 d)
@@ -147,6 +146,7 @@ __device__ __forceinline__ void enter_instance(const SceneView &sv, Trav &s,
     s.spBase = s.sp;
     s.node = 0;
     s.inst = ii;
+    s.triNext = s.triEnd = 0;
     if (kCount) cnt.mesh += 1;
 }
 
@@ -170,72 +170,75 @@ __device__ __forceinline__ void finish_instance(const SceneView &sv, Trav &s)
     }
 }
 
-// One node visit.  Returns true when the ray's walk is complete.
-// kTwoLevel = false walks a single mesh (level stays 1, spBase 0).
+// One step (a node visit or one leaf triangle).  Returns true when the
+// ray's walk is complete.  kTwoLevel = false walks a single mesh (level
+// stays 1, spBase 0).
 template <bool kCount, bool kTwoLevel>
 __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
                                          LaneStack &st, Counts &cnt)
 {
-    const bool blas = !kTwoLevel || s.level != 0;
-    const float4 *N = (blas ? s.bnodes : sv.tlasNodes) + 4 * s.node;
-    v3 o = blas ? s.lo : s.o;
-    v3 inv = blas ? s.linv : s.inv;
-    float tc;
-    if (blas)
+    int32_t next = kNoRef;
+    if (s.triNext < s.triEnd)
     {
-        tc = s.h.t * kCullSlack;
-        tc = tc < s.lim ? tc : s.lim;
+        test_triangle(s.btris, s.triNext, s.lo, s.ld, s.h);
+        if (++s.triNext < s.triEnd) return false;
     }
     else
-        tc = s.out.t > 0.0f ? s.out.t * kCullSlack : INFINITY;
-    float4 A = N[0], B = N[1], C = N[2], D = N[3];
-    int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
-    float e0, e1;
-    bool h0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, inv, &e0) && e0 <= tc;
-    bool h1 = r1 != kEmptyChild &&
-              slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, inv, &e1) && e1 <= tc;
-    if (kCount)
     {
-        uint32_t n = r1 != kEmptyChild ? 2u : 1u;
-        if (blas) cnt.box += n;
-        else cnt.tlasBox += n;
-    }
-    if (blas)
-    {
-        if (h0 && d_is_leaf(r0))
+        const bool blas = !kTwoLevel || s.level != 0;
+        const float4 *N = (blas ? s.bnodes : sv.tlasNodes) + 4 * s.node;
+        v3 o = blas ? s.lo : s.o;
+        v3 inv = blas ? s.linv : s.inv;
+        float tc;
+        if (blas)
         {
-            leaf_triangles<kCount>(s.btris, r0, s.lo, s.ld, s.h, cnt);
-            h0 = false;
+            tc = s.h.t * kCullSlack;
+            tc = tc < s.lim ? tc : s.lim;
         }
-        if (h1 && d_is_leaf(r1))
+        else
+            tc = s.out.t > 0.0f ? s.out.t * kCullSlack : INFINITY;
+        float4 A = N[0], B = N[1], C = N[2], D = N[3];
+        int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
+        float e0, e1;
+        bool h0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, inv, &e0) && e0 <= tc;
+        bool h1 = r1 != kEmptyChild &&
+                  slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, inv, &e1) && e1 <= tc;
+        if (kCount)
         {
-            leaf_triangles<kCount>(s.btris, r1, s.lo, s.ld, s.h, cnt);
-            h1 = false;
+            uint32_t n = r1 != kEmptyChild ? 2u : 1u;
+            if (blas) cnt.box += n;
+            else cnt.tlasBox += n;
         }
-    }
-    int32_t next = kNoRef;
-    if (h0 && h1)
-    {
-        int32_t nearN = r0, farN = r1;
-        if (e1 < e0)
+        if (h0 && h1)
         {
-            nearN = r1;
-            farN = r0;
+            int32_t nearN = r0, farN = r1;
+            if (e1 < e0)
+            {
+                nearN = r1;
+                farN = r0;
+            }
+            st.push(s.sp, farN);
+            next = nearN;
         }
-        st.push(s.sp, farN);
-        next = nearN;
+        else if (h0)
+            next = r0;
+        else if (h1)
+            next = r1;
     }
-    else if (h0)
-        next = r0;
-    else if (h1)
-        next = r1;
     for (;;)
     {
         if (next != kNoRef)
         {
-            if (kTwoLevel && s.level == 0 && d_is_leaf(next))
+            if (d_is_leaf(next))
             {
-                enter_instance<kCount>(sv, s, next, cnt);
+                if (kTwoLevel && s.level == 0)
+                {
+                    enter_instance<kCount>(sv, s, next, cnt);
+                    return false;
+                }
+                s.triNext = d_leaf_first(next);
+                s.triEnd = s.triNext + d_leaf_count(next);
+                if (kCount) cnt.tri += d_leaf_count(next);
                 return false;
             }
             s.node = next;
@@ -273,6 +276,7 @@ __device__ __forceinline__ void trav_begin_scene(Trav &s, v3 o, v3 d)
     s.spBase = 0;
     s.node = 0;
     s.h = Hit{INFINITY, 0.0f, 0.0f, SPG_MISS};
+    s.triNext = s.triEnd = 0;
 }
 
 __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
@@ -289,6 +293,7 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
     s.sp = 0;
     s.spBase = 0;
     s.node = 0;
+    s.triNext = s.triEnd = 0;
 }
 
 // Persistent driver: every wave walks rays [0, n) taken in chunks of
