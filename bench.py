@@ -482,20 +482,29 @@ def make_reference(args, V, scenes):
     dev.close()
 
 
-def time_to_relmse(curve, target):
-    """First crossing of `target` on a (gpu_s, spp, relMSE) curve, log-log
-    interpolated between checkpoints; None if never reached."""
+def time_to_relmse(curve, target, floor=0.0):
+    """Crossing of `target` on a (gpu_s, spp, relMSE) curve: log-log
+    interpolated between checkpoints when reached ("measured"), otherwise
+    extrapolated from the last checkpoint with relMSE - floor falling as
+    1/spp and time growing with spp at the last segment's rate
+    ("extrapolated"); floor is the reference image's own error."""
     prev = None
     for t, n, e in curve:
         if e <= target:
             if prev is None:
-                return t, n
+                return t, n, "measured"
             t0, n0, e0 = prev
             f = (np.log(target) - np.log(e0)) / (np.log(e) - np.log(e0))
             return (float(np.exp(np.log(t0) + f * (np.log(t) - np.log(t0)))),
-                    float(np.exp(np.log(n0) + f * (np.log(n) - np.log(n0)))))
+                    float(np.exp(np.log(n0) + f * (np.log(n) - np.log(n0)))),
+                    "measured")
         prev = (t, n, e)
-    return None
+    if len(curve) < 2 or target <= floor:
+        return None
+    (t0, n0, _), (t1, n1, e1) = curve[-2], curve[-1]
+    need = n1 * (e1 - floor) / (target - floor)
+    rate = (t1 - t0) / (n1 - n0)
+    return t1 + (need - n1) * rate, need, "extrapolated"
 
 
 def bench_relmse(args, V, scenes):
@@ -541,11 +550,14 @@ def bench_relmse(args, V, scenes):
             a.add(64, V.EST_GUIDED, 0x6EED, a.spp, guide=g)
 
     guided = run("guided", guided_step)
-    tp = time_to_relmse(plain, target)
-    tg = time_to_relmse(guided, target)
+    tp = time_to_relmse(plain, target, args.ref_error)
+    tg = time_to_relmse(guided, target, args.ref_error)
     res.update({"plain_curve": plain, "guided_curve": guided,
+                "reference_error": args.ref_error,
                 "plain_time_s": tp[0] if tp else None, "plain_spp": tp[1] if tp else None,
+                "plain_how": tp[2] if tp else None,
                 "guided_time_s": tg[0] if tg else None, "guided_spp": tg[1] if tg else None,
+                "guided_how": tg[2] if tg else None,
                 "speedup": (tp[0] / tg[0]) if tp and tg else None,
                 "train_passes": args.train_passes,
                 "guide": {"lr": args.guide_lr, "batch": args.guide_batch,
@@ -627,6 +639,8 @@ def main():
                     help="time-to-relMSE 1e-3 of the plain and guided tracers "
                          "against this reference instead")
     ap.add_argument("--max-spp", type=int, default=8192)
+    ap.add_argument("--ref-error", type=float, default=2.0e-4,
+                    help="relMSE of the reference itself (half-vs-half estimate)")
     ap.add_argument("--train-passes", type=int, default=64)
     ap.add_argument("--guide-lr", type=float, default=3e-3)
     ap.add_argument("--guide-batch", type=int, default=16384)

@@ -240,6 +240,10 @@ int spo_material_register(spo_Scene *s, const float *albedo,
 int spo_texture_register(spo_Scene *s, const float *rgba, uint32_t w,
                          uint32_t h, uint32_t id, int isCube);
 void spo_set_background(spo_Scene *s, uint32_t id);
+/* GGX half-vector sampling and its density (unbiased estimator). */
+void spo_ggx_direction(const float *n, const float *V, float roughness, float u1,
+                       float u2, int tx, float *out3);
+float spo_ggx_pdf(const float *n, const float *V, const float *L, float roughness);
 void spo_evaluate_material(const spo_Scene *s, uint32_t id, const float *uv,
                            const float *outgoingDir, int tx, float *out7);
 void spo_compute_radiance_for_path(const spo_Scene *s, const float *path15,

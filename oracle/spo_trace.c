@@ -182,6 +182,58 @@ static spo_v3 cosine_direction(spo_v3 n, float u1, float u2, int tx)
     return v3add(v3add(v3mul(t1, lx), v3mul(t2, ly)), v3mul(n, lz));
 }
 
+/* Probability of the GGX lobe on rough materials (unbiased estimator). */
+#define SPEC_PROB 0.25f
+
+/* GGX half-vector sampling (Walter et al. 2007): cos^2 theta_h = (1 - u1) /
+ * (1 + (a^2 - 1) u1), a = roughness^2, phi = 2 pi u2, L = 2 (V.H) H - V. */
+static spo_v3 ggx_direction(spo_v3 n, spo_v3 V, float roughness, float u1,
+                            float u2, int tx)
+{
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float c2 = (1.0f - u1) / (1.0f + (a2 - 1.0f) * u1);
+    float ct = sqrtf(c2);
+    float st = sqrtf(spo_max(0.0f, 1.0f - c2));
+    float phi = 6.28318548f * u2;
+    float hx = st * spo_cos(phi, tx);
+    float hy = st * spo_sin(phi, tx);
+    float sign = copysignf(1.0f, n.z);
+    float aa = -1.0f / (sign + n.z);
+    float b = n.x * n.y * aa;
+    spo_v3 t1 = v3(1.0f + sign * n.x * n.x * aa, sign * b, -sign * n.x);
+    spo_v3 t2 = v3(b, sign + n.y * n.y * aa, -n.y);
+    spo_v3 H = v3add(v3add(v3mul(t1, hx), v3mul(t2, hy)), v3mul(n, ct));
+    float vh = v3dot(V, H);
+    return v3sub(v3mul(H, 2.0f * vh), V);
+}
+
+/* Density of ggx_direction: D(H) (N.H) / (4 V.H), H = normalize(L + V). */
+static float ggx_pdf(spo_v3 n, spo_v3 V, spo_v3 L, float roughness)
+{
+    spo_v3 H = v3norm(v3add(L, V));
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float NdotH = spo_max(v3dot(n, H), 0.0f);
+    float NdotH2 = NdotH * NdotH;
+    float denom = (NdotH2 * (a2 - 1.0f) + 1.0f);
+    denom = SPO_PI * denom * denom;
+    float D = a2 / denom;
+    float VdotH = v3dot(V, H);
+    return VdotH > 0.0f ? D * NdotH / (4.0f * VdotH) : 0.0f;
+}
+
+void spo_ggx_direction(const float *n, const float *V, float roughness, float u1,
+                       float u2, int tx, float *out3)
+{
+    v3put(out3, ggx_direction(v3p(n), v3p(V), roughness, u1, u2, tx));
+}
+
+float spo_ggx_pdf(const float *n, const float *V, const float *L, float roughness)
+{
+    return ggx_pdf(v3p(n), v3p(V), v3p(L), roughness);
+}
+
 static void rng_for_pixel(spo_Rng *rng, const spo_Config *cfg, uint32_t width,
                           uint32_t x, uint32_t y, uint32_t sample,
                           uint32_t *tileState)
@@ -238,12 +290,24 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                 float mat[7];
                 spo_material_eval_internal(s, h.materialId, h.uv, V, tx, mat);
                 L = v3add(L, v3had(beta, v3p(mat + 3)));
+                /* cosine lobe, or GGX with probability ps on rough
+                 * materials; weight X cos / pdf (X pi for ps = 0) */
+                float rough = mat[6];
+                float ps = rough > 0.0f ? SPEC_PROB : 0.0f;
+                float u0 = ps > 0.0f ? rng_u24(rng) : 1.0f;
                 float u1 = rng_u24(rng), u2 = rng_u24(rng);
-                spo_v3 wi = cosine_direction(h.normal, u1, u2, tx);
+                spo_v3 wi = u0 < ps ? ggx_direction(h.normal, V, rough, u1, u2, tx)
+                                    : cosine_direction(h.normal, u1, u2, tx);
                 float cosine = v3dot(h.normal, wi);
                 if (!(cosine > 0.0f)) break;
                 spo_v3 X = brdf_bracket(h.normal, V, wi, mat, tx, 1);
-                beta = v3had(beta, v3mul(X, SPO_PI));
+                float w = SPO_PI;
+                if (ps > 0.0f)
+                {
+                    float pc = (1.0f - ps) * cosine * (1.0f / SPO_PI);
+                    w = cosine / (pc + ps * ggx_pdf(h.normal, V, wi, rough));
+                }
+                beta = v3had(beta, v3mul(X, w));
                 spo_v3 p = v3add(o, v3mul(d, h.t));
                 o = v3add(p, v3mul(h.normal, 0.0001f));
                 d = wi;

@@ -524,6 +524,59 @@ __device__ __forceinline__ v3 cosine_direction(v3 n, float u1, float u2)
     return (t1 * lx + t2 * ly) + n * lz;
 }
 
+// Probability of sampling the GGX lobe instead of the cosine lobe for a
+// material with roughness > 0 (one-sample MIS of the two).
+constexpr float kSpecProb = 0.25f;
+
+// GGX half-vector sampling (Walter et al. 2007, "Microfacet Models for
+// Refraction through Rough Surfaces"): cos^2 theta_h = (1 - u1) /
+// (1 + (a^2 - 1) u1) with a = roughness^2 (the reference's DistributionGGX
+// parameter), phi = 2 pi u2, L = 2 (V.H) H - V.
+template <bool kDouble>
+__device__ __forceinline__ v3 ggx_direction(v3 n, v3 V, float roughness, float u1,
+                                            float u2)
+{
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float c2 = (1.0f - u1) / (1.0f + (a2 - 1.0f) * u1);
+    float ct = sqrtf(c2);
+    float st = sqrtf(fmax_sel(0.0f, 1.0f - c2));
+    float phi = 6.28318548f * u2;
+    float hx = st * Tx<kDouble>::cos_(phi);
+    float hy = st * Tx<kDouble>::sin_(phi);
+    float sign = copysignf(1.0f, n.z);
+    float aa = -1.0f / (sign + n.z);
+    float b = n.x * n.y * aa;
+    v3 t1{1.0f + sign * n.x * n.x * aa, sign * b, -sign * n.x};
+    v3 t2{b, sign + n.y * n.y * aa, -n.y};
+    v3 H = (t1 * hx + t2 * hy) + n * ct;
+    float vh = dot(V, H);
+    return H * (2.0f * vh) - V;
+}
+
+// Density of ggx_direction: D(H) (N.H) / (4 V.H), H = normalize(L + V).
+__device__ __forceinline__ float ggx_pdf(v3 n, v3 V, v3 L, float roughness)
+{
+    v3 H = normalize(L + V);
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float NdotH = fmax_sel(dot(n, H), 0.0f);
+    float NdotH2 = NdotH * NdotH;
+    float denom = (NdotH2 * (a2 - 1.0f) + 1.0f);
+    denom = kPi * denom * denom;
+    float D = a2 / denom;
+    float VdotH = dot(V, H);
+    return VdotH > 0.0f ? D * NdotH / (4.0f * VdotH) : 0.0f;
+}
+
+// The BSDF sampling density: (1 - ps) cos/pi + ps ggx.
+__device__ __forceinline__ float bsdf_pdf(v3 n, v3 V, v3 L, float cosv, float roughness,
+                                          float ps)
+{
+    float pc = (1.0f - ps) * cosv * (1.0f / kPi);
+    return ps > 0.0f ? pc + ps * ggx_pdf(n, V, L, roughness) : pc;
+}
+
 // One bounce of every queued path: surface, material, scatter, vertex record
 // (parity) or throughput update (unbiased), append continuing rays to queue
 // q^1 with a ballot + one atomic per wave.
@@ -624,13 +677,21 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     MatOut m = eval_material<kDouble>(
                         S, S.inst[inst].material, uv, V);
                     L = L + had(beta, m.emission);
+                    // cosine lobe, or GGX with probability ps on rough
+                    // materials; weight X cos / pdf (X pi for ps = 0)
+                    const float ps = m.roughness > 0.0f ? kSpecProb : 0.0f;
+                    float u0 = ps > 0.0f ? rng.u24() : 1.0f;
                     float u1 = rng.u24(), u2 = rng.u24();
-                    v3 wi = cosine_direction<kDouble>(N, u1, u2);
+                    v3 wi = u0 < ps ? ggx_direction<kDouble>(N, V, m.roughness, u1, u2)
+                                    : cosine_direction<kDouble>(N, u1, u2);
                     float cosine = dot(N, wi);
                     if (cosine > 0.0f && vc + 1 < p.bounces)
                     {
                         v3 X = brdf_bracket<kDouble>(N, V, wi, m, true);
-                        beta = had(beta, X * kPi);
+                        float w = ps > 0.0f
+                                      ? cosine / bsdf_pdf(N, V, wi, cosine, m.roughness, ps)
+                                      : kPi;
+                        beta = had(beta, X * w);
                         v3 P = o + d * t;
                         push = true;
                         no = P + N * 0.0001f;
@@ -676,12 +737,19 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                         const LobeRef Lg{p.guide.lobes, cap, i};
                         float u0 = rng.u24(), u1 = rng.u24(), u2 = rng.u24();
                         const float fb = p.guide.bsdfFraction;
-                        v3 wi = u0 < fb ? cosine_direction<kDouble>(N, u1, u2)
-                                        : mixture_sample(Lg, (u0 - fb) / (1.0f - fb),
-                                                         u1, u2);
+                        const float ps = m.roughness > 0.0f ? kSpecProb : 0.0f;
+                        v3 wi;
+                        if (u0 < fb)
+                            wi = u0 < fb * ps
+                                     ? ggx_direction<kDouble>(N, V, m.roughness, u1, u2)
+                                     : cosine_direction<kDouble>(N, u1, u2);
+                        else
+                            wi = mixture_sample(Lg, (u0 - fb) / (1.0f - fb), u1, u2);
                         float cosv = dot(N, wi);
-                        float pdf = fb * cosv * (1.0f / kPi) +
-                                    (1.0f - fb) * mixture_pdf(Lg, wi);
+                        float pdf = cosv > 0.0f
+                                        ? fb * bsdf_pdf(N, V, wi, cosv, m.roughness, ps) +
+                                              (1.0f - fb) * mixture_pdf(Lg, wi)
+                                        : 0.0f;
                         if (cosv > 0.0f && pdf > 0.0f)
                         {
                             X = brdf_bracket<kDouble>(N, V, wi, m, true);
