@@ -212,6 +212,7 @@ def bench_box(args, V, scenes, name):
         V.path_trace_tiles_device(ctx_for(k), tiles, img)
     dev.synchronize()
     ext_ms, ext_n, sh_ms, tot_ms = 0.0, 0, 0.0, 0.0
+    pcls = {"raygen": 0.0, "finalize": 0.0}
     t0 = time.perf_counter()
     for k in range(args.steps):
         V.path_trace_tiles_device(ctx_for(args.warmup + k), tiles, img)
@@ -220,6 +221,8 @@ def bench_box(args, V, scenes, name):
         ext_n += tim["extend"][1]
         sh_ms += tim["shade"][0]
         tot_ms += tim["total"][0]
+        for key in pcls:
+            pcls[key] += tim[key][0]
     dev.synchronize()
     wall = time.perf_counter() - t0
     m = V.path_trace_tiles_device(ctx_for(args.warmup, collect=True), tiles, img)
@@ -260,6 +263,8 @@ def bench_box(args, V, scenes, name):
         "extra": {"Mrays_per_s": rays * args.steps / wall / 1e6,
                   "extend_ms_per_step": ext_ms / args.steps,
                   "shade_ms_per_step": sh_ms / args.steps,
+                  "raygen_ms_per_step": pcls["raygen"] / args.steps,
+                  "finalize_ms_per_step": pcls["finalize"] / args.steps,
                   "gpu_ms_per_step": tot_ms / args.steps,
                   "scene_build_s": build_s,
                   "paths_per_step": m.values[V.METRIC_PATHS]},
@@ -287,9 +292,14 @@ def bench_box(args, V, scenes, name):
     dev.synchronize()
     gwall = time.perf_counter() - t0
     # inference only (frozen guide)
+    fcls = {"raygen": 0.0, "extend": 0.0, "shade": 0.0, "finalize": 0.0, "guide": 0.0,
+            "total": 0.0}
     t0 = time.perf_counter()
     for k in range(args.steps):
         V.path_trace_tiles_device(gctx(args.warmup + k, train=False), tiles, img)
+        tim = dev.last_timings()
+        for key in fcls:
+            fcls[key] += tim[key][0]
     dev.synchronize()
     fwall = time.perf_counter() - t0
     res["extra"]["guided"] = {
@@ -298,6 +308,7 @@ def bench_box(args, V, scenes, name):
         "ms_per_step_training": gwall * 1e3 / args.steps,
         "ms_per_step_frozen": fwall * 1e3 / args.steps,
         "kernel_ms_per_step": {k: v / args.steps for k, v in cls.items()},
+        "frozen_kernel_ms_per_step": {k: v / args.steps for k, v in fcls.items()},
         "training_steps_per_pass": 4, "batch": 16384,
         "collected_last_pass": g.stats()["collected"]}
     g.close()
