@@ -37,12 +37,14 @@ __device__ __forceinline__ f4 mfma16x16x32(h8 a, h8 b, f4 c)
 }
 
 // LDS exchange between lanes of one wave: every LDS access issued before
-// this point has completed (s_waitcnt with all counters 0) and the compiler
-// may not move memory accesses across it.
+// this point has completed (s_waitcnt lgkmcnt(0); outstanding global loads
+// and stores are not waited for) and the compiler may not move memory
+// accesses across it.
+constexpr int kWaitLgkm0 = 0xC07F;  // vmcnt 63, expcnt 7, lgkmcnt 0
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
