@@ -34,6 +34,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s at 1 GPU; time-to-relMSE 1e-3, guided vs plain GPU tracer"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per traced ray of the closest-hit kernel, measured with rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE per dispatch (FETCH scaled x2.0, WRITE
+# x1.0 by a 1 GiB copy kernel on the same box); the roofline's "traffic" is
+# this times the rays of one launch.
+TRAFFIC_BYTES_PER_RAY = {"config5": 1.778e9 / 2.04e6, "soup": 1.685e9 / 4194304}
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same kernel and "
+                  "workload (profiles/hbm_traffic_config5_soup.md), FETCH x2.0 and "
+                  "WRITE x1.0 by a 1 GiB copy calibration")
 
 
 def log(*a):
@@ -254,7 +262,10 @@ def bench_box(args, V, scenes, name):
                    "triangles": desc["triangles"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_extend (closest-hit traversal)",
+                     "traffic": (TRAFFIC_BYTES_PER_RAY["config5"] * rays / launches
+                                 if name == "config5" else None),
+                     "traffic_source": TRAFFIC_SOURCE if name == "config5" else None,
+                     "kernel": "k_extend (closest-hit traversal)",
                      "launch_ms": mean_ms, "algorithmic_bytes_per_launch": per_launch,
                      "rays_per_step": rays, "slab_tests_per_ray": box / max(rays, 1),
                      "tri_tests_per_ray": tri / max(rays, 1),
@@ -617,7 +628,8 @@ def bench_soup(args, V, scenes):
                                "edge lognormal(-5, 0.5)), SAH BVH2"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_closest_mesh",
+                     "traffic": TRAFFIC_BYTES_PER_RAY["soup"] * n,
+                     "traffic_source": TRAFFIC_SOURCE, "kernel": "k_closest_mesh",
                      "launch_ms": ms.value, "algorithmic_bytes_per_launch": bytes_launch,
                      "slab_tests_per_ray": st.boxTests / n,
                      "tri_tests_per_ray": st.triTests / n},
