@@ -276,7 +276,6 @@ static void free_scene_device(spg_Scene *s)
         free_ptr(m.dIdx);
     }
     free_ptr(s->dTlasNodes);
-    free_ptr(s->dTlasInst);
     free_ptr(s->dInst);
     free_ptr(s->dMeshes);
     free_ptr(s->dMats);
@@ -314,6 +313,12 @@ int spg_mesh_create(spg_Scene *s, const float *verts8, uint32_t nv,
     for (uint32_t i = 0; i < ni; ++i)
         if (idx[i] >= nv)
             return fail(SPG_E_INVALID, "spg_mesh_create: index out of range");
+    // finite positions keep every BVH box finite, which the traversal's
+    // min/max slab test relies on
+    for (uint32_t i = 0; i < nv; ++i)
+        if (!std::isfinite(verts8[8 * i]) || !std::isfinite(verts8[8 * i + 1]) ||
+            !std::isfinite(verts8[8 * i + 2]))
+            return fail(SPG_E_INVALID, "spg_mesh_create: non-finite vertex position");
     std::lock_guard<std::mutex> g(s->dev->lock);
     s->meshes.emplace_back();
     HostMesh &m = s->meshes.back();
@@ -341,6 +346,10 @@ int spg_scene_add_object(spg_Scene *s, uint32_t meshId, uint32_t materialId,
 {
     if (!s || !pos || !quat || !scale)
         return fail(SPG_E_INVALID, "spg_scene_add_object: NULL argument");
+    for (int k = 0; k < 4; ++k)
+        if ((k < 3 && (!std::isfinite(pos[k]) || !std::isfinite(scale[k]))) ||
+            !std::isfinite(quat[k]))
+            return fail(SPG_E_INVALID, "spg_scene_add_object: non-finite transform");
     std::lock_guard<std::mutex> g(s->dev->lock);
     if (meshId >= s->meshes.size())
         return fail(SPG_E_NOT_FOUND, "spg_scene_add_object: unknown mesh");
@@ -475,6 +484,8 @@ static int upload_tables(spg_Scene *s)
         d.mesh = o.mesh;
         d.material = find_index(s->materialKeys, o.materialId);
         d.materialId = o.materialId;
+        d.nodes = (const float4 *)s->meshes[o.mesh].dNodes;
+        d.tris = (const float4 *)s->meshes[o.mesh].dTris;
         inst.push_back(d);
     }
     CHK(upload(&s->dTexs, texs.data(), texs.size()));
@@ -535,12 +546,22 @@ int spg_scene_build(spg_Scene *s, const spg_BuildOpts *opts)
         build_bvh(in, s->tlas);
         if (s->tlas.maxDepth > kMaxTlasDepth)
             return fail(SPG_E_CAPACITY, "instance BVH too deep");
-        CHK(upload(&s->dTlasNodes, s->tlas.nodes.data(), s->tlas.nodes.size()));
-        CHK(upload(&s->dTlasInst, s->tlas.prims.data(), s->tlas.prims.size()));
+        // one instance per leaf: the device copy's leaf references name the
+        // instance itself instead of its position in the leaf order
+        std::vector<GpuNode> nodes = s->tlas.nodes;
+        for (GpuNode &nd : nodes)
+            for (float *slot : {&nd.a[3], &nd.b[3]})
+            {
+                int32_t r;
+                memcpy(&r, slot, 4);
+                if (r == kEmptyChild || !ref_is_leaf(r)) continue;
+                r = leaf_ref(s->tlas.prims[leaf_first(r)], 1);
+                memcpy(slot, &r, 4);
+            }
+        CHK(upload(&s->dTlasNodes, nodes.data(), nodes.size()));
     }
     DevScene &ds = s->dev_scene;
     ds.tlasNodes = (const float4 *)s->dTlasNodes;
-    ds.tlasInst = (const uint32_t *)s->dTlasInst;
     ds.meshes = (const DevMesh *)s->dMeshes;
     ds.instCount = no;
     CHK(upload_tables(s));

@@ -58,6 +58,8 @@ struct DevInstance
     uint32_t mesh;
     int32_t material; // dense material slot, -1 = not registered
     uint32_t materialId;
+    const float4 *nodes; // the mesh's BVH and triangles (copied from its
+    const float4 *tris;  // DevMesh so entering an instance is one lookup)
 };
 
 struct DevMaterial
@@ -80,8 +82,7 @@ struct DevTexture
 
 struct DevScene
 {
-    const float4 *tlasNodes;
-    const uint32_t *tlasInst;   // TLAS leaf order -> instance index
+    const float4 *tlasNodes;    // leaf references hold the instance index
     const DevInstance *inst;
     const DevMesh *meshes;
     const DevMaterial *mats;
@@ -272,7 +273,7 @@ struct spg_Scene
     bool dirty = true;
     spg::BvhBuildResult tlas;
     // device-side tables
-    void *dTlasNodes = nullptr, *dTlasInst = nullptr, *dInst = nullptr,
+    void *dTlasNodes = nullptr, *dInst = nullptr,
          *dMeshes = nullptr, *dMats = nullptr, *dTexs = nullptr,
          *dTexels = nullptr;
     spg::DevScene dev_scene{};

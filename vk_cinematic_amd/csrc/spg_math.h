@@ -105,6 +105,31 @@ SPG_HD bool slab_hit(float lox, float loy, float loz, float hix, float hiy,
     return entry <= exit_;
 }
 
+// The same test with IEEE min/max (v_min3/v_max3 on the GPU instead of a
+// compare and a select per operation).  Equal to slab_hit whenever no slab
+// distance is NaN: min/max and compare-and-select differ only on a NaN
+// operand (and on the sign of a zero, which no comparison below sees).  A
+// distance is NaN only from 0 * inf or a non-finite origin, so callers use
+// this form when every component of o and invD is finite (boxes are
+// finite: non-finite vertices and transforms are rejected at input).
+SPG_HD bool slab_hit_mm(float lox, float loy, float loz, float hix, float hiy,
+                        float hiz, v3 o, v3 invD, float *entryOut)
+{
+    float t0x = (lox - o.x) * invD.x, t1x = (hix - o.x) * invD.x;
+    float t0y = (loy - o.y) * invD.y, t1y = (hiy - o.y) * invD.y;
+    float t0z = (loz - o.z) * invD.z, t1z = (hiz - o.z) * invD.z;
+    float entry = fmaxf(fmaxf(0.0f, fminf(t0x, t1x)),
+                        fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
+    float exit_ = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    *entryOut = entry;
+    return entry <= exit_;
+}
+
+SPG_HD bool finite3(v3 a)
+{
+    return isfinite(a.x) && isfinite(a.y) && isfinite(a.z);
+}
+
 // ---- Moller-Trumbore, src/ray_intersection.cpp:156-190, with e1 = b - a
 // and e2 = c - a supplied (precomputed with the same subtraction).
 // Returns t (or -1 on a miss) and u, v.

@@ -26,7 +26,31 @@ namespace spg {
 
 constexpr int kShortStack = 16;
 constexpr uint32_t kRayChunk = 64;
+// A wave refills its idle lanes once this many wait (or all do): each
+// refill stalls the wave on the new rays' loads, so refilling one lane per
+// iteration would add a memory round trip to nearly every step.
+#ifndef SPG_REFILL_MIN
+#define SPG_REFILL_MIN 8
+#endif
+constexpr uint32_t kRefillMin = SPG_REFILL_MIN;
 constexpr int32_t kNoRef = kEmptyChild;
+
+// Node, triangle and stack-spill arrays are reached through pointers read
+// from memory (mesh and scene tables), so the compiler cannot prove them
+// global and would emit flat loads, which also hold the LDS wait counter
+// and serialize with the LDS stack.  These loads name the global address
+// space explicitly (a native vector type, so no HIP vector constructor
+// takes the address back to the generic space).
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) f4v gf4v;
+typedef __attribute__((address_space(1))) int gint;
+typedef __attribute__((address_space(3))) int lint;
+
+__device__ __forceinline__ float4 ld_g4(const float4 *p)
+{
+    f4v v = *(gf4v *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 
 __device__ __forceinline__ bool d_is_leaf(int32_t r) { return r < 0; }
 __device__ __forceinline__ uint32_t d_leaf_first(int32_t r)
@@ -57,22 +81,22 @@ struct Counts
 
 struct LaneStack
 {
-    int *lds;        // this wave's LDS block, entry i of lane l at lds[i*64 + l]
-    int *spill;      // entry i >= kShortStack at spill[(i-kShortStack)*stride + gl]
+    lint *lds;       // this wave's LDS block, entry i of lane l at lds[i*64 + l]
+    gint *spill;     // entry i >= kShortStack at spill[(i-kShortStack)*stride + gl]
     uint32_t stride; // lanes in the grid
     uint32_t gl;     // this lane's index in the grid
     uint32_t lane;
-    __device__ __forceinline__ void push(int &sp, int v)
+    __device__ __forceinline__ void write(int i, int v)
     {
-        if (sp < kShortStack) lds[sp * 64 + lane] = v;
-        else spill[(size_t)(sp - kShortStack) * stride + gl] = v;
-        ++sp;
+        if (i < kShortStack) lds[i * 64 + lane] = v;
+        else spill[(size_t)(i - kShortStack) * stride + gl] = v;
     }
-    __device__ __forceinline__ int pop(int &sp)
+    __device__ __forceinline__ int read(int i)
     {
-        --sp;
-        return sp < kShortStack ? lds[sp * 64 + lane]
-                                : spill[(size_t)(sp - kShortStack) * stride + gl];
+        int v;
+        if (i < kShortStack) v = lds[i * 64 + lane];
+        else v = spill[(size_t)(i - kShortStack) * stride + gl];
+        return v;
     }
 };
 
@@ -81,14 +105,12 @@ struct LaneStack
 struct SceneView
 {
     const float4 *tlasNodes;
-    const uint32_t *tlasInst;
     const DevInstance *inst;
-    const DevMesh *meshes;
 };
 
 __device__ __forceinline__ SceneView scene_view(const DevScene &S)
 {
-    return SceneView{S.tlasNodes, S.tlasInst, S.inst, S.meshes};
+    return SceneView{S.tlasNodes, S.inst};
 }
 
 struct Trav
@@ -102,88 +124,49 @@ struct Trav
     int node, sp, spBase, level;
     uint32_t inst;
     uint32_t triNext, triEnd;  // pending triangles of the current leaf
+    bool mmW, mmL;             // world / mesh-space ray is finite: slab_hit_mm
 };
 
 // One triangle (leaf order index k): Moller-Trumbore
-// (src/ray_intersection.cpp:156-190), keep t > 0 and smaller
-// (src/sp_scene.cpp:189-193); equal t goes to the lower triangle index so
-// the result does not depend on the visit order.
-__device__ __forceinline__ void test_triangle(const float4 *tris, uint32_t k, v3 o,
-                                              v3 d, Hit &best)
+// (src/ray_intersection.cpp:156-190).  Returns whether the hit c beats
+// `best`: t > 0 and smaller (src/sp_scene.cpp:189-193); equal t goes to the
+// lower triangle index so the result does not depend on the visit order.
+__device__ __forceinline__ bool triangle_candidate(const float4 *tris, uint32_t k,
+                                                   v3 o, v3 d, const Hit &best,
+                                                   Hit &c)
 {
     const float4 *T = tris + 3 * k;
-    float4 t0 = T[0], t1 = T[1], t2 = T[2];
+    float4 t0 = ld_g4(T), t1 = ld_g4(T + 1), t2 = ld_g4(T + 2);
     v3 a{t0.x, t0.y, t0.z}, e1v{t0.w, t1.x, t1.y}, e2v{t1.z, t1.w, t2.x};
-    uint32_t prim = __float_as_uint(t2.y);
-    float u, v;
-    float t = mt_test(o, d, a, e1v, e2v, &u, &v);
-    if (t > 0.0f && (t < best.t || (t == best.t && prim < best.prim)))
-    {
-        best.t = t;
-        best.u = u;
-        best.v = v;
-        best.prim = prim;
-    }
+    c.prim = __float_as_uint(t2.y);
+    c.u = c.v = 0.0f;
+    c.t = mt_test(o, d, a, e1v, e2v, &c.u, &c.v);
+    return c.t > 0.0f && (c.t < best.t || (c.t == best.t && c.prim < best.prim));
 }
 
-// Enter the instance of a TLAS leaf: o' = M^-1 o, d' = norm(M^-1 d)
-// (src/sp_scene.cpp:272-276); the mesh walk starts above the current stack.
-template <bool kCount>
-__device__ __forceinline__ void enter_instance(const SceneView &sv, Trav &s,
-                                               int32_t ref, Counts &cnt)
-{
-    uint32_t ii = sv.tlasInst[d_leaf_first(ref)];
-    const DevInstance &I = sv.inst[ii];
-    s.lo = xform_point(I.inv, s.o);
-    s.ld = normalize(xform_vector(I.inv, s.d));
-    s.linv = inverse(s.ld);
-    s.lim = s.out.t > 0.0f ? s.out.t * I.cullScale * kCullSlack : INFINITY;
-    const DevMesh &M = sv.meshes[I.mesh];
-    s.bnodes = M.nodes;
-    s.btris = M.tris;
-    s.h = Hit{INFINITY, 0.0f, 0.0f, SPG_MISS};
-    s.level = 1;
-    s.spBase = s.sp;
-    s.node = 0;
-    s.inst = ii;
-    s.triNext = s.triEnd = 0;
-    if (kCount) cnt.mesh += 1;
-}
-
-// Leave the instance: world t = dot(M (o' + t' d') - o, d) and the closest
-// world hit wins (src/sp_scene.cpp:296-322); equal t goes to the lower
-// instance index.
-__device__ __forceinline__ void finish_instance(const SceneView &sv, Trav &s)
-{
-    if (s.h.prim == SPG_MISS) return;
-    const DevInstance &I = sv.inst[s.inst];
-    v3 lh = s.lo + s.ld * s.h.t;
-    v3 wh = xform_point(I.model, lh);
-    float t = dot(wh - s.o, s.d);
-    if (t < s.out.t || s.out.t < 0.0f || (t == s.out.t && s.inst < s.out.inst))
-    {
-        s.out.t = t;
-        s.out.u = s.h.u;
-        s.out.v = s.h.v;
-        s.out.inst = s.inst;
-        s.out.prim = s.h.prim;
-    }
-}
-
-// One step (a node visit or one leaf triangle).  Returns true when the
-// ray's walk is complete.  kTwoLevel = false walks a single mesh (level
-// stays 1, spBase 0).
+// One step (a node visit or one leaf triangle) of an active lane; an
+// inactive lane's state is left as it is.  Returns true when the ray's walk
+// is complete.  kTwoLevel = false walks a single mesh (level stays 1,
+// spBase 0).
+//
+// Loads and arithmetic sit in branches; the lane state is written through
+// selects after them, and the step runs on every lane with `active` in its
+// predicates.  Written inside divergent branches, every state value needs a
+// second register at each join and the compiler copies the whole state
+// around every branch of the step.
 template <bool kCount, bool kTwoLevel>
 __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
-                                         LaneStack &st, Counts &cnt)
+                                         LaneStack &st, Counts &cnt, bool active)
 {
-    int32_t next = kNoRef;
-    if (s.triNext < s.triEnd)
-    {
-        test_triangle(s.btris, s.triNext, s.lo, s.ld, s.h);
-        if (++s.triNext < s.triEnd) return false;
-    }
-    else
+    // (1) the step's work: one triangle or one node visit
+    const bool isTri = active && s.triNext < s.triEnd;
+    const bool isNode = active && !(s.triNext < s.triEnd);
+    Hit c{0.0f, 0.0f, 0.0f, 0u};
+    bool take = false;
+    int32_t next = kNoRef, far = kNoRef;
+    if (isTri)
+        take = triangle_candidate(s.btris, s.triNext, s.lo, s.ld, s.h, c);
+    else if (isNode)
     {
         const bool blas = !kTwoLevel || s.level != 0;
         const float4 *N = (blas ? s.bnodes : sv.tlasNodes) + 4 * s.node;
@@ -197,72 +180,130 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
         }
         else
             tc = s.out.t > 0.0f ? s.out.t * kCullSlack : INFINITY;
-        float4 A = N[0], B = N[1], C = N[2], D = N[3];
+        // The whole 64-byte node in one round trip: both boxes are tested
+        // unconditionally (an empty second slot holds an unused box), so no
+        // load waits behind the first box's child reference.
+        float4 A = ld_g4(N), B = ld_g4(N + 1), C = ld_g4(N + 2), D = ld_g4(N + 3);
         int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
         float e0, e1;
-        bool h0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, inv, &e0) && e0 <= tc;
-        bool h1 = r1 != kEmptyChild &&
-                  slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, inv, &e1) && e1 <= tc;
+        bool s0 = slab_hit_mm(A.x, A.y, A.z, B.x, B.y, B.z, o, inv, &e0);
+        bool s1 = slab_hit_mm(C.x, C.y, C.z, D.x, D.y, D.z, o, inv, &e1);
+        if (!(blas ? s.mmL : s.mmW))
+        {
+            // a non-finite origin or inverse direction (an axis-parallel
+            // ray): the reference's compare-and-select form
+            float f0, f1;
+            bool q0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, inv, &f0);
+            bool q1 = slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, inv, &f1);
+            s0 = q0;
+            s1 = q1;
+            e0 = f0;
+            e1 = f1;
+        }
+        bool h0 = s0 & (e0 <= tc);
+        bool h1 = (r1 != kEmptyChild) & s1 & (e1 <= tc);
         if (kCount)
         {
             uint32_t n = r1 != kEmptyChild ? 2u : 1u;
             if (blas) cnt.box += n;
             else cnt.tlasBox += n;
         }
-        if (h0 && h1)
-        {
-            int32_t nearN = r0, farN = r1;
-            if (e1 < e0)
-            {
-                nearN = r1;
-                farN = r0;
-            }
-            st.push(s.sp, farN);
-            next = nearN;
-        }
-        else if (h0)
-            next = r0;
-        else if (h1)
-            next = r1;
+        // near child first, the far one waits on the stack
+        const bool both = h0 & h1, swap = e1 < e0;
+        far = both ? (swap ? r0 : r1) : kNoRef;
+        next = both ? (swap ? r1 : r0) : h0 ? r0 : h1 ? r1 : kNoRef;
     }
-    for (;;)
+    s.h.t = take ? c.t : s.h.t;
+    s.h.u = take ? c.u : s.h.u;
+    s.h.v = take ? c.v : s.h.v;
+    s.h.prim = take ? c.prim : s.h.prim;
+    s.triNext += isTri ? 1u : 0u;
+    if (far != kNoRef) st.write(s.sp, far);
+    s.sp += far != kNoRef ? 1 : 0;
+    const bool busy = isTri && s.triNext < s.triEnd;
+
+    // (2) nothing to descend into: leave a finished instance, then pop.  A
+    // popped reference is never empty, so one pop gives the next step work.
+    const bool needPop = active && !busy && next == kNoRef;
+    if (kTwoLevel)
     {
-        if (next != kNoRef)
+        const bool fin = needPop && s.level != 0 && s.sp <= s.spBase;
+        // world t = dot(M (o' + t' d') - o, d) and the closest world hit
+        // wins (src/sp_scene.cpp:296-322); equal t goes to the lower
+        // instance index
+        bool better = false;
+        float tw = 0.0f;
+        if (fin && s.h.prim != SPG_MISS)
         {
-            if (d_is_leaf(next))
-            {
-                if (kTwoLevel && s.level == 0)
-                {
-                    enter_instance<kCount>(sv, s, next, cnt);
-                    return false;
-                }
-                s.triNext = d_leaf_first(next);
-                s.triEnd = s.triNext + d_leaf_count(next);
-                if (kCount) cnt.tri += d_leaf_count(next);
-                return false;
-            }
-            s.node = next;
-            return false;
+            const DevInstance &I = sv.inst[s.inst];
+            v3 lh = s.lo + s.ld * s.h.t;
+            v3 wh = xform_point(I.model, lh);
+            tw = dot(wh - s.o, s.d);
+            better = tw < s.out.t || s.out.t < 0.0f ||
+                     (tw == s.out.t && s.inst < s.out.inst);
         }
-        if (!kTwoLevel)
-        {
-            if (s.sp == 0) return true;
-            next = st.pop(s.sp);
-            continue;
-        }
-        if (s.level != 0)
-        {
-            if (s.sp > s.spBase)
-            {
-                next = st.pop(s.sp);
-                continue;
-            }
-            finish_instance(sv, s);
-            s.level = 0;
-        }
-        if (s.sp == 0) return true;
-        next = st.pop(s.sp);
+        s.out.t = better ? tw : s.out.t;
+        s.out.u = better ? s.h.u : s.out.u;
+        s.out.v = better ? s.h.v : s.out.v;
+        s.out.inst = better ? s.inst : s.out.inst;
+        s.out.prim = better ? s.h.prim : s.out.prim;
+        s.level = fin ? 0 : s.level;
     }
+    const bool done = needPop && s.sp == 0;
+    const bool doPop = needPop && s.sp != 0;
+    int32_t popped = kNoRef;
+    if (doPop) popped = st.read(s.sp - 1);
+    s.sp -= doPop ? 1 : 0;
+    next = doPop ? popped : next;
+
+    // (3) take the next reference: an inner node, a leaf's triangle range,
+    // or (instance level) an instance to enter
+    const bool leaf = d_is_leaf(next);
+    const bool enter = kTwoLevel && leaf && s.level == 0;
+    const bool range = leaf && !enter;
+    if (kTwoLevel)
+    {
+        // o' = M^-1 o, d' = norm(M^-1 d) (src/sp_scene.cpp:272-276); the
+        // mesh walk starts above the current stack
+        v3 nlo{0.0f, 0.0f, 0.0f}, nld{0.0f, 0.0f, 0.0f}, nlinv{0.0f, 0.0f, 0.0f};
+        float nlim = 0.0f;
+        bool nmm = false;
+        const float4 *nb = nullptr, *nt = nullptr;
+        const uint32_t ii = d_leaf_first(next);
+        if (enter)
+        {
+            const DevInstance &I = sv.inst[ii];
+            nlo = xform_point(I.inv, s.o);
+            nld = normalize(xform_vector(I.inv, s.d));
+            nlinv = inverse(nld);
+            nmm = finite3(nlo) && finite3(nlinv);
+            nlim = s.out.t > 0.0f ? s.out.t * I.cullScale * kCullSlack : INFINITY;
+            nb = I.nodes;
+            nt = I.tris;
+        }
+        s.lo = v3{enter ? nlo.x : s.lo.x, enter ? nlo.y : s.lo.y,
+                  enter ? nlo.z : s.lo.z};
+        s.ld = v3{enter ? nld.x : s.ld.x, enter ? nld.y : s.ld.y,
+                  enter ? nld.z : s.ld.z};
+        s.linv = v3{enter ? nlinv.x : s.linv.x, enter ? nlinv.y : s.linv.y,
+                    enter ? nlinv.z : s.linv.z};
+        s.lim = enter ? nlim : s.lim;
+        s.mmL = enter ? nmm : s.mmL;
+        s.bnodes = enter ? nb : s.bnodes;
+        s.btris = enter ? nt : s.btris;
+        s.h.t = enter ? INFINITY : s.h.t;
+        s.h.prim = enter ? SPG_MISS : s.h.prim;
+        s.level = enter ? 1 : s.level;
+        s.spBase = enter ? s.sp : s.spBase;
+        s.inst = enter ? ii : s.inst;
+        if (kCount) cnt.mesh += enter ? 1u : 0u;
+    }
+    const uint32_t first = d_leaf_first(next), count = d_leaf_count(next);
+    s.triNext = range ? first : s.triNext;
+    s.triEnd = range ? first + count : s.triEnd;
+    if (kCount) cnt.tri += range ? count : 0u;
+    s.node = enter ? 0 : (!leaf && next != kNoRef) ? next : s.node;
+    return done;
 }
 
 __device__ __forceinline__ void trav_begin_scene(Trav &s, v3 o, v3 d)
@@ -270,6 +311,8 @@ __device__ __forceinline__ void trav_begin_scene(Trav &s, v3 o, v3 d)
     s.o = o;
     s.d = d;
     s.inv = inverse(d);
+    s.mmW = finite3(o) && finite3(s.inv);
+    s.mmL = false;
     s.out = SceneHit{-1.0f, 0.0f, 0.0f, SPG_MISS, SPG_MISS};
     s.level = 0;
     s.sp = 0;
@@ -285,6 +328,8 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
     s.lo = o;
     s.ld = d;
     s.linv = inverse(d);
+    s.mmL = finite3(o) && finite3(s.linv);
+    s.mmW = false;
     s.bnodes = M.nodes;
     s.btris = M.tris;
     s.lim = INFINITY;
@@ -309,7 +354,8 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
 {
     __shared__ int lds[kShortStack * 64];
     const uint32_t lane = threadIdx.x;
-    LaneStack st{lds, spill, gridDim.x * 64u, blockIdx.x * 64u + lane, lane};
+    LaneStack st{(lint *)lds, (gint *)spill, gridDim.x * 64u, blockIdx.x * 64u + lane,
+                 lane};
     const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t chunkNext = 0, chunkEnd = 0;
     bool exhausted = false;
@@ -319,13 +365,18 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
     {
         bool need = ray == SPG_MISS;
         unsigned long long needMask = __ballot(need);
-        if (needMask && !(exhausted && chunkNext >= chunkEnd))
+        const uint32_t idle = (uint32_t)__popcll(needMask);
+        if ((idle >= kRefillMin || idle == 64u) && idle != 0u &&
+            !(exhausted && chunkNext >= chunkEnd))
         {
             if (chunkNext >= chunkEnd)
             {
                 uint32_t base = 0;
                 if (lane == 0) base = atomicAdd(counter, kRayChunk);
-                base = __shfl(base, 0);
+                // wave-uniform (a scalar register), so the loop's exit is
+                // uniform and the compiler keeps no per-lane copies of the
+                // lane state across it
+                base = __builtin_amdgcn_readfirstlane(base);
                 if (base >= n)
                 {
                     exhausted = true;
@@ -338,7 +389,7 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
                     if (base + kRayChunk >= n) exhausted = true;
                 }
             }
-            uint32_t want = (uint32_t)__popcll(needMask);
+            uint32_t want = idle;
             uint32_t avail = chunkEnd - chunkNext;
             uint32_t rank = (uint32_t)__popcll(needMask & below);
             if (need && rank < avail)
@@ -353,12 +404,12 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
             }
             chunkNext += want < avail ? want : avail;
         }
-        if (__ballot(ray != SPG_MISS) == 0ull)
-        {
-            if (exhausted && chunkNext >= chunkEnd) break;
-            continue;
-        }
-        if (ray != SPG_MISS && trav_step<kCount, kTwoLevel>(sv, s, st, cnt))
+        // one path round the loop: with no active lane the step's branches
+        // are all skipped, and the lane state needs no copies for a second
+        // path back to the loop head
+        if (__ballot(ray != SPG_MISS) == 0ull && exhausted && chunkNext >= chunkEnd)
+            break;
+        if (trav_step<kCount, kTwoLevel>(sv, s, st, cnt, ray != SPG_MISS))
         {
             store(ray, s);
             ray = SPG_MISS;
