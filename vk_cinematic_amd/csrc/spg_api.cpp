@@ -271,7 +271,7 @@ static void free_scene_device(spg_Scene *s)
     for (auto &m : s->meshes)
     {
         free_ptr(m.dNodes);
-        free_ptr(m.dTris);
+        m.dTris = nullptr;
         free_ptr(m.dVerts);
         free_ptr(m.dIdx);
     }
@@ -411,8 +411,13 @@ static int build_mesh(HostMesh &m, const spg_BuildOpts &opt)
         return fail(SPG_E_INVALID, "mesh BVH build failed");
     if (m.bvh.maxDepth > kMaxBlasDepth)
         return fail(SPG_E_CAPACITY, "mesh BVH deeper than the traversal stack");
-    // triangles in leaf order: (a, e1 = b - a, e2 = c - a, prim)
-    std::vector<float4> tris(3 * (size_t)tc);
+    // nodes, then the triangles in leaf order (a, e1 = b - a, e2 = c - a,
+    // prim), then one float4 of padding: the traversal fetches 64 bytes for
+    // every work item, a triangle's 48 and 16 it does not use
+    const size_t nn = m.bvh.nodes.size();
+    std::vector<float4> blob(4 * nn + 3 * (size_t)tc + 1, make_float4(0, 0, 0, 0));
+    memcpy(blob.data(), m.bvh.nodes.data(), nn * sizeof(GpuNode));
+    float4 *tris = blob.data() + 4 * nn;
     for (uint32_t k = 0; k < tc; ++k)
     {
         uint32_t prim = m.bvh.prims[k];
@@ -427,8 +432,9 @@ static int build_mesh(HostMesh &m, const spg_BuildOpts &opt)
         tris[3 * k + 1] = make_float4(e1[1], e1[2], e2[0], e2[1]);
         tris[3 * k + 2] = make_float4(e2[2], primBits, 0.0f, 0.0f);
     }
-    CHK(upload(&m.dNodes, m.bvh.nodes.data(), m.bvh.nodes.size()));
-    CHK(upload(&m.dTris, tris.data(), tris.size()));
+    CHK(upload(&m.dNodes, blob.data(), blob.size()));
+    m.dTris = (float4 *)m.dNodes + 4 * nn;
+    m.nodeCount = (uint32_t)nn;
     CHK(upload(&m.dVerts, m.verts.data(), m.verts.size()));
     CHK(upload(&m.dIdx, m.idx.data(), m.idx.size()));
     m.built = true;
@@ -485,7 +491,7 @@ static int upload_tables(spg_Scene *s)
         d.material = find_index(s->materialKeys, o.materialId);
         d.materialId = o.materialId;
         d.nodes = (const float4 *)s->meshes[o.mesh].dNodes;
-        d.tris = (const float4 *)s->meshes[o.mesh].dTris;
+        d.nodeCount = s->meshes[o.mesh].nodeCount;
         inst.push_back(d);
     }
     CHK(upload(&s->dTexs, texs.data(), texs.size()));

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include <mutex>
@@ -50,17 +51,24 @@ struct DevMesh
     uint32_t triCount;
 };
 
+// 128 bytes.  The traversal reads it in 64-byte pieces: bytes 0..63 when
+// leaving the instance (model rows), bytes 48..111 when entering it (inverse
+// rows, cull scale, the mesh's BVH).
 struct DevInstance
 {
-    m34 model;        // rows 0..2 of M = T*R*S       (src/sp_scene.cpp:91)
-    m34 inv;          // rows 0..2 of S^-1*R(q*)*T(-p) (src/sp_scene.cpp:94-96)
-    float cullScale;  // 1 / min |scale|: world t bound -> local t bound
+    m34 model;          // rows 0..2 of M = T*R*S       (src/sp_scene.cpp:91)
+    m34 inv;            // rows 0..2 of S^-1*R(q*)*T(-p) (src/sp_scene.cpp:94-96)
+    float cullScale;    // 1 / min |scale|: world t bound -> local t bound
+    uint32_t nodeCount; // the mesh's triangles follow its nodes:
+    const float4 *nodes;//   tris = nodes + 4 * nodeCount
     uint32_t mesh;
-    int32_t material; // dense material slot, -1 = not registered
+    int32_t material;   // dense material slot, -1 = not registered
     uint32_t materialId;
-    const float4 *nodes; // the mesh's BVH and triangles (copied from its
-    const float4 *tris;  // DevMesh so entering an instance is one lookup)
+    uint32_t pad;
 };
+static_assert(sizeof(DevInstance) == 128, "DevInstance layout");
+static_assert(offsetof(DevInstance, cullScale) == 96, "DevInstance layout");
+static_assert(offsetof(DevInstance, nodes) == 104, "DevInstance layout");
 
 struct DevMaterial
 {
@@ -242,7 +250,10 @@ struct HostMesh
     spg::BvhBuildResult bvh;
     bool built = false;
     // device copies
+    // one allocation: nodes, triangles in leaf order, one float4 of padding
+    // (dTris points into it)
     void *dNodes = nullptr, *dTris = nullptr, *dVerts = nullptr, *dIdx = nullptr;
+    uint32_t nodeCount = 0;
     float boundsLo[3], boundsHi[3];
 };
 

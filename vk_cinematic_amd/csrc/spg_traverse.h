@@ -1,14 +1,17 @@
 // Closest-hit traversal as a per-lane step machine, run by persistent waves
 // that refill idle lanes with new rays.
 //
-// One loop iteration of a wave = one step per active lane: either a BVH2
-// node visit (both child slab tests) or one triangle of the lane's current
-// leaf.  A hit leaf child becomes the lane's leaf range (the nearer child
-// first, a farther one waits on the stack like an inner node), so an
-// iteration costs one node visit plus one triangle test however the lanes
-// are mixed, instead of a node visit plus a whole leaf.  A lane whose ray
-// finishes takes the next ray from its wave's chunk of the ray list, so the
-// wave's SIMD lanes stay busy instead of idling behind the wave's longest
+// One loop iteration of a wave = one step per active lane, and every step
+// is one 64-byte fetch plus the arithmetic on it: a BVH2 node visit (both
+// child slab tests), one triangle of the lane's current leaf, entering an
+// instance (its inverse transform and BVH) or leaving one after a hit (its
+// model transform, for the world-space t).  All lanes issue their fetch
+// together at the top of the step, so a wave whose lanes hold different
+// kinds of work waits for one memory round trip per iteration, not one per
+// kind.  A hit leaf child becomes the lane's leaf range (the nearer child
+// first, a farther one waits on the stack like an inner node).  A lane whose
+// ray finishes takes the next ray from its wave's chunk of the ray list, so
+// the wave's SIMD lanes stay busy instead of idling behind the wave's longest
 // ray.  The two levels of a scene -- instance hierarchy (TLAS) and mesh
 // hierarchy (BLAS) -- are walked in the same loop: entering an instance
 // transforms the ray into mesh space (sp_RayIntersectScene,
@@ -33,6 +36,14 @@ constexpr uint32_t kRayChunk = 64;
 #define SPG_REFILL_MIN 8
 #endif
 constexpr uint32_t kRefillMin = SPG_REFILL_MIN;
+// Instance entries and exits (about one step in ten) wait until this many
+// lanes of the wave hold one (or no lane has other work): a wave executes
+// the code of every kind of step some lane holds, and these two cost more
+// than a node visit and a triangle together.
+#ifndef SPG_RARE_MIN
+#define SPG_RARE_MIN 1
+#endif
+constexpr uint32_t kRareMin = SPG_RARE_MIN;
 constexpr int32_t kNoRef = kEmptyChild;
 
 // Node, triangle and stack-spill arrays are reached through pointers read
@@ -124,19 +135,25 @@ struct Trav
     int node, sp, spBase, level;
     uint32_t inst;
     uint32_t triNext, triEnd;  // pending triangles of the current leaf
+    uint32_t mode;             // the next step's work (kMode*)
     bool mmW, mmL;             // world / mesh-space ray is finite: slab_hit_mm
 };
 
-// One triangle (leaf order index k): Moller-Trumbore
-// (src/ray_intersection.cpp:156-190).  Returns whether the hit c beats
-// `best`: t > 0 and smaller (src/sp_scene.cpp:189-193); equal t goes to the
-// lower triangle index so the result does not depend on the visit order.
-__device__ __forceinline__ bool triangle_candidate(const float4 *tris, uint32_t k,
+// Kinds of step.
+constexpr uint32_t kModeNode = 0;    // visit node `node` of the current level
+constexpr uint32_t kModeTri = 1;     // test triangle triNext of the leaf range
+constexpr uint32_t kModeEnter = 2;   // enter instance `inst`
+constexpr uint32_t kModeFinish = 3;  // leave instance `inst` with its hit
+
+// Moller-Trumbore on one fetched triangle (a.xyz, e1.x) (e1.yz, e2.xy)
+// (e2.z, prim) (src/ray_intersection.cpp:156-190).  Returns whether the hit
+// c beats `best`: t > 0 and smaller (src/sp_scene.cpp:189-193); equal t
+// goes to the lower triangle index so the result does not depend on the
+// visit order.
+__device__ __forceinline__ bool triangle_candidate(float4 t0, float4 t1, float4 t2,
                                                    v3 o, v3 d, const Hit &best,
                                                    Hit &c)
 {
-    const float4 *T = tris + 3 * k;
-    float4 t0 = ld_g4(T), t1 = ld_g4(T + 1), t2 = ld_g4(T + 2);
     v3 a{t0.x, t0.y, t0.z}, e1v{t0.w, t1.x, t1.y}, e2v{t1.z, t1.w, t2.x};
     c.prim = __float_as_uint(t2.y);
     c.u = c.v = 0.0f;
@@ -144,10 +161,14 @@ __device__ __forceinline__ bool triangle_candidate(const float4 *tris, uint32_t 
     return c.t > 0.0f && (c.t < best.t || (c.t == best.t && c.prim < best.prim));
 }
 
-// One step (a node visit or one leaf triangle) of an active lane; an
-// inactive lane's state is left as it is.  Returns true when the ray's walk
-// is complete.  kTwoLevel = false walks a single mesh (level stays 1,
-// spBase 0).
+__device__ __forceinline__ m34 rows3(float4 a, float4 b, float4 c)
+{
+    return m34{{{a.x, a.y, a.z, a.w}, {b.x, b.y, b.z, b.w}, {c.x, c.y, c.z, c.w}}};
+}
+
+// One step of an active lane (an inactive lane's state is left as it is).
+// Returns true when the ray's walk is complete.  kTwoLevel = false walks a
+// single mesh (level stays 1, spBase 0; only node and triangle steps).
 //
 // Loads and arithmetic sit in branches; the lane state is written through
 // selects after them, and the step runs on every lane with `active` in its
@@ -158,18 +179,40 @@ template <bool kCount, bool kTwoLevel>
 __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
                                          LaneStack &st, Counts &cnt, bool active)
 {
-    // (1) the step's work: one triangle or one node visit
-    const bool isTri = active && s.triNext < s.triEnd;
-    const bool isNode = active && !(s.triNext < s.triEnd);
+    const uint32_t mode = s.mode;
+    const bool blas = !kTwoLevel || s.level != 0;
+
+    // (1) the step's 64-byte fetch: node, triangle (+16 unused bytes; the
+    // triangle array is padded), instance bytes 48..111 (enter) or 0..63
+    // (leave)
+    const float4 *src;
+    if (mode == kModeTri) src = s.btris + 3 * s.triNext;
+    else if (!kTwoLevel || mode == kModeNode)
+        src = (blas ? s.bnodes : sv.tlasNodes) + 4 * s.node;
+    else
+        src = reinterpret_cast<const float4 *>(sv.inst + s.inst) +
+              (mode == kModeEnter ? 3 : 0);
+    float4 A, B, C, D;  // read only by active lanes
+    if (active)
+    {
+        A = ld_g4(src);
+        B = ld_g4(src + 1);
+        C = ld_g4(src + 2);
+        D = ld_g4(src + 3);
+    }
+
+    // (2) the work on it
+    const bool isTri = active && mode == kModeTri;
+    const bool isNode = active && mode == kModeNode;
+    const bool isEnter = kTwoLevel && active && mode == kModeEnter;
+    const bool isFin = kTwoLevel && active && mode == kModeFinish;
     Hit c{0.0f, 0.0f, 0.0f, 0u};
     bool take = false;
     int32_t next = kNoRef, far = kNoRef;
     if (isTri)
-        take = triangle_candidate(s.btris, s.triNext, s.lo, s.ld, s.h, c);
+        take = triangle_candidate(A, B, C, s.lo, s.ld, s.h, c);
     else if (isNode)
     {
-        const bool blas = !kTwoLevel || s.level != 0;
-        const float4 *N = (blas ? s.bnodes : sv.tlasNodes) + 4 * s.node;
         v3 o = blas ? s.lo : s.o;
         v3 inv = blas ? s.linv : s.inv;
         float tc;
@@ -180,10 +223,8 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
         }
         else
             tc = s.out.t > 0.0f ? s.out.t * kCullSlack : INFINITY;
-        // The whole 64-byte node in one round trip: both boxes are tested
-        // unconditionally (an empty second slot holds an unused box), so no
-        // load waits behind the first box's child reference.
-        float4 A = ld_g4(N), B = ld_g4(N + 1), C = ld_g4(N + 2), D = ld_g4(N + 3);
+        // both boxes are tested unconditionally (an empty second slot holds
+        // an unused box)
         int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
         float e0, e1;
         bool s0 = slab_hit_mm(A.x, A.y, A.z, B.x, B.y, B.z, o, inv, &e0);
@@ -222,22 +263,50 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
     s.sp += far != kNoRef ? 1 : 0;
     const bool busy = isTri && s.triNext < s.triEnd;
 
-    // (2) nothing to descend into: leave a finished instance, then pop.  A
-    // popped reference is never empty, so one pop gives the next step work.
-    const bool needPop = active && !busy && next == kNoRef;
     if (kTwoLevel)
     {
-        const bool fin = needPop && s.level != 0 && s.sp <= s.spBase;
-        // world t = dot(M (o' + t' d') - o, d) and the closest world hit
-        // wins (src/sp_scene.cpp:296-322); equal t goes to the lower
-        // instance index
+        // entering: o' = M^-1 o, d' = norm(M^-1 d) (src/sp_scene.cpp:272-276);
+        // the mesh walk starts at its root above the current stack
+        v3 nlo{0.0f, 0.0f, 0.0f}, nld{0.0f, 0.0f, 0.0f}, nlinv{0.0f, 0.0f, 0.0f};
+        float nlim = 0.0f;
+        bool nmm = false;
+        const float4 *nb = nullptr, *nt = nullptr;
+        if (isEnter)
+        {
+            const m34 I = rows3(A, B, C);
+            nlo = xform_point(I, s.o);
+            nld = normalize(xform_vector(I, s.d));
+            nlinv = inverse(nld);
+            nmm = finite3(nlo) && finite3(nlinv);
+            nlim = s.out.t > 0.0f ? s.out.t * D.x * kCullSlack : INFINITY;
+            nb = reinterpret_cast<const float4 *>(
+                ((uint64_t)__float_as_uint(D.w) << 32) | __float_as_uint(D.z));
+            nt = nb + 4 * __float_as_uint(D.y);
+        }
+        s.lo = v3{isEnter ? nlo.x : s.lo.x, isEnter ? nlo.y : s.lo.y,
+                  isEnter ? nlo.z : s.lo.z};
+        s.ld = v3{isEnter ? nld.x : s.ld.x, isEnter ? nld.y : s.ld.y,
+                  isEnter ? nld.z : s.ld.z};
+        s.linv = v3{isEnter ? nlinv.x : s.linv.x, isEnter ? nlinv.y : s.linv.y,
+                    isEnter ? nlinv.z : s.linv.z};
+        s.lim = isEnter ? nlim : s.lim;
+        s.mmL = isEnter ? nmm : s.mmL;
+        s.bnodes = isEnter ? nb : s.bnodes;
+        s.btris = isEnter ? nt : s.btris;
+        s.h.t = isEnter ? INFINITY : s.h.t;
+        s.h.prim = isEnter ? SPG_MISS : s.h.prim;
+        s.spBase = isEnter ? s.sp : s.spBase;
+        if (kCount) cnt.mesh += isEnter ? 1u : 0u;
+
+        // leaving with a hit: world t = dot(M (o' + t' d') - o, d), the
+        // closest world hit wins (src/sp_scene.cpp:296-322), equal t goes to
+        // the lower instance index
         bool better = false;
         float tw = 0.0f;
-        if (fin && s.h.prim != SPG_MISS)
+        if (isFin)
         {
-            const DevInstance &I = sv.inst[s.inst];
             v3 lh = s.lo + s.ld * s.h.t;
-            v3 wh = xform_point(I.model, lh);
+            v3 wh = xform_point(rows3(A, B, C), lh);
             tw = dot(wh - s.o, s.d);
             better = tw < s.out.t || s.out.t < 0.0f ||
                      (tw == s.out.t && s.inst < s.out.inst);
@@ -247,7 +316,20 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
         s.out.v = better ? s.h.v : s.out.v;
         s.out.inst = better ? s.inst : s.out.inst;
         s.out.prim = better ? s.h.prim : s.out.prim;
-        s.level = fin ? 0 : s.level;
+        s.level = isEnter ? 1 : isFin ? 0 : s.level;
+    }
+
+    // (3) the next work.  Nothing to descend into: leave a finished mesh
+    // walk (at once without a hit, by a leave step with one), then pop.  A
+    // popped reference is never empty, so one pop gives the next step work.
+    bool needPop = (isNode && next == kNoRef) || (isTri && !busy) || isFin;
+    bool leaveStep = false;
+    if (kTwoLevel)
+    {
+        const bool leave = needPop && s.level != 0 && s.sp <= s.spBase;
+        leaveStep = leave && s.h.prim != SPG_MISS;
+        s.level = leave && !leaveStep ? 0 : s.level;
+        needPop = needPop && !leaveStep;
     }
     const bool done = needPop && s.sp == 0;
     const bool doPop = needPop && s.sp != 0;
@@ -256,53 +338,22 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
     s.sp -= doPop ? 1 : 0;
     next = doPop ? popped : next;
 
-    // (3) take the next reference: an inner node, a leaf's triangle range,
-    // or (instance level) an instance to enter
     const bool leaf = d_is_leaf(next);
     const bool enter = kTwoLevel && leaf && s.level == 0;
     const bool range = leaf && !enter;
-    if (kTwoLevel)
-    {
-        // o' = M^-1 o, d' = norm(M^-1 d) (src/sp_scene.cpp:272-276); the
-        // mesh walk starts above the current stack
-        v3 nlo{0.0f, 0.0f, 0.0f}, nld{0.0f, 0.0f, 0.0f}, nlinv{0.0f, 0.0f, 0.0f};
-        float nlim = 0.0f;
-        bool nmm = false;
-        const float4 *nb = nullptr, *nt = nullptr;
-        const uint32_t ii = d_leaf_first(next);
-        if (enter)
-        {
-            const DevInstance &I = sv.inst[ii];
-            nlo = xform_point(I.inv, s.o);
-            nld = normalize(xform_vector(I.inv, s.d));
-            nlinv = inverse(nld);
-            nmm = finite3(nlo) && finite3(nlinv);
-            nlim = s.out.t > 0.0f ? s.out.t * I.cullScale * kCullSlack : INFINITY;
-            nb = I.nodes;
-            nt = I.tris;
-        }
-        s.lo = v3{enter ? nlo.x : s.lo.x, enter ? nlo.y : s.lo.y,
-                  enter ? nlo.z : s.lo.z};
-        s.ld = v3{enter ? nld.x : s.ld.x, enter ? nld.y : s.ld.y,
-                  enter ? nld.z : s.ld.z};
-        s.linv = v3{enter ? nlinv.x : s.linv.x, enter ? nlinv.y : s.linv.y,
-                    enter ? nlinv.z : s.linv.z};
-        s.lim = enter ? nlim : s.lim;
-        s.mmL = enter ? nmm : s.mmL;
-        s.bnodes = enter ? nb : s.bnodes;
-        s.btris = enter ? nt : s.btris;
-        s.h.t = enter ? INFINITY : s.h.t;
-        s.h.prim = enter ? SPG_MISS : s.h.prim;
-        s.level = enter ? 1 : s.level;
-        s.spBase = enter ? s.sp : s.spBase;
-        s.inst = enter ? ii : s.inst;
-        if (kCount) cnt.mesh += enter ? 1u : 0u;
-    }
+    const bool inner = !leaf && next != kNoRef;
     const uint32_t first = d_leaf_first(next), count = d_leaf_count(next);
     s.triNext = range ? first : s.triNext;
     s.triEnd = range ? first + count : s.triEnd;
     if (kCount) cnt.tri += range ? count : 0u;
-    s.node = enter ? 0 : (!leaf && next != kNoRef) ? next : s.node;
+    s.inst = enter ? first : s.inst;
+    s.node = inner ? next : isEnter ? 0 : s.node;
+    s.mode = leaveStep ? kModeFinish
+             : enter   ? kModeEnter
+             : range   ? kModeTri
+             : (inner || isEnter) ? kModeNode
+             : busy    ? kModeTri
+                       : s.mode;
     return done;
 }
 
@@ -320,6 +371,7 @@ __device__ __forceinline__ void trav_begin_scene(Trav &s, v3 o, v3 d)
     s.node = 0;
     s.h = Hit{INFINITY, 0.0f, 0.0f, SPG_MISS};
     s.triNext = s.triEnd = 0;
+    s.mode = kModeNode;
 }
 
 __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
@@ -339,6 +391,7 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
     s.spBase = 0;
     s.node = 0;
     s.triNext = s.triEnd = 0;
+    s.mode = kModeNode;
 }
 
 // Persistent driver: every wave walks rays [0, n) taken in chunks of
@@ -409,7 +462,16 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
         // path back to the loop head
         if (__ballot(ray != SPG_MISS) == 0ull && exhausted && chunkNext >= chunkEnd)
             break;
-        if (trav_step<kCount, kTwoLevel>(sv, s, st, cnt, ray != SPG_MISS))
+        const bool act = ray != SPG_MISS;
+        bool go = act;
+        if (kTwoLevel)
+        {
+            const bool rare = act && s.mode >= kModeEnter;
+            const uint32_t nRare = (uint32_t)__popcll(__ballot(rare));
+            const bool common = __ballot(act && !rare) != 0ull;
+            go = act && (!rare || nRare >= kRareMin || !common);
+        }
+        if (trav_step<kCount, kTwoLevel>(sv, s, st, cnt, go))
         {
             store(ray, s);
             ray = SPG_MISS;
