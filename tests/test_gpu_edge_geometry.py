@@ -1,4 +1,4 @@
-"""Axis-parallel rays whose origin lies exactly on a BVH box plane."""
+"""Edge cases of the geometry path: axis-parallel rays whose origin lies exactly on a BVH box plane, and non-finite input geometry."""
 import ctypes as C
 
 import numpy as np
