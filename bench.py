@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # --pmc FETCH_SIZE and WRITE_SIZE per dispatch (FETCH scaled x2.0, WRITE
 # x1.0 by a 1 GiB copy kernel on the same box); the roofline's "traffic" is
 # this times the rays of one launch.
-TRAFFIC_BYTES_PER_RAY = {"config5": 1.778e9 / 2.04e6, "soup": 1.685e9 / 4194304}
+TRAFFIC_BYTES_PER_RAY = {"config5": 1.701e9 / 2.04e6, "soup": 1.735e9 / 4194304}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same kernel and "
                   "workload (profiles/hbm_traffic_config5_soup.md), FETCH x2.0 and "
                   "WRITE x1.0 by a 1 GiB copy calibration")

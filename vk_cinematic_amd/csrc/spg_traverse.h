@@ -30,20 +30,15 @@ namespace spg {
 constexpr int kShortStack = 16;
 constexpr uint32_t kRayChunk = 64;
 // A wave refills its idle lanes once this many wait (or all do): each
-// refill stalls the wave on the new rays' loads, so refilling one lane per
-// iteration would add a memory round trip to nearly every step.
+// refill stalls the wave on the new rays' loads and setup, so refilling one
+// lane per iteration adds a memory round trip to nearly every step, while
+// waiting for all 64 leaves the wave running its longest ray alone.  On the
+// 4M-triangle box scene and the 1M-triangle soup, 48 beat 1, 4, 8, 16, 24,
+// 32 and 64.
 #ifndef SPG_REFILL_MIN
-#define SPG_REFILL_MIN 8
+#define SPG_REFILL_MIN 48
 #endif
 constexpr uint32_t kRefillMin = SPG_REFILL_MIN;
-// Instance entries and exits (about one step in ten) wait until this many
-// lanes of the wave hold one (or no lane has other work): a wave executes
-// the code of every kind of step some lane holds, and these two cost more
-// than a node visit and a triangle together.
-#ifndef SPG_RARE_MIN
-#define SPG_RARE_MIN 1
-#endif
-constexpr uint32_t kRareMin = SPG_RARE_MIN;
 constexpr int32_t kNoRef = kEmptyChild;
 
 // Node, triangle and stack-spill arrays are reached through pointers read
@@ -462,16 +457,7 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
         // path back to the loop head
         if (__ballot(ray != SPG_MISS) == 0ull && exhausted && chunkNext >= chunkEnd)
             break;
-        const bool act = ray != SPG_MISS;
-        bool go = act;
-        if (kTwoLevel)
-        {
-            const bool rare = act && s.mode >= kModeEnter;
-            const uint32_t nRare = (uint32_t)__popcll(__ballot(rare));
-            const bool common = __ballot(act && !rare) != 0ull;
-            go = act && (!rare || nRare >= kRareMin || !common);
-        }
-        if (trav_step<kCount, kTwoLevel>(sv, s, st, cnt, go))
+        if (trav_step<kCount, kTwoLevel>(sv, s, st, cnt, ray != SPG_MISS))
         {
             store(ray, s);
             ray = SPG_MISS;
