@@ -20,7 +20,8 @@
  *                                2^20*s draws into it
  *   tx    SPO_TX_LIBM  sinf/cosf/atan2f/powf from libm (the reference's)
  *         SPO_TX_CR    the same functions evaluated in double and rounded
- *                      once to float (what the GPU path computes)
+ *                      once to float (what the GPU path computes; x^5 by
+ *                      double multiplications)
  *   hit   SPO_HIT_REFERENCE  candidates from the reference's greedy 4-ary BVH,
  *                            first-visited wins ties
  *         SPO_HIT_BRUTE      every triangle is a candidate, ties go to the

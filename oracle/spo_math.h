@@ -40,9 +40,15 @@ static inline float spo_atan2(float y, float x, int tx)
 {
     return tx == SPO_TX_CR ? (float)atan2((double)y, (double)x) : atan2f(y, x);
 }
-static inline float spo_pow(float x, float y, int tx)
+/* powf(x, 5.0f) (Schlick's term).  SPO_TX_CR: x^2 exact in double, then
+ * x^2 * x^2 * x with one rounding each and one rounding to float (the GPU's
+ * tpow5, so the two agree bit for bit; it differs from pow() in double only
+ * in the last double bits) */
+static inline float spo_pow5(float x, int tx)
 {
-    return tx == SPO_TX_CR ? (float)pow((double)x, (double)y) : powf(x, y);
+    if (tx != SPO_TX_CR) return powf(x, 5.0f);
+    double x2 = (double)x * (double)x;
+    return (float)(x2 * x2 * (double)x);
 }
 
 static inline spo_v3 v3(float x, float y, float z)

@@ -40,7 +40,7 @@ static float rng_u24(spo_Rng *r)
 /* src/simd_path_tracer.cpp:65-105 */
 static spo_v3 fresnel_schlick(float cosTheta, spo_v3 F0, int tx)
 {
-    float p = spo_pow(1.0f - cosTheta, 5.0f, tx);
+    float p = spo_pow5(1.0f - cosTheta, tx);
     return v3add(F0, v3mul(v3sub(v3s(1.0f), F0), p));
 }
 static float distribution_ggx(spo_v3 N, spo_v3 H, float roughness)

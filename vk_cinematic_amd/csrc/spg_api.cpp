@@ -411,12 +411,30 @@ static int build_mesh(HostMesh &m, const spg_BuildOpts &opt)
         return fail(SPG_E_INVALID, "mesh BVH build failed");
     if (m.bvh.maxDepth > kMaxBlasDepth)
         return fail(SPG_E_CAPACITY, "mesh BVH deeper than the traversal stack");
-    // nodes, then the triangles in leaf order (a, e1 = b - a, e2 = c - a,
-    // prim), then one float4 of padding: the traversal fetches 64 bytes for
-    // every work item, a triangle's 48 and 16 it does not use
+    // One allocation of 64-byte records: the nodes, then the triangles in
+    // leaf order as (a.xyz, e1.x) (e1.yz, e2.xy) (e2.z, prim, n.xy) (n.z,
+    // 0, 0, 0) with e1 = b - a, e2 = c - a, n = cross(e1, e2) (the
+    // Moller-Trumbore quantities, src/ray_intersection.cpp:156-190, formed
+    // with the same FP32 operations as the test).  Triangle k is record
+    // nn + k, so the device copy's leaf references name records directly
+    // and a traversal step fetches a node or a triangle alike.
     const size_t nn = m.bvh.nodes.size();
-    std::vector<float4> blob(4 * nn + 3 * (size_t)tc + 1, make_float4(0, 0, 0, 0));
+    if (nn + (size_t)tc >= (size_t)1 << 28)
+        return fail(SPG_E_CAPACITY, "mesh too large for 28-bit leaf references");
+    std::vector<float4> blob(4 * (nn + (size_t)tc), make_float4(0, 0, 0, 0));
     memcpy(blob.data(), m.bvh.nodes.data(), nn * sizeof(GpuNode));
+    for (size_t j = 0; j < nn; ++j)
+    {
+        float *slots[2] = {&blob[4 * j].w, &blob[4 * j + 1].w};
+        for (float *slot : slots)
+        {
+            int32_t r;
+            memcpy(&r, slot, 4);
+            if (r == kEmptyChild || !ref_is_leaf(r)) continue;
+            r = leaf_ref((uint32_t)nn + leaf_first(r), leaf_count(r));
+            memcpy(slot, &r, 4);
+        }
+    }
     float4 *tris = blob.data() + 4 * nn;
     for (uint32_t k = 0; k < tc; ++k)
     {
@@ -424,13 +442,15 @@ static int build_mesh(HostMesh &m, const spg_BuildOpts &opt)
         const float *a = &m.verts[8 * (size_t)m.idx[3 * prim]];
         const float *b = &m.verts[8 * (size_t)m.idx[3 * prim + 1]];
         const float *c = &m.verts[8 * (size_t)m.idx[3 * prim + 2]];
-        float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-        float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        v3 e1{b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        v3 e2{c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        v3 n = cross(e1, e2);
         float primBits;
         memcpy(&primBits, &prim, 4);
-        tris[3 * k] = make_float4(a[0], a[1], a[2], e1[0]);
-        tris[3 * k + 1] = make_float4(e1[1], e1[2], e2[0], e2[1]);
-        tris[3 * k + 2] = make_float4(e2[2], primBits, 0.0f, 0.0f);
+        tris[4 * k] = make_float4(a[0], a[1], a[2], e1.x);
+        tris[4 * k + 1] = make_float4(e1.y, e1.z, e2.x, e2.y);
+        tris[4 * k + 2] = make_float4(e2.z, primBits, n.x, n.y);
+        tris[4 * k + 3] = make_float4(n.z, 0.0f, 0.0f, 0.0f);
     }
     CHK(upload(&m.dNodes, blob.data(), blob.size()));
     m.dTris = (float4 *)m.dNodes + 4 * nn;

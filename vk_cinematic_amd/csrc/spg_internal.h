@@ -43,8 +43,8 @@ constexpr float kCullSlack = 1.0f + 1.0f / 4096.0f;
 struct DevMesh
 {
     const float4 *nodes;    // GpuNode, 4 float4 each
-    const float4 *tris;     // 3 float4 per triangle in leaf order:
-                            // (a.xyz, e1.x) (e1.yz, e2.xy) (e2.z, prim, 0, 0)
+    const float4 *tris;     // triangle records (4 float4 each) in leaf order,
+                            // right after the nodes in the same allocation
     const float *verts;     // VertexPNT, 8 floats
     const uint32_t *idx;    // 3 per triangle
     uint32_t smooth;
@@ -53,14 +53,14 @@ struct DevMesh
 
 // 128 bytes.  The traversal reads it in 64-byte pieces: bytes 0..63 when
 // leaving the instance (model rows), bytes 48..111 when entering it (inverse
-// rows, cull scale, the mesh's BVH).
+// rows, cull scale, the mesh's records).
 struct DevInstance
 {
     m34 model;          // rows 0..2 of M = T*R*S       (src/sp_scene.cpp:91)
     m34 inv;            // rows 0..2 of S^-1*R(q*)*T(-p) (src/sp_scene.cpp:94-96)
     float cullScale;    // 1 / min |scale|: world t bound -> local t bound
-    uint32_t nodeCount; // the mesh's triangles follow its nodes:
-    const float4 *nodes;//   tris = nodes + 4 * nodeCount
+    uint32_t nodeCount; // the mesh's node records (triangle k is nodeCount + k)
+    const float4 *nodes;// the mesh's 64-byte records
     uint32_t mesh;
     int32_t material;   // dense material slot, -1 = not registered
     uint32_t materialId;
@@ -250,8 +250,8 @@ struct HostMesh
     spg::BvhBuildResult bvh;
     bool built = false;
     // device copies
-    // one allocation: nodes, triangles in leaf order, one float4 of padding
-    // (dTris points into it)
+    // one allocation of 64-byte records: nodes, then triangles in leaf
+    // order (dTris points into it)
     void *dNodes = nullptr, *dTris = nullptr, *dVerts = nullptr, *dIdx = nullptr;
     uint32_t nodeCount = 0;
     float boundsLo[3], boundsHi[3];

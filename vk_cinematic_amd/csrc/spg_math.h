@@ -6,7 +6,8 @@
 // bit-identical between the host build, the gfx950 build and the reference's
 // own x86 build.  Min/Max are compare-and-select like the reference's
 // (src/math_utils.h:9-19) -- not v_min/v_max, whose NaN and signed-zero
-// handling differ.
+// handling differ -- except in slab_hit_mm, which is used only where no
+// operand can be NaN and the two agree.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -133,12 +134,13 @@ SPG_HD bool finite3(v3 a)
 // ---- Moller-Trumbore, src/ray_intersection.cpp:156-190, with e1 = b - a
 // and e2 = c - a supplied (precomputed with the same subtraction).
 // Returns t (or -1 on a miss) and u, v.
-SPG_HD float mt_test(v3 o, v3 d, v3 a, v3 e1, v3 e2, float *uo, float *vo)
+// n = cross(e1, e2) is an input: the GPU stores it with the triangle
+// (computed with the same operations), the oracle forms it in mt_test.
+SPG_HD float mt_test_n(v3 o, v3 d, v3 a, v3 e1, v3 e2, v3 n, float *uo, float *vo)
 {
     v3 T = o - a;
     v3 p = cross(d, e2);
     v3 q = cross(T, e1);
-    v3 n = cross(e1, e2);
     float mx = dot(q, e2), my = dot(p, T), mz = dot(q, d);
     float det = 1.0f / dot(p, e1);
     float t = det * mx;
@@ -154,6 +156,11 @@ SPG_HD float mt_test(v3 o, v3 d, v3 a, v3 e1, v3 e2, float *uo, float *vo)
         return t;
     }
     return -1.0f;
+}
+
+SPG_HD float mt_test(v3 o, v3 d, v3 a, v3 e1, v3 e2, float *uo, float *vo)
+{
+    return mt_test_n(o, d, a, e1, e2, cross(e1, e2), uo, vo);
 }
 
 // ---- RNG --------------------------------------------------------------
@@ -227,5 +234,22 @@ SPG_HD float tsin(float x) { return (float)sin((double)x); }
 SPG_HD float tcos(float x) { return (float)cos((double)x); }
 SPG_HD float tatan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 SPG_HD float tpow(float x, float y) { return (float)pow((double)x, (double)y); }
+// sin and cos of one argument from one double-precision evaluation
+SPG_HD void tsincos(float x, float *s, float *c)
+{
+    double sd, cd;
+    sincos((double)x, &sd, &cd);
+    *s = (float)sd;
+    *c = (float)cd;
+}
+// x^5 in double, rounded once: x^2 is exact in double, x^4 and x^5 carry
+// one rounding each (relative error below 2^-51, as pow((double)x, 5.0)
+// has below 2^-52), so the float result is the correctly rounded x^5
+// unless x^5 lies within 2^-51 of a float rounding boundary
+SPG_HD float tpow5(float x)
+{
+    double x2 = (double)x * (double)x;
+    return (float)(x2 * x2 * (double)x);
+}
 
 }  // namespace spg

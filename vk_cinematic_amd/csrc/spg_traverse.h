@@ -123,7 +123,7 @@ struct Trav
 {
     v3 o, d, inv;              // world ray
     v3 lo, ld, linv;           // ray in the current mesh's space
-    const float4 *bnodes, *btris;
+    const float4 *bnodes;      // the mesh's records: nodes, then triangles
     float lim;                 // bound on the mesh-space t from the world best
     Hit h;                     // best hit in the current mesh
     SceneHit out;              // best hit in the scene
@@ -140,19 +140,20 @@ constexpr uint32_t kModeTri = 1;     // test triangle triNext of the leaf range
 constexpr uint32_t kModeEnter = 2;   // enter instance `inst`
 constexpr uint32_t kModeFinish = 3;  // leave instance `inst` with its hit
 
-// Moller-Trumbore on one fetched triangle (a.xyz, e1.x) (e1.yz, e2.xy)
-// (e2.z, prim) (src/ray_intersection.cpp:156-190).  Returns whether the hit
-// c beats `best`: t > 0 and smaller (src/sp_scene.cpp:189-193); equal t
-// goes to the lower triangle index so the result does not depend on the
-// visit order.
+// Moller-Trumbore on one fetched triangle record (a.xyz, e1.x) (e1.yz,
+// e2.xy) (e2.z, prim, n.xy) (n.z, ...) with n = cross(e1, e2) stored
+// (src/ray_intersection.cpp:156-190).  Returns whether the hit c beats
+// `best`: t > 0 and smaller (src/sp_scene.cpp:189-193); equal t goes to the
+// lower triangle index so the result does not depend on the visit order.
 __device__ __forceinline__ bool triangle_candidate(float4 t0, float4 t1, float4 t2,
-                                                   v3 o, v3 d, const Hit &best,
-                                                   Hit &c)
+                                                   float4 t3, v3 o, v3 d,
+                                                   const Hit &best, Hit &c)
 {
     v3 a{t0.x, t0.y, t0.z}, e1v{t0.w, t1.x, t1.y}, e2v{t1.z, t1.w, t2.x};
+    v3 n{t2.z, t2.w, t3.x};
     c.prim = __float_as_uint(t2.y);
     c.u = c.v = 0.0f;
-    c.t = mt_test(o, d, a, e1v, e2v, &c.u, &c.v);
+    c.t = mt_test_n(o, d, a, e1v, e2v, n, &c.u, &c.v);
     return c.t > 0.0f && (c.t < best.t || (c.t == best.t && c.prim < best.prim));
 }
 
@@ -177,13 +178,12 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
     const uint32_t mode = s.mode;
     const bool blas = !kTwoLevel || s.level != 0;
 
-    // (1) the step's 64-byte fetch: node, triangle (+16 unused bytes; the
-    // triangle array is padded), instance bytes 48..111 (enter) or 0..63
-    // (leave)
+    // (1) the step's 64-byte fetch: a node or triangle record of the mesh,
+    // a TLAS node, instance bytes 48..111 (enter) or 0..63 (leave)
     const float4 *src;
-    if (mode == kModeTri) src = s.btris + 3 * s.triNext;
-    else if (!kTwoLevel || mode == kModeNode)
-        src = (blas ? s.bnodes : sv.tlasNodes) + 4 * s.node;
+    if (!kTwoLevel || mode <= kModeTri)
+        src = (blas ? s.bnodes : sv.tlasNodes) +
+              4 * (mode == kModeTri ? (int)s.triNext : s.node);
     else
         src = reinterpret_cast<const float4 *>(sv.inst + s.inst) +
               (mode == kModeEnter ? 3 : 0);
@@ -205,7 +205,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
     bool take = false;
     int32_t next = kNoRef, far = kNoRef;
     if (isTri)
-        take = triangle_candidate(A, B, C, s.lo, s.ld, s.h, c);
+        take = triangle_candidate(A, B, C, D, s.lo, s.ld, s.h, c);
     else if (isNode)
     {
         v3 o = blas ? s.lo : s.o;
@@ -265,7 +265,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
         v3 nlo{0.0f, 0.0f, 0.0f}, nld{0.0f, 0.0f, 0.0f}, nlinv{0.0f, 0.0f, 0.0f};
         float nlim = 0.0f;
         bool nmm = false;
-        const float4 *nb = nullptr, *nt = nullptr;
+        const float4 *nb = nullptr;
         if (isEnter)
         {
             const m34 I = rows3(A, B, C);
@@ -276,7 +276,6 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
             nlim = s.out.t > 0.0f ? s.out.t * D.x * kCullSlack : INFINITY;
             nb = reinterpret_cast<const float4 *>(
                 ((uint64_t)__float_as_uint(D.w) << 32) | __float_as_uint(D.z));
-            nt = nb + 4 * __float_as_uint(D.y);
         }
         s.lo = v3{isEnter ? nlo.x : s.lo.x, isEnter ? nlo.y : s.lo.y,
                   isEnter ? nlo.z : s.lo.z};
@@ -287,7 +286,6 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
         s.lim = isEnter ? nlim : s.lim;
         s.mmL = isEnter ? nmm : s.mmL;
         s.bnodes = isEnter ? nb : s.bnodes;
-        s.btris = isEnter ? nt : s.btris;
         s.h.t = isEnter ? INFINITY : s.h.t;
         s.h.prim = isEnter ? SPG_MISS : s.h.prim;
         s.spBase = isEnter ? s.sp : s.spBase;
@@ -378,7 +376,6 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
     s.mmL = finite3(o) && finite3(s.linv);
     s.mmW = false;
     s.bnodes = M.nodes;
-    s.btris = M.tris;
     s.lim = INFINITY;
     s.h = Hit{INFINITY, 0.0f, 0.0f, SPG_MISS};
     s.level = 1;
