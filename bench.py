@@ -101,6 +101,24 @@ def cpu_baseline_soup(threads, tris, o, d, n_rays=65536):
             "sample": f"{n_rays} of the 2^22 rays, same SAH BVH ({sec:.2f} s)"}
 
 
+def cpu_baseline_irradiance(sky, fw, fh):
+    """The oracle's restatement of CreateIrradianceCubeMap (one host thread,
+    double-evaluated transcendentals like the GPU line), the whole cube map."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _libs import oracle
+
+    eh, ew = sky.shape[:2]
+    out = np.zeros(6 * fw * fh * 4, np.float32)
+    t0 = time.perf_counter()
+    oracle().spo_create_irradiance_cubemap(np.ascontiguousarray(sky.ravel()), ew, eh,
+                                           fw, fh, 10.0, 1, out)
+    sec = time.perf_counter() - t0
+    samples = 6 * fw * fh * 63 * 16
+    return {"value": samples / sec / 1e9, "unit": "Gsamples/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the whole {fw}x{fh} cube map ({sec:.2f} s)"}
+
+
 def bench_config1(args, V, scenes):
     dev = V.Device(0)
     desc = scenes.config1(256, 256, env="cube")
@@ -643,13 +661,59 @@ def bench_soup(args, V, scenes):
     return res
 
 
+def bench_irradiance(args, V, scenes):
+    """CreateIrradianceCubeMap on the reference's call (src/main.cpp:1313:
+    32x32 faces) from a 2048x1024 synthetic HDR sky (sky gradient + sun of
+    radiance 50, above the radiance clamp of 10): one step = the whole cube
+    map; double-evaluated transcendentals (the oracle-exact mode)."""
+    dev = V.Device(0)
+    ew, eh, fw, fh = 2048, 1024, 32, 32
+    sky = V.gen_sky_equirect(ew, eh).reshape(eh, ew, 4)
+    tx = V.TX_FAST if args.fast else V.TX_DOUBLE
+    for _ in range(max(1, args.warmup)):
+        dev.create_irradiance_cubemap(sky, fw, fh, 10.0, tx)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, ms = dev.create_irradiance_cubemap(sky, fw, fh, 10.0, tx)
+        kms.append(ms)
+    wall = time.perf_counter() - t0
+    samples = 6 * fw * fh * 63 * 16
+    kernel_ms = float(np.mean(kms))
+    # algorithmic bytes: the four RGBA32F texels of every bilinear lookup
+    # and the output texel (what an implementation without caches would move)
+    bytes_launch = samples * 64 + 6 * fw * fh * 16
+    achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+    res = {
+        "metric": "Gsamples/s (irradiance cube map, kernel)",
+        "value": samples / (kernel_ms * 1e-3) / 1e9,
+        "unit": "Gsamples/s",
+        "ms_per_step": wall * 1e3 / args.steps,
+        "dtype": "f32 (f64 transcendentals)" if tx == V.TX_DOUBLE else "f32",
+        "config": {"workload": f"irradiance cube map: 6 faces of {fw}x{fh} from a "
+                               f"{ew}x{eh} RGBA32F sky equirect, 63x16 samples per "
+                               "texel, radiance clamp 10",
+                   "samples_per_step": samples},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_irradiance", "launch_ms": kernel_ms,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "extra": {"step_ms_with_copies": wall * 1e3 / args.steps},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_irradiance(sky, fw, fh)
+    dev.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="config5",
-                    choices=["config5", "config4", "config3", "config1", "soup"])
+                    choices=["config5", "config4", "config3", "config1", "soup",
+                             "irradiance"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fast", action="store_true",
                     help="device float transcendentals instead of double-evaluated")
@@ -686,7 +750,8 @@ def main():
         res = bench_box(args, V, scenes, args.workload)
     else:
         res = {"config1": bench_config1, "config3": bench_config3,
-               "soup": bench_soup}[args.workload](args, V, scenes)
+               "soup": bench_soup, "irradiance": bench_irradiance}[args.workload](
+                   args, V, scenes)
     out = {"metric": res.pop("metric"), "value": res.pop("value"),
            "unit": res.pop("unit"), "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),

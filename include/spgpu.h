@@ -372,6 +372,24 @@ int spg_guide_train(spg_Guide *g, const float *samples13, uint32_t n,
 int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps,
                     float *loss);
 
+/* ---- environment maps ---- */
+/* CreateIrradianceCubeMap(hdri, arena, faceWidth, faceHeight)
+ * (src/cubemap.cpp:108-233, called at src/main.cpp:1313 with 32x32) with
+ * the reference's production switches: uniform sampling of the hemisphere
+ * about each texel's direction (phi, theta stepped by 0.1) and radiance
+ * clamped to radianceClamp (RADIANCE_CLAMP, src/config.h:37; <= 0: no
+ * clamp).  equirect: RGBA32F equirectangular image (HdrImage pixels,
+ * ew*eh*4 floats, host memory); out: 6 faces of faceWidth*faceHeight
+ * RGBA32F texels (host memory, face-major like HdrCubeMap.images).
+ * transcendentals: SPG_TX_DOUBLE (sums bit-identical to the oracle's
+ * double-evaluated restatement) or SPG_TX_FAST.  kernelMs (may be NULL)
+ * receives the kernel's time. */
+int spg_create_irradiance_cubemap(spg_Device *dev, const float *equirect,
+                                  uint32_t ew, uint32_t eh, uint32_t faceWidth,
+                                  uint32_t faceHeight, float radianceClamp,
+                                  int transcendentals, float *out,
+                                  double *kernelMs);
+
 /* ---- generators used by the benchmark scenes (no reference counterpart
  *      except the icosphere, CreateIcosahedronMesh,
  *      src/mesh_generation.cpp:220-337, and the plane, :1-21) ---- */

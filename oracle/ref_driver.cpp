@@ -248,6 +248,21 @@ void ref_create_cubemap(const float *equirect, unsigned ew, unsigned eh,
         memcpy(out + (u64)f * w * h * 4, cm.images[f].pixels, (u64)w * h * 16);
     free(arena.base);
 }
+
+// Irradiance cube map of an equirectangular RGBA32F image (src/cubemap.cpp
+// CreateIrradianceCubeMap, production switches); out holds 6*w*h*4 floats.
+void ref_create_irradiance_cubemap(const float *equirect, unsigned ew, unsigned eh,
+    unsigned w, unsigned h, float *out)
+{
+    MemoryArena arena;
+    u64 bytes = (u64)6 * w * h * 16 + 64;
+    InitializeMemoryArena(&arena, calloc(1, bytes), bytes);
+    HdrImage src = {(float *)equirect, ew, eh};
+    HdrCubeMap cm = CreateIrradianceCubeMap(src, &arena, w, h);
+    for (u32 f = 0; f < 6; ++f)
+        memcpy(out + (u64)f * w * h * 4, cm.images[f].pixels, (u64)w * h * 16);
+    free(arena.base);
+}
 #endif
 
 unsigned ref_compute_tiles(unsigned W, unsigned H, unsigned tw, unsigned th,

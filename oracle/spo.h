@@ -193,6 +193,9 @@ void spo_cubemap_lookup(const float *faces, uint32_t w, uint32_t h,
                         const float *dir, float *out4);
 void spo_create_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
                         uint32_t w, uint32_t h, float *out);
+void spo_create_irradiance_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
+                                   uint32_t w, uint32_t h, float clampValue, int tx,
+                                   float *out);
 uint32_t spo_compute_tiles(uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
                            uint32_t maxTiles, uint32_t *out4);
 void spo_transform_aabb(const float *mn, const float *mx, const float *pos,

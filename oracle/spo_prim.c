@@ -394,6 +394,71 @@ void spo_create_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
     }
 }
 
+/* CreateIrradianceCubeMap, src/cubemap.cpp:108-233, with the production
+ * switches of src/config.h (IRRADIANCE_CUBEMAP_USE_UNIFORM_SAMPLING 1,
+ * RADIANCE_CLAMP 10; clampValue <= 0 restates a build without the clamp).
+ * Texel (x, y) of face f: normal n = dir as in CreateCubeMap, tangent
+ * t = norm(up x n), bitangent b = norm(n x t); for phi = 0, 0.1, .. < 2 pi
+ * and theta = 0, 0.1, .. < pi/2 (float loop counters, as written), the
+ * direction (n*cos(theta) + t*sin(theta)cos(phi)) + b*sin(theta)sin(phi)
+ * samples the equirect image bilinearly (v flipped), clamped to
+ * [0, clampValue]; irradiance += L*cos(theta)*sin(theta) in loop order;
+ * texel = (pi*irradiance*(1/count), 1). */
+void spo_create_irradiance_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
+                                   uint32_t w, uint32_t h, float clampValue, int tx,
+                                   float *out)
+{
+    const float sampleDelta = 0.1f;
+    for (uint32_t f = 0; f < 6; ++f)
+    {
+        float b[9];
+        spo_cubemap_basis(f, b);
+        spo_v3 fwd = v3p(b), up = v3p(b + 3), right = v3p(b + 6);
+        for (uint32_t y = 0; y < h; ++y)
+            for (uint32_t x = 0; x < w; ++x)
+            {
+                float fx = (float)x / (float)w;
+                float fy = (float)y / (float)h;
+                fy = 1.0f - fy;
+                fx = fx * 2.0f - 1.0f;
+                fy = fy * 2.0f - 1.0f;
+                spo_v3 n = v3norm(v3add(v3add(fwd, v3mul(right, fx)), v3mul(up, fy)));
+                spo_v3 t = v3norm(v3cross(up, n));
+                spo_v3 bt = v3norm(v3cross(n, t));
+                spo_v3 irr = v3s(0.0f);
+                uint32_t count = 0;
+                for (float phi = 0.0f; phi < 2.0f * SPO_PI; phi += sampleDelta)
+                    for (float theta = 0.0f; theta < 0.5f * SPO_PI; theta += sampleDelta)
+                    {
+                        float sc[2] = {phi, theta}, td[3];
+                        spo_spherical_to_cartesian(sc, tx, td);
+                        spo_v3 wd = v3add(v3add(v3mul(n, td[1]), v3mul(t, td[0])),
+                                          v3mul(bt, td[2]));
+                        float dd[3], sph[2], uv[2], smp[4];
+                        v3put(dd, wd);
+                        spo_to_spherical(dd, tx, sph);
+                        spo_map_to_equirect(sph, tx, uv);
+                        uv[1] = 1.0f - uv[1];
+                        spo_sample_bilinear(equirect, ew, eh, uv, smp);
+                        spo_v3 L = {smp[0], smp[1], smp[2]};
+                        if (clampValue > 0.0f)
+                            L = (spo_v3){spo_clamp(L.x, 0.0f, clampValue),
+                                         spo_clamp(L.y, 0.0f, clampValue),
+                                         spo_clamp(L.z, 0.0f, clampValue)};
+                        irr = v3add(irr, v3mul(v3mul(L, spo_cos(theta, tx)),
+                                               spo_sin(theta, tx)));
+                        count++;
+                    }
+                irr = v3mul(v3mul(irr, SPO_PI), 1.0f / (float)count);
+                float *o = out + 4 * (((uint64_t)f * h + y) * w + x);
+                o[0] = irr.x;
+                o[1] = irr.y;
+                o[2] = irr.z;
+                o[3] = 1.0f;
+            }
+    }
+}
+
 /* Nearest cube-map lookup by direction (no reference counterpart: the
  * reference tracer samples an equirect image).  The inverse of CreateCubeMap's
  * texel-to-direction map: major axis picks the face (ties: x, then y, then

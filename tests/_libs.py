@@ -103,6 +103,8 @@ def oracle():
     _sig(L, "spo_cubemap_lookup", None, f32p, C.c_uint32, C.c_uint32, f32p, f32p)
     _sig(L, "spo_create_cubemap", None, f32p, C.c_uint32, C.c_uint32, C.c_uint32,
          C.c_uint32, f32p)
+    _sig(L, "spo_create_irradiance_cubemap", None, f32p, C.c_uint32, C.c_uint32,
+         C.c_uint32, C.c_uint32, C.c_float, C.c_int, f32p)
     _sig(L, "spo_compute_tiles", C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
          C.c_uint32, C.c_uint32, u32p)
     _sig(L, "spo_transform_aabb", None, f32p, f32p, f32p, f32p, f32p, f32p)
@@ -205,6 +207,8 @@ def _load_ref(path):
         _sig(L, "ref_cubemap_basis", None, C.c_uint32, f32p)
         _sig(L, "ref_create_cubemap", None, f32p, C.c_uint32, C.c_uint32, C.c_uint32,
              C.c_uint32, f32p)
+        _sig(L, "ref_create_irradiance_cubemap", None, f32p, C.c_uint32, C.c_uint32,
+             C.c_uint32, C.c_uint32, f32p)
     _sig(L, "ref_compute_tiles", C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
          C.c_uint32, C.c_uint32, u32p)
     _sig(L, "ref_transform_aabb", None, f32p, f32p, f32p, f32p, f32p, f32p)

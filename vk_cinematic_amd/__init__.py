@@ -170,6 +170,9 @@ _SIGS = {
     "spg_gen_triangle_soup": (_int, [C.c_uint64, _u32, C.c_float, C.c_float, _f32p]),
     "spg_gen_sky_cubemap": (_int, [_u32, _f32p, C.c_float, C.c_float, _f32p]),
     "spg_gen_sky_equirect": (_int, [_u32, _u32, _f32p, C.c_float, C.c_float, _f32p]),
+    "spg_create_irradiance_cubemap": (_int, [_vp, _f32p, _u32, _u32, _u32, _u32,
+                                             C.c_float, _int, _f32p,
+                                             C.POINTER(C.c_double)]),
     "spg_pixel_seed": (_u32, [_u32, _u32]),
     "spg_batch_create": (_int, [_vp, _vp, _u32, _vp, _int, C.POINTER(_vp)]),
     "spg_batch_run": (_int, [_vp, _vp]),
@@ -264,6 +267,20 @@ class Device:
         buf = self.alloc(a.nbytes)
         check(lib().spg_copy_to_device(self.handle, buf.ptr, a.ctypes.data, a.nbytes))
         return buf
+
+    def create_irradiance_cubemap(self, equirect, face_width, face_height,
+                                  clamp=10.0, transcendentals=TX_DOUBLE):
+        """CreateIrradianceCubeMap (src/cubemap.cpp:108-233) of an RGBA32F
+        equirect image [h][w][4]; returns the faces [6][fh][fw][4] and the
+        kernel's milliseconds."""
+        img = np.ascontiguousarray(equirect, np.float32)
+        eh, ew = img.shape[0], img.shape[1]
+        out = np.zeros(6 * face_width * face_height * 4, np.float32)
+        ms = C.c_double()
+        check(lib().spg_create_irradiance_cubemap(
+            self.handle, img.ravel(), ew, eh, face_width, face_height, clamp,
+            transcendentals, out, C.byref(ms)))
+        return out.reshape(6, face_height, face_width, 4), ms.value
 
     def last_timings(self):
         ms = (C.c_double * 6)()

@@ -1220,6 +1220,61 @@ int spg_intersect_closest(spg_Scene *s, uint32_t meshId, const float *ox,
     return r;
 }
 
+// CreateIrradianceCubeMap, src/cubemap.cpp:108-233 (uniform sampling).  The
+// sample grid is the reference's float loop counters, generated here the
+// same way and handed to the kernel.
+int spg_create_irradiance_cubemap(spg_Device *d, const float *equirect,
+                                  uint32_t ew, uint32_t eh, uint32_t fw,
+                                  uint32_t fh, float radianceClamp, int tx,
+                                  float *out, double *kernelMs)
+{
+    if (!d || !equirect || !out || ew == 0 || eh == 0 || fw == 0 || fh == 0)
+        return fail(SPG_E_INVALID, "spg_create_irradiance_cubemap: bad argument");
+    if ((uint64_t)ew * eh > (1ull << 28) || (uint64_t)fw * fh > (1ull << 24))
+        return fail(SPG_E_CAPACITY, "spg_create_irradiance_cubemap: image too large");
+    if (tx != SPG_TX_DOUBLE && tx != SPG_TX_FAST)
+        return fail(SPG_E_INVALID, "spg_create_irradiance_cubemap: transcendentals");
+    std::vector<float> grid;
+    const float sampleDelta = 0.1f;
+    for (float phi = 0.0f; phi < 2.0f * kPi; phi += sampleDelta) grid.push_back(phi);
+    const uint32_t nPhi = (uint32_t)grid.size();
+    for (float theta = 0.0f; theta < 0.5f * kPi; theta += sampleDelta)
+        grid.push_back(theta);
+    const uint32_t nTheta = (uint32_t)grid.size() - nPhi;
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    const size_t srcBytes = (size_t)ew * eh * 16, outBytes = (size_t)6 * fw * fh * 16;
+    void *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, srcBytes + outBytes + 4 * grid.size()));
+    char *b = (char *)buf;
+    float4 *dsrc = (float4 *)b, *dout = (float4 *)(b + srcBytes);
+    float *dgrid = (float *)(b + srcBytes + outBytes);
+    int r = hip_check(hipMemcpyAsync(dsrc, equirect, srcBytes, hipMemcpyHostToDevice,
+                                     d->stream), "upload equirect");
+    if (!r)
+        r = hip_check(hipMemcpyAsync(dgrid, grid.data(), 4 * grid.size(),
+                                     hipMemcpyHostToDevice, d->stream), "upload grid");
+    if (!r) r = hip_check(hipEventRecord(d->ev[0], d->stream), "event");
+    if (!r)
+        r = hip_check(launch_irradiance(d->stream, dsrc, ew, eh, fw, fh, dgrid, nPhi,
+                                        dgrid + nPhi, nTheta, radianceClamp,
+                                        tx == SPG_TX_DOUBLE, dout),
+                      "irradiance kernel");
+    if (!r) r = hip_check(hipEventRecord(d->ev[1], d->stream), "event");
+    if (!r)
+        r = hip_check(hipMemcpyAsync(out, dout, outBytes, hipMemcpyDeviceToHost,
+                                     d->stream), "download");
+    if (!r) r = hip_check(hipStreamSynchronize(d->stream), "irradiance");
+    if (!r && kernelMs)
+    {
+        float ms = 0.0f;
+        r = hip_check(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]), "event time");
+        *kernelMs = ms;
+    }
+    hipFree(buf);
+    return r;
+}
+
 int spg_time_intersect_closest(spg_Scene *s, uint32_t meshId, const float *ox,
                                const float *oy, const float *oz,
                                const float *dx, const float *dy,

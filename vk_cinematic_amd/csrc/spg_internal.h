@@ -341,6 +341,10 @@ hipError_t launch_clear_counters(hipStream_t s, uint32_t *counters, int n);
 hipError_t launch_set_counter(hipStream_t s, uint32_t *counter, uint32_t v);
 hipError_t launch_clear_floats(hipStream_t s, float *p, size_t n);
 hipError_t launch_guide_infer(hipStream_t s, const TraceParams &p, int q);
+hipError_t launch_irradiance(hipStream_t s, const float4 *src, uint32_t ew,
+                             uint32_t eh, uint32_t w, uint32_t h, const float *phis,
+                             uint32_t nPhi, const float *thetas, uint32_t nTheta,
+                             float clampValue, bool dbl, float4 *out);
 // guide network (spg_guide.hip)
 hipError_t launch_guide_eval(hipStream_t s, const _Float16 *w16, const float *bias,
                              const float *in9, uint32_t n, float *out48,

@@ -15,6 +15,7 @@
 // tracer given the same random numbers and transcendental results.
 #include "spg_mlp.h"
 #include "spg_traverse.h"
+#include "spg_tx.h"
 
 namespace spg {
 
@@ -38,43 +39,6 @@ __device__ __forceinline__ void wave_add(unsigned long long *dst,
     unsigned long long s = wave_sum(x);
     if (lane_id() == 0 && s) atomicAdd(dst, s);
 }
-
-// Transcendentals: kDouble evaluates in double and rounds once (bit-
-// reproducible across host and device); otherwise device float intrinsics.
-template <bool kDouble> struct Tx;
-template <> struct Tx<true>
-{
-    static __device__ __forceinline__ float sin_(float x) { return tsin(x); }
-    static __device__ __forceinline__ float cos_(float x) { return tcos(x); }
-    static __device__ __forceinline__ float atan2_(float y, float x)
-    {
-        return tatan2(y, x);
-    }
-    static __device__ __forceinline__ void sincos_(float x, float *s, float *c)
-    {
-        tsincos(x, s, c);
-    }
-    // Schlick's (1 - cos)^5 (the reference's powf(x, 5.0f))
-    static __device__ __forceinline__ float pow5_(float x) { return tpow5(x); }
-};
-template <> struct Tx<false>
-{
-    static __device__ __forceinline__ float sin_(float x) { return __sinf(x); }
-    static __device__ __forceinline__ float cos_(float x) { return __cosf(x); }
-    static __device__ __forceinline__ float atan2_(float y, float x)
-    {
-        return atan2f(y, x);
-    }
-    static __device__ __forceinline__ void sincos_(float x, float *s, float *c)
-    {
-        __sincosf(x, s, c);
-    }
-    static __device__ __forceinline__ float pow5_(float x)
-    {
-        float x2 = x * x;
-        return x2 * x2 * x;
-    }
-};
 
 // Surface at a hit (src/sp_scene.cpp:196-213 and :302-312): uv from the
 // barycentrics, smooth normal normalize(w n0 + u n1 + v n2) or the MT normal
