@@ -372,6 +372,39 @@ int spg_guide_train(spg_Guide *g, const float *samples13, uint32_t n,
 int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps,
                     float *loss);
 
+/* ---- after the path ---- */
+/* Progressive accumulation across trace calls (the reference's open TODO,
+ * src/main.cpp:75): dst = (dst * before + src * added) / (before + added)
+ * per RGB channel, alpha 1, on device RGBA32F images of `pixels` texels
+ * (e.g. before = samples already in dst, added = the new call's spp). */
+int spg_accumulate_device(spg_Device *dev, float *dst, const float *src,
+                          uint32_t pixels, uint32_t before, uint32_t added);
+/* PerformToneMapping (src/shaders/post_processing.frag.glsl:19-26), the
+ * viewer's post step for the ray-traced image: c *= exposure,
+ * c = c / (1 + c), c = pow(c, 1/2.2) per RGB channel, then 8 bits as
+ * round-to-nearest(255 c), alpha 255.  rgba: device RGBA32F; out: device
+ * RGBA8.  transcendentals: SPG_TX_DOUBLE (pow in double, rounded once) or
+ * SPG_TX_FAST. */
+int spg_tonemap_device(spg_Device *dev, const float *rgba, uint32_t pixels,
+                       float exposure, int transcendentals, uint8_t *out);
+/* Portable float map (PFM, "PF" colour, little-endian, rows bottom to top)
+ * of a host RGBA32F image in the reference's layout (x + y * width, row 0
+ * at the top); alpha is dropped. */
+int spg_write_pfm(const char *path, uint32_t width, uint32_t height,
+                  const float *rgba);
+
+/* ---- mesh files ---- */
+/* LoadMesh (src/mesh.cpp:5-62) for Wavefront OBJ text: the triangles
+ * assimp's import yields there (aiProcess_Triangulate: polygons split into
+ * fans; aiProcess_JoinIdenticalVertices: one vertex per distinct position +
+ * normal pair, in order of first use), positions and normals of the first
+ * object ('o'), texture coordinates left 0 as LoadMesh leaves them.
+ * Corners without a normal are rejected (LoadMesh asserts normals).  Call
+ * with verts8 = idx = NULL for nv/ni, then with arrays of that size; the
+ * result feeds spg_mesh_create. */
+int spg_obj_parse(const char *text, size_t len, float *verts8, uint32_t vertCap,
+                  uint32_t *idx, uint32_t idxCap, uint32_t *nv, uint32_t *ni);
+
 /* ---- environment maps ---- */
 /* CreateIrradianceCubeMap(hdri, arena, faceWidth, faceHeight)
  * (src/cubemap.cpp:108-233, called at src/main.cpp:1313 with 32x32) with

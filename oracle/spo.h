@@ -196,6 +196,10 @@ void spo_create_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
 void spo_create_irradiance_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
                                    uint32_t w, uint32_t h, float clampValue, int tx,
                                    float *out);
+void spo_tonemap(const float *rgba, uint32_t pixels, float exposure, int tx,
+                 uint8_t *out);
+void spo_accumulate(float *dst, const float *src, uint32_t pixels, uint32_t before,
+                    uint32_t added);
 uint32_t spo_compute_tiles(uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
                            uint32_t maxTiles, uint32_t *out4);
 void spo_transform_aabb(const float *mn, const float *mx, const float *pos,

@@ -459,6 +459,41 @@ void spo_create_irradiance_cubemap(const float *equirect, uint32_t ew, uint32_t 
     }
 }
 
+/* PerformToneMapping, src/shaders/post_processing.frag.glsl:19-26, and an
+ * UNORM8 write (round to nearest even): c *= exposure, c / (1 + c),
+ * pow(c, 1/2.2); NaN and negative results give 0.  tx: SPO_TX_CR pow in
+ * double rounded once, else powf. */
+void spo_tonemap(const float *rgba, uint32_t pixels, float exposure, int tx,
+                 uint8_t *out)
+{
+    for (uint32_t i = 0; i < pixels; ++i)
+    {
+        for (int k = 0; k < 3; ++k)
+        {
+            float x = rgba[4 * i + k] * exposure;
+            x = x / (1.0f + x);
+            float g = tx == SPO_TX_CR ? (float)pow((double)x, (double)(1.0f / 2.2f))
+                                      : powf(x, 1.0f / 2.2f);
+            g = spo_clamp(g, 0.0f, 1.0f);
+            out[4 * i + k] = (uint8_t)(uint32_t)rintf(g * 255.0f);
+        }
+        out[4 * i + 3] = 255;
+    }
+}
+
+/* Progressive accumulation: dst = (dst * before + src * added) / total. */
+void spo_accumulate(float *dst, const float *src, uint32_t pixels, uint32_t before,
+                    uint32_t added)
+{
+    float b = (float)before, a = (float)added, t = (float)before + (float)added;
+    for (uint32_t i = 0; i < pixels; ++i)
+    {
+        for (int k = 0; k < 3; ++k)
+            dst[4 * i + k] = (dst[4 * i + k] * b + src[4 * i + k] * a) / t;
+        dst[4 * i + 3] = 1.0f;
+    }
+}
+
 /* Nearest cube-map lookup by direction (no reference counterpart: the
  * reference tracer samples an equirect image).  The inverse of CreateCubeMap's
  * texel-to-direction map: major axis picks the face (ties: x, then y, then

@@ -105,6 +105,9 @@ def oracle():
          C.c_uint32, f32p)
     _sig(L, "spo_create_irradiance_cubemap", None, f32p, C.c_uint32, C.c_uint32,
          C.c_uint32, C.c_uint32, C.c_float, C.c_int, f32p)
+    _sig(L, "spo_tonemap", None, f32p, C.c_uint32, C.c_float, C.c_int,
+         np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"))
+    _sig(L, "spo_accumulate", None, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32)
     _sig(L, "spo_compute_tiles", C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
          C.c_uint32, C.c_uint32, u32p)
     _sig(L, "spo_transform_aabb", None, f32p, f32p, f32p, f32p, f32p, f32p)

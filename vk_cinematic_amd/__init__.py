@@ -170,6 +170,11 @@ _SIGS = {
     "spg_gen_triangle_soup": (_int, [C.c_uint64, _u32, C.c_float, C.c_float, _f32p]),
     "spg_gen_sky_cubemap": (_int, [_u32, _f32p, C.c_float, C.c_float, _f32p]),
     "spg_gen_sky_equirect": (_int, [_u32, _u32, _f32p, C.c_float, C.c_float, _f32p]),
+    "spg_accumulate_device": (_int, [_vp, _vp, _vp, _u32, _u32, _u32]),
+    "spg_tonemap_device": (_int, [_vp, _vp, _u32, C.c_float, _int, _vp]),
+    "spg_write_pfm": (_int, [C.c_char_p, _u32, _u32, _f32p]),
+    "spg_obj_parse": (_int, [C.c_char_p, C.c_size_t, _vp, _u32, _vp, _u32,
+                             C.POINTER(_u32), C.POINTER(_u32)]),
     "spg_create_irradiance_cubemap": (_int, [_vp, _f32p, _u32, _u32, _u32, _u32,
                                              C.c_float, _int, _f32p,
                                              C.POINTER(C.c_double)]),
@@ -281,6 +286,22 @@ class Device:
             self.handle, img.ravel(), ew, eh, face_width, face_height, clamp,
             transcendentals, out, C.byref(ms)))
         return out.reshape(6, face_height, face_width, 4), ms.value
+
+    def accumulate(self, dst, src, pixels, before, added):
+        """dst = (dst * before + src * added) / (before + added) on device
+        RGBA32F buffers (progressive rendering across trace calls)."""
+        check(lib().spg_accumulate_device(self.handle, dst.ptr, src.ptr, pixels,
+                                          before, added))
+
+    def tonemap(self, src, pixels, exposure=1.0, transcendentals=TX_DOUBLE):
+        """The viewer's tone mapping of a device RGBA32F image; returns
+        RGBA8 [pixels][4] on the host."""
+        out = self.alloc(4 * pixels)
+        check(lib().spg_tonemap_device(self.handle, src.ptr, pixels, exposure,
+                                       transcendentals, out.ptr))
+        px = out.download(np.uint8, 4 * pixels).reshape(-1, 4)
+        out.free()
+        return px
 
     def last_timings(self):
         ms = (C.c_double * 6)()
@@ -693,6 +714,29 @@ def gen_sky_cubemap(size=64, sun=(0.4, 0.75, -0.5), sun_radius=0.0523599,
     check(lib().spg_gen_sky_cubemap(size, _f32(sun, 3), sun_radius, sun_radiance,
                                     out))
     return out
+
+
+def parse_obj(text):
+    """LoadMesh (src/mesh.cpp:5-62) for OBJ text: (verts [n][8] VertexPNT,
+    indices [3m]) as spg_obj_parse triangulates and joins them."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    nv, ni = _u32(), _u32()
+    check(lib().spg_obj_parse(data, len(data), None, 0, None, 0, C.byref(nv), C.byref(ni)))
+    v = np.zeros(nv.value * 8, np.float32)
+    i = np.zeros(ni.value, np.uint32)
+    check(lib().spg_obj_parse(data, len(data), v.ctypes.data, nv.value, i.ctypes.data,
+                              ni.value, C.byref(nv), C.byref(ni)))
+    return v.reshape(-1, 8), i
+
+
+def write_pfm(path, width, height, rgba):
+    """PFM file of a host RGBA32F image (row 0 at the top)."""
+    check(lib().spg_write_pfm(str(path).encode(), width, height, _f32(rgba)))
+
+
+def load_obj(path):
+    with open(path, "rb") as f:
+        return parse_obj(f.read())
 
 
 def gen_sky_equirect(width=256, height=128, sun=(0.4, 0.75, -0.5),

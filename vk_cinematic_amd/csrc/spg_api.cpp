@@ -1220,6 +1220,57 @@ int spg_intersect_closest(spg_Scene *s, uint32_t meshId, const float *ox,
     return r;
 }
 
+int spg_accumulate_device(spg_Device *d, float *dst, const float *src,
+                          uint32_t pixels, uint32_t before, uint32_t added)
+{
+    if (!d || (pixels && (!dst || !src)) || before + (uint64_t)added == 0)
+        return fail(SPG_E_INVALID, "spg_accumulate_device: bad argument");
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    HIPCHK(launch_accumulate(d->stream, (float4 *)dst, (const float4 *)src, pixels,
+                             before, added));
+    return hip_check(hipStreamSynchronize(d->stream), "accumulate");
+}
+
+int spg_tonemap_device(spg_Device *d, const float *rgba, uint32_t pixels,
+                       float exposure, int tx, uint8_t *out)
+{
+    if (!d || (pixels && (!rgba || !out)))
+        return fail(SPG_E_INVALID, "spg_tonemap_device: bad argument");
+    if (tx != SPG_TX_DOUBLE && tx != SPG_TX_FAST)
+        return fail(SPG_E_INVALID, "spg_tonemap_device: transcendentals");
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    HIPCHK(launch_tonemap(d->stream, (const float4 *)rgba, pixels, exposure,
+                          tx == SPG_TX_DOUBLE, (uchar4 *)out));
+    return hip_check(hipStreamSynchronize(d->stream), "tonemap");
+}
+
+int spg_write_pfm(const char *path, uint32_t width, uint32_t height,
+                  const float *rgba)
+{
+    if (!path || !rgba || width == 0 || height == 0)
+        return fail(SPG_E_INVALID, "spg_write_pfm: bad argument");
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(SPG_E_INVALID, "spg_write_pfm: cannot open the file");
+    fprintf(f, "PF\n%u %u\n-1.0\n", width, height);
+    std::vector<float> row(3 * (size_t)width);
+    bool ok = true;
+    for (uint32_t y = height; y-- > 0 && ok;)
+    {
+        const float *src = rgba + 4 * (size_t)y * width;
+        for (uint32_t x = 0; x < width; ++x)
+        {
+            row[3 * x] = src[4 * x];
+            row[3 * x + 1] = src[4 * x + 1];
+            row[3 * x + 2] = src[4 * x + 2];
+        }
+        ok = fwrite(row.data(), 4, row.size(), f) == row.size();
+    }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? SPG_OK : fail(SPG_E_INVALID, "spg_write_pfm: write failed");
+}
+
 // CreateIrradianceCubeMap, src/cubemap.cpp:108-233 (uniform sampling).  The
 // sample grid is the reference's float loop counters, generated here the
 // same way and handed to the kernel.

@@ -341,6 +341,10 @@ hipError_t launch_clear_counters(hipStream_t s, uint32_t *counters, int n);
 hipError_t launch_set_counter(hipStream_t s, uint32_t *counter, uint32_t v);
 hipError_t launch_clear_floats(hipStream_t s, float *p, size_t n);
 hipError_t launch_guide_infer(hipStream_t s, const TraceParams &p, int q);
+hipError_t launch_accumulate(hipStream_t s, float4 *dst, const float4 *src,
+                             uint32_t pixels, uint32_t before, uint32_t added);
+hipError_t launch_tonemap(hipStream_t s, const float4 *src, uint32_t pixels,
+                          float exposure, bool dbl, uchar4 *out);
 hipError_t launch_irradiance(hipStream_t s, const float4 *src, uint32_t ew,
                              uint32_t eh, uint32_t w, uint32_t h, const float *phis,
                              uint32_t nPhi, const float *thetas, uint32_t nTheta,
