@@ -706,6 +706,84 @@ def bench_irradiance(args, V, scenes):
     return res
 
 
+def bench_post(args, V, scenes):
+    """The step after the path on a 1920x1080 RGBA32F image: progressive
+    accumulation of a new image into the running mean, then the viewer's
+    tone mapping to RGBA8 (one step = both), plus the OBJ parser's host
+    throughput on an 81,920-triangle mesh written as text."""
+    dev = V.Device(0)
+    w, h = 1920, 1080
+    n = w * h
+    rng = np.random.default_rng(1)
+    a = rng.exponential(1.0, n * 4).astype(np.float32)
+    b = rng.exponential(1.0, n * 4).astype(np.float32)
+    acc, new = dev.upload(a), dev.upload(b)
+    out = dev.alloc(4 * n)
+    tx = V.TX_FAST if args.fast else V.TX_DOUBLE
+    for _ in range(max(1, args.warmup)):
+        dev.accumulate(acc, new, n, 1, 1)
+        V.check(V.lib().spg_tonemap_device(dev.handle, acc.ptr, n, 1.0, tx, out.ptr))
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        dev.accumulate(acc, new, n, k + 1, 1)
+        V.check(V.lib().spg_tonemap_device(dev.handle, acc.ptr, n, 1.0, tx, out.ptr))
+    wall = (time.perf_counter() - t0) / args.steps
+    # kernel times from HIP events on the library's stream
+    ka, kt = C.c_double(), C.c_double()
+    V.check(V.lib().spg_time_post(dev.handle, acc.ptr, new.ptr, n, 1.0, tx, out.ptr,
+                                  args.steps, C.byref(ka), C.byref(kt)))
+    t_acc, t_tm = ka.value * 1e-3, kt.value * 1e-3
+    # algorithmic bytes: accumulate reads two RGBA32F texels and writes one
+    # (48 B/pixel); tone mapping reads one and writes an RGBA8 texel (20 B)
+    bytes_acc, bytes_tm = n * 48, n * 20
+    # OBJ text of a displaced level-6 icosphere
+    v, idx = scenes.displaced_icosphere(6, seed=2)[:2]
+    v = np.asarray(v, np.float32).reshape(-1, 8)
+    lines = ["o mesh"] + ["v %.9g %.9g %.9g" % tuple(p[:3]) for p in v] + \
+        ["vn %.9g %.9g %.9g" % tuple(p[3:6]) for p in v] + \
+        ["f %d//%d %d//%d %d//%d" % (i + 1, i + 1, j + 1, j + 1, k + 1, k + 1)
+         for i, j, k in np.asarray(idx).reshape(-1, 3)]
+    text = ("\n".join(lines) + "\n").encode()
+    t0 = time.perf_counter()
+    _, pi = V.parse_obj(text)
+    t_obj = time.perf_counter() - t0
+    achieved = bytes_tm / t_tm / 1e9
+    res = {
+        "metric": "Mpixels/s (accumulate + tone map, 1080p)",
+        "value": n / (t_acc + t_tm) / 1e6,
+        "unit": "Mpixels/s",
+        "ms_per_step": (t_acc + t_tm) * 1e3,
+        "dtype": "f32 -> u8 (f64 pow)" if tx == V.TX_DOUBLE else "f32 -> u8",
+        "config": {"workload": "post step on 1920x1080 RGBA32F: running mean of two "
+                               "images, then Reinhard + gamma 1/2.2 to RGBA8"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tonemap", "launch_ms": t_tm * 1e3,
+                     "algorithmic_bytes_per_launch": bytes_tm},
+        "extra": {"accumulate_ms": t_acc * 1e3, "tonemap_ms": t_tm * 1e3,
+                  "accumulate_GBs": bytes_acc / t_acc / 1e9,
+                  "step_ms_with_host_sync": wall * 1e3,
+                  "obj_parse": {"triangles": int(len(pi) // 3), "bytes": len(text),
+                                "ms": t_obj * 1e3,
+                                "Mtris_per_s": len(pi) / 3 / t_obj / 1e6}},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from _libs import oracle
+        O = oracle()
+        d, q = a.copy(), np.zeros(n * 4, np.uint8)
+        t0 = time.perf_counter()
+        O.spo_accumulate(d, b, n, 1, 1)
+        O.spo_tonemap(d, n, 1.0, 1 if tx == V.TX_DOUBLE else 0, q)
+        sec = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n / sec / 1e6, "unit": "Mpixels/s", "cores": 1,
+                               "kind": "port", "sample": f"one 1080p step ({sec:.2f} s)"}
+    for buf in (acc, new, out):
+        buf.free()
+    dev.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -713,7 +791,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="config5",
                     choices=["config5", "config4", "config3", "config1", "soup",
-                             "irradiance"])
+                             "irradiance", "post"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fast", action="store_true",
                     help="device float transcendentals instead of double-evaluated")
@@ -750,7 +828,8 @@ def main():
         res = bench_box(args, V, scenes, args.workload)
     else:
         res = {"config1": bench_config1, "config3": bench_config3,
-               "soup": bench_soup, "irradiance": bench_irradiance}[args.workload](
+               "soup": bench_soup, "irradiance": bench_irradiance,
+               "post": bench_post}[args.workload](
                    args, V, scenes)
     out = {"metric": res.pop("metric"), "value": res.pop("value"),
            "unit": res.pop("unit"), "n_gpus": 1, "steps": args.steps,

@@ -392,6 +392,13 @@ int spg_tonemap_device(spg_Device *dev, const float *rgba, uint32_t pixels,
  * at the top); alpha is dropped. */
 int spg_write_pfm(const char *path, uint32_t width, uint32_t height,
                   const float *rgba);
+/* Times `iterations` accumulate launches, then `iterations` tone-map
+ * launches, with HIP events on the launch stream (arguments as in the two
+ * calls above); returns each kernel's mean ms.  dst keeps being averaged. */
+int spg_time_post(spg_Device *dev, float *dst, const float *src,
+                  uint32_t pixels, float exposure, int transcendentals,
+                  uint8_t *out, uint32_t iterations, double *accumulateMs,
+                  double *tonemapMs);
 
 /* ---- mesh files ---- */
 /* LoadMesh (src/mesh.cpp:5-62) for Wavefront OBJ text: the triangles

@@ -172,6 +172,8 @@ _SIGS = {
     "spg_gen_sky_equirect": (_int, [_u32, _u32, _f32p, C.c_float, C.c_float, _f32p]),
     "spg_accumulate_device": (_int, [_vp, _vp, _vp, _u32, _u32, _u32]),
     "spg_tonemap_device": (_int, [_vp, _vp, _u32, C.c_float, _int, _vp]),
+    "spg_time_post": (_int, [_vp, _vp, _vp, _u32, C.c_float, _int, _vp, _u32,
+                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "spg_write_pfm": (_int, [C.c_char_p, _u32, _u32, _f32p]),
     "spg_obj_parse": (_int, [C.c_char_p, C.c_size_t, _vp, _u32, _vp, _u32,
                              C.POINTER(_u32), C.POINTER(_u32)]),

@@ -1246,6 +1246,36 @@ int spg_tonemap_device(spg_Device *d, const float *rgba, uint32_t pixels,
     return hip_check(hipStreamSynchronize(d->stream), "tonemap");
 }
 
+int spg_time_post(spg_Device *d, float *dst, const float *src, uint32_t pixels,
+                  float exposure, int tx, uint8_t *out, uint32_t iterations,
+                  double *accMs, double *tmMs)
+{
+    if (!d || !pixels || !dst || !src || !out || !iterations || !accMs || !tmMs)
+        return fail(SPG_E_INVALID, "spg_time_post: bad argument");
+    if (tx != SPG_TX_DOUBLE && tx != SPG_TX_FAST)
+        return fail(SPG_E_INVALID, "spg_time_post: transcendentals");
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    hipEvent_t e0 = d->ev[kEvents - 4], e1 = d->ev[kEvents - 3],
+               e2 = d->ev[kEvents - 2];
+    HIPCHK(hipEventRecord(e0, d->stream));
+    for (uint32_t i = 0; i < iterations; ++i)
+        HIPCHK(launch_accumulate(d->stream, (float4 *)dst, (const float4 *)src,
+                                 pixels, i + 1, 1));
+    HIPCHK(hipEventRecord(e1, d->stream));
+    for (uint32_t i = 0; i < iterations; ++i)
+        HIPCHK(launch_tonemap(d->stream, (const float4 *)dst, pixels, exposure,
+                              tx == SPG_TX_DOUBLE, (uchar4 *)out));
+    HIPCHK(hipEventRecord(e2, d->stream));
+    HIPCHK(hipEventSynchronize(e2));
+    float a = 0.0f, b = 0.0f;
+    HIPCHK(hipEventElapsedTime(&a, e0, e1));
+    HIPCHK(hipEventElapsedTime(&b, e1, e2));
+    *accMs = a / iterations;
+    *tmMs = b / iterations;
+    return SPG_OK;
+}
+
 int spg_write_pfm(const char *path, uint32_t width, uint32_t height,
                   const float *rgba)
 {

@@ -189,7 +189,9 @@ struct GuideAdam
 }  // namespace spg
 
 // HIP events per device: timed intervals of a trace use [0, kEvents - 4),
-// the trace total the last two, a closest-hit timing the two before.
+// the trace total the last two, a closest-hit timing the two before; the
+// post-step timing uses kEvents - 4 .. kEvents - 2 (each call reads its
+// events before returning, under the device lock).
 constexpr int kEvents = 256;
 
 struct spg_Device
