@@ -34,6 +34,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s at 1 GPU; time-to-relMSE 1e-3, guided vs plain GPU tracer"
 HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of the closest-hit walk, per unit: a ray's 24 bytes in
+# and its 20-byte hit record out; one child box test = one of the four
+# child slots of a 64-byte four-wide node (16 bytes: its 6 quantized bytes
+# and a quarter of the origin, scales and references); one triangle test =
+# its three vertices (36 bytes); one instance entry = 128 bytes (index,
+# inverse and forward 3x4 rows, cull scale, mesh id and pointers).
+RAY_BYTES, BOX_BYTES, TRI_BYTES, ENTRY_BYTES = 44, 16, 36, 128
 # HBM bytes per traced ray of the closest-hit kernel, measured with rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE per dispatch (FETCH scaled x2.0, WRITE
 # x1.0 by a 1 GiB copy kernel on the same box); the roofline's "traffic" is
@@ -150,7 +157,7 @@ def bench_config1(args, V, scenes):
     rays = m.values[V.METRIC_RAYS]
     box = m.values[V.METRIC_AABB_TESTS]
     tri = dev.last_counters()["tri_tests"]
-    bytes_step = 44 * rays + 24 * box + 36 * tri
+    bytes_step = RAY_BYTES * rays + BOX_BYTES * box + TRI_BYTES * tri
     launches = ext_n / args.steps
     per_launch_bytes = bytes_step / launches
     mean_ms = ext_ms / ext_n
@@ -255,9 +262,8 @@ def bench_box(args, V, scenes, name):
     c = dev.last_counters()
     rays, box, tri = c["rays"], c["slab_tests"], c["tri_tests"]
     tbox, ment = c["tlas_slab_tests"], c["mesh_tests"]
-    # per instance entry: instance index, inverse and forward 3x4 rows,
-    # cull scale and mesh id, the mesh's two pointers = 128 bytes
-    bytes_step = 44 * rays + 24 * (box + tbox) + 36 * tri + 128 * ment
+    bytes_step = (RAY_BYTES * rays + BOX_BYTES * (box + tbox) + TRI_BYTES * tri +
+                  ENTRY_BYTES * ment)
     launches = ext_n / args.steps
     per_launch = bytes_step / launches
     mean_ms = ext_ms / ext_n
@@ -413,8 +419,9 @@ def bench_config3(args, V, scenes):
     tim, _ = b.profile(collect=False)
     _, cnt = b.profile(collect=True)
     rays = cnt["rays"]
-    bytes_render = (44 * rays + 24 * (cnt["slab_tests"] + cnt["tlas_slab_tests"]) +
-                    36 * cnt["tri_tests"] + 128 * cnt["mesh_tests"])
+    bytes_render = (RAY_BYTES * rays +
+                    BOX_BYTES * (cnt["slab_tests"] + cnt["tlas_slab_tests"]) +
+                    TRI_BYTES * cnt["tri_tests"] + ENTRY_BYTES * cnt["mesh_tests"])
     ext_ms, ext_n = tim["extend"]
     per_launch = bytes_render / ext_n
     mean_ms = ext_ms / ext_n
@@ -633,7 +640,7 @@ def bench_soup(args, V, scenes):
                                                outp.ptr, outt.ptr, args.steps,
                                                C.byref(ms)))
     wall = time.perf_counter() - t0
-    bytes_launch = 44 * n + 24 * st.boxTests + 36 * st.triTests
+    bytes_launch = RAY_BYTES * n + BOX_BYTES * st.boxTests + TRI_BYTES * st.triTests
     achieved = bytes_launch / (ms.value * 1e-3) / 1e9
     res = {
         "metric": "Mrays/s (closest hit)",

@@ -176,6 +176,14 @@ int spg_scene_build(spg_Scene *scene, const spg_BuildOpts *opts);
 int spg_bvh_build_host(const float *tris9, uint32_t n, const spg_BuildOpts *opts,
                        float *nodes16, uint32_t nodeCap, uint32_t *order,
                        uint32_t *nodeCount, uint32_t *maxDepth);
+/* The same build collapsed to the traversal's four-wide nodes (64 bytes:
+ * origin, per-axis scale exponents, four child references, 8-bit child
+ * boxes field-major; layout as spg_mesh_wide_export), leaf ranges into
+ * `order`.  maxPending: the walk's stack bound for this tree. */
+int spg_bvh_build_wide_host(const float *tris9, uint32_t n,
+                            const spg_BuildOpts *opts, uint8_t *nodes64,
+                            uint32_t nodeCap, uint32_t *order,
+                            uint32_t *nodeCount, uint32_t *maxPending);
 /* Node/triangle counts and depth of a built mesh BVH. */
 int spg_mesh_bvh_info(spg_Scene *scene, uint32_t meshId, uint32_t *nodes,
                       uint32_t *leaves, uint32_t *maxDepth, double *sahCost);
@@ -185,6 +193,18 @@ int spg_mesh_bvh_info(spg_Scene *scene, uint32_t meshId, uint32_t *nodes,
 int spg_mesh_bvh_export(spg_Scene *scene, uint32_t meshId, float *nodes16,
                         uint32_t nodeCap, uint32_t *primOrder,
                         uint32_t primCap, int32_t *root);
+/* Host copy of the four-wide hierarchy the traversal walks (for
+ * checkers): 64-byte nodes -- origin p (3 floats), per-axis scale
+ * exponents e (s = 2^(e-127), 3 bytes, 1 pad), four child references
+ * (int32, 0x7FFFFFFF absent, leaves as in spg_mesh_bvh_export), then the
+ * child boxes as bytes q[f][c] (field f = lo.x lo.y lo.z hi.x hi.y hi.z of
+ * child c) decoding to p + q * s, 8 pad bytes -- and the primitive order.
+ * With nodes64 and primOrder NULL only the counts are returned;
+ * maxPending is the walk's stack bound for this tree. */
+int spg_mesh_wide_export(spg_Scene *scene, uint32_t meshId, uint8_t *nodes64,
+                         uint32_t nodeCap, uint32_t *primOrder,
+                         uint32_t primCap, uint32_t *nodeCount,
+                         uint32_t *primCount, uint32_t *maxPending);
 
 /* ---- materials: sp_RegisterMaterial / sp_RegisterTexture
  *      (src/sp_material_system.cpp:1-57) and backgroundMaterialId

@@ -291,6 +291,13 @@ void spo_bvh2_closest(const float *nodes16, const uint32_t *order,
                       uint64_t *boxTests, uint64_t *triTests);
 int spo_bvh2_check(const float *nodes16, uint32_t nNodes, const uint32_t *order,
                    const float *tris9, uint32_t nTris);
+/* ---- the GPU's four-wide quantized hierarchy walk (spo_bvh2.c) ---- */
+void spo_wide_closest(const uint8_t *nodes64, const uint32_t *order,
+                      const float *tris9, const float *rays6, uint32_t n,
+                      uint32_t threads, uint32_t *primOut, float *tOut,
+                      uint64_t *boxTests, uint64_t *triTests);
+int spo_wide_check(const uint8_t *nodes64, uint32_t nNodes, const uint32_t *order,
+                   const float *tris9, uint32_t nTris, uint32_t *maxPending);
 
 /* ---- path tracing ---- */
 int spo_path_trace_tile(const spo_Scene *s, const spo_Camera *c,

@@ -21,6 +21,7 @@ REF_TEST_SO = os.path.join(ROOT, "oracle", "_ref", "libref_test.so")
 f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 u32ref = C.POINTER(C.c_uint32)
@@ -155,6 +156,10 @@ def oracle():
     _sig(L, "spo_bvh2_closest", None, f32p, u32p, f32p, f32p, C.c_uint32,
          C.c_uint32, u32p, f32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
     _sig(L, "spo_bvh2_check", C.c_int, f32p, C.c_uint32, u32p, f32p, C.c_uint32)
+    _sig(L, "spo_wide_closest", None, u8p, u32p, f32p, f32p, C.c_uint32,
+         C.c_uint32, u32p, f32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
+    _sig(L, "spo_wide_check", C.c_int, u8p, C.c_uint32, u32p, f32p, C.c_uint32,
+         C.POINTER(C.c_uint32))
     _sig(L, "spo_path_trace_tile", C.c_int, vp, C.POINTER(SpoCamera),
          C.POINTER(SpoConfig), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
          u32ref, f32p, u64p)
@@ -264,3 +269,18 @@ def f32(*a):
 
 def u32(x):
     return C.c_uint32(x)
+
+
+def wide_boxes(nodes64):
+    """Decoded child boxes of exported four-wide nodes ([n, 64] bytes):
+    (lo [n,4,3], hi [n,4,3], present [n,4]), each coordinate p + q * s in
+    float32 (the product is exact, the sum rounds once)."""
+    w = np.ascontiguousarray(nodes64, np.uint8)
+    n = len(w)
+    p = w[:, 0:12].copy().view(np.float32).reshape(n, 3)
+    s = (w[:, 12:15].astype(np.uint32) << 23).view(np.float32)
+    refs = w[:, 16:32].copy().view(np.int32).reshape(n, 4)
+    q = w[:, 32:56].reshape(n, 6, 4).transpose(0, 2, 1).astype(np.float32)
+    lo = p[:, None, :] + q[:, :, 0:3] * s[:, None, :]
+    hi = p[:, None, :] + q[:, :, 3:6] * s[:, None, :]
+    return lo, hi, refs != 0x7FFFFFFF

@@ -138,14 +138,13 @@ def test_full_size_workload(device):
         s.handle, m, *[b.ptr for b in bufs], n, outp.ptr, outt.ptr, None, None, None))
     assert np.array_equal(prim, outp.download(np.uint32, n))
 
-    info = s.bvh_info(m)
-    nodes, order = s.bvh_export(m, info["nodes"], len(tris))
+    nodes, order, _ = s.wide_export(m)
     O = oracle()
     rays = np.ascontiguousarray(np.concatenate([o, d], 1).ravel())
     tp = np.zeros(n, np.uint32)
     tt = np.zeros(n, np.float32)
     box, tri = C.c_uint64(), C.c_uint64()
-    O.spo_bvh2_closest(np.ascontiguousarray(nodes.ravel()), order,
+    O.spo_wide_closest(np.ascontiguousarray(nodes.ravel()), order,
                        np.ascontiguousarray(tris.ravel()), rays, n, 16, tp, tt,
                        C.byref(box), C.byref(tri))
     assert np.array_equal(prim, tp)

@@ -6,7 +6,7 @@ import pytest
 
 import vk_cinematic_amd as V
 from vk_cinematic_amd import scenes
-from _libs import oracle
+from _libs import oracle, wide_boxes
 
 pytestmark = pytest.mark.gpu
 
@@ -23,16 +23,14 @@ def test_axis_parallel_rays_on_box_planes_match_traversal_twin(device):
     v, i = scenes.soup_mesh(np.asarray(tris, np.float32).reshape(-1, 9))
     m = s.create_mesh(v, i, smooth=False)
     s.build()
-    info = s.bvh_info(m)
-    nodes, order = s.bvh_export(m, info["nodes"], len(tris))
-    # plane coordinates of the child boxes: columns 0-2 and 8-10 are lo, 4-6
-    # and 12-14 hi
+    nodes, order, _ = s.wide_export(m)
+    # plane coordinates of the decoded child boxes the traversal tests
+    lo, hi, present = wide_boxes(nodes)
+    planes = np.concatenate([lo, hi], axis=1)[np.concatenate([present, present], 1)]
     n = 8192
     k = rng.integers(0, 3, n)              # the axis the ray is parallel to
-    col = rng.choice([0, 4, 8, 12], n) + k
-    node = rng.integers(0, len(nodes), n)
     o = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
-    o[np.arange(n), k] = nodes[node, col]
+    o[np.arange(n), k] = planes[rng.integers(0, len(planes), n), k]
     d = rng.normal(size=(n, 3))
     d[np.arange(n), k] = 0.0
     d[: n // 8] = 0.0                      # and some along a coordinate axis
@@ -54,7 +52,7 @@ def test_axis_parallel_rays_on_box_planes_match_traversal_twin(device):
     tp = np.zeros(n, np.uint32)
     tt = np.zeros(n, np.float32)
     box, tri = C.c_uint64(), C.c_uint64()
-    O.spo_bvh2_closest(np.ascontiguousarray(nodes.ravel()), order,
+    O.spo_wide_closest(np.ascontiguousarray(nodes.ravel()), order,
                        np.ascontiguousarray(tris.ravel()), rays, n, 16, tp, tt,
                        C.byref(box), C.byref(tri))
     assert np.array_equal(prim, tp)

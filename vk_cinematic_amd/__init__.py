@@ -115,6 +115,7 @@ class TraversalStats(C.Structure):
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 _int = C.c_int
 _u32 = C.c_uint32
@@ -136,6 +137,10 @@ _SIGS = {
     "spg_scene_build": (_int, [_vp, C.POINTER(BuildOpts)]),
     "spg_bvh_build_host": (_int, [_f32p, _u32, C.POINTER(BuildOpts), _f32p, _u32,
                                   _u32p, C.POINTER(_u32), C.POINTER(_u32)]),
+    "spg_bvh_build_wide_host": (_int, [_f32p, _u32, C.POINTER(BuildOpts), _u8p, _u32,
+                                       _u32p, C.POINTER(_u32), C.POINTER(_u32)]),
+    "spg_mesh_wide_export": (_int, [_vp, _u32, _vp, _u32, _vp, _u32, C.POINTER(_u32),
+                                    C.POINTER(_u32), C.POINTER(_u32)]),
     "spg_mesh_bvh_info": (_int, [_vp, _u32, C.POINTER(_u32), C.POINTER(_u32),
                                  C.POINTER(_u32), C.POINTER(C.c_double)]),
     "spg_mesh_bvh_export": (_int, [_vp, _u32, _f32p, _u32, _u32p, _u32,
@@ -412,6 +417,18 @@ class Scene:
                                         n_prims, C.byref(root)))
         return nodes.reshape(-1, 16), prims
 
+    def wide_export(self, mesh):
+        """The four-wide hierarchy the traversal walks: (nodes [n,64]
+        uint8, primitive order, stack bound)."""
+        cnt, nprim, pend = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().spg_mesh_wide_export(self.handle, mesh, None, 0, None, 0,
+                                         C.byref(cnt), C.byref(nprim), C.byref(pend)))
+        nodes = np.zeros(64 * cnt.value, np.uint8)
+        prims = np.zeros(nprim.value, np.uint32)
+        check(lib().spg_mesh_wide_export(self.handle, mesh, nodes.ctypes.data, cnt.value,
+                                         prims.ctypes.data, nprim.value, None, None, None))
+        return nodes.reshape(-1, 64), prims, pend.value
+
     # sp_RegisterMaterial (src/sp_material_system.cpp:1-13)
     def register_material(self, id, albedo=(0, 0, 0), emission=(0, 0, 0),
                           roughness=0.0, albedo_texture=SPG_NO_TEXTURE,
@@ -668,6 +685,21 @@ def path_trace_tiles_device(ctx, tiles, device_image, metrics=None):
     check(lib().spg_path_trace_tiles_device(C.byref(ctx), arr, len(tiles),
                                             C.byref(m), device_image.ptr))
     return m
+
+
+def bvh_build_wide_host(tris9, max_leaf=4, bins=16, threads=0):
+    """The same build collapsed to four-wide quantized nodes; returns
+    (nodes [n,64] uint8, leaf primitive order, stack bound)."""
+    t = _f32(tris9)
+    n = t.size // 9
+    cap = n + 2
+    nodes = np.zeros(64 * cap, np.uint8)
+    order = np.zeros(n, np.uint32)
+    cnt, pend = C.c_uint32(), C.c_uint32()
+    opts = BuildOpts(max_leaf, bins, threads)
+    check(lib().spg_bvh_build_wide_host(t, n, C.byref(opts), nodes, cap, order,
+                                        C.byref(cnt), C.byref(pend)))
+    return nodes[: 64 * cnt.value].reshape(-1, 64), order, pend.value
 
 
 def bvh_build_host(tris9, max_leaf=4, bins=16, threads=0):

@@ -411,30 +411,27 @@ static int build_mesh(HostMesh &m, const spg_BuildOpts &opt)
         return fail(SPG_E_INVALID, "mesh BVH build failed");
     if (m.bvh.maxDepth > kMaxBlasDepth)
         return fail(SPG_E_CAPACITY, "mesh BVH deeper than the traversal stack");
-    // One allocation of 64-byte records: the nodes, then the triangles in
-    // leaf order as (a.xyz, e1.x) (e1.yz, e2.xy) (e2.z, prim, n.xy) (n.z,
+    if (!collapse_wide(m.bvh, m.wide, m.widePending))
+        return fail(SPG_E_INVALID, "mesh BVH quantization failed");
+    if (m.widePending > 3 * (kMaxBlasDepth + 1))
+        return fail(SPG_E_CAPACITY, "mesh hierarchy deeper than the traversal stack");
+    // One allocation of 64-byte records: the wide nodes, then the triangles
+    // in leaf order as (a.xyz, e1.x) (e1.yz, e2.xy) (e2.z, prim, n.xy) (n.z,
     // 0, 0, 0) with e1 = b - a, e2 = c - a, n = cross(e1, e2) (the
     // Moller-Trumbore quantities, src/ray_intersection.cpp:156-190, formed
     // with the same FP32 operations as the test).  Triangle k is record
     // nn + k, so the device copy's leaf references name records directly
     // and a traversal step fetches a node or a triangle alike.
-    const size_t nn = m.bvh.nodes.size();
+    const size_t nn = m.wide.size();
     if (nn + (size_t)tc >= (size_t)1 << 28)
         return fail(SPG_E_CAPACITY, "mesh too large for 28-bit leaf references");
     std::vector<float4> blob(4 * (nn + (size_t)tc), make_float4(0, 0, 0, 0));
-    memcpy(blob.data(), m.bvh.nodes.data(), nn * sizeof(GpuNode));
-    for (size_t j = 0; j < nn; ++j)
-    {
-        float *slots[2] = {&blob[4 * j].w, &blob[4 * j + 1].w};
-        for (float *slot : slots)
-        {
-            int32_t r;
-            memcpy(&r, slot, 4);
-            if (r == kEmptyChild || !ref_is_leaf(r)) continue;
-            r = leaf_ref((uint32_t)nn + leaf_first(r), leaf_count(r));
-            memcpy(slot, &r, 4);
-        }
-    }
+    std::vector<WideNode> nodes = m.wide;
+    for (WideNode &w : nodes)
+        for (int32_t &r : w.ref)
+            if (r != kEmptyChild && ref_is_leaf(r))
+                r = leaf_ref((uint32_t)nn + leaf_first(r), leaf_count(r));
+    memcpy(blob.data(), nodes.data(), nn * sizeof(WideNode));
     float4 *tris = blob.data() + 4 * nn;
     for (uint32_t k = 0; k < tc; ++k)
     {
@@ -572,18 +569,19 @@ int spg_scene_build(spg_Scene *s, const spg_BuildOpts *opts)
         build_bvh(in, s->tlas);
         if (s->tlas.maxDepth > kMaxTlasDepth)
             return fail(SPG_E_CAPACITY, "instance BVH too deep");
+        std::vector<WideNode> nodes;
+        if (!collapse_wide(s->tlas, nodes, s->tlasPending))
+            return fail(SPG_E_INVALID, "instance BVH quantization failed");
+        uint32_t meshPending = 0;
+        for (auto &m : s->meshes) meshPending = std::max(meshPending, m.widePending);
+        if (s->tlasPending + meshPending > (uint32_t)kStackDepth)
+            return fail(SPG_E_CAPACITY, "scene hierarchy deeper than the traversal stack");
         // one instance per leaf: the device copy's leaf references name the
         // instance itself instead of its position in the leaf order
-        std::vector<GpuNode> nodes = s->tlas.nodes;
-        for (GpuNode &nd : nodes)
-            for (float *slot : {&nd.a[3], &nd.b[3]})
-            {
-                int32_t r;
-                memcpy(&r, slot, 4);
-                if (r == kEmptyChild || !ref_is_leaf(r)) continue;
-                r = leaf_ref(s->tlas.prims[leaf_first(r)], 1);
-                memcpy(slot, &r, 4);
-            }
+        for (WideNode &w : nodes)
+            for (int32_t &r : w.ref)
+                if (r != kEmptyChild && ref_is_leaf(r))
+                    r = leaf_ref(s->tlas.prims[leaf_first(r)], 1);
         CHK(upload(&s->dTlasNodes, nodes.data(), nodes.size()));
     }
     DevScene &ds = s->dev_scene;
@@ -595,35 +593,67 @@ int spg_scene_build(spg_Scene *s, const spg_BuildOpts *opts)
     return SPG_OK;
 }
 
+// The mesh build of a host triangle list (tris9 = a,b,c per triangle).
+static int host_build(const char *who, const float *tris9, uint32_t n,
+                      const spg_BuildOpts *opts, BvhBuildResult &b)
+{
+    spg_BuildOpts opt{4, 16, 0};
+    if (opts) opt = *opts;
+    if (opt.maxLeafSize == 0) opt.maxLeafSize = 4;
+    if (opt.maxLeafSize > 8)
+        return fail(SPG_E_INVALID, std::string(who) + ": maxLeafSize > 8");
+    std::vector<float> lo(3 * (size_t)n), hi(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k)
+        {
+            const float *t = tris9 + 9 * (size_t)i;
+            if (!std::isfinite(t[k]) || !std::isfinite(t[3 + k]) ||
+                !std::isfinite(t[6 + k]))
+                return fail(SPG_E_INVALID, std::string(who) + ": non-finite vertex");
+            lo[3 * i + k] = fmin_sel(t[k], fmin_sel(t[3 + k], t[6 + k]));
+            hi[3 * i + k] = fmax_sel(t[k], fmax_sel(t[3 + k], t[6 + k]));
+        }
+    BvhBuildInput in{lo.data(), hi.data(), n, opt.maxLeafSize,
+                     opt.sahBins ? opt.sahBins : 16, opt.threads, kMaxBlasDepth};
+    build_bvh(in, b);
+    return SPG_OK;
+}
+
 int spg_bvh_build_host(const float *tris9, uint32_t n, const spg_BuildOpts *opts,
                        float *nodes16, uint32_t nodeCap, uint32_t *order,
                        uint32_t *nodeCount, uint32_t *maxDepth)
 {
     if (!tris9 || n == 0 || !nodes16 || !order || !nodeCount)
         return fail(SPG_E_INVALID, "spg_bvh_build_host: bad argument");
-    spg_BuildOpts opt{4, 16, 0};
-    if (opts) opt = *opts;
-    if (opt.maxLeafSize == 0) opt.maxLeafSize = 4;
-    if (opt.maxLeafSize > 8)
-        return fail(SPG_E_INVALID, "spg_bvh_build_host: maxLeafSize > 8");
-    std::vector<float> lo(3 * (size_t)n), hi(3 * (size_t)n);
-    for (uint32_t i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k)
-        {
-            const float *t = tris9 + 9 * (size_t)i;
-            lo[3 * i + k] = fmin_sel(t[k], fmin_sel(t[3 + k], t[6 + k]));
-            hi[3 * i + k] = fmax_sel(t[k], fmax_sel(t[3 + k], t[6 + k]));
-        }
     BvhBuildResult b;
-    BvhBuildInput in{lo.data(), hi.data(), n, opt.maxLeafSize,
-                     opt.sahBins ? opt.sahBins : 16, opt.threads, kMaxBlasDepth};
-    build_bvh(in, b);
+    CHK(host_build("spg_bvh_build_host", tris9, n, opts, b));
     if (b.nodes.size() > nodeCap)
         return fail(SPG_E_CAPACITY, "spg_bvh_build_host: nodeCap too small");
     memcpy(nodes16, b.nodes.data(), 64 * b.nodes.size());
     memcpy(order, b.prims.data(), 4 * b.prims.size());
     *nodeCount = (uint32_t)b.nodes.size();
     if (maxDepth) *maxDepth = b.maxDepth;
+    return SPG_OK;
+}
+
+int spg_bvh_build_wide_host(const float *tris9, uint32_t n, const spg_BuildOpts *opts,
+                            uint8_t *nodes64, uint32_t nodeCap, uint32_t *order,
+                            uint32_t *nodeCount, uint32_t *maxPending)
+{
+    if (!tris9 || n == 0 || !nodes64 || !order || !nodeCount)
+        return fail(SPG_E_INVALID, "spg_bvh_build_wide_host: bad argument");
+    BvhBuildResult b;
+    CHK(host_build("spg_bvh_build_wide_host", tris9, n, opts, b));
+    std::vector<WideNode> w;
+    uint32_t pend = 0;
+    if (!collapse_wide(b, w, pend))
+        return fail(SPG_E_INVALID, "spg_bvh_build_wide_host: quantization failed");
+    if (w.size() > nodeCap)
+        return fail(SPG_E_CAPACITY, "spg_bvh_build_wide_host: nodeCap too small");
+    memcpy(nodes64, w.data(), 64 * w.size());
+    memcpy(order, b.prims.data(), 4 * b.prims.size());
+    *nodeCount = (uint32_t)w.size();
+    if (maxPending) *maxPending = pend;
     return SPG_OK;
 }
 
@@ -652,6 +682,26 @@ int spg_mesh_bvh_export(spg_Scene *s, uint32_t meshId, float *nodes16,
     memcpy(nodes16, b.nodes.data(), 64 * b.nodes.size());
     memcpy(primOrder, b.prims.data(), 4 * b.prims.size());
     if (root) *root = 0;
+    return SPG_OK;
+}
+
+int spg_mesh_wide_export(spg_Scene *s, uint32_t meshId, uint8_t *nodes64,
+                         uint32_t nodeCap, uint32_t *primOrder, uint32_t primCap,
+                         uint32_t *nodeCount, uint32_t *primCount,
+                         uint32_t *maxPending)
+{
+    if (!s || meshId >= s->meshes.size() || !s->meshes[meshId].built)
+        return fail(SPG_E_STATE, "spg_mesh_wide_export: mesh not built");
+    const HostMesh &m = s->meshes[meshId];
+    if (nodeCount) *nodeCount = (uint32_t)m.wide.size();
+    if (primCount) *primCount = (uint32_t)m.bvh.prims.size();
+    if (maxPending) *maxPending = m.widePending;
+    if (!nodes64 && !primOrder) return SPG_OK;  // size query
+    if (!nodes64 || !primOrder || nodeCap < m.wide.size() ||
+        primCap < m.bvh.prims.size())
+        return fail(SPG_E_CAPACITY, "spg_mesh_wide_export: buffers too small");
+    memcpy(nodes64, m.wide.data(), 64 * m.wide.size());
+    memcpy(primOrder, m.bvh.prims.data(), 4 * m.bvh.prims.size());
     return SPG_OK;
 }
 

@@ -333,4 +333,141 @@ bool build_bvh(const BvhBuildInput &in, BvhBuildResult &out)
     return true;
 }
 
+namespace {
+
+struct WideChild
+{
+    float lo[3], hi[3];
+    int32_t ref;
+    double area() const
+    {
+        double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1],
+               dz = (double)hi[2] - lo[2];
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+// The two child slots of a BVH2 node (its padded boxes).
+void bvh2_children(const GpuNode &g, std::vector<WideChild> &out)
+{
+    WideChild a;
+    memcpy(a.lo, g.a, 12);
+    memcpy(a.hi, g.b, 12);
+    memcpy(&a.ref, &g.a[3], 4);
+    out.push_back(a);
+    int32_t r1;
+    memcpy(&r1, &g.b[3], 4);
+    if (r1 != kEmptyChild)
+    {
+        WideChild b;
+        memcpy(b.lo, g.c, 12);
+        memcpy(b.hi, g.d, 12);
+        b.ref = r1;
+        out.push_back(b);
+    }
+}
+
+// Outward quantization of the children's boxes on axis k; false if the
+// boxes are not finite.
+bool quantize_axis(WideNode &w, const std::vector<WideChild> &kids, int k)
+{
+    float lo = INFINITY, hi = -INFINITY;
+    for (const WideChild &c : kids)
+    {
+        if (!std::isfinite(c.lo[k]) || !std::isfinite(c.hi[k])) return false;
+        lo = std::min(lo, c.lo[k]);
+        hi = std::max(hi, c.hi[k]);
+    }
+    w.p[k] = lo;
+    const double ext = (double)hi - (double)lo;
+    int e = 1;  // scale 2^-126 at least (normal, so q * s is exact)
+    while (e < 254 && 255.0 * std::ldexp(1.0, e - 127) < ext) ++e;
+    for (; e < 255; ++e)
+    {
+        w.e[k] = (uint8_t)e;
+        const float s = wide_scale(w.e[k]);
+        bool ok = true;
+        for (size_t c = 0; c < kids.size() && ok; ++c)
+        {
+            double ql = std::floor(((double)kids[c].lo[k] - lo) / s);
+            double qh = std::ceil(((double)kids[c].hi[k] - lo) / s);
+            long a = (long)std::max(0.0, std::min(255.0, ql));
+            long b = (long)std::max(0.0, std::min(256.0, qh));
+            // decoded as the traversal decodes: p + q * s in FP32
+            while (a > 0 && lo + (float)a * s > kids[c].lo[k]) --a;
+            while (b <= 255 && lo + (float)b * s < kids[c].hi[k]) ++b;
+            if (b > 255 || lo + (float)a * s > kids[c].lo[k]) ok = false;
+            else
+            {
+                w.q[k][c] = (uint8_t)a;
+                w.q[3 + k][c] = (uint8_t)b;
+            }
+        }
+        if (ok) return true;
+    }
+    return false;
+}
+
+struct WideBuilder
+{
+    const std::vector<GpuNode> &in;
+    std::vector<WideNode> &out;
+    bool ok = true;
+
+    // returns the node's index; pending: its subtree's stack bound
+    uint32_t emit(uint32_t n2, uint32_t &pending)
+    {
+        std::vector<WideChild> kids;
+        bvh2_children(in[n2], kids);
+        while (kids.size() < 4)
+        {
+            int best = -1;
+            double bestArea = -1.0;
+            for (size_t c = 0; c < kids.size(); ++c)
+                if (kids[c].ref >= 0 && kids[c].area() > bestArea)
+                {
+                    best = (int)c;
+                    bestArea = kids[c].area();
+                }
+            if (best < 0) break;
+            std::vector<WideChild> sub;
+            bvh2_children(in[(size_t)kids[best].ref], sub);
+            if (kids.size() + sub.size() - 1 > 4) break;
+            kids[best] = sub[0];
+            for (size_t j = 1; j < sub.size(); ++j)
+                kids.insert(kids.begin() + best + (long)j, sub[j]);
+        }
+        const uint32_t me = (uint32_t)out.size();
+        out.push_back(WideNode{});
+        WideNode w{};
+        for (int c = 0; c < 4; ++c) w.ref[c] = kEmptyChild;
+        for (int k = 0; k < 3 && ok; ++k) ok = quantize_axis(w, kids, k);
+        uint32_t below = 0;
+        for (size_t c = 0; c < kids.size(); ++c)
+        {
+            uint32_t sub = 0;
+            w.ref[c] = kids[c].ref >= 0 ? (int32_t)emit((uint32_t)kids[c].ref, sub)
+                                        : kids[c].ref;
+            below = std::max(below, sub);
+        }
+        pending = (uint32_t)kids.size() - 1 + below;
+        out[me] = w;
+        return me;
+    }
+};
+
+}  // namespace
+
+bool collapse_wide(const BvhBuildResult &bvh2, std::vector<WideNode> &out,
+                   uint32_t &maxPending)
+{
+    out.clear();
+    maxPending = 0;
+    if (bvh2.nodes.empty()) return false;
+    out.reserve(bvh2.nodes.size() / 2 + 1);
+    WideBuilder wb{bvh2.nodes, out};
+    wb.emit(0, maxPending);
+    return wb.ok;
+}
+
 }  // namespace spg

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <stdint.h>
+#include <string.h>
 #include <vector>
 
 namespace spg {
@@ -58,5 +59,56 @@ struct BvhBuildResult
 // 2^-20 so that a ray whose Moller-Trumbore test accepts a triangle also
 // passes the box tests of every node above it.  Returns false if count==0.
 bool build_bvh(const BvhBuildInput &in, BvhBuildResult &out);
+
+// 64-byte four-child node with 8-bit quantized child boxes:
+//   p      quantization origin (float x3)
+//   e      per axis scale s = 2^(e - 127) (byte x3)
+//   ref    child references as in GpuNode (kEmptyChild: absent; present
+//          children first)
+//   q      field-major: q[f][c] is field f (lo.x lo.y lo.z hi.x hi.y hi.z)
+//          of child c, so one 32-bit word holds one field of all four
+//          children; the box is lo = p + q_lo * s, hi = p + q_hi * s in
+//          FP32 (the product is exact), and contains the BVH2 child box it
+//          replaces
+struct alignas(16) WideNode
+{
+    float p[3];
+    uint8_t e[3];
+    uint8_t pad0;
+    int32_t ref[4];
+    uint8_t q[6][4];
+    uint8_t pad1[8];
+};
+static_assert(sizeof(WideNode) == 64, "WideNode must be 64 bytes");
+
+// Collapses a BVH2 into wide nodes: a node starts from the two children of
+// a BVH2 node and opens the inner child of largest surface area until it
+// has four children or only leaves (children keep their expansion order).
+// Depth-first order, root 0; leaf references unchanged.  maxPending: the
+// most references a closest-hit walk holds on its stack (over root-to-node
+// paths, the sum of the nodes' present children less one).  Returns false
+// if quantization fails (non-finite boxes).
+bool collapse_wide(const BvhBuildResult &bvh2, std::vector<WideNode> &out,
+                   uint32_t &maxPending);
+
+// The decoded box of child c on one axis (what the traversal computes).
+inline float wide_lo(const WideNode &n, int c, int k);
+inline float wide_hi(const WideNode &n, int c, int k);
+
+inline float wide_scale(uint8_t e)
+{
+    uint32_t bits = (uint32_t)e << 23;
+    float s;
+    memcpy(&s, &bits, 4);
+    return s;
+}
+inline float wide_lo(const WideNode &n, int c, int k)
+{
+    return n.p[k] + (float)n.q[k][c] * wide_scale(n.e[k]);
+}
+inline float wide_hi(const WideNode &n, int c, int k)
+{
+    return n.p[k] + (float)n.q[3 + k][c] * wide_scale(n.e[k]);
+}
 
 }  // namespace spg

@@ -17,11 +17,15 @@
 
 namespace spg {
 
-// Traversal stack depth per lane, in LDS.  The builder bounds the tree depth
-// so a TLAS path plus a BLAS path always fits (see kMaxBlasDepth).
-constexpr int kStackDepth = 64;
+// Traversal stack depth per lane (the top entries in LDS, the rest in a
+// global spill area).  The builder bounds the binary trees' depths; a
+// four-wide node collapses at least one binary level on every path through
+// it and leaves at most three siblings pending, so a walk holds at most
+// 3 * (kMaxTlasDepth + 1 + kMaxBlasDepth + 1) references.  Scene builds
+// also check the exact bound of their trees.
 constexpr uint32_t kMaxTlasDepth = 20;
-constexpr uint32_t kMaxBlasDepth = kStackDepth - kMaxTlasDepth - 2;
+constexpr uint32_t kMaxBlasDepth = 42;
+constexpr int kStackDepth = 3 * (kMaxTlasDepth + 1 + kMaxBlasDepth + 1);
 constexpr int kTraceBlock = 64;  // traversal blocks are one wave
 // Resident waves per CU a persistent traversal launch may use (sizes the
 // global spill area of the traversal stacks).
@@ -42,7 +46,7 @@ constexpr float kCullSlack = 1.0f + 1.0f / 4096.0f;
 
 struct DevMesh
 {
-    const float4 *nodes;    // GpuNode, 4 float4 each
+    const float4 *nodes;    // WideNode, 4 float4 each
     const float4 *tris;     // triangle records (4 float4 each) in leaf order,
                             // right after the nodes in the same allocation
     const float *verts;     // VertexPNT, 8 floats
@@ -59,7 +63,7 @@ struct DevInstance
     m34 model;          // rows 0..2 of M = T*R*S       (src/sp_scene.cpp:91)
     m34 inv;            // rows 0..2 of S^-1*R(q*)*T(-p) (src/sp_scene.cpp:94-96)
     float cullScale;    // 1 / min |scale|: world t bound -> local t bound
-    uint32_t nodeCount; // the mesh's node records (triangle k is nodeCount + k)
+    uint32_t nodeCount; // the mesh's wide node records (triangle k is nodeCount + k)
     const float4 *nodes;// the mesh's 64-byte records
     uint32_t mesh;
     int32_t material;   // dense material slot, -1 = not registered
@@ -90,7 +94,7 @@ struct DevTexture
 
 struct DevScene
 {
-    const float4 *tlasNodes;    // leaf references hold the instance index
+    const float4 *tlasNodes;    // WideNode; leaf references hold the instance index
     const DevInstance *inst;
     const DevMesh *meshes;
     const DevMaterial *mats;
@@ -250,6 +254,8 @@ struct HostMesh
     std::vector<uint32_t> idx;
     int smooth = 0;
     spg::BvhBuildResult bvh;
+    std::vector<spg::WideNode> wide;  // the traversal's four-wide form of bvh
+    uint32_t widePending = 0;         // its walk's stack bound
     bool built = false;
     // device copies
     // one allocation of 64-byte records: nodes, then triangles in leaf
@@ -285,6 +291,7 @@ struct spg_Scene
     bool built = false;
     bool dirty = true;
     spg::BvhBuildResult tlas;
+    uint32_t tlasPending = 0;       // stack bound of the wide instance tree
     // device-side tables
     void *dTlasNodes = nullptr, *dInst = nullptr,
          *dMeshes = nullptr, *dMats = nullptr, *dTexs = nullptr,

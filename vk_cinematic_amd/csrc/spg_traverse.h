@@ -2,14 +2,15 @@
 // that refill idle lanes with new rays.
 //
 // One loop iteration of a wave = one step per active lane, and every step
-// is one 64-byte fetch plus the arithmetic on it: a BVH2 node visit (both
-// child slab tests), one triangle of the lane's current leaf, entering an
+// is one 64-byte fetch plus the arithmetic on it: a four-wide node visit
+// (decode and slab-test its quantized child boxes, order the hits by entry
+// distance), one triangle of the lane's current leaf, entering an
 // instance (its inverse transform and BVH) or leaving one after a hit (its
 // model transform, for the world-space t).  All lanes issue their fetch
 // together at the top of the step, so a wave whose lanes hold different
 // kinds of work waits for one memory round trip per iteration, not one per
-// kind.  A hit leaf child becomes the lane's leaf range (the nearer child
-// first, a farther one waits on the stack like an inner node).  A lane whose
+// kind.  A hit leaf child becomes the lane's leaf range (the nearest child
+// first, farther ones wait on the stack like inner nodes).  A lane whose
 // ray finishes takes the next ray from its wave's chunk of the ray list, so
 // the wave's SIMD lanes stay busy instead of idling behind the wave's longest
 // ray.  The two levels of a scene -- instance hierarchy (TLAS) and mesh
@@ -19,8 +20,8 @@
 //
 // Stack: the top kShortStack entries of each lane live in LDS (4 KB per
 // wave); deeper entries spill to a global array (rare: the builder bounds
-// tree depth, and a closest-hit walk holds at most one pending far child per
-// level).
+// tree depth, and a closest-hit walk holds at most three pending children
+// per wide level).
 #pragma once
 
 #include "spg_internal.h"
@@ -203,7 +204,10 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
     const bool isFin = kTwoLevel && active && mode == kModeFinish;
     Hit c{0.0f, 0.0f, 0.0f, 0u};
     bool take = false;
-    int32_t next = kNoRef, far = kNoRef;
+    int32_t next = kNoRef;
+    // pending children of a node visit, stored at sp, sp + 1, sp + 2
+    int32_t push0 = kNoRef, push1 = kNoRef, push2 = kNoRef;
+    int npush = 0;
     if (isTri)
         take = triangle_candidate(A, B, C, D, s.lo, s.ld, s.h, c);
     else if (isNode)
@@ -218,44 +222,104 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
         }
         else
             tc = s.out.t > 0.0f ? s.out.t * kCullSlack : INFINITY;
-        // both boxes are tested unconditionally (an empty second slot holds
-        // an unused box)
-        int32_t r0 = __float_as_int(A.w), r1 = __float_as_int(B.w);
-        float e0, e1;
-        bool s0 = slab_hit_mm(A.x, A.y, A.z, B.x, B.y, B.z, o, inv, &e0);
-        bool s1 = slab_hit_mm(C.x, C.y, C.z, D.x, D.y, D.z, o, inv, &e1);
+        // A: origin p and scale exponents; B: child references; C, D: the
+        // child boxes, one field of all four children per 32-bit word.  A
+        // coordinate decodes to p + q * s: q * s is exact (8-bit integer
+        // times a power of two), so the FMA rounds the same sum once.
+        const uint32_t eb = __float_as_uint(A.w);
+        const float sx = __uint_as_float((eb & 0xFFu) << 23);
+        const float sy = __uint_as_float(((eb >> 8) & 0xFFu) << 23);
+        const float sz = __uint_as_float(((eb >> 16) & 0xFFu) << 23);
+        const uint32_t q0 = __float_as_uint(C.x), q1 = __float_as_uint(C.y),
+                       q2 = __float_as_uint(C.z), q3 = __float_as_uint(C.w),
+                       q4 = __float_as_uint(D.x), q5 = __float_as_uint(D.y);
+        const int32_t ref[4] = {__float_as_int(B.x), __float_as_int(B.y),
+                                __float_as_int(B.z), __float_as_int(B.w)};
+        auto box = [&](int k, v3 &lo, v3 &hi) {
+            lo.x = fmaf((float)((q0 >> (8 * k)) & 0xFFu), sx, A.x);
+            lo.y = fmaf((float)((q1 >> (8 * k)) & 0xFFu), sy, A.y);
+            lo.z = fmaf((float)((q2 >> (8 * k)) & 0xFFu), sz, A.z);
+            hi.x = fmaf((float)((q3 >> (8 * k)) & 0xFFu), sx, A.x);
+            hi.y = fmaf((float)((q4 >> (8 * k)) & 0xFFu), sy, A.y);
+            hi.z = fmaf((float)((q5 >> (8 * k)) & 0xFFu), sz, A.z);
+        };
+        float e[4];
+        bool hit[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            v3 lo, hi;
+            box(k, lo, hi);
+            hit[k] = slab_hit_mm(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, o, inv, &e[k]);
+        }
         if (!(blas ? s.mmL : s.mmW))
         {
             // a non-finite origin or inverse direction (an axis-parallel
             // ray): the reference's compare-and-select form
-            float f0, f1;
-            bool q0 = slab_hit(A.x, A.y, A.z, B.x, B.y, B.z, o, inv, &f0);
-            bool q1 = slab_hit(C.x, C.y, C.z, D.x, D.y, D.z, o, inv, &f1);
-            s0 = q0;
-            s1 = q1;
-            e0 = f0;
-            e1 = f1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+            {
+                v3 lo, hi;
+                box(k, lo, hi);
+                hit[k] = slab_hit(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, o, inv, &e[k]);
+            }
         }
-        bool h0 = s0 & (e0 <= tc);
-        bool h1 = (r1 != kEmptyChild) & s1 & (e1 <= tc);
+        // hit children ordered by entry distance: the key is the entry's
+        // bits with the sign cleared (-0 = +0) and the two low bits replaced
+        // by the child slot, so keys are distinct and near-ties go to the
+        // lower slot; a missed or absent child sorts last
+        uint32_t key[4];
+        int m = 0;
+        uint32_t present = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const bool has = ref[k] != kEmptyChild;
+            const bool h = has & hit[k] & (e[k] <= tc);
+            key[k] = h ? ((__float_as_uint(e[k]) & 0x7FFFFFFCu) | (uint32_t)k)
+                       : 0xFFFFFFFFu;
+            m += h ? 1 : 0;
+            present += has ? 1u : 0u;
+        }
         if (kCount)
         {
-            uint32_t n = r1 != kEmptyChild ? 2u : 1u;
-            if (blas) cnt.box += n;
-            else cnt.tlasBox += n;
+            if (blas) cnt.box += present;
+            else cnt.tlasBox += present;
         }
-        // near child first, the far one waits on the stack
-        const bool both = h0 & h1, swap = e1 < e0;
-        far = both ? (swap ? r0 : r1) : kNoRef;
-        next = both ? (swap ? r1 : r0) : h0 ? r0 : h1 ? r1 : kNoRef;
+        auto cx = [](uint32_t &a, uint32_t &b) {
+            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+            a = lo;
+            b = hi;
+        };
+        cx(key[0], key[1]);
+        cx(key[2], key[3]);
+        cx(key[0], key[2]);
+        cx(key[1], key[3]);
+        cx(key[1], key[2]);
+        auto pick = [&](uint32_t k) {
+            const int32_t a = (k & 1u) ? ref[1] : ref[0];
+            const int32_t b = (k & 1u) ? ref[3] : ref[2];
+            return (k & 2u) ? b : a;
+        };
+        const int32_t c0 = pick(key[0]), c1 = pick(key[1]), c2 = pick(key[2]),
+                      c3 = pick(key[3]);
+        // the nearest is next; the others wait on the stack, the nearest
+        // of them on top
+        next = m > 0 ? c0 : kNoRef;
+        push0 = m == 4 ? c3 : m == 3 ? c2 : c1;
+        push1 = m == 4 ? c2 : c1;
+        push2 = c1;
+        npush = m > 1 ? m - 1 : 0;
     }
     s.h.t = take ? c.t : s.h.t;
     s.h.u = take ? c.u : s.h.u;
     s.h.v = take ? c.v : s.h.v;
     s.h.prim = take ? c.prim : s.h.prim;
     s.triNext += isTri ? 1u : 0u;
-    if (far != kNoRef) st.write(s.sp, far);
-    s.sp += far != kNoRef ? 1 : 0;
+    if (npush > 0) st.write(s.sp, push0);
+    if (npush > 1) st.write(s.sp + 1, push1);
+    if (npush > 2) st.write(s.sp + 2, push2);
+    s.sp += npush;
     const bool busy = isTri && s.triNext < s.triEnd;
 
     if (kTwoLevel)
