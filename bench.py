@@ -35,17 +35,21 @@ sys.path.insert(0, ROOT)
 METRIC = "Msamples/s at 1 GPU; time-to-relMSE 1e-3, guided vs plain GPU tracer"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of the closest-hit walk, per unit: a ray's 24 bytes in
-# and its 20-byte hit record out; one child box test = one of the four
-# child slots of a 64-byte four-wide node (16 bytes: its 6 quantized bytes
-# and a quarter of the origin, scales and references); one triangle test =
-# its three vertices (36 bytes); one instance entry = 128 bytes (index,
-# inverse and forward 3x4 rows, cull scale, mesh id and pointers).
-RAY_BYTES, BOX_BYTES, TRI_BYTES, ENTRY_BYTES = 44, 16, 36, 128
+# and its hit record out (8 bytes, triangle and t, for the closest-hit
+# kernel; 20 bytes, t, u, v, instance, triangle, for the path tracer's
+# extension kernel); one box test = a full-precision box (24 bytes; the
+# four-wide nodes store a child box in 6 quantized bytes plus a quarter of
+# the node's 32-byte header, so the kernels fetch less than this); one
+# triangle test = its three vertices (36 bytes); one instance entry = 128
+# bytes (index, inverse and forward 3x4 rows, cull scale, mesh id and
+# pointers).
+RAY_BYTES, HIT_BYTES_MESH, HIT_BYTES_SCENE = 24, 8, 20
+BOX_BYTES, TRI_BYTES, ENTRY_BYTES = 24, 36, 128
 # HBM bytes per traced ray of the closest-hit kernel, measured with rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE per dispatch (FETCH scaled x2.0, WRITE
-# x1.0 by a 1 GiB copy kernel on the same box); the roofline's "traffic" is
-# this times the rays of one launch.
-TRAFFIC_BYTES_PER_RAY = {"config5": 1.701e9 / 2.04e6, "soup": 1.735e9 / 4194304}
+# x1.0 by a 1 GiB copy kernel on the same box; config5: the plain tracer's
+# launches); the roofline's "traffic" is this times the rays of one launch.
+TRAFFIC_BYTES_PER_RAY = {"config5": 1.455e9 / 2037010, "soup": 1.079e9 / 4194304}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same kernel and "
                   "workload (profiles/hbm_traffic_config5_soup.md), FETCH x2.0 and "
                   "WRITE x1.0 by a 1 GiB copy calibration")
@@ -86,26 +90,33 @@ def cpu_baseline_config1(threads, spp=16):
                       f"estimator, median of 3 after 1 warm-up ({sec:.2f} s each)"}
 
 
-def cpu_baseline_soup(threads, tris, o, d, n_rays=65536):
-    """The oracle's restatement of the closest-hit traversal on the same BVH,
-    all host threads, on the first n_rays rays."""
+def cpu_baseline_soup(threads, tris, o, d, repeats=3):
+    """The oracle's restatement of the closest-hit traversal (the binary SAH
+    hierarchy of the same build), all host threads, on all 2^22 rays: one
+    warm-up pass, then the median of `repeats` timed passes."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _libs import oracle
     import vk_cinematic_amd as V
 
     nodes, order, _ = V.bvh_build_host(tris)
-    rays = np.ascontiguousarray(np.concatenate([o[:n_rays], d[:n_rays]], 1).ravel())
-    p = np.zeros(n_rays, np.uint32)
-    t = np.zeros(n_rays, np.float32)
+    n = len(o)
+    rays = np.ascontiguousarray(np.concatenate([o, d], 1).ravel())
+    p = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float32)
     O = oracle()
-    t0 = time.perf_counter()
-    O.spo_bvh2_closest(np.ascontiguousarray(nodes.ravel()), order,
-                       np.ascontiguousarray(tris.ravel()), rays, n_rays, threads, p, t,
-                       None, None)
-    sec = time.perf_counter() - t0
-    return {"value": n_rays / sec / 1e6, "unit": "Mrays/s", "cores": threads,
+    nodes = np.ascontiguousarray(nodes.ravel())
+    tris = np.ascontiguousarray(tris.ravel())
+    secs = []
+    for k in range(repeats + 1):
+        t0 = time.perf_counter()
+        O.spo_bvh2_closest(nodes, order, tris, rays, n, threads, p, t, None, None)
+        if k:
+            secs.append(time.perf_counter() - t0)
+    sec = float(np.median(secs))
+    return {"value": n / sec / 1e6, "unit": "Mrays/s", "cores": threads,
             "kind": "port",
-            "sample": f"{n_rays} of the 2^22 rays, same SAH BVH ({sec:.2f} s)"}
+            "sample": f"all {n} rays, same SAH build, median of {repeats} passes "
+                      f"after a warm-up ({sec:.2f} s per pass)"}
 
 
 def cpu_baseline_irradiance(sky, fw, fh):
@@ -157,7 +168,7 @@ def bench_config1(args, V, scenes):
     rays = m.values[V.METRIC_RAYS]
     box = m.values[V.METRIC_AABB_TESTS]
     tri = dev.last_counters()["tri_tests"]
-    bytes_step = RAY_BYTES * rays + BOX_BYTES * box + TRI_BYTES * tri
+    bytes_step = (RAY_BYTES + HIT_BYTES_SCENE) * rays + BOX_BYTES * box + TRI_BYTES * tri
     launches = ext_n / args.steps
     per_launch_bytes = bytes_step / launches
     mean_ms = ext_ms / ext_n
@@ -193,7 +204,7 @@ def bench_config1(args, V, scenes):
     return res
 
 
-def cpu_baseline_box(desc, threads, spp=4, bounces=6):
+def cpu_baseline_box(desc, threads, spp=8, bounces=6):
     """The oracle on the same scene and estimator (plain unbiased tracer,
     per-pixel PCG) over the same SAH hierarchy (the reference's greedy
     builder is O(n^2 log n) and its 32/128 overlap buffers overflow at this
@@ -262,8 +273,8 @@ def bench_box(args, V, scenes, name):
     c = dev.last_counters()
     rays, box, tri = c["rays"], c["slab_tests"], c["tri_tests"]
     tbox, ment = c["tlas_slab_tests"], c["mesh_tests"]
-    bytes_step = (RAY_BYTES * rays + BOX_BYTES * (box + tbox) + TRI_BYTES * tri +
-                  ENTRY_BYTES * ment)
+    bytes_step = ((RAY_BYTES + HIT_BYTES_SCENE) * rays + BOX_BYTES * (box + tbox) +
+                  TRI_BYTES * tri + ENTRY_BYTES * ment)
     launches = ext_n / args.steps
     per_launch = bytes_step / launches
     mean_ms = ext_ms / ext_n
@@ -419,7 +430,7 @@ def bench_config3(args, V, scenes):
     tim, _ = b.profile(collect=False)
     _, cnt = b.profile(collect=True)
     rays = cnt["rays"]
-    bytes_render = (RAY_BYTES * rays +
+    bytes_render = ((RAY_BYTES + HIT_BYTES_SCENE) * rays +
                     BOX_BYTES * (cnt["slab_tests"] + cnt["tlas_slab_tests"]) +
                     TRI_BYTES * cnt["tri_tests"] + ENTRY_BYTES * cnt["mesh_tests"])
     ext_ms, ext_n = tim["extend"]
@@ -640,7 +651,8 @@ def bench_soup(args, V, scenes):
                                                outp.ptr, outt.ptr, args.steps,
                                                C.byref(ms)))
     wall = time.perf_counter() - t0
-    bytes_launch = RAY_BYTES * n + BOX_BYTES * st.boxTests + TRI_BYTES * st.triTests
+    bytes_launch = ((RAY_BYTES + HIT_BYTES_MESH) * n + BOX_BYTES * st.boxTests +
+                    TRI_BYTES * st.triTests)
     achieved = bytes_launch / (ms.value * 1e-3) / 1e9
     res = {
         "metric": "Mrays/s (closest hit)",
