@@ -126,6 +126,25 @@ SPG_HD bool slab_hit_mm(float lox, float loy, float loz, float hix, float hiy,
     return entry <= exit_;
 }
 
+// slab_hit_mm with each axis's planes given as near and far: the caller
+// passes lo as near where invD > 0 and hi where invD < 0.  With a finite
+// origin and a finite, nonzero inverse direction, (lo - o) * invD and
+// (hi - o) * invD are ordered like lo and hi (monotone rounding of the
+// difference and of the product by a positive factor; reversed by a
+// negative one), so the near distances are slab_hit_mm's per-axis minima
+// and the far ones its maxima: entry, exit and the result are the same.
+SPG_HD bool slab_hit_nf(float nx, float ny, float nz, float fx, float fy,
+                        float fz, v3 o, v3 invD, float *entryOut)
+{
+    float tnx = (nx - o.x) * invD.x, tfx = (fx - o.x) * invD.x;
+    float tny = (ny - o.y) * invD.y, tfy = (fy - o.y) * invD.y;
+    float tnz = (nz - o.z) * invD.z, tfz = (fz - o.z) * invD.z;
+    float entry = fmaxf(fmaxf(0.0f, tnx), fmaxf(tny, tnz));
+    float exit_ = fminf(fminf(tfx, tfy), tfz);
+    *entryOut = entry;
+    return entry <= exit_;
+}
+
 SPG_HD bool finite3(v3 a)
 {
     return isfinite(a.x) && isfinite(a.y) && isfinite(a.z);

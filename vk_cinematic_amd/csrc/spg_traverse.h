@@ -233,24 +233,40 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
         const uint32_t q0 = __float_as_uint(C.x), q1 = __float_as_uint(C.y),
                        q2 = __float_as_uint(C.z), q3 = __float_as_uint(C.w),
                        q4 = __float_as_uint(D.x), q5 = __float_as_uint(D.y);
-        const int32_t ref[4] = {__float_as_int(B.x), __float_as_int(B.y),
-                                __float_as_int(B.z), __float_as_int(B.w)};
-        auto box = [&](int k, v3 &lo, v3 &hi) {
-            lo.x = fmaf((float)((q0 >> (8 * k)) & 0xFFu), sx, A.x);
-            lo.y = fmaf((float)((q1 >> (8 * k)) & 0xFFu), sy, A.y);
-            lo.z = fmaf((float)((q2 >> (8 * k)) & 0xFFu), sz, A.z);
-            hi.x = fmaf((float)((q3 >> (8 * k)) & 0xFFu), sx, A.x);
-            hi.y = fmaf((float)((q4 >> (8 * k)) & 0xFFu), sy, A.y);
-            hi.z = fmaf((float)((q5 >> (8 * k)) & 0xFFu), sz, A.z);
+        int32_t ref[4] = {__float_as_int(B.x), __float_as_int(B.y),
+                          __float_as_int(B.z), __float_as_int(B.w)};
+        auto dq = [](uint32_t w, int k, float sc, float p) {
+            return fmaf((float)((w >> (8 * k)) & 0xFFu), sc, p);
         };
-        float e[4];
-        bool hit[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
+        // a hit child's key: the entry distance's bits with the sign
+        // cleared (-0 = +0) and the two low bits replaced by the child slot,
+        // so keys are distinct and near-ties go to the lower slot; a missed
+        // or absent child gets the largest key
+        auto keyOf = [&](int k, bool h, float e) {
+            return (h & (e <= tc) & (ref[k] != kEmptyChild))
+                       ? ((__float_as_uint(e) & 0x7FFFFFFCu) | (uint32_t)k)
+                       : 0xFFFFFFFFu;
+        };
+        uint32_t key[4];
         {
-            v3 lo, hi;
-            box(k, lo, hi);
-            hit[k] = slab_hit_mm(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, o, inv, &e[k]);
+            // finite origin and inverse direction: per axis the near plane
+            // is lo where inv > 0 (then (lo - o) * inv <= (hi - o) * inv by
+            // monotone rounding) and hi where inv < 0, so choosing the
+            // fields once per node gives slab_hit_mm's values
+            const bool px = inv.x > 0.0f, py = inv.y > 0.0f, pz = inv.z > 0.0f;
+            const uint32_t nx = px ? q0 : q3, fx = px ? q3 : q0;
+            const uint32_t ny = py ? q1 : q4, fy = py ? q4 : q1;
+            const uint32_t nz = pz ? q2 : q5, fz = pz ? q5 : q2;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+            {
+                float e;
+                bool h = slab_hit_nf(dq(nx, k, sx, A.x), dq(ny, k, sy, A.y),
+                                     dq(nz, k, sz, A.z), dq(fx, k, sx, A.x),
+                                     dq(fy, k, sy, A.y), dq(fz, k, sz, A.z), o, inv,
+                                     &e);
+                key[k] = keyOf(k, h, e);
+            }
         }
         if (!(blas ? s.mmL : s.mmW))
         {
@@ -259,57 +275,45 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
 #pragma unroll
             for (int k = 0; k < 4; ++k)
             {
-                v3 lo, hi;
-                box(k, lo, hi);
-                hit[k] = slab_hit(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, o, inv, &e[k]);
+                float e;
+                bool h = slab_hit(dq(q0, k, sx, A.x), dq(q1, k, sy, A.y),
+                                  dq(q2, k, sz, A.z), dq(q3, k, sx, A.x),
+                                  dq(q4, k, sy, A.y), dq(q5, k, sz, A.z), o, inv, &e);
+                key[k] = keyOf(k, h, e);
             }
-        }
-        // hit children ordered by entry distance: the key is the entry's
-        // bits with the sign cleared (-0 = +0) and the two low bits replaced
-        // by the child slot, so keys are distinct and near-ties go to the
-        // lower slot; a missed or absent child sorts last
-        uint32_t key[4];
-        int m = 0;
-        uint32_t present = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-        {
-            const bool has = ref[k] != kEmptyChild;
-            const bool h = has & hit[k] & (e[k] <= tc);
-            key[k] = h ? ((__float_as_uint(e[k]) & 0x7FFFFFFCu) | (uint32_t)k)
-                       : 0xFFFFFFFFu;
-            m += h ? 1 : 0;
-            present += has ? 1u : 0u;
         }
         if (kCount)
         {
+            uint32_t present = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) present += ref[k] != kEmptyChild ? 1u : 0u;
             if (blas) cnt.box += present;
             else cnt.tlasBox += present;
         }
-        auto cx = [](uint32_t &a, uint32_t &b) {
-            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-            a = lo;
-            b = hi;
+        // sort (key, reference) pairs by key: five compare-exchanges
+        auto cx = [&](int a, int b) {
+            const bool sw = key[b] < key[a];
+            const uint32_t lo = sw ? key[b] : key[a], hi = sw ? key[a] : key[b];
+            const int32_t ra = sw ? ref[b] : ref[a], rb = sw ? ref[a] : ref[b];
+            key[a] = lo;
+            key[b] = hi;
+            ref[a] = ra;
+            ref[b] = rb;
         };
-        cx(key[0], key[1]);
-        cx(key[2], key[3]);
-        cx(key[0], key[2]);
-        cx(key[1], key[3]);
-        cx(key[1], key[2]);
-        auto pick = [&](uint32_t k) {
-            const int32_t a = (k & 1u) ? ref[1] : ref[0];
-            const int32_t b = (k & 1u) ? ref[3] : ref[2];
-            return (k & 2u) ? b : a;
-        };
-        const int32_t c0 = pick(key[0]), c1 = pick(key[1]), c2 = pick(key[2]),
-                      c3 = pick(key[3]);
+        cx(0, 1);
+        cx(2, 3);
+        cx(0, 2);
+        cx(1, 3);
+        cx(1, 2);
         // the nearest is next; the others wait on the stack, the nearest
-        // of them on top
-        next = m > 0 ? c0 : kNoRef;
-        push0 = m == 4 ? c3 : m == 3 ? c2 : c1;
-        push1 = m == 4 ? c2 : c1;
-        push2 = c1;
-        npush = m > 1 ? m - 1 : 0;
+        // of them on top: slot 3 at sp, slot 2 above it, slot 1 on top
+        const bool v1 = key[1] != 0xFFFFFFFFu, v2 = key[2] != 0xFFFFFFFFu,
+                   v3 = key[3] != 0xFFFFFFFFu;
+        next = key[0] != 0xFFFFFFFFu ? ref[0] : kNoRef;
+        push0 = v3 ? ref[3] : v2 ? ref[2] : ref[1];
+        push1 = v3 ? ref[2] : ref[1];
+        push2 = ref[1];
+        npush = (v1 ? 1 : 0) + (v2 ? 1 : 0) + (v3 ? 1 : 0);
     }
     s.h.t = take ? c.t : s.h.t;
     s.h.u = take ? c.u : s.h.u;
