@@ -608,8 +608,12 @@ def bench_relmse(args, V, scenes):
             a.add(64, V.EST_GUIDED, 0x6EED, a.spp, guide=g)
 
     guided = run("guided", guided_step)
+    # next-event estimation toward the sky (no learning), for comparison
+    nee = run("nee", lambda a: a.add(64 if a.spp >= 64 else 16, V.EST_NEE, 0x7EED,
+                                     a.spp))
     tp = time_to_relmse(plain, target, args.ref_error)
     tg = time_to_relmse(guided, target, args.ref_error)
+    tn = time_to_relmse(nee, target, args.ref_error)
     res.update({"plain_curve": plain, "guided_curve": guided,
                 "reference_error": args.ref_error,
                 "plain_time_s": tp[0] if tp else None, "plain_spp": tp[1] if tp else None,
@@ -617,6 +621,10 @@ def bench_relmse(args, V, scenes):
                 "guided_time_s": tg[0] if tg else None, "guided_spp": tg[1] if tg else None,
                 "guided_how": tg[2] if tg else None,
                 "speedup": (tp[0] / tg[0]) if tp and tg else None,
+                "nee_curve": nee,
+                "nee_time_s": tn[0] if tn else None, "nee_spp": tn[1] if tn else None,
+                "nee_how": tn[2] if tn else None,
+                "nee_speedup": (tp[0] / tn[0]) if tp and tn else None,
                 "train_passes": args.train_passes,
                 "guide": {"lr": args.guide_lr, "batch": args.guide_batch,
                           "steps_per_pass": args.guide_steps,

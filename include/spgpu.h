@@ -50,7 +50,13 @@ enum spg_Estimator {
     SPG_EST_PARITY = 0,   /* the reference estimator (ComputeRadianceForPath,
                              src/simd_path_tracer.cpp:107-175) */
     SPG_EST_UNBIASED = 1, /* same materials, cosine sampling with 1/pdf */
-    SPG_EST_GUIDED = 2    /* unbiased + learned guiding, one-sample MIS */
+    SPG_EST_GUIDED = 2,   /* unbiased + learned guiding, one-sample MIS */
+    SPG_EST_NEE = 3       /* unbiased + next-event estimation toward the
+                             background's cube-map emission (importance
+                             sampled by texel, shadow rays), combined with
+                             BSDF sampling by the power heuristic; no
+                             reference counterpart (the environment
+                             importance sampling of the survey's next steps) */
 };
 enum spg_Transcendentals {
     SPG_TX_DOUBLE = 1,    /* sin/cos/atan2/pow evaluated in double and
@@ -407,6 +413,19 @@ int spg_accumulate_device(spg_Device *dev, float *dst, const float *src,
  * SPG_TX_FAST. */
 int spg_tonemap_device(spg_Device *dev, const float *rgba, uint32_t pixels,
                        float exposure, int transcendentals, uint8_t *out);
+/* Host-only: the importance-sampling table SPG_EST_NEE builds for a
+ * background cube map (rgba: 6 faces of faceWidth x faceHeight RGBA32F in
+ * the layout spg_cubemap_register takes).  Texel i (face-major, then rows,
+ * then columns) is chosen with probability P_i proportional to its mean
+ * channel value times its face-plane rectangle's area over
+ * (1 + fx^2 + fy^2)^(3/2) at the rectangle's centre; pdf[i] = P_i / area_i
+ * (times (1 + fx^2 + fy^2)^(3/2) this is the density per steradian of a
+ * direction drawn uniformly in the rectangle); prob/alias: Vose's alias
+ * table.  Arrays of 6 * faceWidth * faceHeight entries.  SPG_E_INVALID if
+ * no texel has a positive finite value. */
+int spg_env_table_build(const float *rgba, uint32_t faceWidth,
+                        uint32_t faceHeight, float *pdf, float *prob,
+                        uint32_t *alias);
 /* Portable float map (PFM, "PF" colour, little-endian, rows bottom to top)
  * of a host RGBA32F image in the reference's layout (x + y * width, row 0
  * at the top); alpha is dropped. */

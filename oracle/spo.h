@@ -41,7 +41,7 @@ extern "C" {
 enum { SPO_RNG_XORSHIFT_TILE = 0, SPO_RNG_XORSHIFT_PIXEL = 1, SPO_RNG_PCG = 2 };
 enum { SPO_TX_LIBM = 0, SPO_TX_CR = 1 };
 enum { SPO_HIT_REFERENCE = 0, SPO_HIT_BRUTE = 1 };
-enum { SPO_EST_PARITY = 0, SPO_EST_UNBIASED = 1 };
+enum { SPO_EST_PARITY = 0, SPO_EST_UNBIASED = 1, SPO_EST_NEE = 3 };
 
 #define SPO_MAX_METRICS 12
 #define SPO_NO_TEXTURE 0xFFFFFFFFu
@@ -128,6 +128,11 @@ typedef struct spo_Scene {
     uint32_t backgroundMaterialId;
     /* limits that the reference hard-codes (src/sp_scene.cpp:139,:246) */
     uint32_t maxBroadphaseHits, maxMidphaseHits;
+    /* next-event estimation table of the background's cube map
+     * (spo_scene_build_env); envPdf NULL: the environment is not sampled */
+    float *envPdf, *envProb;
+    uint32_t *envAlias;
+    uint32_t envW, envH;
 } spo_Scene;
 
 /* sp_Camera, src/simd_path_tracer.h:17-32 */
@@ -191,6 +196,16 @@ void spo_sample_bilinear(const float *pixels, uint32_t w, uint32_t h,
 void spo_cubemap_basis(uint32_t face, float *out9);
 void spo_cubemap_lookup(const float *faces, uint32_t w, uint32_t h,
                         const float *dir, float *out4);
+/* ---- environment importance sampling (next-event estimation) ---- */
+int spo_env_table_build(const float *rgba, uint32_t w, uint32_t h, float *pdf,
+                        float *prob, uint32_t *alias);
+float spo_env_pdf(const float *pdfTable, uint32_t w, uint32_t h, const float *dir);
+void spo_env_direction(const float *prob, const uint32_t *alias, uint32_t w,
+                       uint32_t h, uint32_t r, float e1, float e2, float e3,
+                       float *out3);
+/* Builds the scene's table from its background material's cube-map
+ * emission: 1 built, 0 no cube-map background (not sampled). */
+int spo_scene_build_env(spo_Scene *s);
 void spo_create_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
                         uint32_t w, uint32_t h, float *out);
 void spo_create_irradiance_cubemap(const float *equirect, uint32_t ew, uint32_t eh,

@@ -500,8 +500,11 @@ void spo_accumulate(float *dst, const float *src, uint32_t pixels, uint32_t befo
  * z), fx = dot(d,right)/|major|, fy = dot(d,up)/|major|, then the nearest
  * texel corner x = floor((fx+1)*W/2 + 1/2), y = floor((1-fy)*H/2 + 1/2),
  * clamped to the face. */
-void spo_cubemap_lookup(const float *faces, uint32_t w, uint32_t h,
-                        const float *dir, float *out4)
+/* Face and face-plane coordinates of a direction (the inverse of
+ * CreateCubeMap's texel-to-direction map, src/cubemap.cpp:237-291, with the
+ * face bases of src/cubemap.cpp:55-103). */
+static void cube_coord(const float *dir, uint32_t *faceOut, float *fxOut,
+                       float *fyOut)
 {
     float x = dir[0], y = dir[1], z = dir[2];
     float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
@@ -528,8 +531,15 @@ void spo_cubemap_lookup(const float *faces, uint32_t w, uint32_t h,
         sc = z >= 0.0f ? x : -x;
         tc = y;
     }
-    float fx = sc / ma;
-    float fy = tc / ma;
+    *faceOut = face;
+    *fxOut = sc / ma;
+    *fyOut = tc / ma;
+}
+
+/* nearest texel of face-plane point (fx, fy): face-major, rows, columns */
+static uint64_t cube_index(uint32_t w, uint32_t h, uint32_t face, float fx,
+                           float fy)
+{
     float px = (fx + 1.0f) * (0.5f * (float)w) + 0.5f;
     float py = (1.0f - fy) * (0.5f * (float)h) + 0.5f;
     /* clamp in the float domain first so NaN and negatives map to 0 */
@@ -539,7 +549,128 @@ void spo_cubemap_lookup(const float *faces, uint32_t w, uint32_t h,
     py = py < (float)(h - 1) ? py : (float)(h - 1);
     uint32_t ix = (uint32_t)floorf(px);
     uint32_t iy = (uint32_t)floorf(py);
-    memcpy(out4, faces + 4 * (((uint64_t)face * h + iy) * w + ix), 16);
+    return ((uint64_t)face * h + iy) * w + ix;
+}
+
+void spo_cubemap_lookup(const float *faces, uint32_t w, uint32_t h,
+                        const float *dir, float *out4)
+{
+    uint32_t face;
+    float fx, fy;
+    cube_coord(dir, &face, &fx, &fy);
+    memcpy(out4, faces + 4 * cube_index(w, h, face, fx, fy), 16);
+}
+
+/* ---- importance sampling of a cube-map environment (next-event
+ * estimation; no reference counterpart).  Texel (ix, iy) of a w x h face is
+ * the face-plane rectangle the lookup above maps to it: fx in
+ * [(2ix-1)/w - 1, (2ix+1)/w - 1), fy in (1 - (2iy+1)/h, 1 - (2iy-1)/h],
+ * clipped to [-1, 1].  Weight: mean channel value x area / (1 + xc^2 +
+ * yc^2)^(3/2) at the rectangle's centre; P_i = weight / sum (double, texel
+ * order); pdf_i = P_i / area_i; Vose's alias table with stacks filled in
+ * ascending order (A. J. Walker 1977, M. D. Vose 1991). */
+static void texel_rect(uint32_t ix, uint32_t iy, uint32_t w, uint32_t h,
+                       double *x0, double *x1, double *y0, double *y1)
+{
+    *x0 = ix == 0 ? -1.0 : (2.0 * ix - 1.0) / w - 1.0;
+    *x1 = ix + 1 == w ? 1.0 : (2.0 * ix + 1.0) / w - 1.0;
+    *y1 = iy == 0 ? 1.0 : 1.0 - (2.0 * iy - 1.0) / h;
+    *y0 = iy + 1 == h ? -1.0 : 1.0 - (2.0 * iy + 1.0) / h;
+}
+
+int spo_env_table_build(const float *rgba, uint32_t w, uint32_t h, float *pdf,
+                        float *prob, uint32_t *alias)
+{
+    if (!rgba || w < 2 || h < 2) return -1;
+    size_t n = 6 * (size_t)w * h;
+    double *wt = (double *)malloc(n * sizeof(double));
+    double *area = (double *)malloc(n * sizeof(double));
+    double *q = (double *)malloc(n * sizeof(double));
+    uint32_t *small = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint32_t *large = (uint32_t *)malloc(n * sizeof(uint32_t));
+    double total = 0.0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        uint32_t ix = (uint32_t)(i % w), iy = (uint32_t)((i / w) % h);
+        double x0, x1, y0, y1;
+        texel_rect(ix, iy, w, h, &x0, &x1, &y0, &y1);
+        double a = (x1 - x0) * (y1 - y0);
+        double xc = 0.5 * (x0 + x1), yc = 0.5 * (y0 + y1);
+        double t = 1.0 + xc * xc + yc * yc;
+        double lum = ((double)rgba[4 * i] + (double)rgba[4 * i + 1] +
+                      (double)rgba[4 * i + 2]) / 3.0;
+        if (!(lum > 0.0) || !isfinite(lum)) lum = 0.0;
+        area[i] = a;
+        wt[i] = lum * a / (t * sqrt(t));
+        total += wt[i];
+    }
+    int rc = 0;
+    if (!(total > 0.0) || !isfinite(total)) rc = -1;
+    else
+    {
+        size_t ns = 0, nl = 0;
+        for (size_t i = 0; i < n; ++i)
+        {
+            double P = wt[i] / total;
+            pdf[i] = (float)(P / area[i]);
+            q[i] = P * (double)n;
+            prob[i] = 1.0f;
+            alias[i] = (uint32_t)i;
+            if (q[i] < 1.0) small[ns++] = (uint32_t)i;
+            else large[nl++] = (uint32_t)i;
+        }
+        while (ns && nl)
+        {
+            uint32_t sm = small[--ns], lg = large[--nl];
+            prob[sm] = (float)q[sm];
+            alias[sm] = lg;
+            q[lg] = (q[lg] + q[sm]) - 1.0;
+            if (q[lg] < 1.0) small[ns++] = lg;
+            else large[nl++] = lg;
+        }
+    }
+    free(wt); free(area); free(q); free(small); free(large);
+    return rc;
+}
+
+/* Density per steradian of the sampler below at direction dir: pdf_i of the
+ * texel the lookup gives times (1 + fx^2 + fy^2)^(3/2). */
+float spo_env_pdf(const float *pdfTable, uint32_t w, uint32_t h, const float *dir)
+{
+    uint32_t face;
+    float fx, fy;
+    cube_coord(dir, &face, &fx, &fy);
+    float t = (1.0f + fx * fx) + fy * fy;
+    return pdfTable[cube_index(w, h, face, fx, fy)] * (t * sqrtf(t));
+}
+
+/* A direction from the table: texel i = (r * n) >> 32, replaced by its
+ * alias unless e1 < prob[i]; then uniform in its rectangle (e2, e3),
+ * through the face basis, normalized. */
+void spo_env_direction(const float *prob, const uint32_t *alias, uint32_t w,
+                       uint32_t h, uint32_t r, float e1, float e2, float e3,
+                       float *out3)
+{
+    uint32_t n = 6 * w * h;
+    uint32_t i = (uint32_t)(((uint64_t)r * n) >> 32);
+    if (!(e1 < prob[i])) i = alias[i];
+    uint32_t ix = i % w, iy = (i / w) % h, face = i / (w * h);
+    float x0 = ix == 0 ? -1.0f : (float)(2 * ix - 1) / (float)w - 1.0f;
+    float x1 = ix + 1 == w ? 1.0f : (float)(2 * ix + 1) / (float)w - 1.0f;
+    float y1 = iy == 0 ? 1.0f : 1.0f - (float)(2 * iy - 1) / (float)h;
+    float y0 = iy + 1 == h ? -1.0f : 1.0f - (float)(2 * iy + 1) / (float)h;
+    float fx = x0 + e2 * (x1 - x0), fy = y0 + e3 * (y1 - y0);
+    spo_v3 v;
+    switch (face)
+    {
+    case 0: v = v3(1.0f, fy, -fx); break;
+    case 1: v = v3(-1.0f, fy, fx); break;
+    case 2: v = v3(fx, 1.0f, -fy); break;
+    case 3: v = v3(fx, -1.0f, fy); break;
+    case 4: v = v3(fx, fy, 1.0f); break;
+    default: v = v3(-fx, fy, -1.0f); break;
+    }
+    v3put(out3, v3norm(v));
 }
 
 /* ----------------------------------------------------------- tiles etc. */

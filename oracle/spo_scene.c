@@ -32,6 +32,9 @@ void spo_scene_destroy(spo_Scene *s)
     free(s->broadphase.nodes);
     for (uint32_t i = 0; i < s->imageCount; ++i)
         free((void *)s->images[i].pixels);
+    free(s->envPdf);
+    free(s->envProb);
+    free(s->envAlias);
     free(s);
 }
 
@@ -212,6 +215,31 @@ static const spo_Image *find_texture(const spo_Scene *s, uint32_t id)
     for (uint32_t i = 0; i < s->imageCount; ++i)
         if (s->imageKeys[i] == id) return &s->images[i];
     return NULL;
+}
+
+int spo_scene_build_env(spo_Scene *s)
+{
+    free(s->envPdf); free(s->envProb); free(s->envAlias);
+    s->envPdf = s->envProb = NULL;
+    s->envAlias = NULL;
+    const spo_Material *m = find_material(s, s->backgroundMaterialId);
+    const spo_Image *t = m ? find_texture(s, m->emissionTexture) : NULL;
+    if (!t || !t->isCube) return 0;
+    size_t n = 6 * (size_t)t->width * t->height;
+    s->envPdf = (float *)malloc(4 * n);
+    s->envProb = (float *)malloc(4 * n);
+    s->envAlias = (uint32_t *)malloc(4 * n);
+    if (spo_env_table_build(t->pixels, t->width, t->height, s->envPdf,
+                            s->envProb, s->envAlias) < 0)
+    {
+        free(s->envPdf); free(s->envProb); free(s->envAlias);
+        s->envPdf = s->envProb = NULL;
+        s->envAlias = NULL;
+        return 0;
+    }
+    s->envW = t->width;
+    s->envH = t->height;
+    return 1;
 }
 
 /* sp_EvaluateMaterial, src/sp_material_system.cpp:59-105, plus the

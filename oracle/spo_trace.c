@@ -272,8 +272,10 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
     spo_film_position(c, px, py, filmP);
     spo_v3 o = c->position;
     spo_v3 d = v3norm(v3sub(v3p(filmP), c->position));
-    if (cfg->estimator == SPO_EST_UNBIASED)
+    if (cfg->estimator == SPO_EST_UNBIASED || cfg->estimator == SPO_EST_NEE)
     {
+        const int nee = cfg->estimator == SPO_EST_NEE;
+        float lastPdf = 0.0f;
         spo_v3 beta = v3s(1.0f), L = v3(0, 0, 0);
         for (uint32_t bounce = 0; bounce < cfg->bounces; ++bounce)
         {
@@ -299,14 +301,59 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                 spo_v3 wi = u0 < ps ? ggx_direction(h.normal, V, rough, u1, u2, tx)
                                     : cosine_direction(h.normal, u1, u2, tx);
                 float cosine = v3dot(h.normal, wi);
+                if (nee && bounce + 1 < cfg->bounces)
+                {
+                    /* next-event estimation toward the environment: a
+                     * direction from the table, power-heuristic weight
+                     * against the BSDF sampler's density, a shadow ray */
+                    uint32_t r = rng_next(rng);
+                    float e1 = rng_u24(rng), e2 = rng_u24(rng), e3 = rng_u24(rng);
+                    if (s->envPdf)
+                    {
+                        float wl3[3];
+                        spo_env_direction(s->envProb, s->envAlias, s->envW, s->envH,
+                                          r, e1, e2, e3, wl3);
+                        spo_v3 wl = v3p(wl3);
+                        float pe = spo_env_pdf(s->envPdf, s->envW, s->envH, wl3);
+                        float cl = v3dot(h.normal, wl);
+                        if (cl > 0.0f && pe > 0.0f)
+                        {
+                            spo_v3 Xl = brdf_bracket(h.normal, V, wl, mat, tx, 1);
+                            float pcl = (1.0f - ps) * cl * (1.0f / SPO_PI);
+                            float pb = ps > 0.0f ? pcl + ps * ggx_pdf(h.normal, V, wl, rough)
+                                                 : pcl;
+                            float bg[7];
+                            spo_v2 uv0 = {0, 0};
+                            spo_material_eval_internal(s, s->backgroundMaterialId, uv0,
+                                                       v3neg(wl), tx, bg);
+                            float pe2 = pe * pe;
+                            float wmis = pe2 / (pe2 + pb * pb);
+                            spo_v3 C = v3had(v3had(beta, v3mul(Xl, (cl * wmis) / pe)),
+                                             v3p(bg + 3));
+                            spo_v3 sp = v3add(v3add(o, v3mul(d, h.t)),
+                                              v3mul(h.normal, 0.0001f));
+                            float so[3], sd[3];
+                            v3put(so, sp); v3put(sd, wl);
+                            spo_SceneHit sh;
+                            if (spo_scene_intersect(s, so, sd, cfg->hit, tx, &sh,
+                                                    metrics) < 0)
+                            { *fail = 1; return L; }
+                            metrics[2]++;
+                            if (!(sh.t > 0.0f)) L = v3add(L, C);
+                        }
+                    }
+                }
                 if (!(cosine > 0.0f)) break;
                 spo_v3 X = brdf_bracket(h.normal, V, wi, mat, tx, 1);
                 float w = SPO_PI;
+                float pdfc = cosine * (1.0f / SPO_PI);
                 if (ps > 0.0f)
                 {
                     float pc = (1.0f - ps) * cosine * (1.0f / SPO_PI);
-                    w = cosine / (pc + ps * ggx_pdf(h.normal, V, wi, rough));
+                    pdfc = pc + ps * ggx_pdf(h.normal, V, wi, rough);
+                    w = cosine / pdfc;
                 }
+                lastPdf = pdfc;
                 beta = v3had(beta, v3mul(X, w));
                 spo_v3 p = v3add(o, v3mul(d, h.t));
                 o = v3add(p, v3mul(h.normal, 0.0001f));
@@ -319,7 +366,20 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                 spo_v2 uv0 = {0, 0};
                 spo_material_eval_internal(s, s->backgroundMaterialId, uv0, V,
                                            tx, mat);
-                L = v3add(L, v3had(beta, v3p(mat + 3)));
+                if (nee && bounce > 0)
+                {
+                    /* the BSDF-sampled direction's share (power heuristic) */
+                    float dd[3];
+                    v3put(dd, d);
+                    float pe = s->envPdf ? spo_env_pdf(s->envPdf, s->envW, s->envH, dd)
+                                         : 0.0f;
+                    float pb = lastPdf;
+                    float pb2 = pb * pb;
+                    float wm = pb2 / (pb2 + pe * pe);
+                    L = v3add(L, v3had(beta, v3mul(v3p(mat + 3), wm)));
+                }
+                else
+                    L = v3add(L, v3had(beta, v3p(mat + 3)));
                 break;
             }
         }

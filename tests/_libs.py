@@ -29,7 +29,7 @@ u32ref = C.POINTER(C.c_uint32)
 SPO_RNG_XORSHIFT_TILE, SPO_RNG_XORSHIFT_PIXEL, SPO_RNG_PCG = 0, 1, 2
 SPO_TX_LIBM, SPO_TX_CR = 0, 1
 SPO_HIT_REFERENCE, SPO_HIT_BRUTE = 0, 1
-SPO_EST_PARITY, SPO_EST_UNBIASED = 0, 1
+SPO_EST_PARITY, SPO_EST_UNBIASED, SPO_EST_NEE = 0, 1, 3
 NO_TEXTURE = 0xFFFFFFFF
 
 
@@ -102,6 +102,12 @@ def oracle():
     _sig(L, "spo_sample_bilinear", None, f32p, C.c_uint32, C.c_uint32, f32p, f32p)
     _sig(L, "spo_cubemap_basis", None, C.c_uint32, f32p)
     _sig(L, "spo_cubemap_lookup", None, f32p, C.c_uint32, C.c_uint32, f32p, f32p)
+    _sig(L, "spo_env_table_build", C.c_int, f32p, C.c_uint32, C.c_uint32, f32p, f32p,
+         u32p)
+    _sig(L, "spo_env_pdf", C.c_float, f32p, C.c_uint32, C.c_uint32, f32p)
+    _sig(L, "spo_env_direction", None, f32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+         C.c_float, C.c_float, C.c_float, f32p)
+    _sig(L, "spo_scene_build_env", C.c_int, vp)
     _sig(L, "spo_create_cubemap", None, f32p, C.c_uint32, C.c_uint32, C.c_uint32,
          C.c_uint32, f32p)
     _sig(L, "spo_create_irradiance_cubemap", None, f32p, C.c_uint32, C.c_uint32,

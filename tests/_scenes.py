@@ -41,6 +41,7 @@ def load_oracle(desc, build_midphase=True, sah=False):
         assert O.spo_material_register(s, f32(*albedo), atex, f32(*emission), etex,
                                        rough, mid), "oracle material registry full"
     O.spo_set_background(s, desc["background"])
+    O.spo_scene_build_env(s)
     w, h, pos, q, fd = desc["camera"]
     cam = SpoCamera()
     O.spo_camera_configure(C.byref(cam), w, h, f32(*pos), f32(*q), fd)

@@ -24,7 +24,7 @@ SPG_OK = 0
 SPG_NO_TEXTURE = 0xFFFFFFFF
 SPG_MISS = 0xFFFFFFFF
 RNG_XORSHIFT_PIXEL, RNG_PCG = 1, 2
-EST_PARITY, EST_UNBIASED, EST_GUIDED = 0, 1, 2
+EST_PARITY, EST_UNBIASED, EST_GUIDED, EST_NEE = 0, 1, 2, 3
 TX_DOUBLE, TX_FAST = 1, 2
 METRIC_ELAPSED_NS, METRIC_PATHS, METRIC_RAYS, METRIC_HITS, METRIC_MISSES = 0, 1, 2, 3, 4
 METRIC_AABB_TESTS, METRIC_MESH_TESTS = 10, 11
@@ -177,6 +177,7 @@ _SIGS = {
     "spg_gen_sky_equirect": (_int, [_u32, _u32, _f32p, C.c_float, C.c_float, _f32p]),
     "spg_accumulate_device": (_int, [_vp, _vp, _vp, _u32, _u32, _u32]),
     "spg_tonemap_device": (_int, [_vp, _vp, _u32, C.c_float, _int, _vp]),
+    "spg_env_table_build": (_int, [_f32p, _u32, _u32, _f32p, _f32p, _u32p]),
     "spg_time_post": (_int, [_vp, _vp, _vp, _u32, C.c_float, _int, _vp, _u32,
                              C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "spg_write_pfm": (_int, [C.c_char_p, _u32, _u32, _f32p]),
@@ -685,6 +686,17 @@ def path_trace_tiles_device(ctx, tiles, device_image, metrics=None):
     check(lib().spg_path_trace_tiles_device(C.byref(ctx), arr, len(tiles),
                                             C.byref(m), device_image.ptr))
     return m
+
+
+def env_table_build(cube_rgba, size):
+    """The next-event estimation table of a size x size x 6 RGBA32F cube
+    map: (pdf per texel, alias probabilities, alias indices)."""
+    c = _f32(cube_rgba)
+    n = 6 * size * size
+    pdf, prob = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    alias = np.zeros(n, np.uint32)
+    check(lib().spg_env_table_build(c, size, size, pdf, prob, alias))
+    return pdf, prob, alias
 
 
 def bvh_build_wide_host(tris9, max_leaf=4, bins=16, threads=0):

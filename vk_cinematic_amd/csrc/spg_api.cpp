@@ -7,6 +7,7 @@
 #include <thread>
 
 #include "spg_internal.h"
+#include "spg_env_table.h"
 
 namespace spg {
 
@@ -281,6 +282,7 @@ static void free_scene_device(spg_Scene *s)
     free_ptr(s->dMats);
     free_ptr(s->dTexs);
     free_ptr(s->dTexels);
+    free_ptr(s->dEnv);
 }
 
 int spg_scene_destroy(spg_Scene *s)
@@ -522,6 +524,33 @@ static int upload_tables(spg_Scene *s)
     ds.mats = (const DevMaterial *)s->dMats;
     ds.inst = (const DevInstance *)s->dInst;
     ds.background = find_index(s->materialKeys, s->backgroundId);
+    // next-event estimation table of the background's cube-map emission
+    ds.envPdf = ds.envProb = nullptr;
+    ds.envAlias = nullptr;
+    ds.envTex = -1;
+    ds.envCount = 0;
+    free_ptr(s->dEnv);
+    if (ds.background >= 0 && mats[ds.background].emissionTex >= 0)
+    {
+        const int32_t ti = mats[ds.background].emissionTex;
+        const HostTexture &t = s->textures[ti];
+        std::vector<float> pdf, prob;
+        std::vector<uint32_t> alias;
+        if (t.isCube && build_env_table(t.rgba.data(), t.width, t.height, pdf, prob, alias))
+        {
+            const size_t n = pdf.size();
+            HIPCHK(hipMalloc(&s->dEnv, 12 * n));
+            char *e = (char *)s->dEnv;
+            HIPCHK(hipMemcpy(e, pdf.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + 4 * n, prob.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + 8 * n, alias.data(), 4 * n, hipMemcpyHostToDevice));
+            ds.envPdf = (const float *)e;
+            ds.envProb = (const float *)(e + 4 * n);
+            ds.envAlias = (const uint32_t *)(e + 8 * n);
+            ds.envTex = ti;
+            ds.envCount = (uint32_t)n;
+        }
+    }
     s->dirty = false;
     return SPG_OK;
 }
@@ -861,7 +890,9 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
            oVtx = take(4 * P * 7 * bounces), oVc = take(4 * P),
            oBeta = take(12 * P), oRad = take(12 * P), oQo0 = take(12 * P),
            oQo1 = take(12 * P), oQd0 = take(12 * P), oQd1 = take(12 * P),
-           oQp0 = take(4 * P), oQp1 = take(4 * P), oCnt = take(64),
+           oQp0 = take(4 * P), oQp1 = take(4 * P), oQo2 = take(12 * P),
+           oQd2 = take(12 * P), oQp2 = take(4 * P), oPdf = take(4 * P),
+           oNee = take(12 * P), oCnt = take(64),
            oMet = take(128), oHt = take(4 * P), oHu = take(4 * P),
            oHv = take(4 * P), oHi = take(4 * P), oHp = take(4 * P);
     HIPCHK(hipMalloc(&block, off));
@@ -883,6 +914,11 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
     p.qd[1] = (float *)(b + oQd1);
     p.qpath[0] = (uint32_t *)(b + oQp0);
     p.qpath[1] = (uint32_t *)(b + oQp1);
+    p.qo[2] = (float *)(b + oQo2);
+    p.qd[2] = (float *)(b + oQd2);
+    p.qpath[2] = (uint32_t *)(b + oQp2);
+    p.lastPdf = (float *)(b + oPdf);
+    p.nee = (float *)(b + oNee);
     p.counters = (uint32_t *)(b + oCnt);
     p.metrics = (unsigned long long *)(b + oMet);
     p.ht = (float *)(b + oHt);
@@ -950,9 +986,10 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     if (cfg.rngMode != SPG_RNG_XORSHIFT_PIXEL && cfg.rngMode != SPG_RNG_PCG)
         return fail(SPG_E_INVALID, "trace: unknown rngMode");
     if (cfg.estimator != SPG_EST_PARITY && cfg.estimator != SPG_EST_UNBIASED &&
-        cfg.estimator != SPG_EST_GUIDED)
+        cfg.estimator != SPG_EST_GUIDED && cfg.estimator != SPG_EST_NEE)
         return fail(SPG_E_INVALID, "trace: unknown estimator");
     const bool guided = cfg.estimator == SPG_EST_GUIDED;
+    const bool nee = cfg.estimator == SPG_EST_NEE;
     if (guided && (!cfg.guide || cfg.guide->dev != d))
         return fail(SPG_E_INVALID, "trace: the guided estimator needs a guide on the scene's device");
     if (cfg.transcendentals != SPG_TX_DOUBLE && cfg.transcendentals != SPG_TX_FAST)
@@ -1046,11 +1083,24 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
                     tm.end();
                     launches[kGuide]++;
                 }
+                if (nee) HIPCHK(launch_set_counter(d->stream, &p.paths.counters[2], 0));
                 tm.begin(kShade);
                 HIPCHK(launch_shade(d->stream, p, q));
                 tm.end();
                 launches[kExtend]++;
                 launches[kShade]++;
+                if (nee && b + 1 < cfg.bounces)
+                {
+                    // shadow rays of this bounce, then their contributions
+                    tm.begin(kExtend);
+                    HIPCHK(launch_extend(d->stream, p, 2, d->persist));
+                    tm.end();
+                    tm.begin(kShade);
+                    HIPCHK(launch_resolve_shadow(d->stream, p));
+                    tm.end();
+                    launches[kExtend]++;
+                    launches[kShade]++;
+                }
                 q ^= 1;
             }
             tm.begin(kFinal);
@@ -1323,6 +1373,21 @@ int spg_time_post(spg_Device *d, float *dst, const float *src, uint32_t pixels,
     HIPCHK(hipEventElapsedTime(&b, e1, e2));
     *accMs = a / iterations;
     *tmMs = b / iterations;
+    return SPG_OK;
+}
+
+int spg_env_table_build(const float *rgba, uint32_t faceWidth, uint32_t faceHeight,
+                        float *pdf, float *prob, uint32_t *alias)
+{
+    if (!rgba || !pdf || !prob || !alias || faceWidth < 2 || faceHeight < 2)
+        return fail(SPG_E_INVALID, "spg_env_table_build: bad argument");
+    std::vector<float> a, b;
+    std::vector<uint32_t> c;
+    if (!build_env_table(rgba, faceWidth, faceHeight, a, b, c))
+        return fail(SPG_E_INVALID, "spg_env_table_build: no positive finite texel");
+    memcpy(pdf, a.data(), 4 * a.size());
+    memcpy(prob, b.data(), 4 * b.size());
+    memcpy(alias, c.data(), 4 * c.size());
     return SPG_OK;
 }
 

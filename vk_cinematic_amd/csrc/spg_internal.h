@@ -102,6 +102,13 @@ struct DevScene
     const float4 *texels;
     int32_t background;         // dense material slot, -1 = none
     uint32_t instCount;
+    // importance sampling of the background's cube-map emission (next-event
+    // estimation); envTex < 0: no table, the environment is not sampled
+    const float *envPdf;        // per texel: P_i / area_i
+    const float *envProb;       // alias table
+    const uint32_t *envAlias;
+    int32_t envTex;             // texture slot of the cube map
+    uint32_t envCount;          // 6 * w * h
 };
 
 struct DevCamera
@@ -124,9 +131,12 @@ struct DevPaths
     uint32_t *vcount;            // vertices per path
     float *beta;                 // [3][capacity]
     float *radiance;             // [3][capacity]
-    float *qo[2], *qd[2];        // [3][capacity] ray origins / directions
-    uint32_t *qpath[2];          // path index of each queued ray
-    uint32_t *counters;          // [0..1] queue lengths
+    float *qo[3], *qd[3];        // [3][capacity] ray origins / directions
+                                 // (queues 0, 1: path rays; 2: shadow rays)
+    uint32_t *qpath[3];          // path index of each queued ray
+    uint32_t *counters;          // [0..2] queue lengths
+    float *lastPdf;              // density of each path's last sampled direction
+    float *nee;                  // [3][capacity] pending shadow-ray contribution
     unsigned long long *metrics; // rays, hits, misses, box, mesh, tri tests
     float *ht, *hu, *hv;         // hit records per queued ray
     uint32_t *hinst, *hprim;
@@ -293,6 +303,7 @@ struct spg_Scene
     spg::BvhBuildResult tlas;
     uint32_t tlasPending = 0;       // stack bound of the wide instance tree
     // device-side tables
+    void *dEnv = nullptr;           // environment table: pdf, prob, alias
     void *dTlasNodes = nullptr, *dInst = nullptr,
          *dMeshes = nullptr, *dMats = nullptr, *dTexs = nullptr,
          *dTexels = nullptr;
@@ -343,6 +354,7 @@ hipError_t launch_raygen(hipStream_t s, const TraceParams &p);
 hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
                          const Persist &ps);
 hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q);
+hipError_t launch_resolve_shadow(hipStream_t s, const TraceParams &p);
 hipError_t launch_finalize(hipStream_t s, const TraceParams &p);
 hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,
                                float *image);

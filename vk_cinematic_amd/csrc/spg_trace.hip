@@ -94,11 +94,16 @@ __device__ __forceinline__ float4 sample_nearest(const DevScene &S,
     return texel(S, T, idx);
 }
 
-// Nearest cube-map texel by direction: the inverse of CreateCubeMap's
-// texel-to-direction map (src/cubemap.cpp:237-291) with the face bases of
-// src/cubemap.cpp:55-103.
-__device__ __forceinline__ float4 sample_cube(const DevScene &S,
-                                              const DevTexture &T, v3 d)
+// Face and face-plane coordinates of a direction: the inverse of
+// CreateCubeMap's texel-to-direction map (src/cubemap.cpp:237-291) with the
+// face bases of src/cubemap.cpp:55-103.
+struct CubeCoord
+{
+    uint32_t face;
+    float fx, fy;
+};
+
+__device__ __forceinline__ CubeCoord cube_coord(v3 d)
 {
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     uint32_t face;
@@ -124,16 +129,67 @@ __device__ __forceinline__ float4 sample_cube(const DevScene &S,
         sc = d.z >= 0.0f ? d.x : -d.x;
         tc = d.y;
     }
-    float fx = sc / ma;
-    float fy = tc / ma;
-    float px = (fx + 1.0f) * (0.5f * (float)T.width) + 0.5f;
-    float py = (1.0f - fy) * (0.5f * (float)T.height) + 0.5f;
+    return CubeCoord{face, sc / ma, tc / ma};
+}
+
+// Texel index of the nearest cube-map texel (face-major, rows, columns).
+__device__ __forceinline__ uint64_t cube_index(const DevTexture &T, CubeCoord c)
+{
+    float px = (c.fx + 1.0f) * (0.5f * (float)T.width) + 0.5f;
+    float py = (1.0f - c.fy) * (0.5f * (float)T.height) + 0.5f;
     px = px >= 0.0f ? px : 0.0f;
     py = py >= 0.0f ? py : 0.0f;
     px = px < (float)(T.width - 1) ? px : (float)(T.width - 1);
     py = py < (float)(T.height - 1) ? py : (float)(T.height - 1);
     uint32_t ix = (uint32_t)floorf(px), iy = (uint32_t)floorf(py);
-    return texel(S, T, ((uint64_t)face * T.height + iy) * T.width + ix);
+    return ((uint64_t)c.face * T.height + iy) * T.width + ix;
+}
+
+// Nearest cube-map texel by direction.
+__device__ __forceinline__ float4 sample_cube(const DevScene &S,
+                                              const DevTexture &T, v3 d)
+{
+    return texel(S, T, cube_index(T, cube_coord(d)));
+}
+
+// Density per steradian with which the environment sampler draws direction
+// d: the texel's P_i / area_i times (1 + fx^2 + fy^2)^(3/2); 0 without a
+// table.
+__device__ __forceinline__ float env_pdf(const DevScene &S, v3 d)
+{
+    if (S.envTex < 0) return 0.0f;
+    const CubeCoord c = cube_coord(d);
+    const float t = (1.0f + c.fx * c.fx) + c.fy * c.fy;
+    return S.envPdf[cube_index(S.texs[S.envTex], c)] * (t * sqrtf(t));
+}
+
+// A direction drawn from the environment table: texel by the alias table
+// (index from the 32-bit draw r, alias test with e1), then uniform in the
+// texel's face-plane rectangle (e2, e3).
+__device__ __forceinline__ v3 env_direction(const DevScene &S, uint32_t r, float e1,
+                                            float e2, float e3)
+{
+    const DevTexture &T = S.texs[S.envTex];
+    uint32_t i = (uint32_t)(((uint64_t)r * S.envCount) >> 32);
+    if (!(e1 < S.envProb[i])) i = S.envAlias[i];
+    const uint32_t w = T.width, h = T.height;
+    const uint32_t ix = i % w, iy = (i / w) % h, face = i / (w * h);
+    const float x0 = ix == 0 ? -1.0f : (float)(2 * ix - 1) / (float)w - 1.0f;
+    const float x1 = ix + 1 == w ? 1.0f : (float)(2 * ix + 1) / (float)w - 1.0f;
+    const float y1 = iy == 0 ? 1.0f : 1.0f - (float)(2 * iy - 1) / (float)h;
+    const float y0 = iy + 1 == h ? -1.0f : 1.0f - (float)(2 * iy + 1) / (float)h;
+    const float fx = x0 + e2 * (x1 - x0), fy = y0 + e3 * (y1 - y0);
+    v3 v;
+    switch (face)
+    {
+    case 0: v = v3{1.0f, fy, -fx}; break;
+    case 1: v = v3{-1.0f, fy, fx}; break;
+    case 2: v = v3{fx, 1.0f, -fy}; break;
+    case 3: v = v3{fx, -1.0f, fy}; break;
+    case 4: v = v3{fx, fy, 1.0f}; break;
+    default: v = v3{-fx, fy, -1.0f}; break;
+    }
+    return normalize(v);
 }
 
 struct MatOut
@@ -307,7 +363,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_mesh(
         if (kCount && s.h.prim != SPG_MISS) hits++;
     };
     SceneView sv{};
-    persistent_walk<kCount, false>(sv, n, counter, spill, begin, store, cnt, rays);
+    persistent_walk<kCount, false, kMeshRefill, kMeshShortStack>(sv, n, counter, spill,
+                                                                 begin, store, cnt, rays);
     if (kCount)
     {
         wave_add(&stats[0], rays);
@@ -355,7 +412,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_scene(
         if (objOut) objOut[i] = h.inst;
         if (primOut) primOut[i] = h.prim;
     };
-    persistent_walk<false, true>(sv, n, counter, spill, begin, store, cnt, rays);
+    persistent_walk<false, true, kSceneRefill, kSceneShortStack>(sv, n, counter, spill,
+                                                                 begin, store, cnt, rays);
 }
 
 // Logical block index spread so consecutive ranges of work stay on one XCD
@@ -470,7 +528,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
         p.paths.hinst[i] = s.out.inst;
         p.paths.hprim[i] = s.out.prim;
     };
-    persistent_walk<kCount, true>(sv, n, counter, spill, begin, store, cnt, rays);
+    persistent_walk<kCount, true, kSceneRefill, kSceneShortStack>(sv, n, counter, spill,
+                                                                  begin, store, cnt, rays);
     if (kCount)
     {
         wave_add(&p.paths.metrics[3], cnt.box);
@@ -570,8 +629,8 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
     for (uint32_t i0 = xcd_block() * 256; i0 < n; i0 += stride)
     {
         uint32_t i = i0 + threadIdx.x;
-        bool push = false;
-        v3 no{0, 0, 0}, nd{0, 0, 0};
+        bool push = false, shadow = false;
+        v3 no{0, 0, 0}, nd{0, 0, 0}, so{0, 0, 0}, sd{0, 0, 0};
         uint32_t path = 0;
         if (i < n)
         {
@@ -643,7 +702,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                 vt[6 * cap] = cosine;
                 p.paths.vcount[path] = vc + 1;
             }
-            else if (kEst == SPG_EST_UNBIASED)
+            else if (kEst == SPG_EST_UNBIASED || kEst == SPG_EST_NEE)
             {
                 v3 beta{p.paths.beta[path], p.paths.beta[cap + path],
                         p.paths.beta[2 * cap + path]};
@@ -667,13 +726,48 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     v3 wi = u0 < ps ? ggx_direction<kDouble>(N, V, m.roughness, u1, u2)
                                     : cosine_direction<kDouble>(N, u1, u2);
                     float cosine = dot(N, wi);
+                    if (kEst == SPG_EST_NEE && vc + 1 < p.bounces)
+                    {
+                        // next-event estimation: a direction from the
+                        // environment table, weighted by the power
+                        // heuristic against the BSDF sampler's density; its
+                        // contribution waits for the shadow ray's answer
+                        uint32_t r = rng.next();
+                        float e1 = rng.u24(), e2 = rng.u24(), e3 = rng.u24();
+                        if (S.envTex >= 0)
+                        {
+                            v3 wl = env_direction(S, r, e1, e2, e3);
+                            float pe = env_pdf(S, wl);
+                            float cl = dot(N, wl);
+                            if (cl > 0.0f && pe > 0.0f)
+                            {
+                                v3 Xl = brdf_bracket<kDouble>(N, V, wl, m, true);
+                                float pb = bsdf_pdf(N, V, wl, cl, m.roughness, ps);
+                                v2 uv0{0.0f, 0.0f};
+                                MatOut bg = eval_material<kDouble>(S, S.background,
+                                                                   uv0, -wl);
+                                float pe2 = pe * pe;
+                                float wmis = pe2 / (pe2 + pb * pb);
+                                v3 C = had(had(beta, Xl * ((cl * wmis) / pe)),
+                                           bg.emission);
+                                p.paths.nee[path] = C.x;
+                                p.paths.nee[cap + path] = C.y;
+                                p.paths.nee[2 * cap + path] = C.z;
+                                shadow = true;
+                                so = (o + d * t) + N * 0.0001f;
+                                sd = wl;
+                            }
+                        }
+                    }
                     if (cosine > 0.0f && vc + 1 < p.bounces)
                     {
                         v3 X = brdf_bracket<kDouble>(N, V, wi, m, true);
-                        float w = ps > 0.0f
-                                      ? cosine / bsdf_pdf(N, V, wi, cosine, m.roughness, ps)
-                                      : kPi;
+                        float pdfc = ps > 0.0f
+                                         ? bsdf_pdf(N, V, wi, cosine, m.roughness, ps)
+                                         : cosine * (1.0f / kPi);
+                        float w = ps > 0.0f ? cosine / pdfc : kPi;
                         beta = had(beta, X * w);
+                        if (kEst == SPG_EST_NEE) p.paths.lastPdf[path] = pdfc;
                         v3 P = o + d * t;
                         push = true;
                         no = P + N * 0.0001f;
@@ -686,7 +780,18 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     v2 uv0{0.0f, 0.0f};
                     MatOut m = eval_material<kDouble>(S, S.background,
                                                       uv0, V);
-                    L = L + had(beta, m.emission);
+                    if (kEst == SPG_EST_NEE && vc > 0)
+                    {
+                        // the BSDF-sampled direction's share of the
+                        // environment (power heuristic)
+                        float pe = env_pdf(S, d);
+                        float pb = p.paths.lastPdf[path];
+                        float pb2 = pb * pb;
+                        float w = pb2 / (pb2 + pe * pe);
+                        L = L + had(beta, m.emission * w);
+                    }
+                    else
+                        L = L + had(beta, m.emission);
                 }
                 p.paths.beta[path] = beta.x;
                 p.paths.beta[cap + path] = beta.y;
@@ -804,9 +909,51 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
             p.paths.qd[nq][2 * cap + slotq] = nd.z;
             p.paths.qpath[nq][slotq] = path;
         }
+        if (kEst == SPG_EST_NEE)
+        {
+            // shadow rays into queue 2
+            unsigned long long smask = __ballot(shadow);
+            uint32_t stotal = (uint32_t)__popcll(smask);
+            uint32_t sbase = 0;
+            if (stotal)
+            {
+                int leader = __ffsll((long long)smask) - 1;
+                if ((int)lane == leader) sbase = atomicAdd(&p.paths.counters[2], stotal);
+                sbase = __shfl(sbase, leader);
+            }
+            if (shadow)
+            {
+                uint32_t k = sbase + (uint32_t)__popcll(smask & ((1ull << lane) - 1ull));
+                p.paths.qo[2][k] = so.x;
+                p.paths.qo[2][cap + k] = so.y;
+                p.paths.qo[2][2 * cap + k] = so.z;
+                p.paths.qd[2][k] = sd.x;
+                p.paths.qd[2][cap + k] = sd.y;
+                p.paths.qd[2][2 * cap + k] = sd.z;
+                p.paths.qpath[2][k] = path;
+            }
+        }
     }
     wave_add(&p.paths.metrics[1], hits);
     wave_add(&p.paths.metrics[2], misses);
+}
+
+// Shadow rays of next-event estimation (queue 2, traced by k_extend): an
+// unoccluded one adds its pending contribution to the path's radiance.
+__global__ __launch_bounds__(256) void k_resolve_shadow(TraceParams p)
+{
+    const uint32_t n = p.paths.counters[2];
+    const uint32_t cap = p.paths.capacity;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    {
+        const bool blocked = p.paths.hinst[i] != SPG_MISS && p.paths.ht[i] > 0.0f;
+        if (blocked) continue;
+        const uint32_t path = p.paths.qpath[2][i];
+        p.paths.radiance[path] = p.paths.radiance[path] + p.paths.nee[path];
+        p.paths.radiance[cap + path] = p.paths.radiance[cap + path] + p.paths.nee[cap + path];
+        p.paths.radiance[2 * cap + path] =
+            p.paths.radiance[2 * cap + path] + p.paths.nee[2 * cap + path];
+    }
 }
 
 // Per pixel slot: the radiance of each of its paths in sample order
@@ -842,7 +989,7 @@ __global__ __launch_bounds__(256) void k_finalize(TraceParams p)
                     radiance = E + had(X, incoming) * cosine;
                 }
             }
-            else if (p.estimator == SPG_EST_UNBIASED)
+            else if (p.estimator == SPG_EST_UNBIASED || p.estimator == SPG_EST_NEE)
                 radiance = v3{p.paths.radiance[path], p.paths.radiance[cap + path],
                               p.paths.radiance[2 * cap + path]};
             else
@@ -1021,7 +1168,7 @@ template <class K> static uint32_t persistent_grid(K kernel)
 size_t spill_ints()
 {
     return (size_t)device_cus() * kMaxPersistWavesPerCU * kTraceBlock *
-           (size_t)(kStackDepth - kShortStack);
+           (size_t)(kStackDepth - kMinShortStack);
 }
 
 static uint32_t persist_blocks(uint32_t grid, uint64_t n)
@@ -1144,6 +1291,10 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
         if (dbl) SPG_SHADE(true, SPG_EST_UNBIASED);
         else SPG_SHADE(false, SPG_EST_UNBIASED);
         break;
+    case SPG_EST_NEE:
+        if (dbl) SPG_SHADE(true, SPG_EST_NEE);
+        else SPG_SHADE(false, SPG_EST_NEE);
+        break;
     default:
         if (dbl) SPG_SHADE(true, SPG_EST_GUIDED);
         else SPG_SHADE(false, SPG_EST_GUIDED);
@@ -1151,6 +1302,13 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
     }
 #undef SPG_SHADE
 #undef SPG_SHADE_B
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_shadow(hipStream_t s, const TraceParams &p)
+{
+    uint32_t g = grid_for((uint64_t)p.slots * p.sppPass, 256, 4096);
+    hipLaunchKernelGGL(k_resolve_shadow, dim3(g), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

@@ -18,8 +18,8 @@
 // transforms the ray into mesh space (sp_RayIntersectScene,
 // src/sp_scene.cpp:263-276) and the mesh walk continues on the same stack.
 //
-// Stack: the top kShortStack entries of each lane live in LDS (4 KB per
-// wave); deeper entries spill to a global array (rare: the builder bounds
+// Stack: the top entries of each lane live in LDS (4 or 8 KB per wave);
+// deeper entries spill to a global array (rare: the builder bounds
 // tree depth, and a closest-hit walk holds at most three pending children
 // per wide level).
 #pragma once
@@ -28,18 +28,20 @@
 
 namespace spg {
 
-constexpr int kShortStack = 16;
 constexpr uint32_t kRayChunk = 64;
-// A wave refills its idle lanes once this many wait (or all do): each
+// A wave refills its idle lanes once kRefill of them wait (or all do): each
 // refill stalls the wave on the new rays' loads and setup, so refilling one
 // lane per iteration adds a memory round trip to nearly every step, while
-// waiting for all 64 leaves the wave running its longest ray alone.  On the
-// 4M-triangle box scene and the 1M-triangle soup, 48 beat 1, 4, 8, 16, 24,
-// 32 and 64.
-#ifndef SPG_REFILL_MIN
-#define SPG_REFILL_MIN 48
-#endif
-constexpr uint32_t kRefillMin = SPG_REFILL_MIN;
+// waiting for all 64 leaves the wave running its longest ray alone.  The
+// top kShort entries of each lane's stack live in LDS (kShort * 256 bytes
+// per wave), deeper ones spill to global memory.  Measured with four-wide
+// nodes: the scene walk (TLAS + BLAS, path-tracing rays of the 4M-triangle
+// box scene) is fastest at 40 / 16 (48: -4%, 56: -11%; 24 or 32 LDS
+// entries: -1%); the single-mesh walk of random rays (the 1M-triangle soup,
+// deeper stacks, longer rays) at 56 / 32 (+4% and +10% over 48 / 16).
+constexpr uint32_t kSceneRefill = 40, kMeshRefill = 56;
+constexpr int kSceneShortStack = 16, kMeshShortStack = 32;
+constexpr int kMinShortStack = 16;  // sizes the spill area
 constexpr int32_t kNoRef = kEmptyChild;
 
 // Node, triangle and stack-spill arrays are reached through pointers read
@@ -86,23 +88,24 @@ struct Counts
     uint32_t box, tlasBox, tri, mesh;
 };
 
+template <int kShort>
 struct LaneStack
 {
     lint *lds;       // this wave's LDS block, entry i of lane l at lds[i*64 + l]
-    gint *spill;     // entry i >= kShortStack at spill[(i-kShortStack)*stride + gl]
+    gint *spill;     // entry i >= kShort at spill[(i-kShort)*stride + gl]
     uint32_t stride; // lanes in the grid
     uint32_t gl;     // this lane's index in the grid
     uint32_t lane;
     __device__ __forceinline__ void write(int i, int v)
     {
-        if (i < kShortStack) lds[i * 64 + lane] = v;
-        else spill[(size_t)(i - kShortStack) * stride + gl] = v;
+        if (i < kShort) lds[i * 64 + lane] = v;
+        else spill[(size_t)(i - kShort) * stride + gl] = v;
     }
     __device__ __forceinline__ int read(int i)
     {
         int v;
-        if (i < kShortStack) v = lds[i * 64 + lane];
-        else v = spill[(size_t)(i - kShortStack) * stride + gl];
+        if (i < kShort) v = lds[i * 64 + lane];
+        else v = spill[(size_t)(i - kShort) * stride + gl];
         return v;
     }
 };
@@ -172,9 +175,10 @@ __device__ __forceinline__ m34 rows3(float4 a, float4 b, float4 c)
 // predicates.  Written inside divergent branches, every state value needs a
 // second register at each join and the compiler copies the whole state
 // around every branch of the step.
-template <bool kCount, bool kTwoLevel>
+template <bool kCount, bool kTwoLevel, int kShort>
 __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
-                                         LaneStack &st, Counts &cnt, bool active)
+                                         LaneStack<kShort> &st, Counts &cnt,
+                                         bool active)
 {
     const uint32_t mode = s.mode;
     const bool blas = !kTwoLevel || s.level != 0;
@@ -459,15 +463,17 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
 // returns false if it needs no walk (it is then stored at once);
 // store(ray, s) writes the result.  Must be called by every lane of a
 // one-wave block.
-template <bool kCount, bool kTwoLevel, class Begin, class Store>
+template <bool kCount, bool kTwoLevel, uint32_t kRefill, int kShort, class Begin,
+          class Store>
 __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
                                                 uint32_t *counter, int *spill,
                                                 Begin begin, Store store,
                                                 Counts &cnt, uint32_t &rays)
 {
-    __shared__ int lds[kShortStack * 64];
+    __shared__ int lds[kShort * 64];
     const uint32_t lane = threadIdx.x;
-    LaneStack st{(lint *)lds, (gint *)spill, gridDim.x * 64u, blockIdx.x * 64u + lane,
+    LaneStack<kShort> st{(lint *)lds, (gint *)spill, gridDim.x * 64u,
+                         blockIdx.x * 64u + lane,
                  lane};
     const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t chunkNext = 0, chunkEnd = 0;
@@ -479,7 +485,7 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
         bool need = ray == SPG_MISS;
         unsigned long long needMask = __ballot(need);
         const uint32_t idle = (uint32_t)__popcll(needMask);
-        if ((idle >= kRefillMin || idle == 64u) && idle != 0u &&
+        if ((idle >= kRefill || idle == 64u) && idle != 0u &&
             !(exhausted && chunkNext >= chunkEnd))
         {
             if (chunkNext >= chunkEnd)
@@ -522,7 +528,7 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
         // path back to the loop head
         if (__ballot(ray != SPG_MISS) == 0ull && exhausted && chunkNext >= chunkEnd)
             break;
-        if (trav_step<kCount, kTwoLevel>(sv, s, st, cnt, ray != SPG_MISS))
+        if (trav_step<kCount, kTwoLevel, kShort>(sv, s, st, cnt, ray != SPG_MISS))
         {
             store(ray, s);
             ray = SPG_MISS;
