@@ -49,7 +49,7 @@ BOX_BYTES, TRI_BYTES, ENTRY_BYTES = 24, 36, 128
 # --pmc FETCH_SIZE and WRITE_SIZE per dispatch (FETCH scaled x2.0, WRITE
 # x1.0 by a 1 GiB copy kernel on the same box; config5: the plain tracer's
 # launches); the roofline's "traffic" is this times the rays of one launch.
-TRAFFIC_BYTES_PER_RAY = {"config5": 1.455e9 / 2037010, "soup": 1.079e9 / 4194304}
+TRAFFIC_BYTES_PER_RAY = {"config5": 1.522e9 / 2037010, "soup": 1.095e9 / 4194304}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same kernel and "
                   "workload (profiles/hbm_traffic_config5_soup.md), FETCH x2.0 and "
                   "WRITE x1.0 by a 1 GiB copy calibration")
