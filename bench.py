@@ -204,32 +204,58 @@ def bench_config1(args, V, scenes):
     return res
 
 
-def cpu_baseline_box(desc, threads, spp=8, bounces=6):
+def cpu_baseline_box(desc, threads, spp=4, bounces=6, repeats=3):
     """The oracle on the same scene and estimator (plain unbiased tracer,
     per-pixel PCG) over the same SAH hierarchy (the reference's greedy
     builder is O(n^2 log n) and its 32/128 overlap buffers overflow at this
-    scale), 64x64 tiles from an atomic counter, one full frame at `spp`."""
+    scale), 64x64 tiles from an atomic counter on `threads` workers: one
+    warm-up frame at 1 spp, then the median of `repeats` full frames at
+    `spp`; plus one thread on a band of 64 rows at `spp`.  Scene and
+    hierarchy build time is reported separately."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _libs import (SPO_EST_UNBIASED, SPO_HIT_REFERENCE, SPO_RNG_PCG,
                        SPO_TX_LIBM, SpoConfig, oracle)
     from _scenes import load_oracle
 
     O = oracle()
+    t0 = time.perf_counter()
     s, cam = load_oracle(desc, sah=True)
-    cfg = SpoConfig(rng=SPO_RNG_PCG, tx=SPO_TX_LIBM, hit=SPO_HIT_REFERENCE,
-                    estimator=SPO_EST_UNBIASED, samplesPerPixel=spp,
-                    bounces=bounces, radianceClamp=0.0, jitterYFirst=1,
-                    seed=0x5EED)
+    build_s = time.perf_counter() - t0
     w, h = desc["camera"][0], desc["camera"][1]
+
+    def cfg_for(n):
+        return SpoConfig(rng=SPO_RNG_PCG, tx=SPO_TX_LIBM, hit=SPO_HIT_REFERENCE,
+                         estimator=SPO_EST_UNBIASED, samplesPerPixel=n,
+                         bounces=bounces, radianceClamp=0.0, jitterYFirst=1,
+                         seed=0x5EED)
+
     px = np.zeros(w * h * 4, np.float32)
-    sec = O.spo_render_threaded(s, C.byref(cam), C.byref(cfg), 64, 64, threads, px,
-                                np.zeros(12, np.uint64))
+    O.spo_render_threaded(s, C.byref(cam), C.byref(cfg_for(1)), 64, 64, threads, px,
+                          np.zeros(12, np.uint64))
+    secs, rays = [], 0
+    cfg = cfg_for(spp)
+    for _ in range(repeats):
+        met = np.zeros(12, np.uint64)
+        secs.append(O.spo_render_threaded(s, C.byref(cam), C.byref(cfg), 64, 64, threads,
+                                          px, met))
+        rays = int(met[2])
+    sec = float(np.median(secs))
+    band = min(64, h)
+    t0 = time.perf_counter()
+    O.spo_path_trace_tile(s, C.byref(cam), C.byref(cfg), 0, 0, w, band, None, px,
+                          np.zeros(12, np.uint64))
+    sec1 = time.perf_counter() - t0
     O.spo_scene_destroy(s)
     return {"value": w * h * spp / sec / 1e6, "unit": "Msamples/s", "cores": threads,
             "kind": "port",
-            "sample": f"one {w}x{h} frame at {spp} spp, {bounces} bounces, plain "
-                      f"unbiased tracer, libm transcendentals, same SAH BVH "
-                      f"({sec:.2f} s)"}
+            "sample": f"{w}x{h} frames at {spp} spp, {bounces} bounces, plain unbiased "
+                      f"tracer, libm transcendentals, same SAH BVH; median of {repeats} "
+                      f"after a 1-spp warm-up ({sec:.2f} s per frame)",
+            "Mrays_per_s": rays / sec / 1e6,
+            "value_1thread": w * band * spp / sec1 / 1e6,
+            "sample_1thread": f"rows 0-{band - 1} at {spp} spp on one thread "
+                              f"({sec1:.2f} s)",
+            "scene_build_s": build_s}
 
 
 def bench_box(args, V, scenes, name):
@@ -275,6 +301,11 @@ def bench_box(args, V, scenes, name):
     tbox, ment = c["tlas_slab_tests"], c["mesh_tests"]
     bytes_step = ((RAY_BYTES + HIT_BYTES_SCENE) * rays + BOX_BYTES * (box + tbox) +
                   TRI_BYTES * tri + ENTRY_BYTES * ment)
+    # compulsory bytes of one launch: its rays in and hit records out, and
+    # every mesh record (wide nodes and triangles, 64 bytes each) and
+    # instance record (128 bytes) read once
+    scene_bytes = sum(64 * sum(s.wide_counts(k)) for k in range(len(desc["meshes"])))
+    scene_bytes += 128 * len(desc["objects"])
     launches = ext_n / args.steps
     per_launch = bytes_step / launches
     mean_ms = ext_ms / ext_n
@@ -302,6 +333,10 @@ def bench_box(args, V, scenes, name):
                      "traffic_source": TRAFFIC_SOURCE if name == "config5" else None,
                      "kernel": "k_extend (closest-hit traversal)",
                      "launch_ms": mean_ms, "algorithmic_bytes_per_launch": per_launch,
+                     "compulsory_bytes_per_launch": (RAY_BYTES + HIT_BYTES_SCENE) * rays /
+                     launches + scene_bytes,
+                     "measured_limit": "VALU issue (rocprofv3 SQ_ACTIVE_INST_VALU ~99% of "
+                                       "SIMD issue slots, profiles/hbm_traffic_config5_soup.md)",
                      "rays_per_step": rays, "slab_tests_per_ray": box / max(rays, 1),
                      "tri_tests_per_ray": tri / max(rays, 1),
                      "tlas_slab_tests_per_ray": tbox / max(rays, 1),

@@ -418,6 +418,13 @@ class Scene:
                                         n_prims, C.byref(root)))
         return nodes.reshape(-1, 16), prims
 
+    def wide_counts(self, mesh):
+        """(four-wide nodes, triangles) of a built mesh."""
+        cnt, nprim = C.c_uint32(), C.c_uint32()
+        check(lib().spg_mesh_wide_export(self.handle, mesh, None, 0, None, 0,
+                                         C.byref(cnt), C.byref(nprim), None))
+        return cnt.value, nprim.value
+
     def wide_export(self, mesh):
         """The four-wide hierarchy the traversal walks: (nodes [n,64]
         uint8, primitive order, stack bound)."""
