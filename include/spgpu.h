@@ -53,10 +53,11 @@ enum spg_Estimator {
     SPG_EST_GUIDED = 2,   /* unbiased + learned guiding, one-sample MIS */
     SPG_EST_NEE = 3       /* unbiased + next-event estimation toward the
                              background's cube-map emission (importance
-                             sampled by texel, shadow rays), combined with
-                             BSDF sampling by the power heuristic; no
-                             reference counterpart (the environment
-                             importance sampling of the survey's next steps) */
+                             sampled by texel) and toward emissive triangles
+                             (by emission x area), one light sample and one
+                             shadow ray per vertex, combined with BSDF
+                             sampling by the power heuristic; no reference
+                             counterpart */
 };
 enum spg_Transcendentals {
     SPG_TX_DOUBLE = 1,    /* sin/cos/atan2/pow evaluated in double and

@@ -27,7 +27,12 @@
  *         SPO_HIT_BRUTE      every triangle is a candidate, ties go to the
  *                            lower triangle index / lower object index
  *
- * Parity status of each function is recorded in DESIGN.md (oracle table).
+ * Parity status: every function is pinned by a test -- the reference's
+ * golden vectors or its own build in oracle/_ref where it has a reference
+ * counterpart; closed forms, brute force, quadrature, finite differences or
+ * distribution checks where it has none (tests/test_oracle_*.py,
+ * tests/test_wide_bvh.py, tests/test_nee.py, tests/test_post.py).  No
+ * function is left parity unpinned.
  */
 #ifndef SPO_H
 #define SPO_H
@@ -133,6 +138,12 @@ typedef struct spo_Scene {
     float *envPdf, *envProb;
     uint32_t *envAlias;
     uint32_t envW, envH;
+    /* emissive triangles (spo_scene_build_env); emitCount 0: none */
+    float *emitPdfA, *emitProb;
+    uint32_t *emitAlias, *emitPair;
+    int32_t *emitBase;
+    uint32_t emitCount;
+    float envSelect;     /* probability of sampling the environment */
 } spo_Scene;
 
 /* sp_Camera, src/simd_path_tracer.h:17-32 */
@@ -203,9 +214,15 @@ float spo_env_pdf(const float *pdfTable, uint32_t w, uint32_t h, const float *di
 void spo_env_direction(const float *prob, const uint32_t *alias, uint32_t w,
                        uint32_t h, uint32_t r, float e1, float e2, float e3,
                        float *out3);
-/* Builds the scene's table from its background material's cube-map
- * emission: 1 built, 0 no cube-map background (not sampled). */
+/* Builds the scene's light tables: the background material's cube-map
+ * emission (1 if built, else the sky is not sampled) and the emissive
+ * triangles of instances whose material has a constant emission; sets the
+ * environment's selection probability (0.5 with both, else 1 or 0). */
 int spo_scene_build_env(spo_Scene *s);
+/* The emitter table alone (arrays of the scene's emissive triangle count,
+ * base: one entry per object); returns the entry count. */
+uint32_t spo_scene_emitter_table(const spo_Scene *s, float *pdfA, float *prob,
+                                 uint32_t *alias, uint32_t *pair, int32_t *base);
 void spo_create_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
                         uint32_t w, uint32_t h, float *out);
 void spo_create_irradiance_cubemap(const float *equirect, uint32_t ew, uint32_t eh,

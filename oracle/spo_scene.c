@@ -35,6 +35,8 @@ void spo_scene_destroy(spo_Scene *s)
     free(s->envPdf);
     free(s->envProb);
     free(s->envAlias);
+    free(s->emitPdfA); free(s->emitProb); free(s->emitAlias);
+    free(s->emitPair); free(s->emitBase);
     free(s);
 }
 
@@ -217,8 +219,134 @@ static const spo_Image *find_texture(const spo_Scene *s, uint32_t id)
     return NULL;
 }
 
+/* Vose's alias table of P (n entries, sum 1): stacks filled in ascending
+ * order, leftovers keep probability 1. */
+static void alias_table(const double *P, size_t n, float *prob, uint32_t *alias)
+{
+    double *q = (double *)malloc((n ? n : 1) * sizeof(double));
+    uint32_t *sm = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *lg = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    size_t ns = 0, nl = 0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        q[i] = P[i] * (double)n;
+        prob[i] = 1.0f;
+        alias[i] = (uint32_t)i;
+        if (q[i] < 1.0) sm[ns++] = (uint32_t)i;
+        else lg[nl++] = (uint32_t)i;
+    }
+    while (ns && nl)
+    {
+        uint32_t a = sm[--ns], b = lg[--nl];
+        prob[a] = (float)q[a];
+        alias[a] = b;
+        q[b] = (q[b] + q[a]) - 1.0;
+        if (q[b] < 1.0) sm[ns++] = b;
+        else lg[nl++] = b;
+    }
+    free(q); free(sm); free(lg);
+}
+
+/* mean channel of an object's constant emission, 0 if none or textured */
+static double object_emission(const spo_Scene *s, const spo_Object *o)
+{
+    const spo_Material *m = find_material(s, o->material);
+    if (!m || find_texture(s, m->emissionTexture)) return 0.0;
+    return ((double)m->emission.x + (double)m->emission.y + (double)m->emission.z) / 3.0;
+}
+
+/* Emissive triangles of the scene's objects (next-event estimation toward
+ * emitting geometry; no reference counterpart): every triangle of an
+ * object with a positive constant emission, in object then index order;
+ * weight = mean emission x world-space area (vertices through the
+ * column-major model matrix in double); P = weight / sum; pdfA = P / area
+ * (0 for a degenerate triangle); alias table; pair = (object, triangle). */
+uint32_t spo_scene_emitter_table(const spo_Scene *s, float *pdfA, float *prob,
+                                 uint32_t *alias, uint32_t *pair, int32_t *base)
+{
+    size_t n = 0;
+    for (uint32_t i = 0; i < s->objectCount; ++i)
+    {
+        double lum = object_emission(s, &s->objects[i]);
+        base[i] = -1;
+        if (lum > 0.0 && isfinite(lum))
+            n += s->meshes[s->objects[i].mesh].indexCount / 3;
+    }
+    if (n == 0) return 0;
+    double *wt = (double *)malloc(n * sizeof(double));
+    double *area = (double *)malloc(n * sizeof(double));
+    size_t g = 0;
+    for (uint32_t i = 0; i < s->objectCount; ++i)
+    {
+        const spo_Object *o = &s->objects[i];
+        double lum = object_emission(s, o);
+        if (!(lum > 0.0) || !isfinite(lum)) continue;
+        const spo_Mesh *me = &s->meshes[o->mesh];
+        const float *m = o->model.m;
+        base[i] = (int32_t)g;
+        for (uint32_t k = 0; k < me->indexCount / 3; ++k, ++g)
+        {
+            double w[3][3];
+            for (int v = 0; v < 3; ++v)
+            {
+                spo_v3 p = me->vertices[me->indices[3 * k + v]].p;
+                for (int r = 0; r < 3; ++r)
+                    w[v][r] = (double)m[r] * p.x + (double)m[4 + r] * p.y +
+                              (double)m[8 + r] * p.z + (double)m[12 + r];
+            }
+            double ux = w[1][0] - w[0][0], uy = w[1][1] - w[0][1], uz = w[1][2] - w[0][2];
+            double vx = w[2][0] - w[0][0], vy = w[2][1] - w[0][1], vz = w[2][2] - w[0][2];
+            double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+            double a = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            area[g] = a;
+            wt[g] = isfinite(a) ? lum * a : 0.0;
+            pair[2 * g] = i;
+            pair[2 * g + 1] = k;
+        }
+    }
+    double total = 0.0;
+    for (size_t j = 0; j < n; ++j) total += wt[j];
+    uint32_t count = 0;
+    if (total > 0.0 && isfinite(total))
+    {
+        double *P = (double *)malloc(n * sizeof(double));
+        for (size_t j = 0; j < n; ++j)
+        {
+            P[j] = wt[j] / total;
+            pdfA[j] = area[j] > 0.0 ? (float)(P[j] / area[j]) : 0.0f;
+        }
+        alias_table(P, n, prob, alias);
+        free(P);
+        count = (uint32_t)n;
+    }
+    else
+        for (uint32_t i = 0; i < s->objectCount; ++i) base[i] = -1;
+    free(wt); free(area);
+    return count;
+}
+
 int spo_scene_build_env(spo_Scene *s)
 {
+    free(s->emitPdfA); free(s->emitProb); free(s->emitAlias);
+    free(s->emitPair); free(s->emitBase);
+    s->emitPdfA = s->emitProb = NULL;
+    s->emitAlias = s->emitPair = NULL;
+    s->emitBase = NULL;
+    s->emitCount = 0;
+    {
+        size_t n = 0;
+        for (uint32_t i = 0; i < s->objectCount; ++i)
+            n += s->meshes[s->objects[i].mesh].indexCount / 3;
+        size_t nn = n ? n : 1;
+        s->emitPdfA = (float *)malloc(4 * nn);
+        s->emitProb = (float *)malloc(4 * nn);
+        s->emitAlias = (uint32_t *)malloc(4 * nn);
+        s->emitPair = (uint32_t *)malloc(8 * nn);
+        s->emitBase = (int32_t *)malloc(4 * (s->objectCount ? s->objectCount : 1));
+        s->emitCount = spo_scene_emitter_table(s, s->emitPdfA, s->emitProb,
+                                               s->emitAlias, s->emitPair, s->emitBase);
+    }
+    s->envSelect = 0.0f;
     free(s->envPdf); free(s->envProb); free(s->envAlias);
     s->envPdf = s->envProb = NULL;
     s->envAlias = NULL;
@@ -239,6 +367,7 @@ int spo_scene_build_env(spo_Scene *s)
     }
     s->envW = t->width;
     s->envH = t->height;
+    s->envSelect = s->emitCount ? 0.5f : 1.0f;
     return 1;
 }
 

@@ -253,6 +253,55 @@ static void rng_for_pixel(spo_Rng *rng, const spo_Config *cfg, uint32_t width,
         rng->xs = *tileState;
 }
 
+/* world-space vertices of triangle k of object i */
+static void world_triangle(const spo_Scene *s, uint32_t i, uint32_t k, spo_v3 *A,
+                           spo_v3 *B, spo_v3 *C)
+{
+    const spo_Object *o = &s->objects[i];
+    const spo_Mesh *m = &s->meshes[o->mesh];
+    *A = m4point(o->model, m->vertices[m->indices[3 * k]].p);
+    *B = m4point(o->model, m->vertices[m->indices[3 * k + 1]].p);
+    *C = m4point(o->model, m->vertices[m->indices[3 * k + 2]].p);
+}
+
+/* density per steradian of the emitter sampler reaching triangle k of
+ * object i at distance t along unit direction d: pdfA t^2 / |cos| */
+static float emit_pdf(const spo_Scene *s, uint32_t i, uint32_t k, float t, spo_v3 d)
+{
+    int32_t b = s->emitBase[i];
+    if (b < 0) return 0.0f;
+    spo_v3 A, B, C;
+    world_triangle(s, i, k, &A, &B, &C);
+    spo_v3 n = v3cross(v3sub(B, A), v3sub(C, A));
+    float c = fabsf(v3dot(n, d)) / sqrtf(v3dot(n, n));
+    return c > 0.0f ? s->emitPdfA[b + k] * (t * t) / c : 0.0f;
+}
+
+/* a point on an emissive triangle: entry by the alias table (r, e1),
+ * uniform on the triangle (e2, e3); the unit direction from x and its
+ * density per steradian */
+static spo_v3 emit_direction(const spo_Scene *s, uint32_t r, float e1, float e2,
+                             float e3, spo_v3 x, float *pdf, uint32_t *inst,
+                             uint32_t *prim)
+{
+    uint32_t g = (uint32_t)(((uint64_t)r * s->emitCount) >> 32);
+    if (!(e1 < s->emitProb[g])) g = s->emitAlias[g];
+    *inst = s->emitPair[2 * g];
+    *prim = s->emitPair[2 * g + 1];
+    spo_v3 A, B, C;
+    world_triangle(s, *inst, *prim, &A, &B, &C);
+    float su = sqrtf(e2);
+    spo_v3 P = v3add(v3add(v3mul(A, 1.0f - su), v3mul(B, su * (1.0f - e3))),
+                     v3mul(C, su * e3));
+    spo_v3 v = v3sub(P, x);
+    float dist2 = v3dot(v, v);
+    spo_v3 w = v3norm(v);
+    spo_v3 n = v3cross(v3sub(B, A), v3sub(C, A));
+    float c = fabsf(v3dot(n, w)) / sqrtf(v3dot(n, n));
+    *pdf = c > 0.0f && dist2 > 0.0f ? s->emitPdfA[g] * dist2 / c : 0.0f;
+    return w;
+}
+
 /* One sample of one pixel.  Parity estimator: src/simd_path_tracer.cpp:
  * 216-322.  Unbiased estimator: the same material model and camera, cosine
  * sampling and the 1/pdf weight. */
@@ -291,7 +340,19 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                 metrics[3]++;
                 float mat[7];
                 spo_material_eval_internal(s, h.materialId, h.uv, V, tx, mat);
-                L = v3add(L, v3had(beta, v3p(mat + 3)));
+                if (nee && bounce > 0 && s->emitCount && s->emitBase[h.object] >= 0)
+                {
+                    /* a BSDF-sampled ray that reached an emitter: its share
+                     * against the emitter sampler (power heuristic) */
+                    float pl = (1.0f - s->envSelect) *
+                               emit_pdf(s, h.object, h.prim, h.t, d);
+                    float pb = lastPdf;
+                    float pb2 = pb * pb;
+                    float wm = pb2 / (pb2 + pl * pl);
+                    L = v3add(L, v3had(beta, v3mul(v3p(mat + 3), wm)));
+                }
+                else
+                    L = v3add(L, v3had(beta, v3p(mat + 3)));
                 /* cosine lobe, or GGX with probability ps on rough
                  * materials; weight X cos / pdf (X pi for ps = 0) */
                 float rough = mat[6];
@@ -303,43 +364,66 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                 float cosine = v3dot(h.normal, wi);
                 if (nee && bounce + 1 < cfg->bounces)
                 {
-                    /* next-event estimation toward the environment: a
-                     * direction from the table, power-heuristic weight
-                     * against the BSDF sampler's density, a shadow ray */
+                    /* next-event estimation: the sky (probability
+                     * envSelect) or an emissive triangle, power-heuristic
+                     * weight against the BSDF sampler's density, a shadow
+                     * ray that must reach the light */
+                    float e0 = rng_u24(rng);
                     uint32_t r = rng_next(rng);
                     float e1 = rng_u24(rng), e2 = rng_u24(rng), e3 = rng_u24(rng);
-                    if (s->envPdf)
+                    int useEnv = e0 < s->envSelect;
+                    if (useEnv ? s->envPdf != NULL : s->emitCount > 0)
                     {
-                        float wl3[3];
-                        spo_env_direction(s->envProb, s->envAlias, s->envW, s->envH,
-                                          r, e1, e2, e3, wl3);
-                        spo_v3 wl = v3p(wl3);
-                        float pe = spo_env_pdf(s->envPdf, s->envW, s->envH, wl3);
+                        spo_v3 x = v3add(v3add(o, v3mul(d, h.t)), v3mul(h.normal, 0.0001f));
+                        spo_v3 wl, Le;
+                        float pl;
+                        uint32_t ti = 0xFFFFFFFFu, tp = 0;
+                        if (useEnv)
+                        {
+                            float wl3[3];
+                            spo_env_direction(s->envProb, s->envAlias, s->envW, s->envH,
+                                              r, e1, e2, e3, wl3);
+                            wl = v3p(wl3);
+                            pl = s->envSelect * spo_env_pdf(s->envPdf, s->envW, s->envH, wl3);
+                            float bg[7];
+                            spo_v2 uv0 = {0, 0};
+                            spo_material_eval_internal(s, s->backgroundMaterialId, uv0,
+                                                       v3neg(wl), tx, bg);
+                            Le = v3p(bg + 3);
+                        }
+                        else
+                        {
+                            float pg;
+                            wl = emit_direction(s, r, e1, e2, e3, x, &pg, &ti, &tp);
+                            pl = (1.0f - s->envSelect) * pg;
+                            float em[7];
+                            spo_v2 uv0 = {0, 0};
+                            spo_material_eval_internal(s, s->objects[ti].material, uv0,
+                                                       v3neg(wl), tx, em);
+                            Le = v3p(em + 3);
+                        }
                         float cl = v3dot(h.normal, wl);
-                        if (cl > 0.0f && pe > 0.0f)
+                        if (cl > 0.0f && pl > 0.0f)
                         {
                             spo_v3 Xl = brdf_bracket(h.normal, V, wl, mat, tx, 1);
                             float pcl = (1.0f - ps) * cl * (1.0f / SPO_PI);
                             float pb = ps > 0.0f ? pcl + ps * ggx_pdf(h.normal, V, wl, rough)
                                                  : pcl;
-                            float bg[7];
-                            spo_v2 uv0 = {0, 0};
-                            spo_material_eval_internal(s, s->backgroundMaterialId, uv0,
-                                                       v3neg(wl), tx, bg);
-                            float pe2 = pe * pe;
-                            float wmis = pe2 / (pe2 + pb * pb);
-                            spo_v3 C = v3had(v3had(beta, v3mul(Xl, (cl * wmis) / pe)),
-                                             v3p(bg + 3));
-                            spo_v3 sp = v3add(v3add(o, v3mul(d, h.t)),
-                                              v3mul(h.normal, 0.0001f));
+                            float pl2 = pl * pl;
+                            float wmis = pl2 / (pl2 + pb * pb);
+                            spo_v3 C = v3had(v3had(beta, v3mul(Xl, (cl * wmis) / pl)), Le);
                             float so[3], sd[3];
-                            v3put(so, sp); v3put(sd, wl);
+                            v3put(so, x); v3put(sd, wl);
                             spo_SceneHit sh;
                             if (spo_scene_intersect(s, so, sd, cfg->hit, tx, &sh,
                                                     metrics) < 0)
                             { *fail = 1; return L; }
                             metrics[2]++;
-                            if (!(sh.t > 0.0f)) L = v3add(L, C);
+                            int hitAny = sh.t > 0.0f;
+                            int visible = ti == 0xFFFFFFFFu
+                                              ? !hitAny
+                                              : hitAny && sh.object == ti && sh.prim == tp;
+                            if (visible) L = v3add(L, C);
                         }
                     }
                 }
@@ -371,7 +455,8 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                     /* the BSDF-sampled direction's share (power heuristic) */
                     float dd[3];
                     v3put(dd, d);
-                    float pe = s->envPdf ? spo_env_pdf(s->envPdf, s->envW, s->envH, dd)
+                    float pe = s->envPdf ? s->envSelect *
+                                               spo_env_pdf(s->envPdf, s->envW, s->envH, dd)
                                          : 0.0f;
                     float pb = lastPdf;
                     float pb2 = pb * pb;

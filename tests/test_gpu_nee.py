@@ -29,24 +29,31 @@ def test_nee_config1_matches_oracle_bit_exact(device, rng, orng, env):
     assert [m.values[k] for k in (1, 2, 3, 4)] == [int(met[k]) for k in (1, 2, 3, 4)]
 
 
-def test_nee_box_scene_tiles_match_oracle_bit_exact(device):
-    """Config 4 (a million triangles in the closed box, the sky only through
-    the opening): tiles of the 512x512 image at 2 spp and 6 bounces equal
-    the oracle's over the same SAH hierarchy bit for bit."""
-    desc = scenes.config4()
+@pytest.mark.parametrize("name,tiles", [
+    ("config4", [(224, 224, 288, 288), (0, 448, 64, 512)]),
+    ("config5", [(928, 508, 992, 572), (0, 1040, 64, 1080)]),
+])
+def test_nee_box_scene_tiles_match_oracle_bit_exact(device, name, tiles):
+    """The million-triangle box scenes, 2 spp and 6 bounces: config 4 (the
+    sky only, through the opening) and config 5 (the sky and 54 emissive
+    spheres: light samples split between the sky table and the ~1.1M
+    emissive triangles): tiles equal the oracle's over the same SAH
+    hierarchy bit for bit, as do the ray (shadow rays included), hit and
+    miss counts."""
+    desc = getattr(scenes, name)()
     w, h = desc["camera"][0], desc["camera"][1]
     s, cam = scenes.load_gpu(device, desc)
     ctx = V.make_context(cam, s, spp=2, bounces=6, clamp=0.0, estimator=V.EST_NEE,
                          rng=V.RNG_PCG, seed=0x5EED, collect=True)
     os_, ocam = load_oracle(desc, sah=True)
-    for t in [(224, 224, 288, 288), (0, 448, 64, 512)]:
+    for t in tiles:
         tile = V.Tile(*t)
         img = np.zeros(w * h * 4, np.float32)
         m = V.sp_PathTraceTile(ctx, tile, img)
         px, met = _oracle_tile_sah(os_, ocam, desc, tile, 2, 6, SPO_EST_NEE)
         a = img.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
         b = px.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
-        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), t
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (name, t)
         assert [m.values[k] for k in (2, 3, 4)] == [int(met[k]) for k in (2, 3, 4)]
     oracle().spo_scene_destroy(os_)
     s.close()
@@ -74,3 +81,15 @@ def test_nee_cuts_error_at_equal_samples(device):
                    for k in range(4)])
     assert np.mean(en) < np.mean(eu) / 3, (en, eu)
     assert np.mean(mn) == pytest.approx(np.mean(mu), rel=0.01)
+
+
+def test_nee_emitter_scene_matches_oracle_bit_exact(device):
+    """A floor lit only by a small emissive sphere (no sky table: every light
+    sample goes to the emitter table): bit-exact against the oracle."""
+    from test_nee import _lamp_scene
+    desc = _lamp_scene()
+    for rng, orng in [(V.RNG_PCG, SPO_RNG_PCG), (V.RNG_XORSHIFT_PIXEL, SPO_RNG_XORSHIFT_PIXEL)]:
+        img, m = gpu_render(device, desc, 8, 3, V.EST_NEE, rng, clamp=0.0)
+        px, met = oracle_render(desc, 8, 3, SPO_EST_NEE, orng, clamp=0.0)
+        assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
+        assert [m.values[k] for k in (1, 2, 3, 4)] == [int(met[k]) for k in (1, 2, 3, 4)]

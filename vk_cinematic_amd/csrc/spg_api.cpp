@@ -283,6 +283,7 @@ static void free_scene_device(spg_Scene *s)
     free_ptr(s->dTexs);
     free_ptr(s->dTexels);
     free_ptr(s->dEnv);
+    free_ptr(s->dEmit);
 }
 
 int spg_scene_destroy(spg_Scene *s)
@@ -551,6 +552,51 @@ static int upload_tables(spg_Scene *s)
             ds.envCount = (uint32_t)n;
         }
     }
+    // next-event estimation table of the emissive triangles
+    ds.emitPdfA = ds.emitProb = nullptr;
+    ds.emitAlias = ds.emitPair = nullptr;
+    ds.emitBase = nullptr;
+    ds.emitCount = 0;
+    free_ptr(s->dEmit);
+    {
+        std::vector<EmitterInput> ein;
+        for (auto &o : s->objects)
+        {
+            EmitterInput e{o.model, nullptr, nullptr, 0, 0.0};
+            const int slot = find_index(s->materialKeys, o.materialId);
+            const HostMesh &m = s->meshes[o.mesh];
+            if (slot >= 0 && mats[slot].emissionTex < 0)
+            {
+                e.verts = m.verts.data();
+                e.idx = m.idx.data();
+                e.triCount = (uint32_t)(m.idx.size() / 3);
+                e.lum = ((double)mats[slot].emission[0] + (double)mats[slot].emission[1] +
+                         (double)mats[slot].emission[2]) / 3.0;
+            }
+            ein.push_back(e);
+        }
+        std::vector<float> pdfA, prob;
+        std::vector<uint32_t> alias, pair;
+        std::vector<int32_t> base;
+        if (build_emitter_table(ein, pdfA, prob, alias, pair, base))
+        {
+            const size_t n = pdfA.size(), ni = base.size();
+            HIPCHK(hipMalloc(&s->dEmit, 20 * n + 4 * ni));
+            char *e = (char *)s->dEmit;
+            HIPCHK(hipMemcpy(e, pdfA.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + 4 * n, prob.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + 8 * n, alias.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + 12 * n, pair.data(), 8 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + 20 * n, base.data(), 4 * ni, hipMemcpyHostToDevice));
+            ds.emitPdfA = (const float *)e;
+            ds.emitProb = (const float *)(e + 4 * n);
+            ds.emitAlias = (const uint32_t *)(e + 8 * n);
+            ds.emitPair = (const uint32_t *)(e + 12 * n);
+            ds.emitBase = (const int32_t *)(e + 20 * n);
+            ds.emitCount = (uint32_t)n;
+        }
+    }
+    ds.envSelect = ds.envTex >= 0 ? (ds.emitCount ? 0.5f : 1.0f) : 0.0f;
     s->dirty = false;
     return SPG_OK;
 }
@@ -892,7 +938,7 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
            oQo1 = take(12 * P), oQd0 = take(12 * P), oQd1 = take(12 * P),
            oQp0 = take(4 * P), oQp1 = take(4 * P), oQo2 = take(12 * P),
            oQd2 = take(12 * P), oQp2 = take(4 * P), oPdf = take(4 * P),
-           oNee = take(12 * P), oCnt = take(64),
+           oNee = take(12 * P), oTgt = take(8 * P), oCnt = take(64),
            oMet = take(128), oHt = take(4 * P), oHu = take(4 * P),
            oHv = take(4 * P), oHi = take(4 * P), oHp = take(4 * P);
     HIPCHK(hipMalloc(&block, off));
@@ -919,6 +965,7 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
     p.qpath[2] = (uint32_t *)(b + oQp2);
     p.lastPdf = (float *)(b + oPdf);
     p.nee = (float *)(b + oNee);
+    p.neeTarget = (uint32_t *)(b + oTgt);
     p.counters = (uint32_t *)(b + oCnt);
     p.metrics = (unsigned long long *)(b + oMet);
     p.ht = (float *)(b + oHt);

@@ -24,6 +24,37 @@
 
 namespace spg {
 
+// Vose's alias table of probabilities P (sum 1), n entries: prob[i] (float)
+// and alias[i]; small and large work lists are stacks filled in ascending
+// order, leftovers keep probability 1.
+inline void build_alias(const std::vector<double> &P, std::vector<float> &prob,
+                        std::vector<uint32_t> &alias)
+{
+    const size_t n = P.size();
+    prob.assign(n, 1.0f);
+    alias.resize(n);
+    std::vector<double> q(n);
+    std::vector<uint32_t> small, large;
+    for (size_t i = 0; i < n; ++i)
+    {
+        q[i] = P[i] * (double)n;
+        alias[i] = (uint32_t)i;
+        if (q[i] < 1.0) small.push_back((uint32_t)i);
+        else large.push_back((uint32_t)i);
+    }
+    while (!small.empty() && !large.empty())
+    {
+        const uint32_t s = small.back(), l = large.back();
+        small.pop_back();
+        large.pop_back();
+        prob[s] = (float)q[s];
+        alias[s] = l;
+        q[l] = (q[l] + q[s]) - 1.0;
+        if (q[l] < 1.0) small.push_back(l);
+        else large.push_back(l);
+    }
+}
+
 // Face-plane rectangle of texel (ix, iy) of a w x h face.
 inline void env_texel_rect(uint32_t ix, uint32_t iy, uint32_t w, uint32_t h,
                            double &x0, double &x1, double &y0, double &y1)
@@ -64,30 +95,92 @@ inline bool build_env_table(const float *rgba, uint32_t w, uint32_t h,
     }
     if (!(total > 0.0) || !std::isfinite(total)) return false;
     pdf.resize(n);
-    prob.assign(n, 1.0f);
-    alias.resize(n);
-    std::vector<double> q(n);
-    std::vector<uint32_t> small, large;
+    std::vector<double> P(n);
     for (size_t i = 0; i < n; ++i)
     {
-        const double P = wt[i] / total;
-        pdf[i] = (float)(P / area[i]);
-        q[i] = P * (double)n;
-        alias[i] = (uint32_t)i;
-        if (q[i] < 1.0) small.push_back((uint32_t)i);
-        else large.push_back((uint32_t)i);
+        P[i] = wt[i] / total;
+        pdf[i] = (float)(P[i] / area[i]);
     }
-    while (!small.empty() && !large.empty())
+    build_alias(P, prob, alias);
+    return true;
+}
+
+// Emissive triangles (next-event estimation toward emitting geometry): for
+// every instance whose material has a constant emission with a positive
+// mean channel value, every triangle of its mesh in index order, weight =
+// that mean x the triangle's world-space area (vertices transformed by the
+// instance's column-major model matrix in double).  P_g = weight / sum
+// (double, instance-then-triangle order); pdfA[g] = P_g / area_g (density
+// per unit area; 0 for a degenerate triangle); alias table as above;
+// pair[g] = (instance, triangle); base[i] = the first entry of instance i
+// or -1.
+struct EmitterInput
+{
+    const float *model;     // 16 floats, column-major
+    const float *verts;     // 8 floats per vertex
+    const uint32_t *idx;
+    uint32_t triCount;
+    double lum;             // mean emission channel, <= 0: not an emitter
+};
+
+inline bool build_emitter_table(const std::vector<EmitterInput> &inst,
+                                std::vector<float> &pdfA, std::vector<float> &prob,
+                                std::vector<uint32_t> &alias,
+                                std::vector<uint32_t> &pair,
+                                std::vector<int32_t> &base)
+{
+    pdfA.clear();
+    prob.clear();
+    alias.clear();
+    pair.clear();
+    base.assign(inst.size(), -1);
+    std::vector<double> wt, area;
+    for (size_t i = 0; i < inst.size(); ++i)
     {
-        const uint32_t s = small.back(), l = large.back();
-        small.pop_back();
-        large.pop_back();
-        prob[s] = (float)q[s];
-        alias[s] = l;
-        q[l] = (q[l] + q[s]) - 1.0;
-        if (q[l] < 1.0) small.push_back(l);
-        else large.push_back(l);
+        const EmitterInput &e = inst[i];
+        if (!(e.lum > 0.0) || !std::isfinite(e.lum) || e.triCount == 0) continue;
+        base[i] = (int32_t)wt.size();
+        const float *m = e.model;
+        for (uint32_t k = 0; k < e.triCount; ++k)
+        {
+            double w[3][3];
+            for (int v = 0; v < 3; ++v)
+            {
+                const float *p = e.verts + 8 * (size_t)e.idx[3 * k + v];
+                for (int r = 0; r < 3; ++r)
+                    w[v][r] = (double)m[r] * p[0] + (double)m[4 + r] * p[1] +
+                              (double)m[8 + r] * p[2] + (double)m[12 + r];
+            }
+            const double ux = w[1][0] - w[0][0], uy = w[1][1] - w[0][1],
+                         uz = w[1][2] - w[0][2];
+            const double vx = w[2][0] - w[0][0], vy = w[2][1] - w[0][1],
+                         vz = w[2][2] - w[0][2];
+            const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz,
+                         cz = ux * vy - uy * vx;
+            const double a = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            area.push_back(a);
+            wt.push_back(std::isfinite(a) ? e.lum * a : 0.0);
+            pair.push_back((uint32_t)i);
+            pair.push_back(k);
+        }
     }
+    double total = 0.0;
+    for (double w : wt) total += w;
+    if (!(total > 0.0) || !std::isfinite(total))
+    {
+        pair.clear();
+        base.assign(inst.size(), -1);
+        return false;
+    }
+    const size_t n = wt.size();
+    pdfA.resize(n);
+    std::vector<double> P(n);
+    for (size_t g = 0; g < n; ++g)
+    {
+        P[g] = wt[g] / total;
+        pdfA[g] = area[g] > 0.0 ? (float)(P[g] / area[g]) : 0.0f;
+    }
+    build_alias(P, prob, alias);
     return true;
 }
 

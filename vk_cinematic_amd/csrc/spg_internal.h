@@ -109,6 +109,15 @@ struct DevScene
     const uint32_t *envAlias;
     int32_t envTex;             // texture slot of the cube map
     uint32_t envCount;          // 6 * w * h
+    // emissive triangles (next-event estimation toward emitting geometry);
+    // emitCount 0: none
+    const float *emitPdfA;      // per entry: P_g / world area
+    const float *emitProb;      // alias table
+    const uint32_t *emitAlias;
+    const uint32_t *emitPair;   // [2 * g]: instance, triangle
+    const int32_t *emitBase;    // per instance: first entry or -1
+    uint32_t emitCount;
+    float envSelect;            // probability of sampling the environment
 };
 
 struct DevCamera
@@ -137,6 +146,8 @@ struct DevPaths
     uint32_t *counters;          // [0..2] queue lengths
     float *lastPdf;              // density of each path's last sampled direction
     float *nee;                  // [3][capacity] pending shadow-ray contribution
+    uint32_t *neeTarget;         // [2][capacity] instance, triangle the shadow
+                                 // ray must reach (instance SPG_MISS: the sky)
     unsigned long long *metrics; // rays, hits, misses, box, mesh, tri tests
     float *ht, *hu, *hv;         // hit records per queued ray
     uint32_t *hinst, *hprim;
@@ -304,6 +315,7 @@ struct spg_Scene
     uint32_t tlasPending = 0;       // stack bound of the wide instance tree
     // device-side tables
     void *dEnv = nullptr;           // environment table: pdf, prob, alias
+    void *dEmit = nullptr;          // emitter table: pdfA, prob, alias, pair, base
     void *dTlasNodes = nullptr, *dInst = nullptr,
          *dMeshes = nullptr, *dMats = nullptr, *dTexs = nullptr,
          *dTexels = nullptr;

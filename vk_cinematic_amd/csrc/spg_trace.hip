@@ -192,6 +192,59 @@ __device__ __forceinline__ v3 env_direction(const DevScene &S, uint32_t r, float
     return normalize(v);
 }
 
+// World-space vertices of triangle `prim` of instance `inst`.
+__device__ __forceinline__ void world_triangle(const DevScene &S, uint32_t inst,
+                                               uint32_t prim, v3 &A, v3 &B, v3 &C)
+{
+    const DevInstance &I = S.inst[inst];
+    const DevMesh &M = S.meshes[I.mesh];
+    const float *p0 = M.verts + 8 * (size_t)M.idx[3 * prim];
+    const float *p1 = M.verts + 8 * (size_t)M.idx[3 * prim + 1];
+    const float *p2 = M.verts + 8 * (size_t)M.idx[3 * prim + 2];
+    A = xform_point(I.model, v3{p0[0], p0[1], p0[2]});
+    B = xform_point(I.model, v3{p1[0], p1[1], p1[2]});
+    C = xform_point(I.model, v3{p2[0], p2[1], p2[2]});
+}
+
+// Density per steradian with which the emitter sampler reaches triangle
+// `prim` of instance `inst` at distance t along unit direction d:
+// pdfA * t^2 / |cos| at the light, 0 if the triangle is not in the table.
+__device__ __forceinline__ float emit_pdf(const DevScene &S, uint32_t inst, uint32_t prim,
+                                          float t, v3 d)
+{
+    const int32_t b = S.emitBase[inst];
+    if (b < 0) return 0.0f;
+    v3 A, B, C;
+    world_triangle(S, inst, prim, A, B, C);
+    const v3 n = cross(B - A, C - A);
+    const float c = fabsf(dot(n, d)) / sqrtf(dot(n, n));
+    return c > 0.0f ? S.emitPdfA[b + prim] * (t * t) / c : 0.0f;
+}
+
+// A point on an emissive triangle drawn from the table (entry by the alias
+// table from r and e1, uniform on the triangle by e2, e3): the unit
+// direction from x, its density per steradian, the triangle.
+__device__ __forceinline__ v3 emit_direction(const DevScene &S, uint32_t r, float e1,
+                                             float e2, float e3, v3 x, float &pdf,
+                                             uint32_t &inst, uint32_t &prim)
+{
+    uint32_t g = (uint32_t)(((uint64_t)r * S.emitCount) >> 32);
+    if (!(e1 < S.emitProb[g])) g = S.emitAlias[g];
+    inst = S.emitPair[2 * g];
+    prim = S.emitPair[2 * g + 1];
+    v3 A, B, C;
+    world_triangle(S, inst, prim, A, B, C);
+    const float su = sqrtf(e2);
+    const v3 P = (A * (1.0f - su) + B * (su * (1.0f - e3))) + C * (su * e3);
+    const v3 v = P - x;
+    const float dist2 = dot(v, v);
+    const v3 w = normalize(v);
+    const v3 n = cross(B - A, C - A);
+    const float c = fabsf(dot(n, w)) / sqrtf(dot(n, n));
+    pdf = c > 0.0f && dist2 > 0.0f ? S.emitPdfA[g] * dist2 / c : 0.0f;
+    return w;
+}
+
 struct MatOut
 {
     v3 albedo, emission;
@@ -717,7 +770,20 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                                p.paths.hv[i], N, uv);
                     MatOut m = eval_material<kDouble>(
                         S, S.inst[inst].material, uv, V);
-                    L = L + had(beta, m.emission);
+                    if (kEst == SPG_EST_NEE && vc > 0 && S.emitCount > 0 &&
+                        S.emitBase[inst] >= 0)
+                    {
+                        // a BSDF-sampled ray that reached an emitter: its
+                        // share against the emitter sampler (power heuristic)
+                        float pl = (1.0f - S.envSelect) *
+                                   emit_pdf(S, inst, p.paths.hprim[i], t, d);
+                        float pb = p.paths.lastPdf[path];
+                        float pb2 = pb * pb;
+                        float w = pb2 / (pb2 + pl * pl);
+                        L = L + had(beta, m.emission * w);
+                    }
+                    else
+                        L = L + had(beta, m.emission);
                     // cosine lobe, or GGX with probability ps on rough
                     // materials; weight X cos / pdf (X pi for ps = 0)
                     const float ps = m.roughness > 0.0f ? kSpecProb : 0.0f;
@@ -728,33 +794,52 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     float cosine = dot(N, wi);
                     if (kEst == SPG_EST_NEE && vc + 1 < p.bounces)
                     {
-                        // next-event estimation: a direction from the
-                        // environment table, weighted by the power
-                        // heuristic against the BSDF sampler's density; its
-                        // contribution waits for the shadow ray's answer
+                        // next-event estimation: the sky (probability
+                        // envSelect) or an emissive triangle, weighted by
+                        // the power heuristic against the BSDF sampler's
+                        // density; the contribution waits for the shadow
+                        // ray's answer
+                        float e0 = rng.u24();
                         uint32_t r = rng.next();
                         float e1 = rng.u24(), e2 = rng.u24(), e3 = rng.u24();
-                        if (S.envTex >= 0)
+                        const bool useEnv = e0 < S.envSelect;
+                        if (useEnv ? S.envTex >= 0 : S.emitCount > 0)
                         {
-                            v3 wl = env_direction(S, r, e1, e2, e3);
-                            float pe = env_pdf(S, wl);
+                            const v3 x = (o + d * t) + N * 0.0001f;
+                            v3 wl, Le;
+                            float pl;
+                            uint32_t ti = SPG_MISS, tp = 0;
+                            if (useEnv)
+                            {
+                                wl = env_direction(S, r, e1, e2, e3);
+                                pl = S.envSelect * env_pdf(S, wl);
+                                v2 uv0{0.0f, 0.0f};
+                                Le = eval_material<kDouble>(S, S.background, uv0, -wl)
+                                         .emission;
+                            }
+                            else
+                            {
+                                float pg;
+                                wl = emit_direction(S, r, e1, e2, e3, x, pg, ti, tp);
+                                pl = (1.0f - S.envSelect) * pg;
+                                const float *em = S.mats[S.inst[ti].material].emission;
+                                Le = v3{em[0], em[1], em[2]};
+                            }
                             float cl = dot(N, wl);
-                            if (cl > 0.0f && pe > 0.0f)
+                            if (cl > 0.0f && pl > 0.0f)
                             {
                                 v3 Xl = brdf_bracket<kDouble>(N, V, wl, m, true);
                                 float pb = bsdf_pdf(N, V, wl, cl, m.roughness, ps);
-                                v2 uv0{0.0f, 0.0f};
-                                MatOut bg = eval_material<kDouble>(S, S.background,
-                                                                   uv0, -wl);
-                                float pe2 = pe * pe;
-                                float wmis = pe2 / (pe2 + pb * pb);
-                                v3 C = had(had(beta, Xl * ((cl * wmis) / pe)),
-                                           bg.emission);
+                                float pl2 = pl * pl;
+                                float wmis = pl2 / (pl2 + pb * pb);
+                                v3 C = had(had(beta, Xl * ((cl * wmis) / pl)), Le);
                                 p.paths.nee[path] = C.x;
                                 p.paths.nee[cap + path] = C.y;
                                 p.paths.nee[2 * cap + path] = C.z;
+                                p.paths.neeTarget[path] = ti;
+                                p.paths.neeTarget[cap + path] = tp;
                                 shadow = true;
-                                so = (o + d * t) + N * 0.0001f;
+                                so = x;
                                 sd = wl;
                             }
                         }
@@ -784,7 +869,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     {
                         // the BSDF-sampled direction's share of the
                         // environment (power heuristic)
-                        float pe = env_pdf(S, d);
+                        float pe = S.envSelect * env_pdf(S, d);
                         float pb = p.paths.lastPdf[path];
                         float pb2 = pb * pb;
                         float w = pb2 / (pb2 + pe * pe);
@@ -938,17 +1023,24 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
     wave_add(&p.paths.metrics[2], misses);
 }
 
-// Shadow rays of next-event estimation (queue 2, traced by k_extend): an
-// unoccluded one adds its pending contribution to the path's radiance.
+// Shadow rays of next-event estimation (queue 2, traced by k_extend): one
+// that reaches its light (the sky: no hit; an emissive triangle: that
+// triangle is the closest hit) adds its pending contribution to the path's
+// radiance.
 __global__ __launch_bounds__(256) void k_resolve_shadow(TraceParams p)
 {
     const uint32_t n = p.paths.counters[2];
     const uint32_t cap = p.paths.capacity;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
     {
-        const bool blocked = p.paths.hinst[i] != SPG_MISS && p.paths.ht[i] > 0.0f;
-        if (blocked) continue;
         const uint32_t path = p.paths.qpath[2][i];
+        const uint32_t ti = p.paths.neeTarget[path], tp = p.paths.neeTarget[cap + path];
+        const bool hitAny = p.paths.hinst[i] != SPG_MISS && p.paths.ht[i] > 0.0f;
+        // the sky is seen through a miss, a light triangle by hitting it first
+        const bool visible = ti == SPG_MISS
+                                 ? !hitAny
+                                 : hitAny && p.paths.hinst[i] == ti && p.paths.hprim[i] == tp;
+        if (!visible) continue;
         p.paths.radiance[path] = p.paths.radiance[path] + p.paths.nee[path];
         p.paths.radiance[cap + path] = p.paths.radiance[cap + path] + p.paths.nee[cap + path];
         p.paths.radiance[2 * cap + path] =
