@@ -93,3 +93,24 @@ def test_nee_emitter_scene_matches_oracle_bit_exact(device):
         px, met = oracle_render(desc, 8, 3, SPO_EST_NEE, orng, clamp=0.0)
         assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
         assert [m.values[k] for k in (1, 2, 3, 4)] == [int(met[k]) for k in (1, 2, 3, 4)]
+
+
+def test_nee_error_paths(device):
+    """An unknown estimator is an error, not an abort; batches refuse the NEE
+    estimator (they take parity and unbiased only); NEE with float
+    transcendentals renders a finite image."""
+    desc = scenes.config1(32, 24)
+    s, cam = scenes.load_gpu(device, desc)
+    img = np.zeros(32 * 24 * 4, np.float32)
+    ctx = V.make_context(cam, s, spp=1, bounces=3, estimator=7)
+    with pytest.raises(V.SpgError):
+        V.path_trace_tiles(ctx, V.ComputeTiles(32, 24, 64, 64), img)
+    buf = device.alloc(32 * 24 * 16)
+    with pytest.raises(V.SpgError):
+        V.Batch(device, [(cam, s, buf)], spp=1, bounces=3, estimator=V.EST_NEE)
+    ctx = V.make_context(cam, s, spp=4, bounces=3, clamp=0.0, estimator=V.EST_NEE,
+                         transcendentals=V.TX_FAST)
+    V.path_trace_tiles(ctx, V.ComputeTiles(32, 24, 64, 64), img)
+    assert np.isfinite(img).all() and img.reshape(-1, 4)[:, :3].mean() > 0
+    buf.free()
+    s.close()

@@ -387,6 +387,23 @@ int spg_scene_add_object(spg_Scene *s, uint32_t meshId, uint32_t materialId,
     }
     memcpy(o.aabbLo, &rmin, 12);
     memcpy(o.aabbHi, &rmax, 12);
+    // The instance tree bounds each instance by its transformed vertices
+    // (the box of a rotated mesh box can be up to sqrt(3) wider per axis):
+    // fewer rays enter instances they miss.  The result does not depend on
+    // it, since the tree's padded boxes contain every triangle either way.
+    v3 tmin = rmin, tmax = rmax;
+    if (!m.verts.empty())
+    {
+        tmin = tmax = m4point(model, v3{m.verts[0], m.verts[1], m.verts[2]});
+        for (size_t i = 8; i < m.verts.size(); i += 8)
+        {
+            v3 p = m4point(model, v3{m.verts[i], m.verts[i + 1], m.verts[i + 2]});
+            tmin = vmin(tmin, p);
+            tmax = vmax(tmax, p);
+        }
+    }
+    memcpy(o.tightLo, &tmin, 12);
+    memcpy(o.tightHi, &tmax, 12);
     s->objects.push_back(o);
     s->built = false;
     if (objectId) *objectId = (uint32_t)(s->objects.size() - 1);
@@ -636,8 +653,8 @@ int spg_scene_build(spg_Scene *s, const spg_BuildOpts *opts)
         std::vector<float> lo(3 * (size_t)no), hi(3 * (size_t)no);
         for (uint32_t i = 0; i < no; ++i)
         {
-            memcpy(&lo[3 * i], s->objects[i].aabbLo, 12);
-            memcpy(&hi[3 * i], s->objects[i].aabbHi, 12);
+            memcpy(&lo[3 * i], s->objects[i].tightLo, 12);
+            memcpy(&hi[3 * i], s->objects[i].tightHi, 12);
         }
         BvhBuildInput in{lo.data(), hi.data(), no, 1, opt.sahBins, opt.threads,
                          kMaxTlasDepth};

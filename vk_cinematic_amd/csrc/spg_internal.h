@@ -291,7 +291,8 @@ struct HostObject
     uint32_t mesh, materialId;
     float pos[3], quat[4], scale[3];
     float model[16], inv[16];
-    float aabbLo[3], aabbHi[3];
+    float aabbLo[3], aabbHi[3];   // TransformAabb of the mesh bounds
+    float tightLo[3], tightHi[3]; // bounds of the transformed vertices (instance tree)
 };
 
 struct HostTexture
