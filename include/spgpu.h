@@ -414,6 +414,10 @@ int spg_accumulate_device(spg_Device *dev, float *dst, const float *src,
  * SPG_TX_FAST. */
 int spg_tonemap_device(spg_Device *dev, const float *rgba, uint32_t pixels,
                        float exposure, int transcendentals, uint8_t *out);
+/* Host-only: sin and cos of each x as SPG_TX_DOUBLE evaluates them on the
+ * device (the same code: a double-precision evaluation rounded once to
+ * float), for checking them against a correctly rounded library. */
+int spg_sincos_host(const float *x, uint32_t n, float *s, float *c);
 /* Host-only: the importance-sampling table SPG_EST_NEE builds for a
  * background cube map (rgba: 6 faces of faceWidth x faceHeight RGBA32F in
  * the layout spg_cubemap_register takes).  Texel i (face-major, then rows,

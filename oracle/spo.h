@@ -207,6 +207,8 @@ void spo_sample_bilinear(const float *pixels, uint32_t w, uint32_t h,
 void spo_cubemap_basis(uint32_t face, float *out9);
 void spo_cubemap_lookup(const float *faces, uint32_t w, uint32_t h,
                         const float *dir, float *out4);
+/* sin and cos in double, rounded once to float, of an array (SPO_TX_CR) */
+void spo_sincos_cr_array(const float *x, uint32_t n, float *s, float *c);
 /* ---- environment importance sampling (next-event estimation) ---- */
 int spo_env_table_build(const float *rgba, uint32_t w, uint32_t h, float *pdf,
                         float *prob, uint32_t *alias);

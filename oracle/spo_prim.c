@@ -830,3 +830,14 @@ void spo_plane(float *verts8, uint32_t *idx, uint32_t *nv, uint32_t *ni)
     *nv = 4;
     *ni = 6;
 }
+
+/* sin and cos of each x evaluated in double by the C library and rounded
+ * once to float (SPO_TX_CR) */
+void spo_sincos_cr_array(const float *x, uint32_t n, float *s, float *c)
+{
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        s[i] = (float)sin((double)x[i]);
+        c[i] = (float)cos((double)x[i]);
+    }
+}

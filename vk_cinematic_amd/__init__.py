@@ -178,6 +178,7 @@ _SIGS = {
     "spg_accumulate_device": (_int, [_vp, _vp, _vp, _u32, _u32, _u32]),
     "spg_tonemap_device": (_int, [_vp, _vp, _u32, C.c_float, _int, _vp]),
     "spg_env_table_build": (_int, [_f32p, _u32, _u32, _f32p, _f32p, _u32p]),
+    "spg_sincos_host": (_int, [_f32p, _u32, _f32p, _f32p]),
     "spg_time_post": (_int, [_vp, _vp, _vp, _u32, C.c_float, _int, _vp, _u32,
                              C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "spg_write_pfm": (_int, [C.c_char_p, _u32, _u32, _f32p]),

@@ -1440,6 +1440,13 @@ int spg_time_post(spg_Device *d, float *dst, const float *src, uint32_t pixels,
     return SPG_OK;
 }
 
+int spg_sincos_host(const float *x, uint32_t n, float *s, float *c)
+{
+    if (n && (!x || !s || !c)) return fail(SPG_E_INVALID, "spg_sincos_host: NULL array");
+    for (uint32_t i = 0; i < n; ++i) tsincos(x[i], &s[i], &c[i]);
+    return SPG_OK;
+}
+
 int spg_env_table_build(const float *rgba, uint32_t faceWidth, uint32_t faceHeight,
                         float *pdf, float *prob, uint32_t *alias)
 {

@@ -253,11 +253,64 @@ SPG_HD float tsin(float x) { return (float)sin((double)x); }
 SPG_HD float tcos(float x) { return (float)cos((double)x); }
 SPG_HD float tatan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 SPG_HD float tpow(float x, float y) { return (float)pow((double)x, (double)y); }
-// sin and cos of one argument from one double-precision evaluation
+// sin and cos of one argument from one double-precision evaluation, each
+// rounded once to float.  For |x| < 2^20 pi/2: Cody-Waite reduction
+// r = x - k pi/2, k = rint(x 2/pi), with pi/2 split into two 33-bit parts
+// (k times either is exact) and a remainder (split from pi's decimal
+// expansion), then the Taylor series of sin to r^17 and cos to r^18 on
+// |r| <= pi/4 (the first omitted terms are below 1e-19 of the result) by
+// Horner's rule with FMA, and the quadrant's signs and swap (|x| < 1e-9:
+// x and 1 directly).  The double
+// results are within a few units in the last place of sin and cos, as
+// the library's are, so both round to the same float except when the
+// exact value lies within ~2^-50 of a rounding boundary.  Larger |x|: the
+// library's sincos.
 SPG_HD void tsincos(float x, float *s, float *c)
 {
+    const double xd = (double)x;
     double sd, cd;
-    sincos((double)x, &sd, &cd);
+    if (!(fabs(xd) < 1647099.0))  // 2^20 * pi/2; also NaN and infinities
+    {
+        sincos(xd, &sd, &cd);
+        *s = (float)sd;
+        *c = (float)cd;
+        return;
+    }
+    if (fabs(xd) < 1e-9)
+    {
+        // sin x = x and cos x = 1 after rounding to float (keeps the sign
+        // of a zero)
+        *s = x;
+        *c = 1.0f;
+        return;
+    }
+    const double kd = rint(xd * 0.6366197723675814);
+    const double P1 = 1.5707963267341256, P2 = 6.077100506303966e-11,
+                 P3 = 2.0222662487959506e-21;
+    const double r = ((xd - kd * P1) - kd * P2) - kd * P3;
+    const double r2 = r * r;
+    double ps = 2.8114572543455206e-15;
+    ps = fma(ps, r2, -7.647163731819816e-13);
+    ps = fma(ps, r2, 1.6059043836821613e-10);
+    ps = fma(ps, r2, -2.505210838544172e-08);
+    ps = fma(ps, r2, 2.7557319223985893e-06);
+    ps = fma(ps, r2, -0.0001984126984126984);
+    ps = fma(ps, r2, 0.008333333333333333);
+    ps = fma(ps, r2, -0.16666666666666666);
+    const double sr = fma(r * r2, ps, r);
+    double pc = -1.5619206968586225e-16;
+    pc = fma(pc, r2, 4.779477332387385e-14);
+    pc = fma(pc, r2, -1.1470745597729725e-11);
+    pc = fma(pc, r2, 2.08767569878681e-09);
+    pc = fma(pc, r2, -2.755731922398589e-07);
+    pc = fma(pc, r2, 2.48015873015873e-05);
+    pc = fma(pc, r2, -0.001388888888888889);
+    pc = fma(pc, r2, 0.041666666666666664);
+    pc = fma(pc, r2, -0.5);
+    const double cr = fma(r2, pc, 1.0);
+    const int q = (int)((long long)kd & 3);
+    sd = q == 0 ? sr : q == 1 ? cr : q == 2 ? -sr : -cr;
+    cd = q == 0 ? cr : q == 1 ? -sr : q == 2 ? -cr : sr;
     *s = (float)sd;
     *c = (float)cd;
 }
