@@ -955,7 +955,8 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
            oQo1 = take(12 * P), oQd0 = take(12 * P), oQd1 = take(12 * P),
            oQp0 = take(4 * P), oQp1 = take(4 * P), oQo2 = take(12 * P),
            oQd2 = take(12 * P), oQp2 = take(4 * P), oPdf = take(4 * P),
-           oNee = take(12 * P), oTgt = take(8 * P), oCnt = take(64),
+           oNee = take(12 * P), oTgt = take(8 * P), oDist = take(4 * P),
+           oCnt = take(64),
            oMet = take(128), oHt = take(4 * P), oHu = take(4 * P),
            oHv = take(4 * P), oHi = take(4 * P), oHp = take(4 * P);
     HIPCHK(hipMalloc(&block, off));
@@ -983,6 +984,7 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
     p.lastPdf = (float *)(b + oPdf);
     p.nee = (float *)(b + oNee);
     p.neeTarget = (uint32_t *)(b + oTgt);
+    p.neeDist = (float *)(b + oDist);
     p.counters = (uint32_t *)(b + oCnt);
     p.metrics = (unsigned long long *)(b + oMet);
     p.ht = (float *)(b + oHt);

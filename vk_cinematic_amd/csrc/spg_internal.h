@@ -148,6 +148,7 @@ struct DevPaths
     float *nee;                  // [3][capacity] pending shadow-ray contribution
     uint32_t *neeTarget;         // [2][capacity] instance, triangle the shadow
                                  // ray must reach (instance SPG_MISS: the sky)
+    float *neeDist;              // its distance bound (emitter targets)
     unsigned long long *metrics; // rays, hits, misses, box, mesh, tri tests
     float *ht, *hu, *hv;         // hit records per queued ray
     uint32_t *hinst, *hprim;

@@ -226,7 +226,8 @@ __device__ __forceinline__ float emit_pdf(const DevScene &S, uint32_t inst, uint
 // direction from x, its density per steradian, the triangle.
 __device__ __forceinline__ v3 emit_direction(const DevScene &S, uint32_t r, float e1,
                                              float e2, float e3, v3 x, float &pdf,
-                                             uint32_t &inst, uint32_t &prim)
+                                             float &dist, uint32_t &inst,
+                                             uint32_t &prim)
 {
     uint32_t g = (uint32_t)(((uint64_t)r * S.emitCount) >> 32);
     if (!(e1 < S.emitProb[g])) g = S.emitAlias[g];
@@ -242,6 +243,7 @@ __device__ __forceinline__ v3 emit_direction(const DevScene &S, uint32_t r, floa
     const v3 n = cross(B - A, C - A);
     const float c = fabsf(dot(n, w)) / sqrtf(dot(n, n));
     pdf = c > 0.0f && dist2 > 0.0f ? S.emitPdfA[g] * dist2 / c : 0.0f;
+    dist = sqrtf(dist2);
     return w;
 }
 
@@ -551,7 +553,13 @@ __global__ __launch_bounds__(256) void k_raygen(TraceParams p)
 // measured slower on the 4M-triangle box scene: 1.44, 1.80 and 2.43 ms per
 // launch against 1.41; the step is bound by instruction issue, and 7 and 8
 // spill.)
-template <bool kCount, bool kBatch>
+// kShadow: the shadow rays of next-event estimation (queue 2).  A ray
+// toward the sky only asks whether anything is hit, so it stops at the
+// first instance with a hit; a ray toward an emissive triangle starts with
+// its best t at 1.001 x the light's distance, so nothing beyond the light
+// is visited (the closest hit below it is still found: the ray reaches its
+// light iff that hit is the light's triangle).
+template <bool kCount, bool kBatch, bool kShadow>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
                                                         uint32_t *counter,
                                                         int *spill)
@@ -565,6 +573,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
     auto begin = [&](uint32_t i, Trav &s) {
         trav_begin_scene(s, v3{qo[i], qo[cap + i], qo[2 * cap + i]},
                          v3{qd[i], qd[cap + i], qd[2 * cap + i]});
+        if (kShadow)
+        {
+            const uint32_t path = p.paths.qpath[2][i];
+            if (p.paths.neeTarget[path] == SPG_MISS) s.anyHit = true;
+            else s.out.t = p.paths.neeDist[path];
+        }
         if (kBatch)
         {
             const DevScene &S =
@@ -819,8 +833,9 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                             }
                             else
                             {
-                                float pg;
-                                wl = emit_direction(S, r, e1, e2, e3, x, pg, ti, tp);
+                                float pg, dl;
+                                wl = emit_direction(S, r, e1, e2, e3, x, pg, dl, ti, tp);
+                                p.paths.neeDist[path] = dl * 1.001f;
                                 pl = (1.0f - S.envSelect) * pg;
                                 const float *em = S.mats[S.inst[ti].material].emission;
                                 Le = v3{em[0], em[1], em[2]};
@@ -1323,31 +1338,39 @@ hipError_t launch_raygen(hipStream_t s, const TraceParams &p)
 hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
                          const Persist &ps)
 {
-    static uint32_t gCount = persistent_grid(k_extend<true, false>);
-    static uint32_t gPlain = persistent_grid(k_extend<false, false>);
-    static uint32_t gCountB = persistent_grid(k_extend<true, true>);
-    static uint32_t gPlainB = persistent_grid(k_extend<false, true>);
+    static uint32_t gCount = persistent_grid(k_extend<true, false, false>);
+    static uint32_t gPlain = persistent_grid(k_extend<false, false, false>);
+    static uint32_t gCountB = persistent_grid(k_extend<true, true, false>);
+    static uint32_t gPlainB = persistent_grid(k_extend<false, true, false>);
+    static uint32_t gCountS = persistent_grid(k_extend<true, false, true>);
+    static uint32_t gPlainS = persistent_grid(k_extend<false, false, true>);
     uint64_t n = (uint64_t)p.slots * p.sppPass;  // bound on the queue length
     const bool batch = p.slotItem != nullptr;
-    uint32_t g = batch ? (p.collect ? gCountB : gPlainB) : (p.collect ? gCount : gPlain);
+    const bool shadow = q == 2 && !batch;
+    uint32_t g = batch    ? (p.collect ? gCountB : gPlainB)
+                 : shadow ? (p.collect ? gCountS : gPlainS)
+                          : (p.collect ? gCount : gPlain);
     if (ps.maxBlocks && g > ps.maxBlocks) g = ps.maxBlocks;
     dim3 grid(persist_blocks(g, n));
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, ps.counter, 0u);
+#define SPG_EXT(C, B, S)                                                          \
+    hipLaunchKernelGGL((k_extend<C, B, S>), grid, dim3(kTraceBlock), 0, s, p, q, \
+                       ps.counter, ps.spill)
     if (batch)
     {
-        if (p.collect)
-            hipLaunchKernelGGL((k_extend<true, true>), grid, dim3(kTraceBlock), 0, s, p,
-                               q, ps.counter, ps.spill);
-        else
-            hipLaunchKernelGGL((k_extend<false, true>), grid, dim3(kTraceBlock), 0, s,
-                               p, q, ps.counter, ps.spill);
+        if (p.collect) SPG_EXT(true, true, false);
+        else SPG_EXT(false, true, false);
+    }
+    else if (shadow)
+    {
+        if (p.collect) SPG_EXT(true, false, true);
+        else SPG_EXT(false, false, true);
     }
     else if (p.collect)
-        hipLaunchKernelGGL((k_extend<true, false>), grid, dim3(kTraceBlock), 0, s, p, q,
-                           ps.counter, ps.spill);
+        SPG_EXT(true, false, false);
     else
-        hipLaunchKernelGGL((k_extend<false, false>), grid, dim3(kTraceBlock), 0, s, p,
-                           q, ps.counter, ps.spill);
+        SPG_EXT(false, false, false);
+#undef SPG_EXT
     return hipGetLastError();
 }
 

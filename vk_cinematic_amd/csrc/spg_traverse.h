@@ -136,6 +136,7 @@ struct Trav
     uint32_t triNext, triEnd;  // pending triangles of the current leaf
     uint32_t mode;             // the next step's work (kMode*)
     bool mmW, mmL;             // world / mesh-space ray is finite: slab_hit_mm
+    bool anyHit;               // occlusion query: stop at the first instance hit
 };
 
 // Kinds of step.
@@ -330,6 +331,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
     s.sp += npush;
     const bool busy = isTri && s.triNext < s.triEnd;
 
+    bool stop = false;  // an occlusion query found a hit
     if (kTwoLevel)
     {
         // entering: o' = M^-1 o, d' = norm(M^-1 d) (src/sp_scene.cpp:272-276);
@@ -376,6 +378,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
             better = tw < s.out.t || s.out.t < 0.0f ||
                      (tw == s.out.t && s.inst < s.out.inst);
         }
+        stop = s.anyHit & better;
         s.out.t = better ? tw : s.out.t;
         s.out.u = better ? s.h.u : s.out.u;
         s.out.v = better ? s.h.v : s.out.v;
@@ -396,7 +399,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
         s.level = leave && !leaveStep ? 0 : s.level;
         needPop = needPop && !leaveStep;
     }
-    const bool done = needPop && s.sp == 0;
+    const bool done = (needPop && s.sp == 0) || stop;
     const bool doPop = needPop && s.sp != 0;
     int32_t popped = kNoRef;
     if (doPop) popped = st.read(s.sp - 1);
@@ -429,6 +432,7 @@ __device__ __forceinline__ void trav_begin_scene(Trav &s, v3 o, v3 d)
     s.inv = inverse(d);
     s.mmW = finite3(o) && finite3(s.inv);
     s.mmL = false;
+    s.anyHit = false;
     s.out = SceneHit{-1.0f, 0.0f, 0.0f, SPG_MISS, SPG_MISS};
     s.level = 0;
     s.sp = 0;
@@ -447,6 +451,7 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
     s.linv = inverse(d);
     s.mmL = finite3(o) && finite3(s.linv);
     s.mmW = false;
+    s.anyHit = false;
     s.bnodes = M.nodes;
     s.lim = INFINITY;
     s.h = Hit{INFINITY, 0.0f, 0.0f, SPG_MISS};
