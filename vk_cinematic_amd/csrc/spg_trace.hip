@@ -607,56 +607,49 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
     wave_add(&p.paths.metrics[0], rays);
 }
 
-// Cosine-weighted direction about n: r = sqrt(u1), phi = 2 pi u2,
-// z = sqrt(max(0, 1-u1)) in the orthonormal basis of Duff et al. 2017.
-template <bool kDouble>
-__device__ __forceinline__ v3 cosine_direction(v3 n, float u1, float u2)
-{
-    float r = sqrtf(u1);
-    float phi = 6.28318548f * u2;
-    float sp, cp;
-    Tx<kDouble>::sincos_(phi, &sp, &cp);
-    float lx = r * cp;
-    float ly = r * sp;
-    float lz = sqrtf(fmax_sel(0.0f, 1.0f - u1));
-    float sign = copysignf(1.0f, n.z);
-    float a = -1.0f / (sign + n.z);
-    float b = n.x * n.y * a;
-    v3 t1{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
-    v3 t2{b, sign + n.y * n.y * a, -n.y};
-    return (t1 * lx + t2 * ly) + n * lz;
-}
-
 // Probability of sampling the GGX lobe instead of the cosine lobe for a
 // material with roughness > 0 (one-sample MIS of the two).
 constexpr float kSpecProb = 0.25f;
 
-// GGX half-vector sampling (Walter et al. 2007, "Microfacet Models for
-// Refraction through Rough Surfaces"): cos^2 theta_h = (1 - u1) /
-// (1 + (a^2 - 1) u1) with a = roughness^2 (the reference's DistributionGGX
-// parameter), phi = 2 pi u2, L = 2 (V.H) H - V.
+// Direction sampling of the two lobes with one sincos of phi = 2 pi u2 and
+// one orthonormal frame about n (Duff et al. 2017, "Building an
+// Orthonormal Basis, Revisited"):
+//   cosine lobe: r = sqrt(u1), z = sqrt(max(0, 1 - u1)), direction
+//     r cos phi t1 + r sin phi t2 + z n;
+//   GGX half-vector sampling (Walter et al. 2007, "Microfacet Models for
+//     Refraction through Rough Surfaces"): cos^2 theta_h = (1 - u1) /
+//     (1 + (a^2 - 1) u1) with a = roughness^2 (the reference's
+//     DistributionGGX parameter), H in the same frame, L = 2 (V.H) H - V.
 template <bool kDouble>
-__device__ __forceinline__ v3 ggx_direction(v3 n, v3 V, float roughness, float u1,
-                                            float u2)
+__device__ __forceinline__ v3 lobe_direction(v3 n, v3 V, float roughness, bool ggx,
+                                             float u1, float u2)
 {
-    float a = roughness * roughness;
-    float a2 = a * a;
-    float c2 = (1.0f - u1) / (1.0f + (a2 - 1.0f) * u1);
-    float ct = sqrtf(c2);
-    float st = sqrtf(fmax_sel(0.0f, 1.0f - c2));
     float phi = 6.28318548f * u2;
     float sp, cp;
     Tx<kDouble>::sincos_(phi, &sp, &cp);
-    float hx = st * cp;
-    float hy = st * sp;
+    float rad, ax;
+    if (ggx)
+    {
+        float a = roughness * roughness;
+        float a2 = a * a;
+        float c2 = (1.0f - u1) / (1.0f + (a2 - 1.0f) * u1);
+        ax = sqrtf(c2);
+        rad = sqrtf(fmax_sel(0.0f, 1.0f - c2));
+    }
+    else
+    {
+        rad = sqrtf(u1);
+        ax = sqrtf(fmax_sel(0.0f, 1.0f - u1));
+    }
+    float x = rad * cp;
+    float y = rad * sp;
     float sign = copysignf(1.0f, n.z);
     float aa = -1.0f / (sign + n.z);
     float b = n.x * n.y * aa;
     v3 t1{1.0f + sign * n.x * n.x * aa, sign * b, -sign * n.x};
     v3 t2{b, sign + n.y * n.y * aa, -n.y};
-    v3 H = (t1 * hx + t2 * hy) + n * ct;
-    float vh = dot(V, H);
-    return H * (2.0f * vh) - V;
+    v3 D = (t1 * x + t2 * y) + n * ax;
+    return ggx ? D * (2.0f * dot(V, D)) - V : D;
 }
 
 // Density of ggx_direction: D(H) (N.H) / (4 V.H), H = normalize(L + V).
@@ -803,8 +796,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     const float ps = m.roughness > 0.0f ? kSpecProb : 0.0f;
                     float u0 = ps > 0.0f ? rng.u24() : 1.0f;
                     float u1 = rng.u24(), u2 = rng.u24();
-                    v3 wi = u0 < ps ? ggx_direction<kDouble>(N, V, m.roughness, u1, u2)
-                                    : cosine_direction<kDouble>(N, u1, u2);
+                    v3 wi = lobe_direction<kDouble>(N, V, m.roughness, u0 < ps, u1, u2);
                     float cosine = dot(N, wi);
                     if (kEst == SPG_EST_NEE && vc + 1 < p.bounces)
                     {
@@ -927,9 +919,8 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                         const float ps = m.roughness > 0.0f ? kSpecProb : 0.0f;
                         v3 wi;
                         if (u0 < fb)
-                            wi = u0 < fb * ps
-                                     ? ggx_direction<kDouble>(N, V, m.roughness, u1, u2)
-                                     : cosine_direction<kDouble>(N, u1, u2);
+                            wi = lobe_direction<kDouble>(N, V, m.roughness, u0 < fb * ps,
+                                                         u1, u2);
                         else
                             wi = mixture_sample(Lg, (u0 - fb) / (1.0f - fb), u1, u2);
                         float cosv = dot(N, wi);
