@@ -1068,16 +1068,28 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     if (s->dirty) CHK(upload_tables(s));
 
     // pixel slots in tile order (clamped to the image as
-    // sp_PathTraceTile clamps, src/simd_path_tracer.cpp:186-189)
-    std::vector<uint32_t> slots;
-    for (uint32_t k = 0; k < count; ++k)
+    // sp_PathTraceTile clamps, src/simd_path_tracer.cpp:186-189); kept from
+    // the previous call when the tiles and the image size are the same
+    const bool sameTiles =
+        d->slotW == c.width && d->slotH == c.height && d->slotTiles.size() == count &&
+        (count == 0 || memcmp(d->slotTiles.data(), tiles, count * sizeof(spg_Tile)) == 0);
+    if (!sameTiles)
     {
-        uint32_t x1 = std::min(tiles[k].maxX, c.width);
-        uint32_t y1 = std::min(tiles[k].maxY, c.height);
-        for (uint32_t y = tiles[k].minY; y < y1; ++y)
-            for (uint32_t x = tiles[k].minX; x < x1; ++x)
-                slots.push_back(y * c.width + x);
+        d->slots.clear();
+        for (uint32_t k = 0; k < count; ++k)
+        {
+            uint32_t x1 = std::min(tiles[k].maxX, c.width);
+            uint32_t y1 = std::min(tiles[k].maxY, c.height);
+            for (uint32_t y = tiles[k].minY; y < y1; ++y)
+                for (uint32_t x = tiles[k].minX; x < x1; ++x)
+                    d->slots.push_back(y * c.width + x);
+        }
+        d->slotTiles.assign(tiles, tiles + count);
+        d->slotW = c.width;
+        d->slotH = c.height;
+        d->slotsOnDevice = false;
     }
+    const std::vector<uint32_t> &slots = d->slots;
     const uint32_t spp = cfg.samplesPerPixel;
     const bool pcg = cfg.rngMode == SPG_RNG_PCG;
     uint64_t want = (uint64_t)slots.size() * (pcg ? spp : 1);
@@ -1085,7 +1097,9 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
                                                 kMaxPathsInFlight);
     CHK(ensure_paths(d, cap, cfg.bounces));
     cap = d->paths.capacity;
+    const size_t slotHave = d->slotPixelCount;
     CHK(ensure_buffer<uint32_t>(d->slotPixel, d->slotPixelCount, cap));
+    if (d->slotPixelCount != slotHave) d->slotsOnDevice = false;  // reallocated
     CHK(ensure_buffer<float>(d->accum, d->accumCount, 3 * (size_t)cap));
     HIPCHK(launch_clear_counters(d->stream, (uint32_t *)d->paths.metrics, 32));
 
@@ -1113,8 +1127,11 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     for (size_t s0 = 0; s0 < slots.size(); s0 += cap)
     {
         uint32_t nslots = (uint32_t)std::min<size_t>(cap, slots.size() - s0);
-        HIPCHK(hipMemcpyAsync(d->slotPixel, slots.data() + s0, 4 * (size_t)nslots,
-                              hipMemcpyHostToDevice, d->stream));
+        const bool whole = s0 == 0 && slots.size() <= cap;
+        if (!(whole && d->slotsOnDevice))
+            HIPCHK(hipMemcpyAsync(d->slotPixel, slots.data() + s0, 4 * (size_t)nslots,
+                                  hipMemcpyHostToDevice, d->stream));
+        d->slotsOnDevice = whole;
         HIPCHK(launch_clear_floats(d->stream, p.accum, 3 * (size_t)nslots));
         p.slots = nslots;
         uint32_t sppPass = pcg ? std::max(1u, std::min(spp, cap / nslots)) : 1u;

@@ -234,6 +234,12 @@ struct spg_Device
     size_t imageCount = 0;
     void *slotPixel = nullptr;   // pixel index per slot
     size_t slotPixelCount = 0;
+    // the slot list of the last trace call (pixels of its tiles in tile
+    // order) and whether slotPixel already holds all of it
+    std::vector<spg_Tile> slotTiles;
+    uint32_t slotW = 0, slotH = 0;
+    std::vector<uint32_t> slots;
+    bool slotsOnDevice = false;
     void *accum = nullptr;       // per-slot radiance sums
     size_t accumCount = 0;
     void *persistBlock = nullptr; // launch counters + traversal stack spill
