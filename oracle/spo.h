@@ -300,6 +300,10 @@ void spo_film_position(const spo_Camera *c, float px, float py, float *out3);
 int spo_scene_intersect(const spo_Scene *s, const float *o, const float *d,
                         int hitMode, int tx, spo_SceneHit *out,
                         uint64_t *metrics12);
+/* 1 if some object's closest mesh hit lies at a positive world t (the
+ * shadow test toward the sky), 0 if none, -1 on a broadphase overflow */
+int spo_scene_occluded(const spo_Scene *s, const float *o, const float *d, int hitMode,
+                       uint64_t *metrics12);
 float spo_mesh_intersect(const spo_Scene *s, uint32_t mesh, const float *o,
                          const float *d, int hitMode, float *out6,
                          uint32_t *prim, uint64_t *metrics12);

@@ -683,3 +683,47 @@ int spo_scene_intersect(const spo_Scene *s, const float *o, const float *d,
     }
     return 0;
 }
+
+/* Occlusion query of a shadow ray toward the sky: 1 if some object's
+ * closest mesh hit lies at a positive world t (the instance visit order
+ * does not matter), 0 if none, -1 on a broadphase overflow.  The GPU's
+ * occlusion walk stops at the first such instance
+ * (vk_cinematic_amd/csrc/spg_traverse.h, anyHit). */
+int spo_scene_occluded(const spo_Scene *s, const float *o, const float *d, int hitMode,
+                       uint64_t *metrics12)
+{
+    uint32_t objs[1024];
+    uint32_t n;
+    if (hitMode == SPO_HIT_BRUTE)
+    {
+        n = s->objectCount < 1024 ? s->objectCount : 1024;
+        for (uint32_t i = 0; i < n; ++i) objs[i] = i;
+    }
+    else
+    {
+        uint32_t err = 0, tests = 0;
+        uint32_t cap = s->maxBroadphaseHits < 1024 ? s->maxBroadphaseHits : 1024;
+        n = spo_bvh_intersect(&s->broadphase, o, d, cap, objs, &tests, &err);
+        if (err) return -1;
+    }
+    spo_v3 ro = v3p(o), rd = v3p(d);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const spo_Object *obj = &s->objects[objs[i]];
+        spo_v3 lo = m4point(obj->invModel, ro);
+        spo_v3 ld = v3norm(m4vector(obj->invModel, rd));
+        float lof[3], ldf[3];
+        v3put(lof, lo);
+        v3put(ldf, ld);
+        mesh_hit h;
+        if (mesh_query(s, &s->meshes[obj->mesh], lof, ldf, hitMode, &h, metrics12) < 0)
+            return -1;
+        if (metrics12) metrics12[11] += 1;
+        if (h.t >= 0.0f)
+        {
+            spo_v3 worldHit = m4point(obj->model, v3add(lo, v3mul(ld, h.t)));
+            if (v3dot(v3sub(worldHit, ro), rd) > 0.0f) return 1;
+        }
+    }
+    return 0;
+}

@@ -414,15 +414,25 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                             spo_v3 C = v3had(v3had(beta, v3mul(Xl, (cl * wmis) / pl)), Le);
                             float so[3], sd[3];
                             v3put(so, x); v3put(sd, wl);
-                            spo_SceneHit sh;
-                            if (spo_scene_intersect(s, so, sd, cfg->hit, tx, &sh,
-                                                    metrics) < 0)
-                            { *fail = 1; return L; }
+                            /* the sky is seen unless some object lies at a
+                             * positive distance along the ray; an emitter
+                             * triangle must be the closest hit */
+                            int visible;
+                            if (ti == 0xFFFFFFFFu)
+                            {
+                                int occ = spo_scene_occluded(s, so, sd, cfg->hit, metrics);
+                                if (occ < 0) { *fail = 1; return L; }
+                                visible = !occ;
+                            }
+                            else
+                            {
+                                spo_SceneHit sh;
+                                if (spo_scene_intersect(s, so, sd, cfg->hit, tx, &sh,
+                                                        metrics) < 0)
+                                { *fail = 1; return L; }
+                                visible = sh.t > 0.0f && sh.object == ti && sh.prim == tp;
+                            }
                             metrics[2]++;
-                            int hitAny = sh.t > 0.0f;
-                            int visible = ti == 0xFFFFFFFFu
-                                              ? !hitAny
-                                              : hitAny && sh.object == ti && sh.prim == tp;
                             if (visible) L = v3add(L, C);
                         }
                     }

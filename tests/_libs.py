@@ -3,8 +3,11 @@
 * ``oracle()``  -- oracle/liboracle.so, the C restatement of the reference
   tracer (the checker; see oracle/spo.h).
 * ``ref()`` / ``ref_test()`` -- oracle/_ref/libref*.so, the reference's own
-  sources compiled by oracle/Makefile.  Only present where the reference
-  checkout exists (never on the GPU box); tests that need them skip otherwise.
+  sources compiled by oracle/Makefile where the reference checkout exists.
+  The built files travel with the repository's snapshot like the other
+  built libraries, but only CPU tests (not marked ``gpu``) load them, and
+  only when they run (never at collection); tests that need them skip when
+  the build is absent.
 
 Nothing in the product imports this module.
 """
@@ -156,6 +159,7 @@ def oracle():
          C.POINTER(SpoSceneHit), u64p)
     _sig(L, "spo_mesh_intersect", C.c_float, vp, C.c_uint32, f32p, f32p, C.c_int,
          f32p, u32ref, u64p)
+    _sig(L, "spo_scene_occluded", C.c_int, vp, f32p, f32p, C.c_int, u64p)
     _sig(L, "spo_closest_hit_brute", None, f32p, C.c_uint32, f32p, C.c_uint32,
          C.c_uint32, u32p, f32p)
     _sig(L, "spo_closest_hit_brute_f64", None, f32p, C.c_uint32, f32p, C.c_uint32,

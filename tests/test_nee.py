@@ -280,3 +280,44 @@ def test_emitter_nee_agrees_with_unbiased_and_cuts_error():
     en = np.mean([((x - ref) ** 2).mean() for x in n])
     assert en < eu / 3, (en, eu)
     O.spo_scene_destroy(s)
+
+
+def test_sky_occlusion_query_is_any_positive_hit():
+    """The sky shadow test (spo_scene_occluded): occluded iff some object's
+    closest mesh hit lies at a positive world t, whatever the order in
+    which objects are visited -- the broadphase walk and the brute-force
+    object loop agree, a ray whose closest hit is at t > 0 is occluded, and
+    a ray with no hit is not.  Rays start on the ground next to the
+    sphere's contact point (the concave joint) and at random points."""
+    from _libs import SPO_HIT_REFERENCE, SpoSceneHit
+
+    desc = scenes.config1(16, 16, env="cube", env_size=8)
+    s, _ = load_oracle(desc)
+    O = oracle()
+    rng = np.random.default_rng(5)
+    n_occ = 0
+    for k in range(3000):
+        if k % 2:
+            r = rng.uniform(0.0, 0.3)
+            a = rng.uniform(0, 2 * math.pi)
+            o = f32(r * math.cos(a), 1e-4, r * math.sin(a))
+        else:
+            o = f32(*rng.uniform(-3, 3, 3))
+            o[1] = abs(o[1]) + 1e-4
+        d = rng.normal(size=3)
+        d[1] = abs(d[1])
+        d = f32(*(d / np.linalg.norm(d)))
+        met = np.zeros(12, np.uint64)
+        occ_ref = O.spo_scene_occluded(s, o, d, SPO_HIT_REFERENCE, met)
+        occ_brute = O.spo_scene_occluded(s, o, d, SPO_HIT_BRUTE, met)
+        assert occ_ref == occ_brute
+        h = SpoSceneHit()
+        assert O.spo_scene_intersect(s, o, d, SPO_HIT_BRUTE, SPO_TX_CR, C.byref(h),
+                                     met) == 0
+        if h.t > 0.0:
+            assert occ_ref == 1
+        if h.object == 0xFFFFFFFF:
+            assert occ_ref == 0
+        n_occ += occ_ref
+    assert 300 < n_occ < 2700
+    O.spo_scene_destroy(s)

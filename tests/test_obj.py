@@ -92,3 +92,31 @@ def test_reference_assets(name, tris):
     corners = np.array([c for f in faces for c in f])
     assert np.array_equal(v[i, 0:3], pos[corners[:, 0]])
     assert np.array_equal(v[i, 3:6], nrm[corners[:, 1]])
+
+
+def test_text_without_terminator_is_read_only_to_its_length():
+    """The text is (pointer, length): a buffer whose last line ends in a
+    digit with more digits right after it in memory must be read only up to
+    the length (the bytes after it would make index 37 and a vertex 0.55)."""
+    import ctypes as C
+
+    body = b"vn 0 0 1\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1//1 2//1 3//1"
+    buf = C.create_string_buffer(body + b"7 v 0.5 5", len(body) + 10)
+    nv, ni = C.c_uint32(), C.c_uint32()
+    V.check(V.lib().spg_obj_parse(buf, len(body), None, 0, None, 0, C.byref(nv),
+                                  C.byref(ni)))
+    assert (nv.value, ni.value) == (3, 3)
+    # a vertex line cut at the length: "v 0 0 0.5" is read as 0.5, not 0.55
+    body2 = b"vn 0 0 1\nv 0 0 0\nv 1 0 0\nv 0 1 0.5"
+    tail = b"5\nf 1//1 2//1 3//1\n"
+    buf2 = C.create_string_buffer(body2 + tail, len(body2) + len(tail))
+    # the cut text has no faces: rejected, but without reading the tail's face
+    assert V.lib().spg_obj_parse(buf2, len(body2), None, 0, None, 0, C.byref(nv),
+                                 C.byref(ni)) == -1  # SPG_E_INVALID
+    full = body2 + b"\nf 1//1 2//1 3//1"
+    buf3 = C.create_string_buffer(full + b"5", len(full) + 1)
+    v = np.zeros(24, np.float32)
+    i = np.zeros(3, np.uint32)
+    V.check(V.lib().spg_obj_parse(buf3, len(full), v.ctypes.data, 3, i.ctypes.data, 3,
+                                  C.byref(nv), C.byref(ni)))
+    assert v.reshape(3, 8)[2, 2] == np.float32(0.5)

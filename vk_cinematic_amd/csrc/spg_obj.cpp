@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -46,11 +47,17 @@ int parse(const char *text, size_t len, ObjMesh &m)
     int objects = 0;
     const char *p = text, *end = text + len;
     std::vector<uint32_t> poly;
+    // each line is parsed from a NUL-terminated copy, so strtof/strtol stop
+    // at the line's end even when the text is not NUL-terminated (e.g. an
+    // exactly sized mapping whose last line ends in a digit)
+    std::string line;
     while (p < end)
     {
-        const char *eol = (const char *)memchr(p, '\n', (size_t)(end - p));
-        if (!eol) eol = end;
-        const char *q = p;
+        const char *nl = (const char *)memchr(p, '\n', (size_t)(end - p));
+        if (!nl) nl = end;
+        line.assign(p, (size_t)(nl - p));
+        const char *q = line.c_str();
+        const char *eol = q + line.size();
         while (q < eol && is_space(*q)) ++q;
         if (q + 1 < eol && q[0] == 'o' && is_space(q[1]))
         {
@@ -134,7 +141,7 @@ int parse(const char *text, size_t len, ObjMesh &m)
                 m.idx.push_back(poly[k + 1]);
             }
         }
-        p = eol < end ? eol + 1 : end;
+        p = nl < end ? nl + 1 : end;
     }
     if (m.idx.empty()) return fail(SPG_E_INVALID, "spg_obj_parse: no faces");
     return SPG_OK;

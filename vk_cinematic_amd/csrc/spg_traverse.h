@@ -136,7 +136,8 @@ struct Trav
     uint32_t triNext, triEnd;  // pending triangles of the current leaf
     uint32_t mode;             // the next step's work (kMode*)
     bool mmW, mmL;             // world / mesh-space ray is finite: slab_hit_mm
-    bool anyHit;               // occlusion query: stop at the first instance hit
+    bool anyHit;               // occlusion query: stop at the first instance
+                               // hit with a positive world t
 };
 
 // Kinds of step.
@@ -375,8 +376,12 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
             v3 lh = s.lo + s.ld * s.h.t;
             v3 wh = xform_point(rows3(A, B, C), lh);
             tw = dot(wh - s.o, s.d);
-            better = tw < s.out.t || s.out.t < 0.0f ||
-                     (tw == s.out.t && s.inst < s.out.inst);
+            // an occlusion query takes the first instance hit at a positive
+            // world t (what k_resolve_shadow counts as a blocker) and stops;
+            // a hit whose world t rounds to <= 0 neither stops nor blocks
+            better = s.anyHit ? tw > 0.0f
+                              : tw < s.out.t || s.out.t < 0.0f ||
+                                    (tw == s.out.t && s.inst < s.out.inst);
         }
         stop = s.anyHit & better;
         s.out.t = better ? tw : s.out.t;
