@@ -15,8 +15,11 @@
  *     pixels are RGBA32F row-major x + y*width (src/simd_path_tracer.h:3-8).
  *   - Data passed in is copied to HBM (the reference borrows the caller's
  *     arrays, src/sp_scene.cpp:8-19).
- *   - Calls on one device are serialized by the library; handles may be
- *     shared between host threads.
+ *   - Handles may be shared between host threads.  Device work of one
+ *     device is serialized by the library, except that concurrent
+ *     spg_path_trace_tile / spg_path_trace_tiles calls with the same scene,
+ *     camera, options and host image (the reference's worker pool,
+ *     src/main.cpp:731-759) are traced together as one wavefront.
  */
 #ifndef SPGPU_H
 #define SPGPU_H
@@ -239,7 +242,15 @@ int spg_compute_tiles(uint32_t width, uint32_t height, uint32_t tileWidth,
 
 /* ---- tracing ---- */
 /* sp_PathTraceTile(ctx, tile, rng, metrics), src/simd_path_tracer.cpp:178:
- * overwrites the tile's pixels of the host image (width*height RGBA32F). */
+ * overwrites the tile's pixels of the host image (width*height RGBA32F).
+ * Safe to call from many threads at once on one context (the reference's
+ * WorkerThread, src/main.cpp:744): calls that arrive while a trace runs are
+ * traced together by the next one, with the same pixels as one
+ * spg_path_trace_tiles call over their tiles (per-pixel random streams).
+ * metrics: paths exactly; rays, hits, misses and test counts of a combined
+ * trace are shared out in proportion to each call's paths (the shares sum
+ * to the combined counts); elapsed time from the call's own start.  A
+ * guided call with training on is never combined. */
 int spg_path_trace_tile(const spg_Context *ctx, spg_Tile tile,
                         spg_Metrics *metrics, float *hostPixels);
 /* The work queue of src/main.cpp:819-844 in one call: all tiles are traced
@@ -398,6 +409,17 @@ int spg_guide_train(spg_Guide *g, const float *samples13, uint32_t n,
  * taken, mean loss of the last step. */
 int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps,
                     float *loss);
+/* Guide checkpoint (no reference counterpart; the survey's optimizer state
+ * that must be serializable): the configuration, the FP32 master
+ * parameters, both Adam moments and the step count, in a little-endian file
+ * ("SPGGUIDE", version 1).  Training steps are deterministic (fixed-order
+ * reductions), so a guide loaded from a checkpoint continues training
+ * bit-identically to the guide that wrote it. */
+int spg_guide_save(spg_Guide *g, const char *path);
+/* Restores a checkpoint into g; its configuration replaces g's.
+ * SPG_E_INVALID for a file that is not a version-1 checkpoint of this
+ * network. */
+int spg_guide_load(spg_Guide *g, const char *path);
 
 /* ---- after the path ---- */
 /* Progressive accumulation across trace calls (the reference's open TODO,

@@ -206,6 +206,8 @@ _SIGS = {
     "spg_guide_train": (_int, [_vp, _f32p, _u32, _u32, C.POINTER(C.c_float)]),
     "spg_guide_stats": (_int, [_vp, C.POINTER(_u32), C.POINTER(_u32),
                                C.POINTER(C.c_float)]),
+    "spg_guide_save": (_int, [_vp, C.c_char_p]),
+    "spg_guide_load": (_int, [_vp, C.c_char_p]),
 }
 
 _lib = None
@@ -648,6 +650,14 @@ class Guide:
         check(lib().spg_guide_train(self.handle, x, x.size // GUIDE_SAMPLE_FLOATS,
                                     steps, C.byref(loss)))
         return loss.value
+
+    def save(self, path):
+        """Checkpoint: configuration, parameters, Adam moments and steps."""
+        check(lib().spg_guide_save(self.handle, os.fsencode(path)))
+
+    def load(self, path):
+        """Restores a checkpoint written by save (its configuration too)."""
+        check(lib().spg_guide_load(self.handle, os.fsencode(path)))
 
     def stats(self):
         c, st, loss = C.c_uint32(), C.c_uint32(), C.c_float()

@@ -1037,6 +1037,8 @@ struct LaunchTimer
     }
 };
 
+// devImage == nullptr: render into the device's staging image (sized here,
+// under the device lock) and copy the tiles out to hostImage.
 static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
                  spg_Metrics *metrics, float *devImage, float *hostImage)
 {
@@ -1062,10 +1064,18 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
         return fail(SPG_E_INVALID, "trace: unknown transcendentals mode");
     if (c.width == 0 || c.height == 0)
         return fail(SPG_E_INVALID, "trace: camera not configured");
+    if (!devImage && !hostImage)
+        return fail(SPG_E_INVALID, "trace: no output image");
     auto t0 = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> g(d->lock);
     HIPCHK(hipSetDevice(d->ordinal));
     if (s->dirty) CHK(upload_tables(s));
+    if (!devImage)
+    {
+        CHK(ensure_buffer<float>(d->image, d->imageCount,
+                                 (size_t)c.width * c.height * 4));
+        devImage = (float *)d->image;
+    }
 
     // pixel slots in tile order (clamped to the image as
     // sp_PathTraceTile clamps, src/simd_path_tracer.cpp:186-189); kept from
@@ -1193,6 +1203,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
             if (train)
             {
                 tm.begin(kGuide);
+                HIPCHK(launch_guide_collect(d->stream, p));
                 CHK(guide_train_pass(cfg.guide, d->stream));
                 tm.end();
                 launches[kGuide]++;
@@ -1249,19 +1260,141 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     return SPG_OK;
 }
 
+// Whether two host-pixel requests can be traced as one wavefront: the
+// same scene, camera, options and host image.  A guided call that trains
+// changes the guide, so it always runs alone.
+static bool combinable(const TileRequest &a, const TileRequest &b)
+{
+    const spg_TraceConfig &ca = a.ctx->config, &cb = b.ctx->config;
+    if (ca.estimator == SPG_EST_GUIDED && (ca.guideTrain || cb.guideTrain)) return false;
+    return a.ctx->scene == b.ctx->scene && a.hostPixels == b.hostPixels &&
+           memcmp(a.ctx->camera, b.ctx->camera, sizeof(spg_Camera)) == 0 &&
+           memcmp(&ca, &cb, sizeof(spg_TraceConfig)) == 0;
+}
+
+// Traces a group of combinable requests in one call and hands each its
+// share of the counters: paths exactly (its pixels x spp), rays, hits,
+// misses and test counts in proportion to its paths (the shares add up to
+// the group's totals), elapsed time from its own start.
+static void trace_group(spg_Device *d, std::vector<TileRequest *> &group)
+{
+    std::vector<spg_Tile> tiles;
+    for (TileRequest *r : group) tiles.insert(tiles.end(), r->tiles, r->tiles + r->count);
+    spg_Metrics m{};
+    const TileRequest &lead = *group[0];
+    int st = trace(lead.ctx, tiles.data(), (uint32_t)tiles.size(), &m, nullptr,
+                   lead.hostPixels);
+    std::string err = st ? std::string(g_err) : std::string();
+    auto t1 = std::chrono::steady_clock::now();
+    const spg_Camera &c = *lead.ctx->camera;
+    const uint64_t spp = lead.ctx->config.samplesPerPixel;
+    std::vector<uint64_t> paths(group.size());
+    uint64_t allPaths = 0;
+    for (size_t k = 0; k < group.size(); ++k)
+    {
+        uint64_t px = 0;
+        for (uint32_t j = 0; j < group[k]->count; ++j)
+        {
+            const spg_Tile &t = group[k]->tiles[j];
+            uint64_t x1 = std::min(t.maxX, c.width), y1 = std::min(t.maxY, c.height);
+            if (x1 > t.minX && y1 > t.minY) px += (x1 - t.minX) * (y1 - t.minY);
+        }
+        paths[k] = px * spp;
+        allPaths += paths[k];
+    }
+    const int shared[] = {SPG_METRIC_RAYS_TRACED, SPG_METRIC_RAY_HIT_COUNT,
+                          SPG_METRIC_RAY_MISS_COUNT, SPG_METRIC_MIDPHASE_AABB_TESTS,
+                          SPG_METRIC_MESH_TESTS};
+    uint64_t given[5] = {0, 0, 0, 0, 0};
+    uint64_t before = 0;
+    std::lock_guard<std::mutex> g(d->qlock);
+    for (size_t k = 0; k < group.size(); ++k)
+    {
+        TileRequest *r = group[k];
+        if (!st && r->metrics)
+        {
+            spg_Metrics &o = *r->metrics;
+            o.values[SPG_METRIC_ELAPSED_NS] += (uint64_t)
+                std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - r->start).count();
+            o.values[SPG_METRIC_PATHS_TRACED] += paths[k];
+            before += paths[k];
+            for (int j = 0; j < 5; ++j)
+            {
+                // cumulative share up to this request, so the shares sum
+                // exactly to the group's count
+                uint64_t tot = m.values[shared[j]];
+                uint64_t upto = k + 1 == group.size() || allPaths == 0
+                                    ? tot
+                                    : (uint64_t)((unsigned __int128)tot * before / allPaths);
+                o.values[shared[j]] += upto - given[j];
+                given[j] = upto;
+            }
+        }
+        r->status = st;
+        r->error = err;
+        r->done = true;
+    }
+}
+
+// Host-pixel trace through the device's request queue: the caller that
+// finds no trace running takes every queued request combinable with the
+// oldest one, traces them as one wavefront and wakes their callers; the
+// others wait.  Concurrent callers on one context (the reference's worker
+// threads, src/main.cpp:731-759) therefore share launch sequences instead
+// of queueing whole sequences behind the device lock.
 static int host_trace(const spg_Context *ctx, const spg_Tile *tiles,
                       uint32_t count, spg_Metrics *metrics, float *hostPixels)
 {
-    if (!ctx || !ctx->camera || !ctx->scene || !hostPixels)
+    if (!ctx || !ctx->camera || !ctx->scene || !hostPixels || (!tiles && count))
         return fail(SPG_E_INVALID, "trace: NULL argument");
     spg_Device *d = ctx->scene->dev;
-    size_t n = (size_t)ctx->camera->width * ctx->camera->height * 4;
+    TileRequest req{ctx, tiles, count, metrics, hostPixels,
+                    std::chrono::steady_clock::now()};
+    std::unique_lock<std::mutex> q(d->qlock);
+    d->queue.push_back(&req);
+    while (!req.done)
     {
-        std::lock_guard<std::mutex> g(d->lock);
-        HIPCHK(hipSetDevice(d->ordinal));
-        CHK(ensure_buffer<float>(d->image, d->imageCount, n));
+        if (d->leaderActive)
+        {
+            d->qcv.wait(q);
+            continue;
+        }
+        d->leaderActive = true;
+        std::vector<TileRequest *> group;
+        TileRequest *first = d->queue.front();
+        for (auto it = d->queue.begin(); it != d->queue.end();)
+        {
+            if (*it == first || combinable(*first, **it))
+            {
+                group.push_back(*it);
+                it = d->queue.erase(it);
+            }
+            else
+                ++it;
+        }
+        q.unlock();
+        try
+        {
+            trace_group(d, group);
+        }
+        catch (...)
+        {
+            // host allocation failure: fail the group, keep the queue alive
+            std::lock_guard<std::mutex> g(d->qlock);
+            for (TileRequest *r : group)
+                if (!r->done)
+                {
+                    r->status = SPG_E_CAPACITY;
+                    r->error = "trace: host allocation failed";
+                    r->done = true;
+                }
+        }
+        q.lock();
+        d->leaderActive = false;
+        d->qcv.notify_all();
     }
-    return trace(ctx, tiles, count, metrics, (float *)d->image, hostPixels);
+    if (req.status) g_err = req.error;
+    return req.status;
 }
 
 }  // namespace spg

@@ -28,6 +28,9 @@ constexpr int kGParams = 4 * (64 * 64 + 64) + 48 * 64 + 48;
 constexpr int kGLobeFloats = 40;                 // pi[8] kappa[8] mu[8][3]
 constexpr int kGSampleFloats = 13;               // pn n v wi weight
 constexpr int kGVertexFloats = 12;               // pn n v wi
+constexpr int kGradChunk = 512;                  // samples per partial weight gradient
+constexpr int kGScanSeg = 4096;                  // paths per segment of the sample scan
+constexpr int kGScanBlocks = 1024;               // segments (paths per pass <= 2^22)
 constexpr float kLogKappa0 = -2.0f;
 constexpr float kLogKappa1 = 8.0f;
 
@@ -45,6 +48,13 @@ struct DevGuide
     float *samples;        // [kGSampleFloats][sampleCap]
     uint32_t *sampleCount;
     uint32_t sampleCap;
+    // deterministic collection: per pass, path p's sample count goes to
+    // gcnt[v] with v = p * permInv mod paths (a bijection spreading
+    // consecutive v over the image: p = v * permMul mod paths), and the
+    // samples are stored in v order after an exclusive scan of gcnt
+    uint32_t *gcnt;        // [capacity]
+    uint32_t *scanBase;    // [kGScanBlocks] per 4096-entry segment: sum, then base
+    uint32_t permMul, permInv;
     uint32_t selectBelow;  // a path feeds training if hash(path, salt) < this
     uint32_t salt;
     uint32_t train;        // record vertices and append training samples

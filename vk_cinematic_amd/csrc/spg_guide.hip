@@ -11,7 +11,9 @@
 //                   layer's input H_l and output gradient G_l (FP16,
 //                   feature-major [64][B])
 //   k_guide_wgrad   dW_l = G_l H_l^T and db_l = G_l 1 as MFMA products with
-//                   the samples on K, accumulated into the FP32 gradient
+//                   the samples on K, one FP32 partial gradient per chunk
+//   k_guide_greduce the partials added in chunk order (the step's result
+//                   does not depend on scheduling)
 //   k_guide_adam    Adam on the FP32 master parameters, refreshes the FP16
 //                   forward and transposed copies, clears the gradient
 // Gradients carry a fixed loss scale (kLossScale) while in FP16 and are
@@ -23,17 +25,11 @@ namespace spg {
 
 constexpr int kEvalWaves = 4;
 constexpr int kTrainWaves = 4;
-constexpr int kWgradChunk = 512;   // samples per k_guide_wgrad block
 constexpr float kFp16Max = 60000.0f;
 
 __device__ __forceinline__ float clamp16(float x)
 {
     return fminf(fmaxf(x, -kFp16Max), kFp16Max);
-}
-
-__global__ void k_clear_f32_guide(float *p, uint32_t n)
-{
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0f;
 }
 
 // Parameter k (flat layout) into the FP16 forward copy, the FP16 transposed
@@ -125,14 +121,24 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_guide_eval(
 }
 
 // ------------------------------------------------------------ training
+// Sum of the batch's weights in a fixed order (one block: each thread sums
+// a fixed stride of the batch, then a fixed tree), so a training step's
+// result does not depend on scheduling.
 __global__ __launch_bounds__(256) void k_guide_wsum(GuideTrainArgs a)
 {
+    __shared__ float part[256];
     const uint32_t n = batch_count(a);
     float s = 0.0f;
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256)
+    for (uint32_t j = threadIdx.x; j < n; j += 256)
         s += a.samples[(size_t)12 * a.sampleCap + a.first + j];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0 && s != 0.0f) atomicAdd(a.wsum, s);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1)
+    {
+        if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *a.wsum = part[0];
 }
 
 // d(-w log q)/d(outputs) of one sample (o[k] at stride 64), FP32; returns
@@ -298,8 +304,10 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_fwdbwd(GuideTrainArg
     if (lane == 0 && lossSum != 0.0f) atomicAdd(a.loss, lossSum);
 }
 
-// dW_l = G_l H_l^T, db_l = G_l 1 over one chunk of the batch; block = one
-// wave, blockIdx.y = layer, kMb = output rows / 16.
+// dW_l = G_l H_l^T, db_l = G_l 1 over one chunk of the batch, stored as
+// that chunk's partial gradient (every element of the layer written once);
+// k_guide_greduce adds the chunks in order.  Block = one wave, blockIdx.y =
+// layer, kMb = output rows / 16.
 template <int kMb>
 __device__ __forceinline__ void wgrad_chunk(const GuideTrainArgs &a, int l,
                                             uint32_t k0, uint32_t k1)
@@ -329,17 +337,18 @@ __device__ __forceinline__ void wgrad_chunk(const GuideTrainArgs &a, int l,
             for (int i = 0; i < 8; ++i) db[mb] += (float)av[mb][i];
         }
     }
-    float *dW = a.grad + g_w_off(l);
-    float *dB = a.grad + g_b_off(l);
+    float *part = a.part + (size_t)(k0 / kGradChunk) * kGParams;
+    float *dW = part + g_w_off(l);
+    float *dB = part + g_b_off(l);
     for (int mb = 0; mb < kMb; ++mb)
     {
         float v = db[mb];
         v += __shfl_xor(v, 16);
         v += __shfl_xor(v, 32);
-        if (g == 0) atomicAdd(dB + mb * 16 + r, v);
+        if (g == 0) dB[mb * 16 + r] = v;
         for (int nb = 0; nb < 4; ++nb)
             for (int i = 0; i < 4; ++i)
-                atomicAdd(dW + (mb * 16 + 4 * g + i) * 64 + nb * 16 + r, acc[mb][nb][i]);
+                dW[(mb * 16 + 4 * g + i) * 64 + nb * 16 + r] = acc[mb][nb][i];
     }
 }
 
@@ -347,12 +356,24 @@ __global__ __launch_bounds__(64) void k_guide_wgrad(GuideTrainArgs a)
 {
     const uint32_t n = batch_count(a);
     const uint32_t nPad = (n + 63) / 64 * 64;
-    const uint32_t k0 = blockIdx.x * kWgradChunk;
+    const uint32_t k0 = blockIdx.x * kGradChunk;
     if (k0 >= nPad) return;
-    const uint32_t k1 = min(k0 + (uint32_t)kWgradChunk, nPad);
+    const uint32_t k1 = min(k0 + (uint32_t)kGradChunk, nPad);
     const int l = blockIdx.y;
     if (l < 4) wgrad_chunk<4>(a, l, k0, k1);
     else wgrad_chunk<3>(a, l, k0, k1);
+}
+
+// The gradient: the chunks' partial gradients added in chunk order.
+__global__ __launch_bounds__(256) void k_guide_greduce(GuideTrainArgs a)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= (uint32_t)kGParams) return;
+    const uint32_t nPad = (batch_count(a) + 63) / 64 * 64;
+    const uint32_t chunks = (nPad + kGradChunk - 1) / kGradChunk;
+    float g = 0.0f;
+    for (uint32_t c = 0; c < chunks; ++c) g += a.part[(size_t)c * kGParams + k];
+    a.grad[k] = g;
 }
 
 // Adam on the master parameters, then the FP16 forward / transposed copies
@@ -409,17 +430,15 @@ hipError_t launch_guide_convert(hipStream_t s, const GuideTrainArgs &a)
 hipError_t launch_guide_gradient(hipStream_t s, const GuideTrainArgs &a,
                                  uint32_t cus)
 {
-    if (a.wsum)
-    {
-        hipLaunchKernelGGL(k_clear_f32_guide, dim3(1), dim3(64), 0, s, a.wsum, 1);
-        hipLaunchKernelGGL(k_guide_wsum, dim3(64), dim3(256), 0, s, a);
-    }
+    if (a.wsum) hipLaunchKernelGGL(k_guide_wsum, dim3(1), dim3(256), 0, s, a);
     uint32_t tiles = (a.batch + 63) / 64;
     uint32_t blocks = (tiles + kTrainWaves - 1) / kTrainWaves;
     if (blocks > cus) blocks = cus;
     hipLaunchKernelGGL(k_guide_fwdbwd, dim3(blocks), dim3(64 * kTrainWaves), 0, s, a);
-    uint32_t chunks = (a.batch + kWgradChunk - 1) / kWgradChunk;
+    uint32_t chunks = (a.batch + kGradChunk - 1) / kGradChunk;
     hipLaunchKernelGGL(k_guide_wgrad, dim3(chunks, kGLayers), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_guide_greduce, dim3((kGParams + 255) / 256), dim3(256), 0, s,
+                       a);
     return hipGetLastError();
 }
 

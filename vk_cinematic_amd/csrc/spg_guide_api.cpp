@@ -3,6 +3,7 @@
 // steps after each pass).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -86,6 +87,7 @@ GuideTrainArgs make_args(spg_Guide *g, uint32_t first, uint32_t batch,
     a.wmax = wmax;
     a.H = g->H;
     a.G = g->G;
+    a.part = g->part;
     a.loss = g->loss;
     return a;
 }
@@ -107,9 +109,12 @@ int ensure_hg(spg_Guide *g, uint32_t batch)
     free_dev(g->hgBlock);
     g->batchCap = 0;
     size_t half = (size_t)kGLayers * 64 * batch;
-    HIPCHK(hipMalloc(&g->hgBlock, 2 * half * sizeof(_Float16)));
+    size_t hgBytes = (2 * half * sizeof(_Float16) + 255) / 256 * 256;
+    size_t chunks = (batch + kGradChunk - 1) / kGradChunk;
+    HIPCHK(hipMalloc(&g->hgBlock, hgBytes + chunks * kGParams * sizeof(float)));
     g->H = (_Float16 *)g->hgBlock;
     g->G = g->H + half;
+    g->part = (float *)((char *)g->hgBlock + hgBytes);
     g->batchCap = batch;
     return SPG_OK;
 }
@@ -179,9 +184,13 @@ int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
         size_t off = 0;
         size_t oL = take(off, (size_t)kGLobeFloats * capacity * 4);
         size_t oV = take(off, (size_t)kGVertexFloats * bounces * capacity * 4);
+        size_t oC = take(off, (size_t)capacity * 4);
+        size_t oS = take(off, (size_t)kGScanBlocks * 4);
         HIPCHK(hipMalloc(&g->pathBlock, off));
         g->lobes = (float *)((char *)g->pathBlock + oL);
         g->gvtx = (float *)((char *)g->pathBlock + oV);
+        g->gcnt = (uint32_t *)((char *)g->pathBlock + oC);
+        g->scanBase = (uint32_t *)((char *)g->pathBlock + oS);
         g->pathCap = capacity;
         g->pathBounces = bounces;
     }
@@ -203,6 +212,9 @@ int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
     dg.bias = g->bias;
     dg.lobes = g->lobes;
     dg.gvtx = g->gvtx;
+    dg.gcnt = g->gcnt;
+    dg.scanBase = g->scanBase;
+    dg.permMul = dg.permInv = 1;
     dg.samples = g->samples;
     dg.sampleCount = g->count;
     dg.sampleCap = g->sampleCap;
@@ -223,6 +235,33 @@ int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
     return SPG_OK;
 }
 
+// A multiplier m coprime to n near n / golden ratio and its inverse mod n:
+// v -> v * m mod n visits 0..n-1 once, consecutive v far apart (a Weyl
+// sequence), so the training samples stored in v order are spread over the
+// image in every prefix.
+static void weyl_permutation(uint32_t n, uint32_t &mul, uint32_t &inv)
+{
+    mul = inv = n > 1 ? 1u : 0u;
+    if (n < 3) return;
+    auto gcd = [](uint64_t a, uint64_t b) {
+        while (b) { uint64_t t = a % b; a = b; b = t; }
+        return a;
+    };
+    uint64_t m = (uint64_t)((double)n * 0.6180339887498949) | 1u;
+    while (gcd(m % n, n) != 1) m += 2;
+    m %= n;
+    // inverse by the extended Euclidean algorithm
+    int64_t r0 = (int64_t)n, r1 = (int64_t)m, t0 = 0, t1 = 1;
+    while (r1)
+    {
+        int64_t q = r0 / r1, r2 = r0 - q * r1, t2 = t0 - q * t1;
+        r0 = r1; r1 = r2; t0 = t1; t1 = t2;
+    }
+    if (t0 < 0) t0 += (int64_t)n;
+    mul = (uint32_t)m;
+    inv = (uint32_t)t0;
+}
+
 // Each pass feeds training from a hashed subset of its paths sized to
 // about batchSize * stepsPerPass paths (a path yields a sample per
 // scattering vertex that saw light).
@@ -230,6 +269,7 @@ int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt,
                      uint32_t paths)
 {
     dg.salt = salt;
+    weyl_permutation(paths, dg.permMul, dg.permInv);
     double want = (double)g->cfg.batchSize * g->cfg.stepsPerPass;
     double psel = paths ? std::min(1.0, want / (double)paths) : 1.0;
     dg.selectBelow = (uint32_t)std::min(4294967295.0, psel * 4294967296.0);
@@ -470,6 +510,75 @@ int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps, float *l
         uint32_t n = have > first ? std::min(have - first, g->cfg.batchSize) : 0;
         *loss = n ? l / (float)n : 0.0f;
     }
+    return SPG_OK;
+}
+
+// Checkpoint layout: magic, version, parameter count, Adam steps, the
+// configuration, then params, m, v (FP32, kGParams each).
+static const char kGuideMagic[8] = {'S', 'P', 'G', 'G', 'U', 'I', 'D', 'E'};
+constexpr uint32_t kGuideVersion = 1;
+
+int spg_guide_save(spg_Guide *g, const char *path)
+{
+    if (!g || !path) return fail(SPG_E_INVALID, "spg_guide_save: NULL");
+    std::vector<float> st((size_t)3 * kGParams);
+    uint32_t steps;
+    spg_GuideConfig cfg;
+    {
+        std::lock_guard<std::mutex> lk(g->dev->lock);
+        HIPCHK(hipSetDevice(g->dev->ordinal));
+        const float *src[3] = {g->params, g->m, g->v};
+        for (int k = 0; k < 3; ++k)
+            HIPCHK(hipMemcpyAsync(st.data() + (size_t)k * kGParams, src[k],
+                                  sizeof(float) * kGParams, hipMemcpyDeviceToHost,
+                                  g->dev->stream));
+        HIPCHK(hipStreamSynchronize(g->dev->stream));
+        steps = g->steps;
+        cfg = g->cfg;
+    }
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(SPG_E_INVALID, std::string("spg_guide_save: cannot open ") + path);
+    const uint32_t head[3] = {kGuideVersion, (uint32_t)kGParams, steps};
+    bool ok = fwrite(kGuideMagic, 8, 1, f) == 1 && fwrite(head, sizeof head, 1, f) == 1 &&
+              fwrite(&cfg, sizeof cfg, 1, f) == 1 &&
+              fwrite(st.data(), sizeof(float), st.size(), f) == st.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? SPG_OK : fail(SPG_E_INVALID, std::string("spg_guide_save: write failed: ") + path);
+}
+
+int spg_guide_load(spg_Guide *g, const char *path)
+{
+    if (!g || !path) return fail(SPG_E_INVALID, "spg_guide_load: NULL");
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(SPG_E_INVALID, std::string("spg_guide_load: cannot open ") + path);
+    char magic[8];
+    uint32_t head[3];
+    spg_GuideConfig cfg;
+    std::vector<float> st((size_t)3 * kGParams);
+    bool ok = fread(magic, 8, 1, f) == 1 && fread(head, sizeof head, 1, f) == 1 &&
+              fread(&cfg, sizeof cfg, 1, f) == 1 &&
+              memcmp(magic, kGuideMagic, 8) == 0 && head[0] == kGuideVersion &&
+              head[1] == (uint32_t)kGParams &&
+              fread(st.data(), sizeof(float), st.size(), f) == st.size();
+    char extra;
+    ok = ok && fread(&extra, 1, 1, f) == 0;  // nothing after the moments
+    fclose(f);
+    if (!ok) return fail(SPG_E_INVALID, std::string("spg_guide_load: not a guide checkpoint: ") + path);
+    if (cfg.batchSize == 0 || cfg.batchSize % 64 || cfg.batchSize > (1u << 20) ||
+        !(cfg.bsdfFraction >= 0.0f && cfg.bsdfFraction <= 1.0f) || !(cfg.learningRate >= 0.0f))
+        return fail(SPG_E_INVALID, "spg_guide_load: checkpoint configuration out of range");
+    spg_Device *d = g->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    float *dst[3] = {g->params, g->m, g->v};
+    for (int k = 0; k < 3; ++k)
+        HIPCHK(hipMemcpyAsync(dst[k], st.data() + (size_t)k * kGParams,
+                              sizeof(float) * kGParams, hipMemcpyHostToDevice, d->stream));
+    HIPCHK(launch_clear_floats(d->stream, g->grad, kGParams));
+    HIPCHK(launch_guide_convert(d->stream, make_args(g, 0, 64, false, 0.0f)));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    g->cfg = cfg;
+    g->steps = head[2];
     return SPG_OK;
 }
 

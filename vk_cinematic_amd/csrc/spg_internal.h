@@ -3,9 +3,13 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <chrono>
 #include <stddef.h>
 #include <stdint.h>
 
+#include <condition_variable>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -203,6 +207,7 @@ struct GuideTrainArgs
     float *wsum;                 // batch weight sum (normalize) or nullptr
     float wmax;                  // clamp of normalized weights, 0 = off
     _Float16 *H, *G;             // [5][64][batch] layer inputs / out grads
+    float *part;                 // [batch / 512][kGParams] partial gradients
     float *loss;                 // += sum_j w_j (-log q_j)
 };
 
@@ -220,9 +225,32 @@ struct GuideAdam
 // events before returning, under the device lock).
 constexpr int kEvents = 256;
 
+// One caller's host-pixel trace waiting to be combined with other callers'
+// (spg_path_trace_tile / spg_path_trace_tiles): compatible requests -- the
+// same scene, camera, trace options and host image -- are traced as one
+// wavefront by whichever caller finds the device idle.
+struct TileRequest
+{
+    const spg_Context *ctx;
+    const spg_Tile *tiles;
+    uint32_t count;
+    spg_Metrics *metrics;
+    float *hostPixels;
+    std::chrono::steady_clock::time_point start;
+    bool done = false;
+    int status = 0;
+    std::string error;
+};
+
 struct spg_Device
 {
     int ordinal = 0;
+    // host-pixel trace requests of concurrent callers (guarded by qlock,
+    // never held while tracing)
+    std::mutex qlock;
+    std::condition_variable qcv;
+    std::deque<TileRequest *> queue;
+    bool leaderActive = false;
     int liveScenes = 0;          // guarded by lock
     bool destroyRequested = false;
     hipStream_t stream = nullptr;
@@ -268,10 +296,12 @@ struct spg_Guide
     // per-layer activations / gradients of a batch, [5][64][batchCap] each
     void *hgBlock = nullptr;
     _Float16 *H = nullptr, *G = nullptr;
+    float *part = nullptr;            // per-chunk partial gradients
     uint32_t batchCap = 0;
     // per-path buffers of the tracer (lobes per queued ray, vertex records)
     void *pathBlock = nullptr;
     float *lobes = nullptr, *gvtx = nullptr;
+    uint32_t *gcnt = nullptr, *scanBase = nullptr;  // training-sample scan
     uint32_t pathCap = 0, pathBounces = 0;
     uint32_t steps = 0;               // Adam steps taken
 };
@@ -376,6 +406,8 @@ hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
 hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q);
 hipError_t launch_resolve_shadow(hipStream_t s, const TraceParams &p);
 hipError_t launch_finalize(hipStream_t s, const TraceParams &p);
+// guided training: the pass's samples in a fixed order (after finalize)
+hipError_t launch_guide_collect(hipStream_t s, const TraceParams &p);
 hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,
                                float *image);
 hipError_t launch_clear_counters(hipStream_t s, uint32_t *counters, int n);

@@ -1054,6 +1054,36 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(TraceParams p)
     }
 }
 
+// The guided estimator's reverse recurrence of one path (cos/pdf in place
+// of the cosine, no clamp): returns its radiance and, when the path is
+// selected for training, calls sample(j, w) for each scattering vertex j
+// (in reverse order) whose training weight w = Y(f cos Li / pdf) = Y(T_j)
+// is positive.
+template <class F>
+__device__ __forceinline__ v3 guided_recurrence(const TraceParams &p, uint32_t path,
+                                                F sample)
+{
+    const uint32_t cap = p.paths.capacity;
+    const int32_t vc = (int32_t)p.paths.vcount[path];
+    const bool sel = p.guide.train && guide_hash(path ^ p.guide.salt) < p.guide.selectBelow;
+    v3 radiance{0.0f, 0.0f, 0.0f};
+    for (int32_t j = vc - 1; j >= 0; --j)
+    {
+        const float *vt = p.paths.vtx + (size_t)j * 7 * cap + path;
+        v3 E{vt[0], vt[cap], vt[2 * cap]};
+        v3 X{vt[3 * cap], vt[4 * cap], vt[5 * cap]};
+        float c = vt[6 * cap];
+        v3 T = had(X, radiance) * c;
+        if (sel && c > 0.0f)
+        {
+            float w = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+            if (w > 0.0f) sample(j, w);
+        }
+        radiance = E + T;
+    }
+    return radiance;
+}
+
 // Per pixel slot: the radiance of each of its paths in sample order
 // (parity: ComputeRadianceForPath's reverse recurrence with the clamp on the
 // incoming radiance, src/simd_path_tracer.cpp:113-172), then
@@ -1093,39 +1123,13 @@ __global__ __launch_bounds__(256) void k_finalize(TraceParams p)
             else
             {
                 // guided: the same recurrence with cos/pdf in place of the
-                // cosine and no clamp; each scattering vertex j yields a
-                // training sample with weight Y(f cos Li / pdf) = Y(T_j)
-                int32_t vc = (int32_t)p.paths.vcount[path];
-                const bool sel = p.guide.train &&
-                                 guide_hash(path ^ p.guide.salt) < p.guide.selectBelow;
-                for (int32_t j = vc - 1; j >= 0; --j)
-                {
-                    const float *vt = p.paths.vtx + (size_t)j * 7 * cap + path;
-                    v3 E{vt[0], vt[cap], vt[2 * cap]};
-                    v3 X{vt[3 * cap], vt[4 * cap], vt[5 * cap]};
-                    float c = vt[6 * cap];
-                    v3 T = had(X, radiance) * c;
-                    if (sel && c > 0.0f)
-                    {
-                        float w = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
-                        if (w > 0.0f)
-                        {
-                            uint32_t at = atomicAdd(p.guide.sampleCount, 1u);
-                            if (at < p.guide.sampleCap)
-                            {
-                                const size_t stride = (size_t)p.bounces * cap;
-                                const float *gv =
-                                    p.guide.gvtx + (size_t)j * cap + path;
-                                float *dst = p.guide.samples + at;
-                                const size_t C = p.guide.sampleCap;
-                                for (int k = 0; k < kGVertexFloats; ++k)
-                                    dst[k * C] = gv[k * stride];
-                                dst[12 * C] = w;
-                            }
-                        }
-                    }
-                    radiance = E + T;
-                }
+                // cosine and no clamp; a selected path's training samples
+                // are counted here and written by k_guide_emit
+                uint32_t emitted = 0;
+                radiance = guided_recurrence(p, path, [&](int32_t, float) { emitted++; });
+                if (p.guide.train)
+                    p.guide.gcnt[(uint32_t)((uint64_t)path * p.guide.permInv %
+                                            ((uint64_t)p.slots * p.sppPass))] = emitted;
             }
             total = total + radiance * invSpp;
             lastXs = p.paths.rngXs[path];
@@ -1134,6 +1138,98 @@ __global__ __launch_bounds__(256) void k_finalize(TraceParams p)
         p.accum[p.slots + slot] = total.y;
         p.accum[2 * p.slots + slot] = total.z;
         if (p.rngMode != SPG_RNG_PCG) p.paths.slotRng[slot] = lastXs;
+    }
+}
+
+// Training samples of a pass in a fixed order.  gcnt[v] holds the count of
+// the path p = v * permMul mod paths (k_finalize); the samples are stored
+// in v order: k_gscan_sums adds each 4096-entry segment, k_gscan_top turns
+// the sums into segment bases (and the total into the sample count),
+// k_guide_emit scans its segment and writes each path's samples at its
+// offset (those past sampleCap are dropped).  Consecutive v are spread over
+// the image, so every training batch -- a contiguous range -- covers all
+// of it, and the result does not depend on scheduling.
+__device__ __forceinline__ uint32_t block_exclusive_scan_1024(uint32_t x, uint32_t *sh,
+                                                              uint32_t &total)
+{
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = x;
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        uint32_t y = __shfl_up(inc, o);
+        if ((int)lane >= o) inc += y;
+    }
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    if (wave == 0)
+    {
+        uint32_t w = lane < 16 ? sh[lane] : 0u, wi = w;
+        for (int o = 1; o < 16; o <<= 1)
+        {
+            uint32_t y = __shfl_up(wi, o);
+            if ((int)lane >= o) wi += y;
+        }
+        if (lane < 16) sh[16 + lane] = wi - w;
+        if (lane == 15) sh[32] = wi;
+    }
+    __syncthreads();
+    total = sh[32];
+    return sh[16 + wave] + inc - x;
+}
+
+__global__ __launch_bounds__(1024) void k_gscan_sums(TraceParams p)
+{
+    __shared__ uint32_t sh[40];
+    const uint32_t N = p.slots * p.sppPass;
+    const uint32_t v0 = blockIdx.x * kGScanSeg + threadIdx.x * 4;
+    uint32_t c = 0;
+    for (int k = 0; k < 4; ++k)
+        if (v0 + k < N) c += p.guide.gcnt[v0 + k];
+    uint32_t total;
+    block_exclusive_scan_1024(c, sh, total);
+    if (threadIdx.x == 0) p.guide.scanBase[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_gscan_top(TraceParams p, uint32_t segs)
+{
+    __shared__ uint32_t sh[40];
+    const uint32_t x = threadIdx.x < segs ? p.guide.scanBase[threadIdx.x] : 0u;
+    uint32_t total;
+    const uint32_t e = block_exclusive_scan_1024(x, sh, total);
+    if (threadIdx.x < segs) p.guide.scanBase[threadIdx.x] = e;
+    if (threadIdx.x == 0) *p.guide.sampleCount = total;
+}
+
+__global__ __launch_bounds__(1024) void k_guide_emit(TraceParams p)
+{
+    __shared__ uint32_t sh[40];
+    const uint64_t N = (uint64_t)p.slots * p.sppPass;
+    const uint32_t v0 = blockIdx.x * kGScanSeg + threadIdx.x * 4;
+    uint32_t c[4], sum = 0;
+    for (int k = 0; k < 4; ++k)
+    {
+        c[k] = v0 + k < N ? p.guide.gcnt[v0 + k] : 0u;
+        sum += c[k];
+    }
+    uint32_t total;
+    uint32_t at = p.guide.scanBase[blockIdx.x] + block_exclusive_scan_1024(sum, sh, total);
+    const uint32_t cap = p.paths.capacity;
+    const size_t stride = (size_t)p.bounces * cap;
+    const size_t C = p.guide.sampleCap;
+    for (int k = 0; k < 4; ++k)
+    {
+        if (!c[k]) continue;
+        const uint32_t path = (uint32_t)((uint64_t)(v0 + k) * p.guide.permMul % N);
+        guided_recurrence(p, path, [&](int32_t j, float w) {
+            if (at < C)
+            {
+                const float *gv = p.guide.gvtx + (size_t)j * cap + path;
+                float *dst = p.guide.samples + at;
+                for (int f = 0; f < kGVertexFloats; ++f) dst[f * C] = gv[f * stride];
+                dst[12 * C] = w;
+            }
+            ++at;
+        });
     }
 }
 
@@ -1434,6 +1530,18 @@ hipError_t launch_finalize(hipStream_t s, const TraceParams &p)
 {
     hipLaunchKernelGGL(k_finalize, dim3(grid_for(p.slots, 256, 4096)), dim3(256),
                        0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_collect(hipStream_t s, const TraceParams &p)
+{
+    const uint64_t n = (uint64_t)p.slots * p.sppPass;
+    const uint32_t segs = (uint32_t)((n + kGScanSeg - 1) / kGScanSeg);
+    if (segs == 0) return hipSuccess;
+    if (segs > (uint32_t)kGScanBlocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gscan_sums, dim3(segs), dim3(1024), 0, s, p);
+    hipLaunchKernelGGL(k_gscan_top, dim3(1), dim3(1024), 0, s, p, segs);
+    hipLaunchKernelGGL(k_guide_emit, dim3(segs), dim3(1024), 0, s, p);
     return hipGetLastError();
 }
 
