@@ -279,6 +279,25 @@ int spg_intersect_closest_device(spg_Scene *scene, uint32_t meshId,
                                  const float *dy, const float *dz, uint32_t n,
                                  uint32_t *primId, float *t, float *u,
                                  float *v, spg_TraversalStats *stats);
+/* spg_intersect_closest_device without any allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * persistent launch's work counter and traversal-stack spill live in a
+ * static device array of the library (SPG_NOALLOC_SCRATCH_BYTES on every
+ * GPU it runs on, held until the process ends); no hipMalloc, hipFree or
+ * allocating library call is made.  Limits: a mesh whose walk needs more
+ * than SPG_NOALLOC_STACK stack entries is rejected with SPG_E_CAPACITY; n
+ * is unbounded (the launch is persistent).  All calls on one GPU share the
+ * scratch: calls that overlap on one GPU (from different spg_Device handles
+ * of that GPU) race.  No traversal statistics. */
+#define SPG_NOALLOC_STACK 56
+#define SPG_NOALLOC_SCRATCH_BYTES (256u + 4096u * 64u * 24u * 4u)
+int spg_intersect_closest_device_noalloc(spg_Scene *scene, uint32_t meshId,
+                                         const float *ox, const float *oy,
+                                         const float *oz, const float *dx,
+                                         const float *dy, const float *dz,
+                                         uint32_t n, uint32_t *primId, float *t,
+                                         float *u, float *v);
 /* Scene-level closest hit in world space: sp_RayIntersectScene
  * (src/sp_scene.cpp:229-339).  out7 per ray: t, materialId bits,
  * normal.xyz, uv (7 floats); object/prim may be NULL. */
@@ -409,6 +428,28 @@ int spg_guide_train(spg_Guide *g, const float *samples13, uint32_t n,
  * taken, mean loss of the last step. */
 int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps,
                     float *loss);
+/* The training samples collected by the last traced pass, in the order the
+ * training steps read them (rows of SPG_GUIDE_SAMPLE_FLOATS: pn, n, v, wi,
+ * weight); *count = samples available (at most the pass's sample
+ * capacity), rows copied = min(*count, cap). */
+int spg_guide_samples(spg_Guide *g, float *rows13, uint32_t cap, uint32_t *count);
+/* Verification hooks of the guided estimator (no reference counterpart; the
+ * tests use them to check the GPU against the oracle).  flags
+ * SPG_GUIDE_ANALYTIC_MIXTURE: every vertex samples a fixed analytic mixture
+ * instead of the network's (lobe 0 toward the point (0.3, 2.5, -0.4),
+ * kappa 40, weight 0.5; lobe 1 about the normal, kappa 3, weight 0.3;
+ * lobe 2 about the mirror direction 2 (n.v) n - v, kappa 120, weight 0.2).
+ * recordCap > 0: from now on the guided estimator records its first
+ * recordCap scatter events (SPG_GUIDE_RECORD_FLOATS floats each: pn[3]
+ * n[3] v[3] roughness u0 u1 u2 lobes[40] wi[3] pdf hit[3] pad[4]); 0
+ * stops recording.  Each call restarts the record count. */
+#define SPG_GUIDE_ANALYTIC_MIXTURE 1u
+#define SPG_GUIDE_RECORD_FLOATS 64
+int spg_guide_debug(spg_Guide *g, uint32_t flags, uint32_t recordCap);
+/* The recorded scatter events: *count = recorded so far, rows copied =
+ * min(*count, cap). */
+int spg_guide_debug_records(spg_Guide *g, float *records, uint32_t cap,
+                            uint32_t *count);
 /* Guide checkpoint (no reference counterpart; the survey's optimizer state
  * that must be serializable): the configuration, the FP32 master
  * parameters, both Adam moments and the step count, in a little-endian file

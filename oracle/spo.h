@@ -46,7 +46,7 @@ extern "C" {
 enum { SPO_RNG_XORSHIFT_TILE = 0, SPO_RNG_XORSHIFT_PIXEL = 1, SPO_RNG_PCG = 2 };
 enum { SPO_TX_LIBM = 0, SPO_TX_CR = 1 };
 enum { SPO_HIT_REFERENCE = 0, SPO_HIT_BRUTE = 1 };
-enum { SPO_EST_PARITY = 0, SPO_EST_UNBIASED = 1, SPO_EST_NEE = 3 };
+enum { SPO_EST_PARITY = 0, SPO_EST_UNBIASED = 1, SPO_EST_GUIDED = 2, SPO_EST_NEE = 3 };
 
 #define SPO_MAX_METRICS 12
 #define SPO_NO_TEXTURE 0xFFFFFFFFu
@@ -161,6 +161,7 @@ typedef struct {
     float radianceClamp;      /* 0 = no clamp */
     int jitterYFirst;         /* argument evaluation order of the jitter draws */
     uint32_t seed;            /* xorshift tile state, or base seed */
+    float bsdfFraction;       /* SPO_EST_GUIDED: MIS probability of the BSDF */
 } spo_Config;
 
 /* closest-hit record, sp_RayIntersectSceneResult (src/sp_scene.h:39-53)
@@ -300,6 +301,25 @@ void spo_film_position(const spo_Camera *c, float px, float py, float *out3);
 int spo_scene_intersect(const spo_Scene *s, const float *o, const float *d,
                         int hitMode, int tx, spo_SceneHit *out,
                         uint64_t *metrics12);
+/* The guided estimator's scatter (FP32, the product's operation order):
+ * with probability fb the BSDF lobe (cosine, or GGX when u0 < fb * ps on
+ * rough materials), otherwise the vMF mixture lobes40 (pi[8] kappa[8]
+ * mu[8][3]) with u0' = (u0 - fb) / (1 - fb); pdf = fb * p_bsdf + (1 - fb) *
+ * q(wi) where cos(n, wi) > 0, else 0. */
+void spo_guided_scatter(const float *lobes40, const float *n, const float *v,
+                        float roughness, float fb, float u0, float u1, float u2,
+                        int tx, float *wi3, float *pdf);
+/* The verification mixture of the product's SPG_GUIDE_ANALYTIC_MIXTURE at
+ * hit point P with normal n and view direction v. */
+void spo_guide_analytic_lobes(const float *P, const float *n, const float *v,
+                              float *lobes40);
+/* SPO_EST_GUIDED renders with the verification mixture; candidate training
+ * samples (every scattering vertex with a positive weight, in the order the
+ * paths are traced) are appended to this sink, 16 floats each: hit point
+ * P[3], n[3], v[3], wi[3], weight, pixel index, sample index, vertex index
+ * (the last three as float).  cap 0 turns the sink off. */
+void spo_guided_sink(float *rows16, uint32_t cap);
+uint32_t spo_guided_sink_count(void);
 /* 1 if some object's closest mesh hit lies at a positive world t (the
  * shadow test toward the sky), 0 if none, -1 on a broadphase overflow */
 int spo_scene_occluded(const spo_Scene *s, const float *o, const float *d, int hitMode,

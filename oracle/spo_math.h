@@ -40,6 +40,20 @@ static inline float spo_atan2(float y, float x, int tx)
 {
     return tx == SPO_TX_CR ? (float)atan2((double)y, (double)x) : atan2f(y, x);
 }
+/* exp, log, expm1 of the guide's vMF lobes (SPO_TX_CR: double, rounded
+ * once, as the product's double-transcendental mode) */
+static inline float spo_exp(float x, int tx)
+{
+    return tx == SPO_TX_CR ? (float)exp((double)x) : expf(x);
+}
+static inline float spo_log(float x, int tx)
+{
+    return tx == SPO_TX_CR ? (float)log((double)x) : logf(x);
+}
+static inline float spo_expm1(float x, int tx)
+{
+    return tx == SPO_TX_CR ? (float)expm1((double)x) : expm1f(x);
+}
 /* powf(x, 5.0f) (Schlick's term).  SPO_TX_CR: x^2 exact in double, then
  * x^2 * x^2 * x with one rounding each and one rounding to float (the GPU's
  * tpow5, so the two agree bit for bit; it differs from pow() in double only

@@ -32,7 +32,7 @@ u32ref = C.POINTER(C.c_uint32)
 SPO_RNG_XORSHIFT_TILE, SPO_RNG_XORSHIFT_PIXEL, SPO_RNG_PCG = 0, 1, 2
 SPO_TX_LIBM, SPO_TX_CR = 0, 1
 SPO_HIT_REFERENCE, SPO_HIT_BRUTE = 0, 1
-SPO_EST_PARITY, SPO_EST_UNBIASED, SPO_EST_NEE = 0, 1, 3
+SPO_EST_PARITY, SPO_EST_UNBIASED, SPO_EST_GUIDED, SPO_EST_NEE = 0, 1, 2, 3
 NO_TEXTURE = 0xFFFFFFFF
 
 
@@ -40,7 +40,8 @@ class SpoConfig(C.Structure):
     _fields_ = [("rng", C.c_int), ("tx", C.c_int), ("hit", C.c_int),
                 ("estimator", C.c_int), ("samplesPerPixel", C.c_uint32),
                 ("bounces", C.c_uint32), ("radianceClamp", C.c_float),
-                ("jitterYFirst", C.c_int), ("seed", C.c_uint32)]
+                ("jitterYFirst", C.c_int), ("seed", C.c_uint32),
+                ("bsdfFraction", C.c_float)]
 
 
 class SpoCamera(C.Structure):
@@ -160,6 +161,11 @@ def oracle():
     _sig(L, "spo_mesh_intersect", C.c_float, vp, C.c_uint32, f32p, f32p, C.c_int,
          f32p, u32ref, u64p)
     _sig(L, "spo_scene_occluded", C.c_int, vp, f32p, f32p, C.c_int, u64p)
+    _sig(L, "spo_guided_scatter", None, f32p, f32p, f32p, C.c_float, C.c_float,
+         C.c_float, C.c_float, C.c_float, C.c_int, f32p, C.POINTER(C.c_float))
+    _sig(L, "spo_guide_analytic_lobes", None, f32p, f32p, f32p, f32p)
+    _sig(L, "spo_guided_sink", None, C.c_void_p, C.c_uint32)
+    _sig(L, "spo_guided_sink_count", C.c_uint32)
     _sig(L, "spo_closest_hit_brute", None, f32p, C.c_uint32, f32p, C.c_uint32,
          C.c_uint32, u32p, f32p)
     _sig(L, "spo_closest_hit_brute_f64", None, f32p, C.c_uint32, f32p, C.c_uint32,

@@ -137,3 +137,70 @@ def test_batch_rejects_bad_items(device):
     unbuilt.close()
     img.free()
     s.close()
+
+
+def test_batch_item_tiles_match_oracle(device):
+    """A batch item against the oracle over the same SAH hierarchy (plain
+    unbiased tracer, double-evaluated transcendentals, per-pixel PCG): two
+    tiles of two items of a config-3 style batch, bit for bit."""
+    import ctypes as C
+
+    from _libs import SPO_HIT_REFERENCE, SPO_RNG_PCG, SPO_TX_CR, SpoConfig, oracle
+    from _scenes import load_oracle
+
+    descs = scenes.config3(count=4, seed=21, min_side=96, max_side=160,
+                           min_tris=2e4, max_tris=3e5)
+    loaded = _load(device, descs)
+    kw = dict(spp=2, bounces=5, clamp=0.0, estimator=V.EST_UNBIASED, rng=V.RNG_PCG,
+              seed=0x5EED, transcendentals=V.TX_DOUBLE)
+    b = V.Batch(device, [(cam, s, img) for s, cam, img, _, _ in loaded],
+                autotune=False, **kw)
+    b.run()
+    got = _images(loaded)
+    O = oracle()
+    for k in (1, 3):
+        w, h = loaded[k][3], loaded[k][4]
+        os_, ocam = load_oracle(descs[k], sah=True)
+        cfg = SpoConfig(rng=SPO_RNG_PCG, tx=SPO_TX_CR, hit=SPO_HIT_REFERENCE,
+                        estimator=V.EST_UNBIASED, samplesPerPixel=2, bounces=5,
+                        radianceClamp=0.0, jitterYFirst=1, seed=0x5EED)
+        for t in ((0, 0, 32, 32), (w // 2 - 16, h // 2 - 16, w // 2 + 16, h // 2 + 16)):
+            px = np.zeros(w * h * 4, np.float32)
+            assert O.spo_path_trace_tile(os_, C.byref(ocam), C.byref(cfg), *t, None, px,
+                                         np.zeros(12, np.uint64)) == 0
+            a = got[k].reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
+            r = px.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
+            assert np.array_equal(a.view(np.uint32), r.view(np.uint32)), (k, t)
+        O.spo_scene_destroy(os_)
+    b.close()
+    _close(loaded)
+
+
+def test_full_config3_batch(device):
+    """The full batched workload: 64 seeded scenes of 10k-2M triangles,
+    images 128^2..1024^2, 16 spp, 5 bounces, one autotuned graph.  Every
+    pixel finite, every path traced once per sample, every ray a hit or a
+    miss, and the largest item equal to its single-scene render."""
+    descs = scenes.config3()
+    loaded = _load(device, descs)
+    kw = dict(spp=16, bounces=5, clamp=0.0, estimator=V.EST_UNBIASED, rng=V.RNG_PCG,
+              seed=0x5EED, transcendentals=V.TX_FAST)
+    b = V.Batch(device, [(cam, s, img) for s, cam, img, _, _ in loaded],
+                autotune=True, **kw)
+    t = b.tuning()
+    pixels = sum(w * h for *_, w, h in loaded)
+    assert t["pixels"] == pixels
+    m = b.run()
+    assert m.values[V.METRIC_PATHS] == pixels * 16
+    rays = m.values[V.METRIC_RAYS]
+    assert rays == m.values[V.METRIC_HITS] + m.values[V.METRIC_MISSES]
+    assert pixels * 16 <= rays <= pixels * 16 * 5
+    got = _images(loaded)
+    assert all(np.isfinite(g).all() for g in got)
+    assert all(g.reshape(-1, 4)[:, :3].mean() > 0 for g in got)
+    k = int(np.argmax([w * h for *_, w, h in loaded]))
+    s, cam, _, w, h = loaded[k]
+    want = _single(device, s, cam, w, h, **kw)
+    assert np.array_equal(got[k].view(np.uint32), want.view(np.uint32))
+    b.close()
+    _close(loaded)

@@ -1,0 +1,103 @@
+"""Albedo textures on the GPU (sp_EvaluateMaterial's texture branch,
+src/sp_material_system.cpp:65-75): the reference's TestMaterialAlbedoTexture
+(unit_tests/test_simd_path_tracer.cpp:308-331: a 1x1 red image as the albedo
+texture evaluates to albedo (1, 0, 0)) restated through the tracer, and
+textured renders against the oracle bit for bit."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import vk_cinematic_amd as V
+from vk_cinematic_amd import scenes
+from _libs import (SPO_HIT_BRUTE, SPO_RNG_PCG, SPO_RNG_XORSHIFT_PIXEL, SPO_TX_CR,
+                   SpoConfig, oracle)
+from _scenes import load_oracle
+
+pytestmark = pytest.mark.gpu
+
+NT = V.SPG_NO_TEXTURE
+
+
+def _oracle_render(desc, spp, bounces, est, orng, seed, clamp):
+    O = oracle()
+    s, cam = load_oracle(desc)
+    w, h = desc["camera"][0], desc["camera"][1]
+    cfg = SpoConfig(rng=orng, tx=SPO_TX_CR, hit=SPO_HIT_BRUTE, estimator=est,
+                    samplesPerPixel=spp, bounces=bounces, radianceClamp=clamp,
+                    jitterYFirst=1, seed=seed)
+    px = np.zeros(w * h * 4, np.float32)
+    assert O.spo_path_trace_tile(s, C.byref(cam), C.byref(cfg), 0, 0, w, h, None, px,
+                                 np.zeros(12, np.uint64)) == 0
+    O.spo_scene_destroy(s)
+    return px
+
+
+def _gpu_render(device, desc, spp, bounces, est, rng, seed, clamp):
+    s, cam = scenes.load_gpu(device, desc)
+    w, h = desc["camera"][0], desc["camera"][1]
+    ctx = V.make_context(cam, s, spp=spp, bounces=bounces, clamp=clamp, estimator=est,
+                         rng=rng, seed=seed)
+    img = np.zeros(w * h * 4, np.float32)
+    V.path_trace_tiles(ctx, V.ComputeTiles(w, h, 64, 64), img)
+    s.close()
+    return img
+
+
+def _textured_floor(texture, tw, th, w=48, h=32):
+    """An 8x8 floor quad (uv over [0,1]^2) seen from above under a constant
+    white sky, its material's albedo from `texture`."""
+    pv, pi = V.gen_plane()
+    down = scenes.quat_axis_angle((1, 0, 0), -np.pi / 2)
+    look = scenes.quat_axis_angle((1, 0, 0), -np.pi / 2)
+    return {"meshes": [(pv, pi, True)],
+            "objects": [(0, 3, (0, 0, 0), tuple(down), (8, 8, 8))],
+            "textures": [(5, texture, tw, th)],
+            "materials": [(3, (0, 0, 0), (0, 0, 0), 0.0, 5, NT),
+                          (9, (0, 0, 0), (1, 1, 1), 0.0, NT, NT)],
+            "background": 9,
+            "camera": (w, h, (0, 3, 0), tuple(look), 0.8)}
+
+
+def test_albedo_texture_known_answer(device):
+    """TestMaterialAlbedoTexture through the tracer: a 1x1 red albedo
+    texture.  Under a constant sky of radiance 1 the unbiased estimator's
+    two-ray sample of the floor is (1 - F) * albedo (the GGX term is dropped
+    at roughness 0): green and blue exactly 0 and red in (0, 0.96] on every
+    floor pixel; equal to the oracle bit for bit."""
+    desc = _textured_floor(np.float32([1, 0, 0, 1]), 1, 1)
+    img = _gpu_render(device, desc, 4, 2, V.EST_UNBIASED, V.RNG_PCG, 7, 0.0)
+    px = _oracle_render(desc, 4, 2, V.EST_UNBIASED, SPO_RNG_PCG, 7, 0.0)
+    assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
+    rgb = img.reshape(-1, 4)[:, :3]
+    floor = rgb[:, 0] != 1.0           # pixels that see the sky read (1, 1, 1)
+    assert floor.mean() > 0.9
+    assert np.all(rgb[floor, 1] == 0.0) and np.all(rgb[floor, 2] == 0.0)
+    assert np.all(rgb[floor, 0] > 0.0) and np.all(rgb[floor, 0] <= 0.96)
+
+
+@pytest.mark.parametrize("est,rng,orng,clamp", [
+    (V.EST_PARITY, V.RNG_PCG, SPO_RNG_PCG, 10.0),
+    (V.EST_PARITY, V.RNG_XORSHIFT_PIXEL, SPO_RNG_XORSHIFT_PIXEL, 10.0),
+    (V.EST_UNBIASED, V.RNG_PCG, SPO_RNG_PCG, 0.0),
+])
+def test_albedo_texture_renders_match_oracle(device, est, rng, orng, clamp):
+    """Nearest-texel lookup at the hit's uv (a 5x3 texture of distinct
+    colours on the floor, a 4x4 one on the config-1 sphere): every pixel bit
+    for bit against the oracle."""
+    r = np.random.default_rng(2)
+    floor = _textured_floor(r.uniform(0.05, 0.95, (3, 5, 4)).astype(np.float32).ravel(),
+                            5, 3)
+    img = _gpu_render(device, floor, 4, 3, est, rng, 11, clamp)
+    px = _oracle_render(floor, 4, 3, est, orng, 11, clamp)
+    assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
+    # distinct texels show up as distinct colours on the floor
+    assert len(np.unique(img.reshape(-1, 4)[:, :3].round(2), axis=0)) > 6
+    c1 = scenes.config1(64, 48, env="cube", env_size=32)
+    c1["textures"] = c1.get("textures", []) + [
+        (6, r.uniform(0.05, 0.95, (4, 4, 4)).astype(np.float32).ravel(), 4, 4)]
+    c1["materials"] = [(m[0], m[1], m[2], m[3], 6 if m[0] == scenes.ALBEDO_ID else m[4],
+                        m[5]) for m in c1["materials"]]
+    img = _gpu_render(device, c1, 4, 3, est, rng, 13, clamp)
+    px = _oracle_render(c1, 4, 3, est, orng, 13, clamp)
+    assert np.array_equal(img.view(np.uint32), px.view(np.uint32))

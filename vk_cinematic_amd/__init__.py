@@ -164,6 +164,8 @@ _SIGS = {
     "spg_intersect_closest_device": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp,
                                             _u32, _vp, _vp, _vp, _vp,
                                             C.POINTER(TraversalStats)]),
+    "spg_intersect_closest_device_noalloc": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp,
+                                                    _vp, _u32, _vp, _vp, _vp, _vp]),
     "spg_intersect_scene": (_int, [_vp, _f32p, _u32, _f32p, _vp, _vp]),
     "spg_time_intersect_closest": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp,
                                           _u32, _vp, _vp, _u32, C.POINTER(C.c_double)]),
@@ -206,6 +208,9 @@ _SIGS = {
     "spg_guide_train": (_int, [_vp, _f32p, _u32, _u32, C.POINTER(C.c_float)]),
     "spg_guide_stats": (_int, [_vp, C.POINTER(_u32), C.POINTER(_u32),
                                C.POINTER(C.c_float)]),
+    "spg_guide_samples": (_int, [_vp, _vp, _u32, C.POINTER(_u32)]),
+    "spg_guide_debug": (_int, [_vp, _u32, _u32]),
+    "spg_guide_debug_records": (_int, [_vp, _vp, _u32, C.POINTER(_u32)]),
     "spg_guide_save": (_int, [_vp, C.c_char_p]),
     "spg_guide_load": (_int, [_vp, C.c_char_p]),
 }
@@ -600,6 +605,8 @@ GUIDE_PARAMS = 19760
 GUIDE_OUTPUTS = 48
 GUIDE_LOBE_FLOATS = 40
 GUIDE_SAMPLE_FLOATS = 13
+GUIDE_ANALYTIC_MIXTURE = 1
+GUIDE_RECORD_FLOATS = 64
 
 
 class Guide:
@@ -650,6 +657,33 @@ class Guide:
         check(lib().spg_guide_train(self.handle, x, x.size // GUIDE_SAMPLE_FLOATS,
                                     steps, C.byref(loss)))
         return loss.value
+
+    def samples(self):
+        """The training samples of the last traced pass, in training order
+        (n x 13: pn, n, v, wi, weight)."""
+        n = _u32()
+        check(lib().spg_guide_samples(self.handle, None, 0, C.byref(n)))
+        out = np.zeros((n.value, GUIDE_SAMPLE_FLOATS), np.float32)
+        if n.value:
+            check(lib().spg_guide_samples(self.handle, out.ctypes.data, n.value,
+                                          C.byref(n)))
+        return out
+
+    def debug(self, analytic=False, record=0):
+        """Verification hooks: the analytic mixture instead of the network;
+        record the first `record` scatter events of the next traces."""
+        check(lib().spg_guide_debug(self.handle, GUIDE_ANALYTIC_MIXTURE if analytic
+                                    else 0, record))
+
+    def records(self):
+        """Recorded scatter events (n x 64, see spg_guide_debug)."""
+        n = _u32()
+        check(lib().spg_guide_debug_records(self.handle, None, 0, C.byref(n)))
+        out = np.zeros((n.value, GUIDE_RECORD_FLOATS), np.float32)
+        if n.value:
+            check(lib().spg_guide_debug_records(self.handle, out.ctypes.data, n.value,
+                                                C.byref(n)))
+        return out
 
     def save(self, path):
         """Checkpoint: configuration, parameters, Adam moments and steps."""

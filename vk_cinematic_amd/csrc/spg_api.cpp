@@ -173,6 +173,17 @@ int spg_device_create(int ordinal, spg_Device **out)
     }
     d->persist.counter = (uint32_t *)d->persistBlock;
     d->persist.spill = (int *)((char *)d->persistBlock + 256);
+    // the allocation-free entry point's scratch: this device's copy of the
+    // static array, looked up once with the device current
+    e = noalloc_scratch_address(&d->noallocScratch);
+    if (e != hipSuccess)
+    {
+        for (int i = 0; i < kEvents; ++i) hipEventDestroy(d->ev[i]);
+        hipStreamDestroy(d->stream);
+        hipFree(d->persistBlock);
+        delete d;
+        return hip_check(e, "hipGetSymbolAddress(scratch)");
+    }
     *out = d;
     return SPG_OK;
 }
@@ -1498,6 +1509,33 @@ int spg_intersect_closest_device(spg_Scene *s, uint32_t meshId,
         stats->nodeFetches = h[1] / 2;
         stats->hits = h[4];
     }
+    return hip_check(hipStreamSynchronize(d->stream), "closest hit");
+}
+
+int spg_intersect_closest_device_noalloc(spg_Scene *s, uint32_t meshId,
+                                         const float *ox, const float *oy,
+                                         const float *oz, const float *dx,
+                                         const float *dy, const float *dz,
+                                         uint32_t n, uint32_t *primId, float *t,
+                                         float *u, float *v)
+{
+    CHK(check_mesh(s, meshId));
+    if (n && (!ox || !oy || !oz || !dx || !dy || !dz || !primId || !t))
+        return fail(SPG_E_INVALID, "spg_intersect_closest_device_noalloc: NULL array");
+    const HostMesh &hm = s->meshes[meshId];
+    if (hm.widePending > kNoAllocStack)
+        return fail(SPG_E_CAPACITY,
+                    "spg_intersect_closest_device_noalloc: the mesh's walk needs " +
+                        std::to_string(hm.widePending) + " stack entries, the static "
+                        "scratch holds " + std::to_string(kNoAllocStack));
+    spg_Device *d = s->dev;
+    if (!d->noallocScratch)
+        return fail(SPG_E_STATE, "spg_intersect_closest_device_noalloc: no scratch address");
+    std::lock_guard<std::mutex> g(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    DevMesh m = dev_mesh(hm);
+    HIPCHK(launch_closest_mesh_noalloc(d->stream, &m, ox, oy, oz, dx, dy, dz, n, primId,
+                                       t, u, v, d->noallocScratch));
     return hip_check(hipStreamSynchronize(d->stream), "closest hit");
 }
 

@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include "spg_math.h"
+#include "spg_tx.h"
 
 namespace spg {
 
@@ -29,6 +30,7 @@ constexpr int kGLobeFloats = 40;                 // pi[8] kappa[8] mu[8][3]
 constexpr int kGSampleFloats = 13;               // pn n v wi weight
 constexpr int kGVertexFloats = 12;               // pn n v wi
 constexpr int kGradChunk = 512;                  // samples per partial weight gradient
+constexpr int kGRecordFloats = 64;               // one recorded scatter event
 constexpr int kGScanSeg = 4096;                  // paths per segment of the sample scan
 constexpr int kGScanBlocks = 1024;               // segments (paths per pass <= 2^22)
 constexpr float kLogKappa0 = -2.0f;
@@ -59,6 +61,13 @@ struct DevGuide
     uint32_t salt;
     uint32_t train;        // record vertices and append training samples
     float bsdfFraction;    // one-sample MIS probability of BSDF sampling
+    // verification hooks (off in normal use): analytic != 0 replaces the
+    // network by guide_analytic_lobes; rec != nullptr records up to recCap
+    // scatter events (kGRecordFloats each) of the guided estimator
+    uint32_t analytic;
+    float *rec;
+    uint32_t *recCount;
+    uint32_t recCap;
     float lo[3], invExt[3];// scene bounds -> [0,1]^3
 };
 
@@ -85,14 +94,17 @@ struct LobeRef
     }
 };
 
-// kappa / (2 pi (1 - e^-2kappa)) * e^(kappa (mu.w - 1))
+// kappa / (2 pi (1 - e^-2kappa)) * e^(kappa (mu.w - 1)); kDouble: exp and
+// expm1 evaluated in double and rounded once (bit-reproducible on the host)
+template <bool kDouble>
 __device__ __forceinline__ float vmf_pdf(float kappa, const float *mu, v3 w)
 {
     float d = mu[0] * w.x + mu[1] * w.y + mu[2] * w.z;
-    return kappa / (6.28318548f * -expm1f(-2.0f * kappa)) *
-           expf(kappa * (d - 1.0f));
+    return kappa / (6.28318548f * -Tx<kDouble>::expm1_(-2.0f * kappa)) *
+           Tx<kDouble>::exp_(kappa * (d - 1.0f));
 }
 
+template <bool kDouble>
 __device__ __forceinline__ float mixture_pdf(const LobeRef &L, v3 w)
 {
     float q = 0.0f;
@@ -100,7 +112,7 @@ __device__ __forceinline__ float mixture_pdf(const LobeRef &L, v3 w)
     {
         float m[3];
         L.mu(k, m);
-        q += L.pi(k) * vmf_pdf(L.kappa(k), m, w);
+        q += L.pi(k) * vmf_pdf<kDouble>(L.kappa(k), m, w);
     }
     return q;
 }
@@ -116,21 +128,26 @@ __device__ __forceinline__ void onb(v3 n, v3 &t1, v3 &t2)
 }
 
 // Jakob 2012: cos = 1 + log(u1 + (1 - u1) e^-2kappa) / kappa, uniform
-// azimuth about mu.
+// azimuth 2 pi u2 about mu.  kDouble: log, exp and the azimuth's sin/cos in
+// double, each rounded once.
+template <bool kDouble>
 __device__ __forceinline__ v3 vmf_sample(float kappa, const float *mu, float u1,
                                          float u2)
 {
-    float c = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kappa)) / kappa;
+    float c = 1.0f + Tx<kDouble>::log_(u1 + (1.0f - u1) * Tx<kDouble>::exp_(-2.0f * kappa)) /
+                         kappa;
     c = fminf(fmaxf(c, -1.0f), 1.0f);
     float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
     float sp, cp;
-    sincospif(2.0f * u2, &sp, &cp);
+    if (kDouble) tsincos(6.28318548f * u2, &sp, &cp);
+    else sincospif(2.0f * u2, &sp, &cp);
     v3 m{mu[0], mu[1], mu[2]}, t1, t2;
     onb(m, t1, t2);
     return (t1 * (s * cp) + t2 * (s * sp)) + m * c;
 }
 
 // u0 in [0,1) picks the lobe by the cumulative weights.
+template <bool kDouble>
 __device__ __forceinline__ v3 mixture_sample(const LobeRef &L, float u0, float u1,
                                              float u2)
 {
@@ -147,7 +164,35 @@ __device__ __forceinline__ v3 mixture_sample(const LobeRef &L, float u0, float u
     }
     float m[3];
     L.mu(k, m);
-    return vmf_sample(L.kappa(k), m, u1, u2);
+    return vmf_sample<kDouble>(L.kappa(k), m, u1, u2);
+}
+
+// The verification mixture (no network): lobe 0 toward the fixed point
+// (0.3, 2.5, -0.4) from the hit, kappa 40, weight 0.5; lobe 1 about the
+// normal, kappa 3, weight 0.3; lobe 2 about the mirror direction
+// 2 (N.V) N - V, kappa 120, weight 0.2; lobes 3..7 weight 0.  Unit vectors
+// as (x, y, z) * (1 / sqrt(x^2 + y^2 + z^2)).
+__device__ __forceinline__ v3 unit_rcp(v3 a)
+{
+    float inv = 1.0f / sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+    return v3{a.x * inv, a.y * inv, a.z * inv};
+}
+__device__ __forceinline__ void guide_analytic_lobes(v3 P, v3 N, v3 V, Lobes &L)
+{
+    const v3 c0{0.3f, 2.5f, -0.4f};
+    const float nv = N.x * V.x + N.y * V.y + N.z * V.z;
+    const v3 mus[3] = {unit_rcp(v3{c0.x - P.x, c0.y - P.y, c0.z - P.z}), N,
+                       unit_rcp(v3{2.0f * nv * N.x - V.x, 2.0f * nv * N.y - V.y,
+                                   2.0f * nv * N.z - V.z})};
+    const float pis[3] = {0.5f, 0.3f, 0.2f}, kaps[3] = {40.0f, 3.0f, 120.0f};
+    for (int k = 0; k < kGLobes; ++k)
+    {
+        L.pi[k] = k < 3 ? pis[k] : 0.0f;
+        L.kappa[k] = k < 3 ? kaps[k] : 1.0f;
+        L.mu[k][0] = k < 3 ? mus[k].x : 0.0f;
+        L.mu[k][1] = k < 3 ? mus[k].y : 0.0f;
+        L.mu[k][2] = k < 3 ? mus[k].z : 1.0f;
+    }
 }
 
 __device__ __forceinline__ uint32_t guide_hash(uint32_t x)

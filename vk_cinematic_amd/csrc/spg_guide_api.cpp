@@ -221,6 +221,10 @@ int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
     dg.selectBelow = 0xFFFFFFFFu;
     dg.train = train ? 1u : 0u;
     dg.bsdfFraction = g->cfg.bsdfFraction;
+    dg.analytic = g->analytic;
+    dg.rec = g->recCap ? (float *)((char *)g->recBlock + 256) : nullptr;
+    dg.recCount = g->recCap ? (uint32_t *)g->recBlock : nullptr;
+    dg.recCap = g->recCap;
     for (int a = 0; a < 3; ++a)
     {
         if (!(lo[a] <= hi[a]))
@@ -375,6 +379,7 @@ int spg_guide_destroy(spg_Guide *g)
         free_dev(g->samplesBlock);
         free_dev(g->hgBlock);
         free_dev(g->pathBlock);
+        free_dev(g->recBlock);
         delete g;
         d->liveScenes--;
         last = d->destroyRequested && d->liveScenes == 0;
@@ -510,6 +515,79 @@ int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps, float *l
         uint32_t n = have > first ? std::min(have - first, g->cfg.batchSize) : 0;
         *loss = n ? l / (float)n : 0.0f;
     }
+    return SPG_OK;
+}
+
+int spg_guide_debug(spg_Guide *g, uint32_t flags, uint32_t recordCap)
+{
+    if (!g || (flags & ~SPG_GUIDE_ANALYTIC_MIXTURE))
+        return fail(SPG_E_INVALID, "spg_guide_debug: NULL guide or unknown flag");
+    if (recordCap > (1u << 24))
+        return fail(SPG_E_CAPACITY, "spg_guide_debug: at most 2^24 records");
+    spg_Device *d = g->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    HIPCHK(hipSetDevice(d->ordinal));
+    g->analytic = flags & SPG_GUIDE_ANALYTIC_MIXTURE;
+    if (recordCap != g->recCap)
+    {
+        HIPCHK(hipStreamSynchronize(d->stream));
+        free_dev(g->recBlock);
+        g->recCap = 0;
+        if (recordCap)
+            HIPCHK(hipMalloc(&g->recBlock, 256 + (size_t)recordCap * kGRecordFloats * 4));
+        g->recCap = recordCap;
+    }
+    if (g->recCap) HIPCHK(launch_set_counter(d->stream, (uint32_t *)g->recBlock, 0u));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return SPG_OK;
+}
+
+int spg_guide_debug_records(spg_Guide *g, float *records, uint32_t cap, uint32_t *count)
+{
+    if (!g || !count) return fail(SPG_E_INVALID, "spg_guide_debug_records: NULL");
+    spg_Device *d = g->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    *count = 0;
+    if (!g->recCap) return SPG_OK;
+    HIPCHK(hipSetDevice(d->ordinal));
+    uint32_t n = 0;
+    HIPCHK(hipMemcpyAsync(&n, g->recBlock, 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    n = std::min(n, g->recCap);
+    *count = n;
+    if (records && cap)
+    {
+        HIPCHK(hipMemcpyAsync(records, (char *)g->recBlock + 256,
+                              (size_t)std::min(n, cap) * kGRecordFloats * 4,
+                              hipMemcpyDeviceToHost, d->stream));
+        HIPCHK(hipStreamSynchronize(d->stream));
+    }
+    return SPG_OK;
+}
+
+int spg_guide_samples(spg_Guide *g, float *rows13, uint32_t cap, uint32_t *count)
+{
+    if (!g || !count) return fail(SPG_E_INVALID, "spg_guide_samples: NULL");
+    spg_Device *d = g->dev;
+    std::lock_guard<std::mutex> lk(d->lock);
+    *count = 0;
+    if (!g->samplesBlock) return SPG_OK;
+    HIPCHK(hipSetDevice(d->ordinal));
+    uint32_t n = 0;
+    HIPCHK(hipMemcpyAsync(&n, g->count, 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    n = std::min(n, g->sampleCap);
+    *count = n;
+    if (!rows13 || !cap || !n) return SPG_OK;
+    const uint32_t m = std::min(n, cap);
+    std::vector<float> soa((size_t)kGSampleFloats * m);
+    for (int k = 0; k < kGSampleFloats; ++k)
+        HIPCHK(hipMemcpyAsync(soa.data() + (size_t)k * m, g->samples + (size_t)k * g->sampleCap,
+                              (size_t)m * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    for (uint32_t j = 0; j < m; ++j)
+        for (int k = 0; k < kGSampleFloats; ++k)
+            rows13[(size_t)j * kGSampleFloats + k] = soa[(size_t)k * m + j];
     return SPG_OK;
 }
 

@@ -43,6 +43,13 @@ struct Persist
     uint32_t maxBlocks;  // cap on the resident one-wave blocks, 0 = occupancy
 };
 
+// Limits of the allocation-free closest-hit entry point: resident one-wave
+// blocks of its persistent launch, and the traversal-stack entries per lane
+// its scratch holds (LDS and spill together; a mesh whose walk needs more
+// is rejected).
+constexpr uint32_t kNoAllocWaves = 4096;
+constexpr uint32_t kNoAllocStack = 56;
+
 // Relative slack on the t-max cull: a box is skipped only if its entry
 // exceeds the best t by more than 2^-12 of it, so rounding differences
 // between the slab entry and the Moller-Trumbore t never cull a closer hit.
@@ -271,6 +278,8 @@ struct spg_Device
     void *accum = nullptr;       // per-slot radiance sums
     size_t accumCount = 0;
     void *persistBlock = nullptr; // launch counters + traversal stack spill
+    void *noallocScratch = nullptr; // device address of the static scratch of
+                                    // spg_intersect_closest_device_noalloc
     spg::Persist persist{};
     hipEvent_t ev[kEvents];
     bool eventsMade = false;
@@ -304,6 +313,10 @@ struct spg_Guide
     uint32_t *gcnt = nullptr, *scanBase = nullptr;  // training-sample scan
     uint32_t pathCap = 0, pathBounces = 0;
     uint32_t steps = 0;               // Adam steps taken
+    // verification hooks (spg_guide_debug)
+    uint32_t analytic = 0;
+    void *recBlock = nullptr;         // count (256 B), then the records
+    uint32_t recCap = 0;
 };
 
 struct HostMesh
@@ -395,6 +408,15 @@ hipError_t launch_closest_mesh(hipStream_t s, const DevMesh *mesh,
                                const float *dy, const float *dz, uint32_t n,
                                uint32_t *prim, float *t, float *u, float *v,
                                unsigned long long *stats, const Persist &ps);
+// device address of the allocation-free entry point's scratch (call with
+// the device current)
+hipError_t noalloc_scratch_address(void **ptr);
+hipError_t launch_closest_mesh_noalloc(hipStream_t s, const DevMesh *mesh,
+                                       const float *ox, const float *oy,
+                                       const float *oz, const float *dx,
+                                       const float *dy, const float *dz, uint32_t n,
+                                       uint32_t *prim, float *t, float *u, float *v,
+                                       void *scratchPtr);
 hipError_t launch_intersect_scene(hipStream_t s, const DevScene &scene,
                                   const float *rays6, uint32_t n, float *out7,
                                   uint32_t *object, uint32_t *prim,

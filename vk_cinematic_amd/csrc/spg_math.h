@@ -253,6 +253,9 @@ SPG_HD float tsin(float x) { return (float)sin((double)x); }
 SPG_HD float tcos(float x) { return (float)cos((double)x); }
 SPG_HD float tatan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 SPG_HD float tpow(float x, float y) { return (float)pow((double)x, (double)y); }
+SPG_HD float texp(float x) { return (float)exp((double)x); }
+SPG_HD float tlog(float x) { return (float)log((double)x); }
+SPG_HD float texpm1(float x) { return (float)expm1((double)x); }
 // sin and cos of one argument from one double-precision evaluation, each
 // rounded once to float.  For |x| < 2^20 pi/2: Cody-Waite reduction
 // r = x - k pi/2, k = rint(x 2/pi), with pi/2 split into two 33-bit parts

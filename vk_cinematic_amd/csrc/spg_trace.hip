@@ -405,7 +405,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_mesh(
 {
     Counts cnt{0, 0, 0, 0};
     uint32_t rays = 0, hits = 0;
-    DevScene none{};
     auto begin = [&](uint32_t i, Trav &s) {
         trav_begin_mesh(m, s, v3{ox[i], oy[i], oz[i]}, v3{dx[i], dy[i], dz[i]});
         return true;
@@ -675,6 +674,30 @@ __device__ __forceinline__ float bsdf_pdf(v3 n, v3 V, v3 L, float cosv, float ro
     return ps > 0.0f ? pc + ps * ggx_pdf(n, V, L, roughness) : pc;
 }
 
+// Verification hook of the guided estimator: appends one scatter event
+// (pn, N, V, roughness, the three uniforms, the mixture, the sampled
+// direction and its density, the hit point) to the guide's record buffer.
+__device__ __noinline__ void record_scatter(const DevGuide &G, v3 pn, v3 N, v3 V,
+                                            float rough, float u0, float u1, float u2,
+                                            const LobeRef &L, v3 wi, float pdf, v3 P)
+{
+    const uint32_t at = atomicAdd(G.recCount, 1u);
+    if (at >= G.recCap) return;
+    float *r = G.rec + (size_t)at * kGRecordFloats;
+    const float head[13] = {pn.x, pn.y, pn.z, N.x, N.y, N.z, V.x, V.y, V.z,
+                            rough, u0, u1, u2};
+    for (int k = 0; k < 13; ++k) r[k] = head[k];
+    for (int k = 0; k < kGLobeFloats; ++k) r[13 + k] = L.at(k);
+    r[53] = wi.x;
+    r[54] = wi.y;
+    r[55] = wi.z;
+    r[56] = pdf;
+    r[57] = P.x;
+    r[58] = P.y;
+    r[59] = P.z;
+    for (int k = 60; k < kGRecordFloats; ++k) r[k] = 0.0f;
+}
+
 // One bounce of every queued path: surface, material, scatter, vertex record
 // (parity) or throughput update (unbiased), append continuing rays to queue
 // q^1 with a ballot + one atomic per wave.
@@ -922,12 +945,16 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                             wi = lobe_direction<kDouble>(N, V, m.roughness, u0 < fb * ps,
                                                          u1, u2);
                         else
-                            wi = mixture_sample(Lg, (u0 - fb) / (1.0f - fb), u1, u2);
+                            wi = mixture_sample<kDouble>(Lg, (u0 - fb) / (1.0f - fb), u1,
+                                                         u2);
                         float cosv = dot(N, wi);
                         float pdf = cosv > 0.0f
                                         ? fb * bsdf_pdf(N, V, wi, cosv, m.roughness, ps) +
-                                              (1.0f - fb) * mixture_pdf(Lg, wi)
+                                              (1.0f - fb) * mixture_pdf<kDouble>(Lg, wi)
                                         : 0.0f;
+                        if (p.guide.rec)
+                            record_scatter(p.guide, guide_pn(p.guide, o + d * t), N, V,
+                                           m.roughness, u0, u1, u2, Lg, wi, pdf, o + d * t);
                         if (cosv > 0.0f && pdf > 0.0f)
                         {
                             X = brdf_bracket<kDouble>(N, V, wi, m, true);
@@ -1242,14 +1269,14 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
                                                                   int q)
 {
     __shared__ _Float16 sW[kWLdsHalves];
-    __shared__ float scratch[kInferWaves][kWaveScratchFloats];
+    __shared__ float waveOut[kInferWaves][kWaveScratchFloats];
     stage_rows(sW, p.guide.w16, kGRows);
     __syncthreads();
     const uint32_t n = p.paths.counters[q];
     const uint32_t cap = p.paths.capacity;
     const float *qo = p.paths.qo[q], *qd = p.paths.qd[q];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float *scr = scratch[wave];
+    float *scr = waveOut[wave];
     _Float16 *enc = (_Float16 *)scr;
     for (uint32_t tile = xcd_block() * kInferWaves + wave; tile * 64 < n;
          tile += gridDim.x * kInferWaves)
@@ -1272,8 +1299,16 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
                            p.paths.hv[i], N, uv);
                 pn = guide_pn(p.guide, o + d * t);
                 V = -d;
+                if (p.guide.analytic)
+                {
+                    // verification mode: the analytic mixture, no network
+                    Lobes L;
+                    guide_analytic_lobes(o + d * t, N, V, L);
+                    store_lobes(p.guide.lobes, cap, i, L);
+                }
             }
         }
+        if (p.guide.analytic) continue;  // wave-uniform
         encode_row(enc + lane * kLdsPitch, valid, pn, N, V);
         wave_lds_sync();
         h8 B[4][2];
@@ -1393,6 +1428,47 @@ hipError_t launch_closest_mesh(hipStream_t s, const DevMesh *mesh,
         hipLaunchKernelGGL(k_closest_mesh<false>, dim3(persist_blocks(gPlain, n)),
                            dim3(kTraceBlock), 0, s, *mesh, ox, oy, oz, dx, dy, dz,
                            n, prim, t, u, v, stats, ps.counter, ps.spill);
+    return hipGetLastError();
+}
+
+// ---- allocation-free closest hit (spg_intersect_closest_device_noalloc)
+// Its scratch lives in this file-scope array, one copy per GPU for the
+// life of the process: the persistent launch's work counter (first 256
+// bytes), then the traversal-stack spill of at most kNoAllocWaves one-wave
+// blocks, kNoAllocSpill entries per lane beyond the LDS stack (entry i of
+// grid lane gl at spill[(i - kMeshShortStack) * gridLanes + gl]).  Host code
+// never uses the array's name as an address: noalloc_scratch_address
+// returns the device address (hipGetSymbolAddress), which the launcher
+// passes to the kernels.
+constexpr uint32_t kNoAllocSpill = kNoAllocStack - kMeshShortStack;
+static_assert(kNoAllocStack > (uint32_t)kMeshShortStack, "spill entries");
+constexpr size_t kNoAllocScratchBytes =
+    256 + (size_t)kNoAllocWaves * kTraceBlock * kNoAllocSpill * sizeof(int);
+static_assert(kNoAllocScratchBytes == SPG_NOALLOC_SCRATCH_BYTES &&
+                  kNoAllocStack == SPG_NOALLOC_STACK,
+              "the header's limits");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch_address(void **ptr)
+{
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+hipError_t launch_closest_mesh_noalloc(hipStream_t s, const DevMesh *mesh,
+                                       const float *ox, const float *oy,
+                                       const float *oz, const float *dx,
+                                       const float *dy, const float *dz, uint32_t n,
+                                       uint32_t *prim, float *t, float *u, float *v,
+                                       void *scratchPtr)
+{
+    if (n == 0) return hipSuccess;
+    static uint32_t g = std::min(persistent_grid(k_closest_mesh<false>), kNoAllocWaves);
+    uint32_t *counter = (uint32_t *)scratchPtr;
+    int *spill = (int *)((char *)scratchPtr + 256);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, counter, 0u);
+    hipLaunchKernelGGL(k_closest_mesh<false>, dim3(persist_blocks(g, n)),
+                       dim3(kTraceBlock), 0, s, *mesh, ox, oy, oz, dx, dy, dz, n, prim,
+                       t, u, v, nullptr, counter, spill);
     return hipGetLastError();
 }
 

@@ -22,6 +22,10 @@ template <> struct Tx<true>
     }
     // Schlick's (1 - cos)^5 (the reference's powf(x, 5.0f))
     static __device__ __forceinline__ float pow5_(float x) { return tpow5(x); }
+    // the guide's vMF density and sampling
+    static __device__ __forceinline__ float exp_(float x) { return texp(x); }
+    static __device__ __forceinline__ float log_(float x) { return tlog(x); }
+    static __device__ __forceinline__ float expm1_(float x) { return texpm1(x); }
 };
 template <> struct Tx<false>
 {
@@ -40,6 +44,9 @@ template <> struct Tx<false>
         float x2 = x * x;
         return x2 * x2 * x;
     }
+    static __device__ __forceinline__ float exp_(float x) { return expf(x); }
+    static __device__ __forceinline__ float log_(float x) { return logf(x); }
+    static __device__ __forceinline__ float expm1_(float x) { return expm1f(x); }
 };
 
 }  // namespace spg
