@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s at 1 GPU; time-to-relMSE 1e-3, guided vs plain GPU tracer"
 HBM_PEAK_GBS = 8000.0
+MFMA_FP16_PEAK_TFLOPS = 2500.0   # dense FP16 MFMA, MI355X
 # Algorithmic bytes of the closest-hit walk, per unit: a ray's 24 bytes in
 # and its hit record out (8 bytes, triangle and t, for the closest-hit
 # kernel; 20 bytes, t, u, v, instance, triangle, for the path tracer's
@@ -50,6 +51,15 @@ BOX_BYTES, TRI_BYTES, ENTRY_BYTES = 24, 36, 128
 # x1.0 by a 1 GiB copy kernel on the same box; config5: the plain tracer's
 # launches); the roofline's "traffic" is this times the rays of one launch.
 TRAFFIC_BYTES_PER_RAY = {"config5": 1.522e9 / 2037010, "soup": 1.095e9 / 4194304}
+# What bounds the closest-hit kernel in fact, from counter runs of the same
+# kernel and workload: the VALU issue rate, not HBM.  valu_busy: share of
+# the SIMDs' issue cycles with a VALU instruction (SQ_ACTIVE_INST_VALU over
+# SQ_BUSY_CYCLES x 4 SIMDs); valu_insts_per_ray: SQ_INSTS_VALU per traced
+# ray; hbm_frac: counter-measured bytes over the launch time, against the
+# 8 TB/s peak (the "traffic" figure); the roofline "frac" above prices the
+# algorithmic bytes instead.
+LIMITER = {"config5": {"kind": "VALU issue", "valu_busy": 0.96, "valu_insts_per_ray": 285,
+                       "source": "profiles/hbm_traffic_config5_soup.md"}}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same kernel and "
                   "workload (profiles/hbm_traffic_config5_soup.md), FETCH x2.0 and "
                   "WRITE x1.0 by a 1 GiB copy calibration")
@@ -57,6 +67,40 @@ TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same kernel an
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """The host CPUs this process may use: the CPU model, the machine's
+    logical CPU count (nproc), the CPUs in this process's affinity mask, the
+    cgroup quota in CPUs where one is set, and the worker count the CPU
+    baselines use -- the process's CPU share (OMP_NUM_THREADS where the
+    host sets it, else the affinity mask), so a baseline never runs more
+    threads than the cores it is given."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    share = affinity
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        share = min(share, int(os.environ["OMP_NUM_THREADS"]))
+    if quota:
+        share = min(share, max(1, int(quota)))
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "cgroup_cpus": quota, "threads": max(1, share)}
 
 
 def cpu_baseline_config1(threads, spp=16):
@@ -197,7 +241,9 @@ def bench_config1(args, V, scenes):
                   "step_ms": steps_ms},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_config1(min(16, os.cpu_count() or 1))
+        cpu = host_cpu()
+        res["cpu_baseline"] = cpu_baseline_config1(cpu["threads"])
+        res["cpu_baseline"]["host"] = cpu
     img.free()
     s.close()
     dev.close()
@@ -331,12 +377,14 @@ def bench_box(args, V, scenes, name):
                      "traffic": (TRAFFIC_BYTES_PER_RAY["config5"] * rays / launches
                                  if name == "config5" else None),
                      "traffic_source": TRAFFIC_SOURCE if name == "config5" else None,
+                     "traffic_frac": (TRAFFIC_BYTES_PER_RAY["config5"] * rays / launches /
+                                      (mean_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                      if name == "config5" else None),
                      "kernel": "k_extend (closest-hit traversal)",
                      "launch_ms": mean_ms, "algorithmic_bytes_per_launch": per_launch,
                      "compulsory_bytes_per_launch": (RAY_BYTES + HIT_BYTES_SCENE) * rays /
                      launches + scene_bytes,
-                     "measured_limit": "VALU issue (rocprofv3 SQ_ACTIVE_INST_VALU ~99% of "
-                                       "SIMD issue slots, profiles/hbm_traffic_config5_soup.md)",
+                     "limiter": LIMITER.get(name),
                      "rays_per_step": rays, "slab_tests_per_ray": box / max(rays, 1),
                      "tri_tests_per_ray": tri / max(rays, 1),
                      "tlas_slab_tests_per_ray": tbox / max(rays, 1),
@@ -372,17 +420,54 @@ def bench_box(args, V, scenes, name):
             cls[key] += tim[key][0]
     dev.synchronize()
     gwall = time.perf_counter() - t0
+    train_samples = g.samples()
     # inference only (frozen guide)
     fcls = {"raygen": 0.0, "extend": 0.0, "shade": 0.0, "finalize": 0.0, "guide": 0.0,
             "total": 0.0}
+    infer_launches, evals = 0, 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         V.path_trace_tiles_device(gctx(args.warmup + k, train=False), tiles, img)
         tim = dev.last_timings()
         for key in fcls:
             fcls[key] += tim[key][0]
+        infer_launches += tim["guide"][1]
+        evals += dev.last_counters()["guide_evals"]
     dev.synchronize()
     fwall = time.perf_counter() - t0
+    # MFMA rooflines of the network: inference (k_guide_infer: 2 (64*64*4 +
+    # 64*48) FLOP per evaluated hit, mean launch time from the trace's own
+    # events) and one training step on this workload's collected samples
+    # (k_guide_wsum + k_guide_fwdbwd + k_guide_wgrad + k_guide_greduce +
+    # k_guide_adam: about 3x the forward FLOP per sample; host wall clock
+    # of 50 steps including launch gaps)
+    flop_inf = 2 * (64 * 64 * 4 + 64 * 48)
+    infer_ms = fcls["guide"] / max(infer_launches, 1)
+    infer_flop = flop_inf * evals / max(infer_launches, 1)
+    gt = V.Guide(dev, seed=1, max_weight=10.0)
+    gt.train(train_samples, steps=5)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    gt.train(train_samples, steps=50)
+    dev.synchronize()
+    step_s = (time.perf_counter() - t0) / 50
+    gt.close()
+    train_flop = 3 * flop_inf * 16384
+    res["extra"]["guide_mfma"] = {
+        "peak_tflops": MFMA_FP16_PEAK_TFLOPS,
+        "infer": {"kernel": "k_guide_infer", "launch_ms": infer_ms,
+                  "evals_per_launch": evals / max(infer_launches, 1),
+                  "flop_per_launch": infer_flop,
+                  "achieved_tflops": infer_flop / (infer_ms * 1e-3) / 1e12,
+                  "frac": infer_flop / (infer_ms * 1e-3) / 1e12 / MFMA_FP16_PEAK_TFLOPS},
+        "train_step": {"kernels": "k_guide_wsum, k_guide_fwdbwd, k_guide_wgrad, "
+                                  "k_guide_greduce, k_guide_adam",
+                       "batch": 16384, "step_ms": step_s * 1e3,
+                       "flop_per_step": train_flop,
+                       "achieved_tflops": train_flop / step_s / 1e12,
+                       "frac": train_flop / step_s / 1e12 / MFMA_FP16_PEAK_TFLOPS,
+                       "samples_from": f"{len(train_samples)} samples collected by the "
+                                       f"last training pass of this workload"}}
     res["extra"]["guided"] = {
         "Msamples_per_s_training": samples * args.steps / gwall / 1e6,
         "Msamples_per_s_frozen": samples * args.steps / fwall / 1e6,
@@ -394,8 +479,9 @@ def bench_box(args, V, scenes, name):
         "collected_last_pass": g.stats()["collected"]}
     g.close()
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_box(desc, min(16, os.cpu_count() or 1),
-                                               bounces=bounces)
+        cpu = host_cpu()
+        res["cpu_baseline"] = cpu_baseline_box(desc, cpu["threads"], bounces=bounces)
+        res["cpu_baseline"]["host"] = cpu
     img.free()
     s.close()
     dev.close()
@@ -506,7 +592,9 @@ def bench_config3(args, V, scenes):
                   "scene_build_s": build_s, "batch_create_s": create_s},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_config3(descs, min(16, os.cpu_count() or 1))
+        cpu = host_cpu()
+        res["cpu_baseline"] = cpu_baseline_config3(descs, cpu["threads"])
+        res["cpu_baseline"]["host"] = cpu
     b.close()
     for img in imgs:
         img.free()
@@ -524,65 +612,72 @@ def relmse(img, ref):
     return float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2)))
 
 
-class Accumulator:
-    """Renders chunks of samples of one scene into a device image and
-    accumulates them on the host; GPU time per chunk from the trace call's
-    own events (host copies excluded)."""
+class DeviceAccumulator:
+    """Renders chunks of samples of one scene and keeps their running mean
+    on the device (spg_accumulate_device).  `wall` is the host wall clock
+    around the render and accumulate calls (each trace call returns when its
+    work is done); reading the mean back for a relMSE checkpoint is outside
+    it."""
 
-    def __init__(self, V, dev, s, cam, w, h):
+    def __init__(self, V, dev, s, cam, w, h, bounces=6):
         self.V, self.dev, self.s, self.cam, self.w, self.h = V, dev, s, cam, w, h
+        self.bounces = bounces
         self.tiles = V.ComputeTiles(w, h, 64, 64)
         self.img = dev.alloc(w * h * 16)
-        self.sum = np.zeros((h, w, 3))
+        self.acc = dev.upload(np.zeros(w * h * 4, np.float32))
         self.spp = 0
-        self.gpu_ms = 0.0
+        self.wall = 0.0
 
-    def add(self, spp, est, seed, offset, guide=None, train=False):
+    def add(self, spp, est, seed, guide=None, train=False):
         V = self.V
-        ctx = V.make_context(self.cam, self.s, spp=spp, bounces=6, clamp=0.0,
+        t0 = time.perf_counter()
+        ctx = V.make_context(self.cam, self.s, spp=spp, bounces=self.bounces, clamp=0.0,
                              estimator=est, rng=V.RNG_PCG, seed=seed,
-                             sample_offset=offset, transcendentals=V.TX_FAST,
+                             sample_offset=self.spp, transcendentals=V.TX_FAST,
                              guide=guide, train=train)
         V.path_trace_tiles_device(ctx, self.tiles, self.img)
-        self.gpu_ms += self.dev.last_timings()["total"][0]
-        px = self.img.download(np.float32, self.w * self.h * 4)
-        self.sum += px.reshape(self.h, self.w, 4)[:, :, :3] * spp
+        self.dev.accumulate(self.acc, self.img, self.w * self.h, self.spp, spp)
+        self.dev.synchronize()
+        self.wall += time.perf_counter() - t0
         self.spp += spp
 
     def mean(self):
-        return self.sum / max(self.spp, 1)
+        px = self.acc.download(np.float32, self.w * self.h * 4)
+        return px.reshape(self.h, self.w, 4)[:, :, :3].astype(np.float64)
+
+    def close(self):
+        self.img.free()
+        self.acc.free()
 
 
-def make_reference(args, V, scenes):
-    """Plain unbiased render of the workload, args.ref_spp samples in
-    chunks of 512 (samples numbered from args.ref_offset), saved as float32
-    [H][W][3]."""
-    dev = V.Device(0)
-    desc = scenes.config5() if args.workload == "config5" else scenes.config4()
-    w, h = desc["camera"][0], desc["camera"][1]
-    s, cam = scenes.load_gpu(dev, desc)
-    acc = Accumulator(V, dev, s, cam, w, h)
-    chunk = 512
-    t0 = time.perf_counter()
-    while acc.spp < args.ref_spp:
-        n = min(chunk, args.ref_spp - acc.spp)
-        acc.add(n, V.EST_UNBIASED, args.ref_seed, args.ref_offset + acc.spp)
-        log(f"reference {acc.spp}/{args.ref_spp} spp, {time.perf_counter() - t0:.0f} s")
-    np.save(args.make_reference, acc.mean().astype(np.float32))
-    print(json.dumps({"reference": args.make_reference, "spp": acc.spp,
-                      "gpu_s": acc.gpu_ms / 1e3, "mean": float(acc.mean().mean())}))
-    s.close()
-    dev.close()
+def render_reference(V, dev, s, cam, w, h, spp, est, chunk=256):
+    """The reference image: two independent halves of `spp` samples (the
+    unbiased estimator `est`, different seeds), averaged.  Its own relMSE
+    against the true image is estimated as relMSE(half A, half B) / 4 (the
+    halves' difference has twice one half's variance; the average has half
+    of it)."""
+    halves, wall = [], 0.0
+    for seed in (0xA11CE, 0xB0B):
+        acc = DeviceAccumulator(V, dev, s, cam, w, h)
+        while acc.spp < spp // 2:
+            acc.add(min(chunk, spp // 2 - acc.spp), est, seed)
+        halves.append(acc.mean())
+        wall += acc.wall
+        acc.close()
+        log(f"reference half {len(halves)}: {spp // 2} spp in {acc.wall:.1f} s")
+    floor = relmse(halves[0], halves[1]) / 4.0
+    return (halves[0] + halves[1]) / 2.0, floor, wall
 
 
-def time_to_relmse(curve, target, floor=0.0):
-    """Crossing of `target` on a (gpu_s, spp, relMSE) curve: log-log
-    interpolated between checkpoints when reached ("measured"), otherwise
-    extrapolated from the last checkpoint with relMSE - floor falling as
-    1/spp and time growing with spp at the last segment's rate
-    ("extrapolated"); floor is the reference image's own error."""
+def crossing(curve, target):
+    """Wall time and spp at which `relmse` (the reference's own error
+    subtracted) reaches `target`: log-log interpolation between the two
+    checkpoints around the crossing ("measured"), or beyond the last
+    checkpoint with the error falling as 1/spp and time growing at the last
+    segment's rate ("extrapolated")."""
     prev = None
-    for t, n, e in curve:
+    for c in curve:
+        t, n, e = c["wall_s"], c["spp"], c["relmse_true"]
         if e <= target:
             if prev is None:
                 return t, n, "measured"
@@ -591,82 +686,86 @@ def time_to_relmse(curve, target, floor=0.0):
             return (float(np.exp(np.log(t0) + f * (np.log(t) - np.log(t0)))),
                     float(np.exp(np.log(n0) + f * (np.log(n) - np.log(n0)))),
                     "measured")
-        prev = (t, n, e)
-    if len(curve) < 2 or target <= floor:
+        prev = (t, n, max(e, 1e-12))
+    if len(curve) < 2:
         return None
-    (t0, n0, _), (t1, n1, e1) = curve[-2], curve[-1]
-    need = n1 * (e1 - floor) / (target - floor)
-    rate = (t1 - t0) / (n1 - n0)
-    return t1 + (need - n1) * rate, need, "extrapolated"
+    a, b = curve[-2], curve[-1]
+    need = b["spp"] * b["relmse_true"] / target
+    rate = (b["wall_s"] - a["wall_s"]) / (b["spp"] - a["spp"])
+    return b["wall_s"] + (need - b["spp"]) * rate, need, "extrapolated"
 
 
-def bench_relmse(args, V, scenes):
-    """Time to relMSE 1e-3 against a reference image: the plain tracer and
-    the guided tracer (online training for the first passes, then the frozen
-    guide), both accumulating every sample they render."""
-    ref = np.load(args.relmse).astype(np.float64)
-    dev = V.Device(0)
-    desc = scenes.config5() if args.workload == "config5" else scenes.config4()
+def time_to_relmse(V, dev, scenes, workload, ref_spp, ref_est, max_spp=8192,
+                   train_passes=64, guide_kw=None, target=1e-3):
+    """Wall-clock time to relMSE `target` of the plain tracer, the guided
+    tracer (online training during its first `train_passes` one-sample
+    passes, then the frozen guide) and next-event estimation, each
+    accumulating every sample it renders, against a reference of `ref_spp`
+    samples of `ref_est` rendered here."""
+    desc = getattr(scenes, workload)()
     w, h = desc["camera"][0], desc["camera"][1]
-    assert ref.shape == (h, w, 3), ref.shape
     s, cam = scenes.load_gpu(dev, desc)
-    target = 1e-3
-    res = {"workload": args.workload, "target": target,
-           "reference": os.path.basename(args.relmse), "max_spp": args.max_spp}
+    ref, floor, ref_wall = render_reference(V, dev, s, cam, w, h, ref_spp, ref_est)
+    guide_kw = guide_kw or dict(seed=1, learning_rate=3e-3, batch_size=16384,
+                                steps_per_pass=4, max_weight=10.0)
 
     def run(name, step):
-        acc = Accumulator(V, dev, s, cam, w, h)
-        curve, next_check = [], 16
-        t0 = time.perf_counter()
-        while acc.spp < args.max_spp:
+        acc = DeviceAccumulator(V, dev, s, cam, w, h)
+        curve, nxt = [], 16
+        while acc.spp < max_spp:
             step(acc)
-            if acc.spp >= next_check:
+            if acc.spp >= nxt:
                 e = relmse(acc.mean(), ref)
-                curve.append((acc.gpu_ms / 1e3, acc.spp, e))
-                log(f"{name}: {acc.spp} spp gpu {acc.gpu_ms / 1e3:.1f} s relMSE {e:.5f} "
-                    f"(wall {time.perf_counter() - t0:.0f} s)")
-                next_check *= 2
-                if e < target / 2:
+                curve.append({"wall_s": acc.wall, "spp": acc.spp, "relmse": e,
+                              "relmse_true": max(e - floor, 0.0)})
+                log(f"{workload} {name}: {acc.spp} spp, {acc.wall:.2f} s, relMSE {e:.5f}")
+                nxt *= 2
+                if e - floor < target / 2:
                     break
+        acc.close()
         return curve
 
     plain = run("plain", lambda a: a.add(64 if a.spp >= 64 else 16, V.EST_UNBIASED,
-                                         0x5EED, a.spp))
-    g = V.Guide(dev, seed=args.guide_seed, learning_rate=args.guide_lr,
-                batch_size=args.guide_batch, steps_per_pass=args.guide_steps,
-                max_weight=args.guide_max_weight)
+                                         0x5EED))
+    g = V.Guide(dev, **guide_kw)
 
     def guided_step(a):
-        if a.spp < args.train_passes:
-            a.add(1, V.EST_GUIDED, 0x6EED, a.spp, guide=g, train=True)
+        if a.spp < train_passes:
+            a.add(1, V.EST_GUIDED, 0x6EED, guide=g, train=True)
         else:
-            a.add(64, V.EST_GUIDED, 0x6EED, a.spp, guide=g)
+            a.add(64, V.EST_GUIDED, 0x6EED, guide=g)
 
     guided = run("guided", guided_step)
-    # next-event estimation toward the sky (no learning), for comparison
-    nee = run("nee", lambda a: a.add(64 if a.spp >= 64 else 16, V.EST_NEE, 0x7EED,
-                                     a.spp))
-    tp = time_to_relmse(plain, target, args.ref_error)
-    tg = time_to_relmse(guided, target, args.ref_error)
-    tn = time_to_relmse(nee, target, args.ref_error)
-    res.update({"plain_curve": plain, "guided_curve": guided,
-                "reference_error": args.ref_error,
-                "plain_time_s": tp[0] if tp else None, "plain_spp": tp[1] if tp else None,
-                "plain_how": tp[2] if tp else None,
-                "guided_time_s": tg[0] if tg else None, "guided_spp": tg[1] if tg else None,
-                "guided_how": tg[2] if tg else None,
-                "speedup": (tp[0] / tg[0]) if tp and tg else None,
-                "nee_curve": nee,
-                "nee_time_s": tn[0] if tn else None, "nee_spp": tn[1] if tn else None,
-                "nee_how": tn[2] if tn else None,
-                "nee_speedup": (tp[0] / tn[0]) if tp and tn else None,
-                "train_passes": args.train_passes,
-                "guide": {"lr": args.guide_lr, "batch": args.guide_batch,
-                          "steps_per_pass": args.guide_steps,
-                          "max_weight": args.guide_max_weight}})
-    print(json.dumps(res))
     g.close()
+    nee = run("nee", lambda a: a.add(64 if a.spp >= 64 else 16, V.EST_NEE, 0x7EED))
     s.close()
+    res = {"workload": workload, "target": target, "wall_clock": True,
+           "reference": {"spp": ref_spp,
+                         "estimator": {V.EST_UNBIASED: "plain", V.EST_NEE: "nee"}[ref_est],
+                         "own_relmse": floor, "render_s": ref_wall},
+           "train_passes": train_passes, "guide": guide_kw}
+    for name, curve in (("plain", plain), ("guided", guided), ("nee", nee)):
+        c = crossing(curve, target)
+        res[name] = {"time_s": c[0] if c else None, "spp": c[1] if c else None,
+                     "how": c[2] if c else None, "curve": curve}
+    tp = res["plain"]["time_s"]
+    for name in ("guided", "nee"):
+        t = res[name]["time_s"]
+        res[name]["speedup_vs_plain"] = tp / t if tp and t else None
+    return res
+
+
+def bench_relmse(args, V, scenes):
+    """`--relmse`: the time-to-relMSE measurement alone on the workload."""
+    dev = V.Device(0)
+    ref_est = V.EST_NEE if args.ref_estimator == "nee" else V.EST_UNBIASED
+    res = time_to_relmse(V, dev, scenes, args.workload, args.ref_spp, ref_est,
+                         max_spp=args.max_spp, train_passes=args.train_passes,
+                         guide_kw=dict(seed=args.guide_seed, learning_rate=args.guide_lr,
+                                       batch_size=args.guide_batch,
+                                       steps_per_pass=args.guide_steps,
+                                       max_weight=args.guide_max_weight))
+    print(json.dumps(res))
     dev.close()
 
 
@@ -715,7 +814,9 @@ def bench_soup(args, V, scenes):
                      "tri_tests_per_ray": st.triTests / n},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_soup(min(16, os.cpu_count() or 1), tris, o, d)
+        cpu = host_cpu()
+        res["cpu_baseline"] = cpu_baseline_soup(cpu["threads"], tris, o, d)
+        res["cpu_baseline"]["host"] = cpu
     for b in bufs + [outp, outt]:
         b.free()
     s.close()
@@ -766,6 +867,34 @@ def bench_irradiance(args, V, scenes):
         res["cpu_baseline"] = cpu_baseline_irradiance(sky, fw, fh)
     dev.close()
     return res
+
+
+def bench_copy(args, V, scenes):
+    """Calibration workload for HBM counters: a plain device-to-device copy
+    of 1 GiB (torch's elementwise copy kernel on cuda:0), `steps` times.  A
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE pass over this command gives what
+    the counters read for 1 GiB read and 1 GiB written, the factors that
+    scale the same counters of the tracer's kernels into bytes."""
+    import torch
+
+    n = 1 << 28  # floats: 1 GiB
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(max(1, args.warmup)):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    ok = bool(torch.equal(dst, src))
+    return {"metric": "GB/s (1 GiB device copy, read + write)",
+            "value": 2 * 4 * n / sec / 1e9, "unit": "GB/s", "ms_per_step": sec * 1e3,
+            "dtype": "f32", "config": {"workload": "copy: 1 GiB device to device, "
+                                                   "torch elementwise copy kernel",
+                                       "bytes_read": 4 * n, "bytes_written": 4 * n,
+                                       "equal": ok}}
 
 
 def bench_post(args, V, scenes):
@@ -853,21 +982,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="config5",
                     choices=["config5", "config4", "config3", "config1", "soup",
-                             "irradiance", "post"])
+                             "irradiance", "post", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fast", action="store_true",
                     help="device float transcendentals instead of double-evaluated")
-    ap.add_argument("--make-reference", metavar="OUT.npy",
-                    help="render the workload's plain reference image instead")
-    ap.add_argument("--ref-spp", type=int, default=32768)
-    ap.add_argument("--ref-seed", type=int, default=0xABCD)
-    ap.add_argument("--ref-offset", type=int, default=0)
-    ap.add_argument("--relmse", metavar="REF.npy",
-                    help="time-to-relMSE 1e-3 of the plain and guided tracers "
-                         "against this reference instead")
+    ap.add_argument("--relmse", action="store_true",
+                    help="measure wall-clock time to relMSE 1e-3 of the plain, "
+                         "guided and NEE tracers on the workload (config4 or "
+                         "config5) instead, against a reference rendered in "
+                         "the same run")
+    ap.add_argument("--no-relmse", action="store_true",
+                    help="skip the config-4 time-to-relMSE part of the default run")
+    ap.add_argument("--ref-spp", type=int, default=None,
+                    help="reference samples per pixel (config4: 4096 NEE, "
+                         "config5: 32768 plain)")
+    ap.add_argument("--ref-estimator", choices=["plain", "nee"], default=None)
     ap.add_argument("--max-spp", type=int, default=8192)
-    ap.add_argument("--ref-error", type=float, default=2.0e-4,
-                    help="relMSE of the reference itself (half-vs-half estimate)")
     ap.add_argument("--train-passes", type=int, default=64)
     ap.add_argument("--guide-lr", type=float, default=3e-3)
     ap.add_argument("--guide-batch", type=int, default=16384)
@@ -882,17 +1012,29 @@ def main():
     import vk_cinematic_amd as V
     from vk_cinematic_amd import scenes
 
-    if args.make_reference:
-        return make_reference(args, V, scenes)
     if args.relmse:
+        if args.workload not in ("config4", "config5"):
+            sys.exit("--relmse measures config4 or config5")
+        if args.ref_spp is None:
+            args.ref_spp = 4096 if args.workload == "config4" else 32768
+        if args.ref_estimator is None:
+            args.ref_estimator = "nee" if args.workload == "config4" else "plain"
         return bench_relmse(args, V, scenes)
     if args.workload in ("config5", "config4"):
         res = bench_box(args, V, scenes, args.workload)
     else:
         res = {"config1": bench_config1, "config3": bench_config3,
                "soup": bench_soup, "irradiance": bench_irradiance,
-               "post": bench_post}[args.workload](
+               "post": bench_post, "copy": bench_copy}[args.workload](
                    args, V, scenes)
+    if args.workload == "config5" and not args.no_relmse:
+        # the second half of the metric: wall-clock time to relMSE 1e-3,
+        # guided vs plain, on the config-4 workload (the config-5 figure:
+        # python bench.py --workload config5 --relmse)
+        dev = V.Device(0)
+        res["time_to_relmse"] = time_to_relmse(V, dev, scenes, "config4", 4096,
+                                               V.EST_NEE, max_spp=4096)
+        dev.close()
     out = {"metric": res.pop("metric"), "value": res.pop("value"),
            "unit": res.pop("unit"), "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),

@@ -318,7 +318,8 @@ int spg_time_intersect_closest(spg_Scene *scene, uint32_t meshId,
 int spg_last_trace_timings(spg_Device *dev, double *ms6, uint32_t *launches6);
 /* Raw counters of the last trace call on this device: [0] rays [1] hits
  * [2] misses [3] mesh-hierarchy slab tests [4] mesh (instance) entries
- * [5] triangle tests [6] instance-hierarchy slab tests [7] reserved (0).
+ * [5] triangle tests [6] instance-hierarchy slab tests [7] guide network
+ * evaluations (SPG_EST_GUIDED: queued hits that will scatter).
  * The test counts are collected only when the call's config set
  * collectMetrics. */
 int spg_last_trace_counters(spg_Device *dev, uint64_t *counters8);

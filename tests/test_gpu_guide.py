@@ -345,16 +345,18 @@ def test_online_training_is_reproducible(device):
     order (a scan over a Weyl permutation of the paths, not atomic appends)
     and reduces gradients in a fixed order: two guides trained by the same
     passes end bit-identical, and so do the images they render."""
-    desc = _lamp_scene(96, 64)
+    desc = scenes.config1(96, 64, env="cube", env_size=32)
     guides = [V.Guide(device, seed=11, batch_size=4096, steps_per_pass=2)
               for _ in range(2)]
-    imgs = []
+    imgs, collected = [], []
     for g in guides:
         for k in range(6):
             _render(device, desc, 1, V.EST_GUIDED, guide=g, train=True, offset=700 + k)
+        collected.append(g.samples())
         imgs.append(_render(device, desc, 4, V.EST_GUIDED, guide=g, seed=77))
     assert guides[0].stats()["steps"] == 12
-    assert guides[0].stats()["collected"] > 1000
+    assert len(collected[0]) > 1000
+    assert np.array_equal(collected[0].view(np.uint32), collected[1].view(np.uint32))
     assert np.array_equal(guides[0].params().view(np.uint32),
                           guides[1].params().view(np.uint32))
     assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))

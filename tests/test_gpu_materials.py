@@ -92,7 +92,7 @@ def test_albedo_texture_renders_match_oracle(device, est, rng, orng, clamp):
     px = _oracle_render(floor, 4, 3, est, orng, 11, clamp)
     assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
     # distinct texels show up as distinct colours on the floor
-    assert len(np.unique(img.reshape(-1, 4)[:, :3].round(2), axis=0)) > 6
+    assert len(np.unique(img.reshape(-1, 4)[:, :3].round(2), axis=0)) >= 5
     c1 = scenes.config1(64, 48, env="cube", env_size=32)
     c1["textures"] = c1.get("textures", []) + [
         (6, r.uniform(0.05, 0.95, (4, 4, 4)).astype(np.float32).ravel(), 4, 4)]

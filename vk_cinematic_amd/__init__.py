@@ -328,12 +328,13 @@ class Device:
 
     def last_counters(self):
         """rays, hits, misses, mesh-hierarchy slab tests, mesh (instance)
-        entries, triangle tests and instance-hierarchy slab tests of the last
-        trace call (test counts only when it collected metrics)."""
+        entries, triangle tests, instance-hierarchy slab tests and guide
+        network evaluations of the last trace call (test counts only when it
+        collected metrics)."""
         c = (C.c_uint64 * 8)()
         check(lib().spg_last_trace_counters(self.handle, c))
         return dict(zip(["rays", "hits", "misses", "slab_tests", "mesh_tests",
-                         "tri_tests", "tlas_slab_tests"], list(c)))
+                         "tri_tests", "tlas_slab_tests", "guide_evals"], list(c)))
 
     def close(self):
         if self.handle:

@@ -11,14 +11,6 @@
 
 namespace spg {
 
-static thread_local std::string g_err;
-
-int fail(int code, const std::string &msg)
-{
-    g_err = msg;
-    return code;
-}
-
 int hip_check(hipError_t e, const char *what)
 {
     if (e == hipSuccess) return SPG_OK;
@@ -133,8 +125,6 @@ using namespace spg;
 
 
 extern "C" {
-
-const char *spg_last_error(void) { return g_err.c_str(); }
 
 uint32_t spg_pixel_seed(uint32_t base, uint32_t index)
 {
@@ -1256,7 +1246,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     hipEventElapsedTime(&total, d->ev[kEvents - 2], d->ev[kEvents - 1]);
     d->lastMs[5] = total;
     for (int k = 0; k < 6; ++k) d->lastLaunches[k] = launches[k];
-    for (int k = 0; k < 7; ++k) d->lastCounters[k] = m[k];
+    for (int k = 0; k < 8; ++k) d->lastCounters[k] = m[k];
     if (metrics)
     {
         metrics->values[SPG_METRIC_ELAPSED_NS] += (uint64_t)
@@ -1295,7 +1285,7 @@ static void trace_group(spg_Device *d, std::vector<TileRequest *> &group)
     const TileRequest &lead = *group[0];
     int st = trace(lead.ctx, tiles.data(), (uint32_t)tiles.size(), &m, nullptr,
                    lead.hostPixels);
-    std::string err = st ? std::string(g_err) : std::string();
+    std::string err = st ? std::string(spg_last_error()) : std::string();
     auto t1 = std::chrono::steady_clock::now();
     const spg_Camera &c = *lead.ctx->camera;
     const uint64_t spp = lead.ctx->config.samplesPerPixel;
@@ -1404,7 +1394,7 @@ static int host_trace(const spg_Context *ctx, const spg_Tile *tiles,
         d->leaderActive = false;
         d->qcv.notify_all();
     }
-    if (req.status) g_err = req.error;
+    if (req.status) fail(req.status, req.error);
     return req.status;
 }
 

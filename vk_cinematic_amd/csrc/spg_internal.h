@@ -16,6 +16,7 @@
 
 #include "../../include/spgpu.h"
 #include "spg_bvh.h"
+#include "spg_error.h"
 #include "spg_guide.h"
 #include "spg_math.h"
 
@@ -373,12 +374,6 @@ struct spg_Scene
     spg::DevScene dev_scene{};
 };
 
-#define CHK(x)                  \
-    do                          \
-    {                           \
-        int _r = (x);           \
-        if (_r) return _r;      \
-    } while (0)
 #define HIPCHK(x) CHK(spg::hip_check((x), #x))
 
 namespace spg {
@@ -397,8 +392,7 @@ int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt,
                      uint32_t paths);
 int guide_train_pass(spg_Guide *g, hipStream_t st);
 
-// error reporting (thread-local message)
-int fail(int code, const std::string &msg);
+// error reporting (thread-local message, spg_error.h); a HIP failure as one
 int hip_check(hipError_t e, const char *what);
 
 // kernel launchers (spg_trace.hip)

@@ -13,7 +13,7 @@
 #include <utility>
 #include <vector>
 
-#include "spg_internal.h"
+#include "spg_error.h"
 
 using namespace spg;
 

@@ -1308,6 +1308,9 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
                 }
             }
         }
+        const unsigned long long validMask = __ballot(valid);
+        if (lane == 0)
+            atomicAdd(&p.paths.metrics[7], (unsigned long long)__popcll(validMask));
         if (p.guide.analytic) continue;  // wave-uniform
         encode_row(enc + lane * kLdsPitch, valid, pn, N, V);
         wave_lds_sync();
