@@ -353,8 +353,16 @@ typedef struct {
     uint32_t graphNodes;         /* kernels in the captured graph */
 } spg_BatchTuning;
 
-/* cfg: samplesPerPixel, bounces, radianceClamp, estimator (parity or
- * unbiased), rngMode, seed, transcendentals, sampleOffset, collectMetrics. */
+/* cfg: samplesPerPixel, bounces, radianceClamp, estimator, rngMode, seed,
+ * transcendentals, sampleOffset, collectMetrics; guide and guideTrain for
+ * SPG_EST_GUIDED.  With the guided estimator every item must trace the same
+ * scene (one guide learns one scene); guide inference at every bounce and,
+ * with guideTrain, the pass's sample collection and training steps are
+ * part of the captured graph, so each replay trains the guide further.
+ * The guide is pinned while the batch lives: spg_guide_destroy fails and
+ * its training buffers cannot grow (spg_guide_train with more samples than
+ * the batch's passes collect fails).  The autotuner's timing runs do not
+ * train. */
 int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
                      const spg_TraceConfig *cfg, int autotune, spg_Batch **out);
 /* Replays the render; metrics (may be NULL) as sp_PathTraceTile's. */

@@ -204,3 +204,110 @@ def test_full_config3_batch(device):
     assert np.array_equal(got[k].view(np.uint32), want.view(np.uint32))
     b.close()
     _close(loaded)
+
+
+def test_nee_batch_items_equal_single_scene_renders(device):
+    """Next-event estimation in the batch (shadow rays of every item through
+    the batch variant of the extension kernel): each item equals its
+    single-scene NEE render bit for bit."""
+    descs = scenes.config3(count=3, seed=17, min_side=48, max_side=120,
+                           min_tris=2e4, max_tris=1e5)
+    loaded = _load(device, descs)
+    kw = dict(spp=2, bounces=5, clamp=0.0, estimator=V.EST_NEE, rng=V.RNG_PCG,
+              seed=0x5EED, transcendentals=V.TX_DOUBLE)
+    b = V.Batch(device, [(cam, s, img) for s, cam, img, _, _ in loaded],
+                autotune=False, **kw)
+    m = b.run()
+    got = _images(loaded)
+    for (s, cam, _, w, h), g in zip(loaded, got):
+        want = _single(device, s, cam, w, h, **kw)
+        assert np.array_equal(g.view(np.uint32), want.view(np.uint32))
+    assert m.values[V.METRIC_RAYS] == m.values[V.METRIC_HITS] + m.values[V.METRIC_MISSES]
+    b.close()
+    _close(loaded)
+
+
+def _guided_setup(device, w, h):
+    desc = scenes.config1(w, h, env="cube", env_size=32)
+    s, cam = scenes.load_gpu(device, desc)
+    return desc, s, cam
+
+
+def test_guided_batch_frozen_equals_eager(device):
+    """The guided estimator in the graph-captured batch with a frozen guide
+    (inference every bounce, no training): two cameras on one scene, each
+    item equal to the eager guided render bit for bit."""
+    desc, s, cam = _guided_setup(device, 96, 64)
+    cam2 = scenes.sp_ConfigureCamera(64, 80, (1.0, 1.5, 4.0), (0, 0, 0, 1), 0.8)
+    g = V.Guide(device, seed=3, batch_size=4096, steps_per_pass=2)
+    for k in range(4):  # a few training passes so the mixture is not the initial one
+        V.path_trace_tiles(V.make_context(cam, s, spp=1, bounces=4, clamp=0.0,
+                                          estimator=V.EST_GUIDED, rng=V.RNG_PCG,
+                                          seed=9, sample_offset=k, guide=g, train=True),
+                           V.ComputeTiles(96, 64, 64, 64), np.zeros(96 * 64 * 4, np.float32))
+    kw = dict(spp=3, bounces=4, clamp=0.0, estimator=V.EST_GUIDED, rng=V.RNG_PCG,
+              seed=0x77, transcendentals=V.TX_DOUBLE, guide=g)
+    imgs = [device.alloc(96 * 64 * 16), device.alloc(64 * 80 * 16)]
+    b = V.Batch(device, [(cam, s, imgs[0]), (cam2, s, imgs[1])], autotune=False, **kw)
+    b.run()
+    for c, img, w, h in ((cam, imgs[0], 96, 64), (cam2, imgs[1], 64, 80)):
+        got = img.download(np.float32, w * h * 4)
+        want = _single(device, s, c, w, h, **kw)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        assert np.isfinite(got).all()
+    # the guide is pinned while the batch lives
+    with pytest.raises(V.SpgError):
+        V.check(V.lib().spg_guide_destroy(g.handle))
+    b.close()
+    for img in imgs:
+        img.free()
+    g.close()
+    s.close()
+
+
+def test_guided_batch_training_replays_equal_eager_training(device):
+    """Online training inside the captured graph: each replay is one guided
+    pass with its sample collection and Adam steps.  Replaying it R times
+    leaves the guide bit-identical to R eager training traces of the same
+    slots (one tile over the image, sample offset 0), and the images of the
+    last pass equal."""
+    desc, s, cam = _guided_setup(device, 96, 64)
+    w, h = 96, 64
+    kw = dict(spp=1, bounces=4, clamp=0.0, estimator=V.EST_GUIDED, rng=V.RNG_PCG,
+              seed=0x99, transcendentals=V.TX_FAST)
+    ga = V.Guide(device, seed=5, batch_size=2048, steps_per_pass=2)
+    gb = V.Guide(device, seed=5, batch_size=2048, steps_per_pass=2)
+    img = device.alloc(w * h * 16)
+    b = V.Batch(device, [(cam, s, img)], autotune=False, guide=ga, train=True, **kw)
+    assert ga.stats()["steps"] == 0       # creating the batch did not train
+    eager = device.alloc(w * h * 16)
+    for _ in range(5):
+        b.run()
+        V.path_trace_tiles_device(V.make_context(cam, s, guide=gb, train=True, **kw),
+                                  [V.Tile(0, 0, w, h)], eager)
+    assert ga.stats()["steps"] == gb.stats()["steps"] == 10
+    assert np.array_equal(ga.params().view(np.uint32), gb.params().view(np.uint32))
+    a = img.download(np.float32, w * h * 4)
+    e = eager.download(np.float32, w * h * 4)
+    assert np.array_equal(a.view(np.uint32), e.view(np.uint32))
+    b.close()
+    for x in (img, eager):
+        x.free()
+    ga.close()
+    gb.close()
+    s.close()
+
+
+def test_guided_batch_needs_one_scene(device):
+    d1, s1, c1 = _guided_setup(device, 32, 32)
+    d2, s2, c2 = _guided_setup(device, 32, 32)
+    g = V.Guide(device, seed=1)
+    img = [device.alloc(32 * 32 * 16) for _ in range(2)]
+    with pytest.raises(V.SpgError):
+        V.Batch(device, [(c1, s1, img[0]), (c2, s2, img[1])], estimator=V.EST_GUIDED,
+                guide=g, autotune=False)
+    for x in img:
+        x.free()
+    g.close()
+    s1.close()
+    s2.close()

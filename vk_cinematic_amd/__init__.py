@@ -225,7 +225,10 @@ def _release_all():
     # Release GPU objects while the HIP runtime is still up: scenes and
     # guides first, then devices (interpreter teardown would otherwise order
     # them freely).
-    for s in list(_live_scenes):
+    live = list(_live_scenes)
+    for s in [x for x in live if isinstance(x, Batch)]:  # batches pin guides
+        s.close()
+    for s in live:
         s.close()
     for d in list(_live_devices):
         d.close()
@@ -549,18 +552,21 @@ class Batch:
     def __init__(self, device, items, spp=16, bounces=5, clamp=0.0,
                  estimator=EST_UNBIASED, rng=RNG_PCG, seed=0x5EED,
                  transcendentals=TX_FAST, sample_offset=0, collect=False,
-                 autotune=True):
+                 autotune=True, guide=None, train=False):
         self.device = device
         arr = (BatchItem * len(items))()
         for k, (cam, scene, img) in enumerate(items):
             arr[k] = BatchItem(C.pointer(cam), scene.handle, img.ptr)
+        if guide is not None and estimator == EST_UNBIASED:
+            estimator = EST_GUIDED
         cfg = TraceConfig(spp, bounces, clamp, estimator, rng, seed, transcendentals,
-                          sample_offset, int(bool(collect)), 0, None)
+                          sample_offset, int(bool(collect)), int(bool(train)),
+                          guide.handle if guide is not None else None)
         h = C.c_void_p()
         check(lib().spg_batch_create(device.handle, C.byref(arr), len(items),
                                      C.byref(cfg), int(bool(autotune)), C.byref(h)))
         self.handle = h
-        self._keep = items
+        self._keep = (items, guide)
         _live_scenes.add(self)
 
     def run(self, metrics=None):

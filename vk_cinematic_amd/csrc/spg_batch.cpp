@@ -29,6 +29,11 @@ struct spg_Batch
     void *pathsBlock = nullptr;
     size_t pathsBytes = 0;
     void *accum = nullptr;
+    // guided estimator: the guide (pinned while the batch lives), the
+    // batch's own per-path guide buffers and the guide view of the kernels
+    spg_Guide *guide = nullptr;
+    void *guideBlock = nullptr;
+    DevGuide dg{};
     uint32_t maxBlocks = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -49,10 +54,14 @@ int alloc_state(spg_Batch *b, uint32_t capacity)
 {
     free_dev(b->pathsBlock);
     free_dev(b->accum);
+    free_dev(b->guideBlock);
     memset(&b->paths, 0, sizeof b->paths);
     CHK(alloc_path_state(b->paths, b->pathsBlock, b->pathsBytes, capacity,
                          b->cfg.bounces));
     HIPCHK(hipMalloc(&b->accum, (size_t)3 * b->paths.capacity * sizeof(float)));
+    if (b->guide)
+        CHK(guide_prepare_trace(b->guide, b->scenes[0], b->paths.capacity, b->cfg.bounces,
+                                b->cfg.guideTrain != 0, b->dg, &b->guideBlock));
     return SPG_OK;
 }
 
@@ -120,6 +129,14 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
     p.images = b->dImages;
     Persist ps = b->dev->persist;
     ps.maxBlocks = b->maxBlocks;
+    const bool guided = cfg.estimator == SPG_EST_GUIDED;
+    const bool nee = cfg.estimator == SPG_EST_NEE;
+    const bool train = guided && cfg.guideTrain;
+    if (guided)
+    {
+        p.guide = b->dg;
+        p.guide.train = train ? 1u : 0u;
+    }
     uint32_t k = 0;
     HIPCHK(launch_clear_counters(st, (uint32_t *)b->paths.metrics, 32));
     ++k;
@@ -140,6 +157,15 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
             p.sppPass = sp;
             p.sampleBase = cfg.sampleOffset + done;
             HIPCHK(launch_set_counter(st, &p.paths.counters[0], nslots * sp));
+            if (guided)
+            {
+                // the same pass salt as an eager trace of these slots
+                CHK(guide_begin_pass(b->guide, st, p.guide,
+                                     (p.sampleBase + 1u) * 0x9E3779B9u ^ cfg.seed ^
+                                         (uint32_t)s0,
+                                     nslots * sp));
+                k += train ? 1 : 0;
+            }
             CHK(T0(kRaygen));
             HIPCHK(launch_raygen(st, p));
             CHK(T1());
@@ -151,16 +177,48 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
                 HIPCHK(launch_extend(st, p, q, ps));
                 CHK(T1());
                 HIPCHK(launch_set_counter(st, &p.paths.counters[q ^ 1], 0));
+                k += 3;  // the extend launch includes its counter reset
+                if (guided)
+                {
+                    CHK(T0(kGuide));
+                    HIPCHK(launch_guide_infer(st, p, q));
+                    CHK(T1());
+                    ++k;
+                }
+                if (nee)
+                {
+                    HIPCHK(launch_set_counter(st, &p.paths.counters[2], 0));
+                    ++k;
+                }
                 CHK(T0(kShade));
                 HIPCHK(launch_shade(st, p, q));
                 CHK(T1());
-                k += 4;  // the extend launch includes its counter reset
+                ++k;
+                if (nee && bb + 1 < cfg.bounces)
+                {
+                    // shadow rays of this bounce, then their contributions
+                    CHK(T0(kExtend));
+                    HIPCHK(launch_extend(st, p, 2, ps));
+                    CHK(T1());
+                    CHK(T0(kShade));
+                    HIPCHK(launch_resolve_shadow(st, p));
+                    CHK(T1());
+                    k += 3;
+                }
                 q ^= 1;
             }
             CHK(T0(kFinal));
             HIPCHK(launch_finalize(st, p));
             CHK(T1());
             ++k;
+            if (train)
+            {
+                CHK(T0(kGuide));
+                HIPCHK(launch_guide_collect(st, p));
+                CHK(guide_train_pass(b->guide, st));
+                CHK(T1());
+                k += 3 + 7 * b->guide->cfg.stepsPerPass;
+            }
             done += sp;
         }
         HIPCHK(launch_write_pixels(st, p, nullptr));
@@ -188,6 +246,7 @@ int autotune(spg_Batch *b, uint32_t cus)
     spg_TraceConfig one = b->cfg;
     one.samplesPerPixel = 1;
     one.collectMetrics = 0;
+    one.guideTrain = 0;  // timing runs leave the guide as it is
     std::vector<uint32_t> caps;
     for (uint32_t c : {1u << 20, 1u << 21, 1u << 22})
     {
@@ -233,6 +292,7 @@ void release(spg_Batch *b)
     free_dev(b->tables);
     free_dev(b->pathsBlock);
     free_dev(b->accum);
+    free_dev(b->guideBlock);
 }
 
 }  // namespace
@@ -247,8 +307,19 @@ int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
     const spg_TraceConfig &c = *cfg;
     if (c.samplesPerPixel == 0 || c.bounces == 0 || c.bounces > kMaxBounces)
         return fail(SPG_E_INVALID, "spg_batch_create: samplesPerPixel > 0, bounces 1..16");
-    if (c.estimator != SPG_EST_PARITY && c.estimator != SPG_EST_UNBIASED)
-        return fail(SPG_E_INVALID, "spg_batch_create: parity or unbiased estimator only");
+    if (c.estimator != SPG_EST_PARITY && c.estimator != SPG_EST_UNBIASED &&
+        c.estimator != SPG_EST_NEE && c.estimator != SPG_EST_GUIDED)
+        return fail(SPG_E_INVALID, "spg_batch_create: unknown estimator");
+    const bool guided = c.estimator == SPG_EST_GUIDED;
+    if (guided && (!c.guide || c.guide->dev != dev))
+        return fail(SPG_E_INVALID,
+                    "spg_batch_create: the guided estimator needs a guide on the device");
+    if (guided)
+        for (uint32_t k = 1; k < n; ++k)
+            if (items[k].scene != items[0].scene)
+                return fail(SPG_E_INVALID, "spg_batch_create: with the guided estimator "
+                                           "every item traces the same scene (one "
+                                           "guide learns one scene)");
     if (c.rngMode != SPG_RNG_XORSHIFT_PIXEL && c.rngMode != SPG_RNG_PCG)
         return fail(SPG_E_INVALID, "spg_batch_create: unknown rngMode");
     if (c.transcendentals != SPG_TX_DOUBLE && c.transcendentals != SPG_TX_FAST)
@@ -302,6 +373,7 @@ int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
     spg_Batch *b = new spg_Batch();
     b->dev = dev;
     b->cfg = c;
+    b->guide = guided ? c.guide : nullptr;
     b->n = n;
     b->slots = slots;
     for (uint32_t k = 0; k < n; ++k) b->scenes.push_back(items[k].scene);
@@ -350,6 +422,7 @@ int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
         // persistent grids outside the capture
         spg_TraceConfig one = c;
         one.samplesPerPixel = 1;
+        one.guideTrain = 0;  // leaves the guide as it is
         float ms = 0.0f;
         if (!r) r = timed(b, one, &ms);
         b->tuning.candidates = 1;
@@ -384,6 +457,7 @@ int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
     b->tuning.extendBlocks = b->maxBlocks;
     b->tuning.pixels = slots;
     b->tuning.graphNodes = kernels;
+    if (b->guide) b->guide->pins++;
     dev->liveScenes++;
     *out = b;
     return SPG_OK;
@@ -410,8 +484,7 @@ int spg_batch_run(spg_Batch *b, spg_Metrics *metrics)
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     b->lastMs = ms;
-    for (int k = 0; k < 7; ++k) d->lastCounters[k] = m[k];
-    d->lastCounters[7] = 0;
+    for (int k = 0; k < 8; ++k) d->lastCounters[k] = m[k];
     if (metrics)
     {
         metrics->values[SPG_METRIC_ELAPSED_NS] += (uint64_t)
@@ -465,8 +538,7 @@ int spg_batch_profile(spg_Batch *b, int collect, double *ms6, uint32_t *launches
     }
     if (counters8)
     {
-        for (int k = 0; k < 7; ++k) counters8[k] = m[k];
-        counters8[7] = 0;
+        for (int k = 0; k < 8; ++k) counters8[k] = m[k];
     }
     return SPG_OK;
 }
@@ -480,6 +552,7 @@ int spg_batch_destroy(spg_Batch *b)
         std::lock_guard<std::mutex> lk(d->lock);
         hipSetDevice(d->ordinal);
         hipStreamSynchronize(d->stream);
+        if (b->guide) b->guide->pins--;
         release(b);
         delete b;
         d->liveScenes--;

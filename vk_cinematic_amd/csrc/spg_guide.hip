@@ -387,12 +387,16 @@ __global__ __launch_bounds__(256) void k_guide_adam(GuideTrainArgs a, GuideAdam 
     float p = a.params[k];
     if (n > 0 && h.apply)
     {
+        // bias corrections 1 - beta^t in double, rounded once
+        const double t = (double)*h.step;
+        const float c1 = (float)(1.0 - pow((double)h.beta1, t));
+        const float c2 = (float)(1.0 - pow((double)h.beta2, t));
         float gr = a.grad[k] * (1.0f / ((float)n * kLossScale));
         float m = h.beta1 * a.m[k] + (1.0f - h.beta1) * gr;
         float v = h.beta2 * a.v[k] + (1.0f - h.beta2) * gr * gr;
         a.m[k] = m;
         a.v[k] = v;
-        p -= h.lr * (m / h.c1) / (sqrtf(v / h.c2) + h.eps);
+        p -= h.lr * (m / c1) / (sqrtf(v / c2) + h.eps);
         a.params[k] = p;
     }
     a.grad[k] = 0.0f;
@@ -439,6 +443,14 @@ hipError_t launch_guide_gradient(hipStream_t s, const GuideTrainArgs &a,
     hipLaunchKernelGGL(k_guide_wgrad, dim3(chunks, kGLayers), dim3(64), 0, s, a);
     hipLaunchKernelGGL(k_guide_greduce, dim3((kGParams + 255) / 256), dim3(256), 0, s,
                        a);
+    return hipGetLastError();
+}
+
+__global__ void k_guide_step(uint32_t *step) { *step += 1u; }
+
+hipError_t launch_guide_step(hipStream_t s, uint32_t *step)
+{
+    hipLaunchKernelGGL(k_guide_step, dim3(1), dim3(1), 0, s, step);
     return hipGetLastError();
 }
 

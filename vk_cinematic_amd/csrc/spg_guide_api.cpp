@@ -95,6 +95,9 @@ GuideTrainArgs make_args(spg_Guide *g, uint32_t first, uint32_t batch,
 int ensure_samples(spg_Guide *g, uint32_t cap)
 {
     if (g->samplesBlock && g->sampleCap >= cap) return SPG_OK;
+    if (g->pins)
+        return fail(SPG_E_STATE, "guide: a batch has captured its training buffers, "
+                                 "which cannot grow while it lives");
     free_dev(g->samplesBlock);
     g->sampleCap = 0;
     HIPCHK(hipMalloc(&g->samplesBlock, (size_t)kGSampleFloats * cap * sizeof(float)));
@@ -106,6 +109,9 @@ int ensure_samples(spg_Guide *g, uint32_t cap)
 int ensure_hg(spg_Guide *g, uint32_t batch)
 {
     if (g->hgBlock && g->batchCap >= batch) return SPG_OK;
+    if (g->pins)
+        return fail(SPG_E_STATE, "guide: a batch has captured its training buffers, "
+                                 "which cannot grow while it lives");
     free_dev(g->hgBlock);
     g->batchCap = 0;
     size_t half = (size_t)kGLayers * 64 * batch;
@@ -133,19 +139,26 @@ int upload_samples(spg_Guide *g, const float *rows, uint32_t n)
     return SPG_OK;
 }
 
-int adam_step(spg_Guide *g, const GuideTrainArgs &a)
+int adam_step(spg_Guide *g, const GuideTrainArgs &a, hipStream_t st)
 {
-    g->steps++;
     GuideAdam h{};
     h.lr = g->cfg.learningRate;
     h.beta1 = kBeta1;
     h.beta2 = kBeta2;
     h.eps = kAdamEps;
-    h.c1 = (float)(1.0 - std::pow((double)kBeta1, (double)g->steps));
-    h.c2 = (float)(1.0 - std::pow((double)kBeta2, (double)g->steps));
+    h.step = g->stepDev;
     h.apply = 1;
-    HIPCHK(launch_guide_adam(g->dev->stream, a, h));
+    HIPCHK(launch_guide_step(st, g->stepDev));
+    HIPCHK(launch_guide_adam(st, a, h));
     return SPG_OK;
+}
+
+uint32_t read_steps(spg_Guide *g)
+{
+    uint32_t n = 0;
+    hipMemcpyAsync(&n, g->stepDev, 4, hipMemcpyDeviceToHost, g->dev->stream);
+    hipStreamSynchronize(g->dev->stream);
+    return n;
 }
 
 uint32_t cus_of(spg_Device *d)
@@ -163,7 +176,7 @@ int set_params_locked(spg_Guide *g, const float *p)
     HIPCHK(hipMemsetAsync(g->m, 0, sizeof(float) * kGParams, d->stream));
     HIPCHK(hipMemsetAsync(g->v, 0, sizeof(float) * kGParams, d->stream));
     HIPCHK(hipMemsetAsync(g->grad, 0, sizeof(float) * kGParams, d->stream));
-    g->steps = 0;
+    HIPCHK(launch_set_counter(d->stream, g->stepDev, 0u));
     HIPCHK(launch_guide_convert(d->stream, make_args(g, 0, 64, false, 0.0f)));
     HIPCHK(hipStreamSynchronize(d->stream));
     return SPG_OK;
@@ -174,25 +187,53 @@ int set_params_locked(spg_Guide *g, const float *p)
 // ------------------------------------------------------------ tracer hooks
 namespace spg {
 
-int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
-                        uint32_t bounces, bool train, DevGuide &out)
+// Per-path buffers of the guided estimator (lobes per queued ray, vertex
+// records, the training-sample scan) for `capacity` paths in one
+// allocation *block; the pointers into it go to dg.
+static int alloc_path_buffers(void *&block, uint32_t capacity, uint32_t bounces,
+                              DevGuide &dg)
 {
-    if (!g->pathBlock || g->pathCap < capacity || g->pathBounces < bounces)
+    size_t off = 0;
+    size_t oL = take(off, (size_t)kGLobeFloats * capacity * 4);
+    size_t oV = take(off, (size_t)kGVertexFloats * bounces * capacity * 4);
+    size_t oC = take(off, (size_t)capacity * 4);
+    size_t oS = take(off, (size_t)kGScanBlocks * 4);
+    HIPCHK(hipMalloc(&block, off));
+    dg.lobes = (float *)((char *)block + oL);
+    dg.gvtx = (float *)((char *)block + oV);
+    dg.gcnt = (uint32_t *)((char *)block + oC);
+    dg.scanBase = (uint32_t *)((char *)block + oS);
+    return SPG_OK;
+}
+
+// ownBlock != nullptr: the per-path buffers are allocated there for the
+// caller (a batch, whose captured graph must not see them move) instead of
+// in the guide's own block.
+int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
+                        uint32_t bounces, bool train, DevGuide &out, void **ownBlock)
+{
+    DevGuide pb{};
+    if (ownBlock)
+        CHK(alloc_path_buffers(*ownBlock, capacity, bounces, pb));
+    else
     {
-        free_dev(g->pathBlock);
-        g->pathCap = g->pathBounces = 0;
-        size_t off = 0;
-        size_t oL = take(off, (size_t)kGLobeFloats * capacity * 4);
-        size_t oV = take(off, (size_t)kGVertexFloats * bounces * capacity * 4);
-        size_t oC = take(off, (size_t)capacity * 4);
-        size_t oS = take(off, (size_t)kGScanBlocks * 4);
-        HIPCHK(hipMalloc(&g->pathBlock, off));
-        g->lobes = (float *)((char *)g->pathBlock + oL);
-        g->gvtx = (float *)((char *)g->pathBlock + oV);
-        g->gcnt = (uint32_t *)((char *)g->pathBlock + oC);
-        g->scanBase = (uint32_t *)((char *)g->pathBlock + oS);
-        g->pathCap = capacity;
-        g->pathBounces = bounces;
+        if (!g->pathBlock || g->pathCap < capacity || g->pathBounces < bounces)
+        {
+            free_dev(g->pathBlock);
+            g->pathCap = g->pathBounces = 0;
+            DevGuide t{};
+            CHK(alloc_path_buffers(g->pathBlock, capacity, bounces, t));
+            g->lobes = t.lobes;
+            g->gvtx = t.gvtx;
+            g->gcnt = t.gcnt;
+            g->scanBase = t.scanBase;
+            g->pathCap = capacity;
+            g->pathBounces = bounces;
+        }
+        pb.lobes = g->lobes;
+        pb.gvtx = g->gvtx;
+        pb.gcnt = g->gcnt;
+        pb.scanBase = g->scanBase;
     }
     const uint32_t B = g->cfg.batchSize, steps = g->cfg.stepsPerPass;
     if (train)
@@ -210,10 +251,10 @@ int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
     DevGuide dg{};
     dg.w16 = g->w16;
     dg.bias = g->bias;
-    dg.lobes = g->lobes;
-    dg.gvtx = g->gvtx;
-    dg.gcnt = g->gcnt;
-    dg.scanBase = g->scanBase;
+    dg.lobes = pb.lobes;
+    dg.gvtx = pb.gvtx;
+    dg.gcnt = pb.gcnt;
+    dg.scanBase = pb.scanBase;
     dg.permMul = dg.permInv = 1;
     dg.samples = g->samples;
     dg.sampleCount = g->count;
@@ -290,7 +331,7 @@ int guide_train_pass(spg_Guide *g, hipStream_t st)
         GuideTrainArgs a = make_args(g, k * B, B, true, g->cfg.maxWeight);
         HIPCHK(launch_clear_floats(st, g->loss, 1));
         HIPCHK(launch_guide_gradient(st, a, cus));
-        CHK(adam_step(g, a));
+        CHK(adam_step(g, a, st));
     }
     return SPG_OK;
 }
@@ -332,7 +373,7 @@ int spg_guide_create(spg_Device *dev, const spg_GuideConfig *cfg, spg_Guide **ou
     const size_t P = (size_t)kGParams * 4;
     size_t oP = take(off, P), oM = take(off, P), oV = take(off, P), oG = take(off, P),
            oB = take(off, (size_t)kGRows * 4), oS = take(off, 16), oL = take(off, 16),
-           oC = take(off, 16), oW = take(off, (size_t)kGRows * 64 * 2),
+           oC = take(off, 16), oT0 = take(off, 16), oW = take(off, (size_t)kGRows * 64 * 2),
            oT = take(off, (size_t)kGLayers * 64 * 64 * 2);
     hipError_t e = hipMalloc(&g->block, off);
     if (e != hipSuccess)
@@ -349,6 +390,7 @@ int spg_guide_create(spg_Device *dev, const spg_GuideConfig *cfg, spg_Guide **ou
     g->wsum = (float *)(b + oS);
     g->loss = (float *)(b + oL);
     g->count = (uint32_t *)(b + oC);
+    g->stepDev = (uint32_t *)(b + oT0);
     g->w16 = (_Float16 *)(b + oW);
     g->wt16 = (_Float16 *)(b + oT);
     std::vector<float> p;
@@ -373,6 +415,8 @@ int spg_guide_destroy(spg_Guide *g)
     bool last;
     {
         std::lock_guard<std::mutex> lk(d->lock);
+        if (g->pins)
+            return fail(SPG_E_STATE, "spg_guide_destroy: a batch still uses the guide");
         hipSetDevice(d->ordinal);
         hipStreamSynchronize(d->stream);
         free_dev(g->block);
@@ -485,7 +529,7 @@ int spg_guide_train(spg_Guide *g, const float *samples, uint32_t n, uint32_t ste
         GuideTrainArgs a = make_args(g, first, B, true, g->cfg.maxWeight);
         HIPCHK(launch_clear_floats(d->stream, g->loss, 1));
         HIPCHK(launch_guide_gradient(d->stream, a, cus));
-        CHK(adam_step(g, a));
+        CHK(adam_step(g, a, d->stream));
     }
     float l = 0.0f;
     HIPCHK(hipMemcpyAsync(&l, g->loss, 4, hipMemcpyDeviceToHost, d->stream));
@@ -506,7 +550,7 @@ int spg_guide_stats(spg_Guide *g, uint32_t *collected, uint32_t *steps, float *l
     HIPCHK(hipMemcpyAsync(&l, g->loss, 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
     if (collected) *collected = c;
-    if (steps) *steps = g->steps;
+    if (steps) *steps = read_steps(g);
     if (loss)
     {
         uint32_t have = std::min(c, g->sampleCap);
@@ -611,7 +655,7 @@ int spg_guide_save(spg_Guide *g, const char *path)
                                   sizeof(float) * kGParams, hipMemcpyDeviceToHost,
                                   g->dev->stream));
         HIPCHK(hipStreamSynchronize(g->dev->stream));
-        steps = g->steps;
+        steps = read_steps(g);
         cfg = g->cfg;
     }
     FILE *f = fopen(path, "wb");
@@ -653,10 +697,10 @@ int spg_guide_load(spg_Guide *g, const char *path)
         HIPCHK(hipMemcpyAsync(dst[k], st.data() + (size_t)k * kGParams,
                               sizeof(float) * kGParams, hipMemcpyHostToDevice, d->stream));
     HIPCHK(launch_clear_floats(d->stream, g->grad, kGParams));
+    HIPCHK(launch_set_counter(d->stream, g->stepDev, head[2]));
     HIPCHK(launch_guide_convert(d->stream, make_args(g, 0, 64, false, 0.0f)));
     HIPCHK(hipStreamSynchronize(d->stream));
     g->cfg = cfg;
-    g->steps = head[2];
     return SPG_OK;
 }
 

@@ -219,9 +219,13 @@ struct GuideTrainArgs
     float *loss;                 // += sum_j w_j (-log q_j)
 };
 
+// Adam hyperparameters; the step count t lives on the device (advanced by
+// the step's first launch, read by the update), so a captured training
+// sequence replays with the right bias corrections 1 - beta^t.
 struct GuideAdam
 {
-    float lr, beta1, beta2, eps, c1, c2;
+    float lr, beta1, beta2, eps;
+    uint32_t *step;
     uint32_t apply;
 };
 
@@ -299,6 +303,7 @@ struct spg_Guide
     float *bias = nullptr, *wsum = nullptr, *loss = nullptr;
     _Float16 *w16 = nullptr, *wt16 = nullptr;
     uint32_t *count = nullptr;        // training samples collected
+    uint32_t *stepDev = nullptr;      // Adam steps taken (device)
     // training samples, SoA [13][sampleCap]
     void *samplesBlock = nullptr;
     float *samples = nullptr;
@@ -313,7 +318,7 @@ struct spg_Guide
     float *lobes = nullptr, *gvtx = nullptr;
     uint32_t *gcnt = nullptr, *scanBase = nullptr;  // training-sample scan
     uint32_t pathCap = 0, pathBounces = 0;
-    uint32_t steps = 0;               // Adam steps taken
+    uint32_t pins = 0;                // live batches capturing this guide's buffers
     // verification hooks (spg_guide_debug)
     uint32_t analytic = 0;
     void *recBlock = nullptr;         // count (256 B), then the records
@@ -387,7 +392,8 @@ constexpr uint32_t kMaxBounces = 16;
 
 // guide hooks of the tracer (spg_guide_api.cpp)
 int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
-                        uint32_t bounces, bool train, DevGuide &out);
+                        uint32_t bounces, bool train, DevGuide &out,
+                        void **ownBlock = nullptr);
 int guide_begin_pass(spg_Guide *g, hipStream_t st, DevGuide &dg, uint32_t salt,
                      uint32_t paths);
 int guide_train_pass(spg_Guide *g, hipStream_t st);
@@ -447,4 +453,6 @@ hipError_t launch_guide_gradient(hipStream_t s, const GuideTrainArgs &a,
                                  uint32_t cus);
 hipError_t launch_guide_adam(hipStream_t s, const GuideTrainArgs &a,
                              const GuideAdam &h);
+// device Adam step count: += 1 before an update
+hipError_t launch_guide_step(hipStream_t s, uint32_t *step);
 }  // namespace spg

@@ -1265,6 +1265,7 @@ __global__ __launch_bounds__(1024) void k_guide_emit(TraceParams p)
 // per queue entry for k_shade.  Blocks of kInferWaves waves share the
 // weights in LDS; each wave evaluates 64 rays per step.
 constexpr int kInferWaves = 8;
+template <bool kBatch>
 __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
                                                                   int q)
 {
@@ -1295,8 +1296,9 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
                 v3 o{qo[i], qo[cap + i], qo[2 * cap + i]};
                 v3 d{qd[i], qd[cap + i], qd[2 * cap + i]};
                 v2 uv;
-                surface_at(p.scene, inst, p.paths.hprim[i], p.paths.hu[i],
-                           p.paths.hv[i], N, uv);
+                const DevScene &S =
+                    kBatch ? p.scenes[p.slotItem[p.paths.pathSlot[path]]] : p.scene;
+                surface_at(S, inst, p.paths.hprim[i], p.paths.hu[i], p.paths.hv[i], N, uv);
                 pn = guide_pn(p.guide, o + d * t);
                 V = -d;
                 if (p.guide.analytic)
@@ -1510,10 +1512,13 @@ hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
     static uint32_t gPlainB = persistent_grid(k_extend<false, true, false>);
     static uint32_t gCountS = persistent_grid(k_extend<true, false, true>);
     static uint32_t gPlainS = persistent_grid(k_extend<false, false, true>);
+    static uint32_t gCountBS = persistent_grid(k_extend<true, true, true>);
+    static uint32_t gPlainBS = persistent_grid(k_extend<false, true, true>);
     uint64_t n = (uint64_t)p.slots * p.sppPass;  // bound on the queue length
     const bool batch = p.slotItem != nullptr;
-    const bool shadow = q == 2 && !batch;
-    uint32_t g = batch    ? (p.collect ? gCountB : gPlainB)
+    const bool shadow = q == 2;
+    uint32_t g = batch ? (shadow ? (p.collect ? gCountBS : gPlainBS)
+                                 : (p.collect ? gCountB : gPlainB))
                  : shadow ? (p.collect ? gCountS : gPlainS)
                           : (p.collect ? gCount : gPlain);
     if (ps.maxBlocks && g > ps.maxBlocks) g = ps.maxBlocks;
@@ -1522,7 +1527,12 @@ hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
 #define SPG_EXT(C, B, S)                                                          \
     hipLaunchKernelGGL((k_extend<C, B, S>), grid, dim3(kTraceBlock), 0, s, p, q, \
                        ps.counter, ps.spill)
-    if (batch)
+    if (batch && shadow)
+    {
+        if (p.collect) SPG_EXT(true, true, true);
+        else SPG_EXT(false, true, true);
+    }
+    else if (batch)
     {
         if (p.collect) SPG_EXT(true, true, false);
         else SPG_EXT(false, true, false);
@@ -1551,15 +1561,25 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
     hipLaunchKernelGGL((k_shade<D, E, true>), dim3(g), dim3(256), 0, s, p, q)
     if (p.slotItem)
     {
-        if (p.estimator == SPG_EST_PARITY)
+        switch (p.estimator)
         {
+        case SPG_EST_PARITY:
             if (dbl) SPG_SHADE_B(true, SPG_EST_PARITY);
             else SPG_SHADE_B(false, SPG_EST_PARITY);
+            break;
+        case SPG_EST_UNBIASED:
+            if (dbl) SPG_SHADE_B(true, SPG_EST_UNBIASED);
+            else SPG_SHADE_B(false, SPG_EST_UNBIASED);
+            break;
+        case SPG_EST_NEE:
+            if (dbl) SPG_SHADE_B(true, SPG_EST_NEE);
+            else SPG_SHADE_B(false, SPG_EST_NEE);
+            break;
+        default:
+            if (dbl) SPG_SHADE_B(true, SPG_EST_GUIDED);
+            else SPG_SHADE_B(false, SPG_EST_GUIDED);
+            break;
         }
-        else if (dbl)
-            SPG_SHADE_B(true, SPG_EST_UNBIASED);
-        else
-            SPG_SHADE_B(false, SPG_EST_UNBIASED);
         return hipGetLastError();
     }
     switch (p.estimator)
@@ -1600,8 +1620,12 @@ hipError_t launch_guide_infer(hipStream_t s, const TraceParams &p, int q)
     uint32_t cap = device_cus();
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_guide_infer, dim3((uint32_t)blocks), dim3(64 * kInferWaves),
-                       0, s, p, q);
+    if (p.slotItem)
+        hipLaunchKernelGGL(k_guide_infer<true>, dim3((uint32_t)blocks),
+                           dim3(64 * kInferWaves), 0, s, p, q);
+    else
+        hipLaunchKernelGGL(k_guide_infer<false>, dim3((uint32_t)blocks),
+                           dim3(64 * kInferWaves), 0, s, p, q);
     return hipGetLastError();
 }
 
