@@ -516,6 +516,24 @@ int spg_time_post(spg_Device *dev, float *dst, const float *src,
                   uint8_t *out, uint32_t iterations, double *accumulateMs,
                   double *tonemapMs);
 
+/* ---- the parked Vulkan compute tracer (src/shaders/test_compute.comp.glsl),
+ *      a brute-force parity harness: no hierarchy, the shader's hard-coded
+ *      materials ----
+ * Traces the listed tiles (index t: the 16x16 tile (t % 16, t / 16) of the
+ * shader's fixed 256x192 image) as the shader's main/PathTraceTile do: one
+ * xorshift32 stream per tile seeded (t + 1) * 0xF51C0E49, one sample per
+ * pixel, up to 3 rays per path, every object of the scene an entity tested
+ * triangle by triangle (no back-face culling, flat normals), bounce
+ * direction normalize(n + (B, B, B)) into n's hemisphere, materials by the
+ * object's material id: 0 red albedo (0.18, 0.1, 0.1), 2 the 256x256
+ * checkerboard (src/image.cpp:134-160) sampled bilinearly with repeat,
+ * 5 white light (emission 1), others black.  cameraTransform: the
+ * uniform's column-major mat4 (columns right, up, forward, position).
+ * image: host RGBA32F 256x192; only the tiles' pixels are written. */
+int spg_compute_tracer_tiles(spg_Scene *scene, const float cameraTransform[16],
+                             const uint32_t *tileIndices, uint32_t count,
+                             float *image);
+
 /* ---- mesh files ---- */
 /* LoadMesh (src/mesh.cpp:5-62) for Wavefront OBJ text: the triangles
  * assimp's import yields there (aiProcess_Triangulate: polygons split into

@@ -166,6 +166,8 @@ def oracle():
     _sig(L, "spo_guide_analytic_lobes", None, f32p, f32p, f32p, f32p)
     _sig(L, "spo_guided_sink", None, C.c_void_p, C.c_uint32)
     _sig(L, "spo_guided_sink_count", C.c_uint32)
+    _sig(L, "spo_compute_tracer_tile", None, vp, f32p, C.c_uint32, f32p)
+    _sig(L, "spo_checker_bilinear", C.c_float, C.c_float, C.c_float)
     _sig(L, "spo_closest_hit_brute", None, f32p, C.c_uint32, f32p, C.c_uint32,
          C.c_uint32, u32p, f32p)
     _sig(L, "spo_closest_hit_brute_f64", None, f32p, C.c_uint32, f32p, C.c_uint32,

@@ -108,6 +108,9 @@ template <bool kDouble>
 __device__ __forceinline__ float mixture_pdf(const LobeRef &L, v3 w)
 {
     float q = 0.0f;
+    // not unrolled: eight inlined copies of the double-evaluated lobe density
+    // would hold the shading kernel at one wave per SIMD
+#pragma unroll 1
     for (int k = 0; k < kGLobes; ++k)
     {
         float m[3];

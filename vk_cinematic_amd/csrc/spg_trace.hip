@@ -677,7 +677,7 @@ __device__ __forceinline__ float bsdf_pdf(v3 n, v3 V, v3 L, float cosv, float ro
 // Verification hook of the guided estimator: appends one scatter event
 // (pn, N, V, roughness, the three uniforms, the mixture, the sampled
 // direction and its density, the hit point) to the guide's record buffer.
-__device__ __noinline__ void record_scatter(const DevGuide &G, v3 pn, v3 N, v3 V,
+__device__ __forceinline__ void record_scatter(const DevGuide &G, v3 pn, v3 N, v3 V,
                                             float rough, float u0, float u1, float u2,
                                             const LobeRef &L, v3 wi, float pdf, v3 P)
 {
