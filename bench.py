@@ -399,14 +399,27 @@ def bench_box(args, V, scenes, name):
                   "paths_per_step": m.values[V.METRIC_PATHS]},
     }
     # the guided tracer on the same workload: guide inference every bounce
-    # and online training (stepsPerPass Adam steps) after every step
+    # and online training (stepsPerPass Adam steps) after every step; device
+    # float transcendentals, as the time-to-relMSE runs use, with the plain
+    # tracer in the same mode beside it
+    gtx = V.TX_FAST
     g = V.Guide(dev, seed=1, max_weight=10.0)
 
     def gctx(step, train=True):
         return V.make_context(cam, s, spp=1, bounces=bounces, clamp=0.0,
                               estimator=V.EST_GUIDED, rng=V.RNG_PCG, seed=0x6EED,
-                              transcendentals=tx, sample_offset=step, guide=g,
+                              transcendentals=gtx, sample_offset=step, guide=g,
                               train=train)
+
+    pf_ms = 0.0
+    for k in range(args.warmup + args.steps):
+        V.path_trace_tiles_device(V.make_context(cam, s, spp=1, bounces=bounces,
+                                                 clamp=0.0, estimator=V.EST_UNBIASED,
+                                                 rng=V.RNG_PCG, seed=0x5EED,
+                                                 transcendentals=gtx, sample_offset=k),
+                                  tiles, img)
+        if k >= args.warmup:
+            pf_ms += dev.last_timings()["total"][0]
 
     for k in range(args.warmup):
         V.path_trace_tiles_device(gctx(k), tiles, img)
@@ -469,6 +482,8 @@ def bench_box(args, V, scenes, name):
                        "samples_from": f"{len(train_samples)} samples collected by the "
                                        f"last training pass of this workload"}}
     res["extra"]["guided"] = {
+        "transcendentals": "device float",
+        "plain_gpu_ms_per_step_same_mode": pf_ms / args.steps,
         "Msamples_per_s_training": samples * args.steps / gwall / 1e6,
         "Msamples_per_s_frozen": samples * args.steps / fwall / 1e6,
         "ms_per_step_training": gwall * 1e3 / args.steps,
@@ -478,6 +493,32 @@ def bench_box(args, V, scenes, name):
         "training_steps_per_pass": 4, "batch": 16384,
         "collected_last_pass": g.stats()["collected"]}
     g.close()
+    if name == "config5":
+        # the full guided + batched form: the whole image as one batch item,
+        # guide inference, sample collection and training captured in the
+        # HIP graph; a step is one replay (one sample per pixel, one pass of
+        # online training)
+        gb = V.Guide(dev, seed=1, max_weight=10.0)
+        b = V.Batch(dev, [(cam, s, img)], spp=1, bounces=bounces, clamp=0.0,
+                    estimator=V.EST_GUIDED, rng=V.RNG_PCG, seed=0x6EED,
+                    transcendentals=gtx, autotune=True, guide=gb, train=True)
+        for _ in range(args.warmup):
+            b.run()
+        bms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+            bms += b.last_ms()
+        bwall = time.perf_counter() - t0
+        res["extra"]["guided_batched"] = {
+            "Msamples_per_s": samples * args.steps / bwall / 1e6,
+            "ms_per_step": bwall * 1e3 / args.steps,
+            "gpu_ms_per_step": bms / args.steps,
+            "tuning": b.tuning(), "adam_steps": gb.stats()["steps"],
+            "workload": "config5 image as one batch item, guided estimator with "
+                        "online training, graph replays of one pass"}
+        b.close()
+        gb.close()
     if not args.no_cpu:
         cpu = host_cpu()
         res["cpu_baseline"] = cpu_baseline_box(desc, cpu["threads"], bounces=bounces)

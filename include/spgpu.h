@@ -585,6 +585,13 @@ int spg_gen_sky_equirect(uint32_t width, uint32_t height,
                          const float sunDir[3], float sunAngularRadius,
                          float sunRadiance, float *rgba);
 
+/* Self-check of the device arithmetic that bit-exact results rely on (no
+ * reference counterpart): the tracer's fast reciprocal (hardware reciprocal
+ * plus one FMA Newton step, used where x and 1/x are normal) against the
+ * IEEE division for all 2^32 float bit patterns; counts2[0] = patterns in
+ * the fast range tested, counts2[1] = results that differ (0 expected). */
+int spg_check_arithmetic(spg_Device *dev, uint64_t *counts2);
+
 /* xorshift32 seed of pixel `index` in SPG_RNG_XORSHIFT_PIXEL mode. */
 uint32_t spg_pixel_seed(uint32_t base, uint32_t index);
 

@@ -85,3 +85,17 @@ def test_non_finite_geometry_is_rejected(device):
     s.add_object(m, 7, (0, 0, 0))
     s.build()
     s.close()
+
+
+def test_fast_reciprocal_equals_ieee_division_everywhere(device):
+    """The tracer's reciprocal (hardware estimate + one FMA Newton step,
+    spg_math.h rcp_cr) against the IEEE division for every float bit
+    pattern of its fast range (exponent field 2..252): no difference, so
+    normalize, the inverse direction and Moller-Trumbore's determinant keep
+    their bit-exact results."""
+    import ctypes as C
+
+    counts = (C.c_uint64 * 2)()
+    V.check(V.lib().spg_check_arithmetic(device.handle, counts))
+    assert counts[0] == 2 * 251 * (1 << 23)
+    assert counts[1] == 0

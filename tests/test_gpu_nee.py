@@ -96,9 +96,8 @@ def test_nee_emitter_scene_matches_oracle_bit_exact(device):
 
 
 def test_nee_error_paths(device):
-    """An unknown estimator is an error, not an abort; batches refuse the NEE
-    estimator (they take parity and unbiased only); NEE with float
-    transcendentals renders a finite image."""
+    """An unknown estimator is an error, not an abort, in a trace call and
+    in a batch; NEE with float transcendentals renders a finite image."""
     desc = scenes.config1(32, 24)
     s, cam = scenes.load_gpu(device, desc)
     img = np.zeros(32 * 24 * 4, np.float32)
@@ -107,7 +106,7 @@ def test_nee_error_paths(device):
         V.path_trace_tiles(ctx, V.ComputeTiles(32, 24, 64, 64), img)
     buf = device.alloc(32 * 24 * 16)
     with pytest.raises(V.SpgError):
-        V.Batch(device, [(cam, s, buf)], spp=1, bounces=3, estimator=V.EST_NEE)
+        V.Batch(device, [(cam, s, buf)], spp=1, bounces=3, estimator=7)
     ctx = V.make_context(cam, s, spp=4, bounces=3, clamp=0.0, estimator=V.EST_NEE,
                          transcendentals=V.TX_FAST)
     V.path_trace_tiles(ctx, V.ComputeTiles(32, 24, 64, 64), img)

@@ -209,6 +209,7 @@ _SIGS = {
     "spg_guide_stats": (_int, [_vp, C.POINTER(_u32), C.POINTER(_u32),
                                C.POINTER(C.c_float)]),
     "spg_compute_tracer_tiles": (_int, [_vp, _f32p, _u32p, _u32, _f32p]),
+    "spg_check_arithmetic": (_int, [_vp, C.POINTER(C.c_uint64)]),
     "spg_guide_samples": (_int, [_vp, _vp, _u32, C.POINTER(_u32)]),
     "spg_guide_debug": (_int, [_vp, _u32, _u32]),
     "spg_guide_debug_records": (_int, [_vp, _vp, _u32, C.POINTER(_u32)]),

@@ -25,6 +25,42 @@ constexpr float kF32Max = 3.40282347e+38f;
 struct v3 { float x, y, z; };
 struct v2 { float x, y; };
 
+// 1 / x rounded to nearest, equal to the IEEE division.  On the device:
+// the hardware reciprocal (v_rcp_f32, within one unit in the last place)
+// refined by one Newton step with FMA, r = y + y (1 - x y), which rounds to
+// the IEEE quotient whenever x and 1/x are normal -- the exponent field of
+// x in [2, 252] (every such x is checked on the device by
+// spg_check_arithmetic); lanes outside that range (zero, denormals, |x| >=
+// 2^126, infinities, NaN) take the IEEE division, behind a wave vote so
+// waves without such a lane skip it.  Three VALU operations instead of the
+// division's eleven.  On the host: the division.
+__device__ __forceinline__ float rcp_newton(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float y = __builtin_amdgcn_rcpf(x);
+#else
+    const float y = 1.0f / x;
+#endif
+    const float e = fmaf(-x, y, 1.0f);
+    return fmaf(e, y, y);
+}
+__device__ __forceinline__ bool rcp_fast_range(float x)
+{
+    const uint32_t ex = (__float_as_uint(x) >> 23) & 0xFFu;
+    return ex >= 2u && ex <= 252u;
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ float rcp_cr(float x)
+{
+    float r = rcp_newton(x);
+    const bool slow = !rcp_fast_range(x);
+    if (__ballot(slow)) r = slow ? 1.0f / x : r;
+    return r;
+}
+#else
+inline float rcp_cr(float x) { return 1.0f / x; }
+#endif
+
 SPG_HD v3 mk(float x, float y, float z) { return v3{x, y, z}; }
 SPG_HD v3 splat(float s) { return v3{s, s, s}; }
 SPG_HD float fmin_sel(float a, float b) { return a < b ? a : b; }
@@ -64,11 +100,11 @@ SPG_HD v3 normalize(v3 v)
 {
     float length = sqrtf(dot(v, v));
     v3 r{0.0f, 0.0f, 0.0f};
-    if (length > kEpsilon) r = v * (1.0f / length);
+    if (length > kEpsilon) r = v * rcp_cr(length);
     return r;
 }
 // Inverse(vec3), src/math_lib.h:911-915
-SPG_HD v3 inverse(v3 a) { return v3{1.0f / a.x, 1.0f / a.y, 1.0f / a.z}; }
+SPG_HD v3 inverse(v3 a) { return v3{rcp_cr(a.x), rcp_cr(a.y), rcp_cr(a.z)}; }
 
 // Rows 0..2 of a column-major mat4 (src/math_lib.h:93-97), i.e. what
 // TransformPoint/TransformVector read (src/math_lib.h:748-758).
@@ -161,7 +197,7 @@ SPG_HD float mt_test_n(v3 o, v3 d, v3 a, v3 e1, v3 e2, v3 n, float *uo, float *v
     v3 p = cross(d, e2);
     v3 q = cross(T, e1);
     float mx = dot(q, e2), my = dot(p, T), mz = dot(q, d);
-    float det = 1.0f / dot(p, e1);
+    float det = rcp_cr(dot(p, e1));
     float t = det * mx;
     float u = det * my;
     float v = det * mz;

@@ -82,8 +82,23 @@ __device__ __forceinline__ void stage_rows(_Float16 *dst, const _Float16 *src,
     }
 }
 
+// sin and cos of 2 pi x with the hardware's v_sin_f32 / v_cos_f32 (x in
+// revolutions, |x| <= 256; absolute error near 2^-20, far below the FP16
+// the features are stored in).
+__device__ __forceinline__ void sincos_rev(float x, float *s, float *c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    *s = __builtin_amdgcn_sinf(x);
+    *c = __builtin_amdgcn_cosf(x);
+#else
+    *s = sinf(6.28318531f * x);
+    *c = cosf(6.28318531f * x);
+#endif
+}
+
 // Encoding of one hit into 64 features (see spg_guide.h), written as one
-// LDS row of the wave's encoding tile.  Invalid rays get zeros.
+// LDS row of the wave's encoding tile.  Invalid rays get zeros.  sin/cos(2^l
+// pi p) for p in [0, 1] is 2^(l-1) p revolutions (at most 64).
 __device__ __forceinline__ void encode_row(_Float16 *row, bool valid, v3 pn, v3 n,
                                            v3 v)
 {
@@ -93,7 +108,7 @@ __device__ __forceinline__ void encode_row(_Float16 *row, bool valid, v3 pn, v3 
         for (int a = 0; a < 3; ++a)
         {
             float s, c;
-            sincospif(ldexpf(p[a], l), &s, &c);
+            sincos_rev(ldexpf(p[a], l - 1), &s, &c);
             e[6 * l + 2 * a] = s;
             e[6 * l + 2 * a + 1] = c;
         }
@@ -193,30 +208,60 @@ __device__ __forceinline__ void store_outputs(float *scr, const f4 (&acc)[4][3])
                 scr[(mb * 16 + 4 * g + i) * 64 + rb * 16 + r] = acc[rb][mb][i];
 }
 
+// exp, 1/x and 1/sqrt(x) with the hardware's v_exp_f32 / v_rcp_f32 /
+// v_rsq_f32 (within a few units in the last place): the mixture is a
+// proposal, sampled and
+// evaluated from the same stored numbers, so its rounding does not bias the
+// estimator.
+__device__ __forceinline__ float fast_exp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __expf(x);
+#else
+    return expf(x);
+#endif
+}
+__device__ __forceinline__ float fast_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+__device__ __forceinline__ float fast_rsqrt(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+
 // The mixture of one ray from its 48 outputs (o[k] at stride 64).
 __device__ __forceinline__ void outputs_to_lobes(const float *o, Lobes &L)
 {
     float mx = o[0];
     for (int k = 1; k < kGLobes; ++k) mx = fmaxf(mx, o[k * 64]);
     float s = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) s += (L.pi[k] = expf(o[k * 64] - mx));
-    float inv = 1.0f / s;
+    for (int k = 0; k < kGLobes; ++k) s += (L.pi[k] = fast_exp(o[k * 64] - mx));
+    float inv = fast_rcp(s);
     for (int k = 0; k < kGLobes; ++k)
     {
         L.pi[k] *= inv;
-        float sg = 1.0f / (1.0f + expf(-o[(8 + k) * 64]));
-        L.kappa[k] = expf(kLogKappa0 + kLogKappa1 * sg);
+        float sg = fast_rcp(1.0f + fast_exp(-o[(8 + k) * 64]));
+        L.kappa[k] = fast_exp(kLogKappa0 + kLogKappa1 * sg);
         float mx3 = o[(16 + 3 * k) * 64], my = o[(17 + 3 * k) * 64],
               mz = o[(18 + 3 * k) * 64];
-        float len = sqrtf(mx3 * mx3 + my * my + mz * mz);
-        if (len < 1e-20f)
+        float len2 = mx3 * mx3 + my * my + mz * mz;
+        if (len2 < 1e-38f)
         {
             mx3 = 0.0f;
             my = 0.0f;
             mz = 1.0f;
-            len = 1.0f;
+            len2 = 1.0f;
         }
-        float il = 1.0f / len;
+        float il = fast_rsqrt(len2);
         L.mu[k][0] = mx3 * il;
         L.mu[k][1] = my * il;
         L.mu[k][2] = mz * il;
