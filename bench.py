@@ -398,6 +398,11 @@ def bench_box(args, V, scenes, name):
                   "scene_build_s": build_s,
                   "paths_per_step": m.values[V.METRIC_PATHS]},
     }
+    if args.no_guided:
+        img.free()
+        s.close()
+        dev.close()
+        return res
     # the guided tracer on the same workload: guide inference every bounce
     # and online training (stepsPerPass Adam steps) after every step; device
     # float transcendentals, as the time-to-relMSE runs use, with the plain
@@ -1032,6 +1037,8 @@ def main():
                          "guided and NEE tracers on the workload (config4 or "
                          "config5) instead, against a reference rendered in "
                          "the same run")
+    ap.add_argument("--no-guided", action="store_true",
+                    help="config4/config5: the plain tracer only (for counter runs)")
     ap.add_argument("--no-relmse", action="store_true",
                     help="skip the config-4 time-to-relMSE part of the default run")
     ap.add_argument("--ref-spp", type=int, default=None,

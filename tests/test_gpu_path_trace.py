@@ -290,3 +290,45 @@ def test_wavefront_conserves_rays(device, est):
     if g is not None:
         g.close()
     s.close()
+
+
+def test_slot_list_cache_invalidation(device):
+    """The device keeps the last call's tile-to-pixel slot list and reuses
+    it for a call with the same tiles and image size.  Interleaved calls
+    that change the tiles, return to earlier tiles, keep the tiles but change
+    the image size, trace more paths than fit in flight (slot chunks) and
+    then fewer again, each equal a fresh render bit for bit."""
+    big = scenes.config1(2048, 1088, env="cube", env_size=16)
+    small = scenes.config1(256, 192, env="cube", env_size=16)
+    s_big, cam_big = scenes.load_gpu(device, big)
+    s_small, cam_small = scenes.load_gpu(device, small)
+    cam_small2 = V.sp_ConfigureCamera(192, 256, (0.0, 1.0, 4.0), (0, 0, 0, 1), 0.8)
+
+    def render(cam, s, tiles, spp, w, h):
+        ctx = V.make_context(cam, s, spp=spp, bounces=3, estimator=V.EST_PARITY,
+                             rng=V.RNG_PCG, seed=0x5EED)
+        img = np.zeros(w * h * 4, np.float32)
+        V.path_trace_tiles(ctx, tiles, img)
+        return img
+
+    t1 = V.ComputeTiles(256, 192, 64, 64)
+    t2 = V.ComputeTiles(256, 192, 32, 32)[5:40]
+    t_all_big = V.ComputeTiles(2048, 1088, 64, 64)
+    seq = [(cam_small, s_small, t1, 2, 256, 192), (cam_small, s_small, t2, 2, 256, 192),
+           (cam_small, s_small, t1, 2, 256, 192), (cam_small2, s_small, t1, 2, 192, 256),
+           (cam_big, s_big, t_all_big, 1, 2048, 1088),   # 2.2M slots: two chunks
+           (cam_small, s_small, t1, 2, 256, 192), (cam_small, s_small, t1, 2, 256, 192)]
+    results = [render(*a) for a in seq]
+    fresh = {}
+    for a, r in zip(seq, results):
+        key = (id(a[0]), id(a[2]))
+        if key not in fresh:
+            # a fresh render: a different device, so no cache is shared
+            d2 = V.Device(0)
+            s2, _ = scenes.load_gpu(d2, big if a[0] is cam_big else small)
+            fresh[key] = render(a[0], s2, a[2], a[3], a[4], a[5])
+            s2.close()
+            d2.close()
+        assert np.array_equal(r.view(np.uint32), fresh[key].view(np.uint32))
+    s_big.close()
+    s_small.close()
