@@ -48,17 +48,17 @@ RAY_BYTES, HIT_BYTES_MESH, HIT_BYTES_SCENE = 24, 8, 20
 BOX_BYTES, TRI_BYTES, ENTRY_BYTES = 24, 36, 128
 # HBM bytes per traced ray of the closest-hit kernel, measured with rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE per dispatch (FETCH scaled x2.0, WRITE
-# x1.0 by a 1 GiB copy kernel on the same box; config5: the plain tracer's
-# launches); the roofline's "traffic" is this times the rays of one launch.
-TRAFFIC_BYTES_PER_RAY = {"config5": 1.522e9 / 2037010, "soup": 1.095e9 / 4194304}
+# x1.0 by the 1 GiB copy of `--workload copy` on the same box; config5: the
+# plain tracer's launches, `--no-guided`); the roofline's "traffic" is this
+# times the rays of one launch.
+TRAFFIC_BYTES_PER_RAY = {"config5": 1.5237e9 / 2036866, "soup": 1.0953e9 / 4194304}
 # What bounds the closest-hit kernel in fact, from counter runs of the same
 # kernel and workload: the VALU issue rate, not HBM.  valu_busy: share of
-# the SIMDs' issue cycles with a VALU instruction (SQ_ACTIVE_INST_VALU over
-# SQ_BUSY_CYCLES x 4 SIMDs); valu_insts_per_ray: SQ_INSTS_VALU per traced
-# ray; hbm_frac: counter-measured bytes over the launch time, against the
-# 8 TB/s peak (the "traffic" figure); the roofline "frac" above prices the
-# algorithmic bytes instead.
-LIMITER = {"config5": {"kind": "VALU issue", "valu_busy": 0.96, "valu_insts_per_ray": 285,
+# the SIMDs' cycles issuing VALU work (SQ_INSTS_VALU x 4 cycles per wave64
+# instruction over 1024 SIMDs x the dispatches' mean duration x 2.4 GHz);
+# valu_insts_per_ray: SQ_INSTS_VALU per traced ray.  The roofline "frac"
+# prices the algorithmic bytes, "traffic_frac" the counter-measured ones.
+LIMITER = {"config5": {"kind": "VALU issue", "valu_busy": 0.92, "valu_insts_per_ray": 274,
                        "source": "profiles/hbm_traffic_config5_soup.md"}}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same kernel and "
                   "workload (profiles/hbm_traffic_config5_soup.md), FETCH x2.0 and "
@@ -707,6 +707,8 @@ def render_reference(V, dev, s, cam, w, h, spp, est, chunk=256):
         acc = DeviceAccumulator(V, dev, s, cam, w, h)
         while acc.spp < spp // 2:
             acc.add(min(chunk, spp // 2 - acc.spp), est, seed)
+            if acc.spp % 4096 == 0:
+                log(f"reference half {len(halves) + 1}: {acc.spp} spp, {acc.wall:.1f} s")
         halves.append(acc.mean())
         wall += acc.wall
         acc.close()
