@@ -305,8 +305,10 @@ def cpu_baseline_box(desc, threads, spp=4, bounces=6, repeats=3):
 
 
 def bench_box(args, V, scenes, name):
-    """config5 (or config4): the plain (unbiased) GPU tracer, one sample per
-    pixel per step over the whole image."""
+    """config5 (or config4): the plain (unbiased) GPU tracer, `--spp` samples
+    per pixel per step over the whole image (config5: 1, the headline
+    configuration; config4: 2, so that one launch holds 524,288 paths and
+    fills the persistent grid's 327,680 lanes)."""
     dev = V.Device(0)
     desc = scenes.config5() if name == "config5" else scenes.config4()
     w, h = desc["camera"][0], desc["camera"][1]
@@ -317,11 +319,12 @@ def bench_box(args, V, scenes, name):
     tiles = V.ComputeTiles(w, h, 64, 64)
     img = dev.alloc(w * h * 16)
     tx = V.TX_FAST if args.fast else V.TX_DOUBLE
+    spp = args.spp or (1 if name == "config5" else 2)
 
     def ctx_for(step, collect=False):
-        return V.make_context(cam, s, spp=1, bounces=bounces, clamp=0.0,
+        return V.make_context(cam, s, spp=spp, bounces=bounces, clamp=0.0,
                               estimator=V.EST_UNBIASED, rng=V.RNG_PCG, seed=0x5EED,
-                              transcendentals=tx, sample_offset=step,
+                              transcendentals=tx, sample_offset=step * spp,
                               collect=collect)
 
     for k in range(args.warmup):
@@ -356,7 +359,7 @@ def bench_box(args, V, scenes, name):
     per_launch = bytes_step / launches
     mean_ms = ext_ms / ext_n
     achieved = per_launch / (mean_ms * 1e-3) / 1e9
-    samples = w * h
+    samples = w * h * spp
     res = {
         "metric": METRIC,
         "value": samples * args.steps / wall / 1e6,
@@ -366,11 +369,11 @@ def bench_box(args, V, scenes, name):
         "config": {"workload": f"{name}: {desc['triangles']} triangles ("
                                f"{len(desc['objects']) - 1} displaced icospheres of "
                                f"20480 in a closed box with a 1x1 opening), sky cube "
-                               f"6x64x64 (sun 50), {w}x{h}, 1 spp per step, "
+                               f"6x64x64 (sun 50), {w}x{h}, {spp} spp per step, "
                                f"{bounces} bounces, plain unbiased tracer, per-pixel "
                                f"PCG, {'fast' if args.fast else 'double-evaluated'} "
                                f"transcendentals",
-                   "image": [w, h], "spp_per_step": 1, "bounces": bounces,
+                   "image": [w, h], "spp_per_step": spp, "bounces": bounces,
                    "triangles": desc["triangles"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -1039,6 +1042,9 @@ def main():
                          "guided and NEE tracers on the workload (config4 or "
                          "config5) instead, against a reference rendered in "
                          "the same run")
+    ap.add_argument("--spp", type=int, default=None,
+                    help="config4/config5: samples per pixel per step of the plain "
+                         "tracer (default 1 on config5, 2 on config4)")
     ap.add_argument("--no-guided", action="store_true",
                     help="config4/config5: the plain tracer only (for counter runs)")
     ap.add_argument("--no-relmse", action="store_true",
@@ -1048,7 +1054,9 @@ def main():
                          "config5: 32768 plain)")
     ap.add_argument("--ref-estimator", choices=["plain", "nee"], default=None)
     ap.add_argument("--max-spp", type=int, default=8192)
-    ap.add_argument("--train-passes", type=int, default=64)
+    ap.add_argument("--train-passes", type=int, default=None,
+                    help="one-sample passes of online training before the guide "
+                         "is frozen (default 64 on config4, 256 on config5)")
     ap.add_argument("--guide-lr", type=float, default=3e-3)
     ap.add_argument("--guide-batch", type=int, default=16384)
     ap.add_argument("--guide-steps", type=int, default=4)
@@ -1069,6 +1077,8 @@ def main():
             args.ref_spp = 4096 if args.workload == "config4" else 32768
         if args.ref_estimator is None:
             args.ref_estimator = "nee" if args.workload == "config4" else "plain"
+        if args.train_passes is None:
+            args.train_passes = 64 if args.workload == "config4" else 256
         return bench_relmse(args, V, scenes)
     if args.workload in ("config5", "config4"):
         res = bench_box(args, V, scenes, args.workload)

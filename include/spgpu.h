@@ -346,11 +346,17 @@ typedef struct {
 
 typedef struct {
     uint32_t pathsInFlight;      /* wavefront capacity (paths) */
-    uint32_t extendBlocks;       /* resident one-wave traversal blocks */
-    uint32_t candidates;         /* launch shapes timed by the autotuner */
+    uint32_t extendBlocks;       /* resident one-wave traversal blocks
+                                    (0 = as many as fit) */
+    uint32_t candidates;         /* launch shapes timed by the autotuner
+                                    (0 when the shape came from its cache) */
     float tunedMsPerSample;      /* the chosen shape's time for 1 spp */
     uint64_t pixels;             /* slots over all items */
     uint32_t graphNodes;         /* kernels in the captured graph */
+    uint32_t refill;             /* idle lanes a traversal wave waits for
+                                    before taking new rays (0 = default) */
+    uint32_t cached;             /* 1: the shape tuned for an earlier batch
+                                    of the same configuration was reused */
 } spg_BatchTuning;
 
 /* cfg: samplesPerPixel, bounces, radianceClamp, estimator, rngMode, seed,

@@ -106,21 +106,39 @@ def test_batch_larger_than_the_wavefront(device):
 
 
 def test_autotuner_choice(device):
+    """The tuner times capacity x traversal grid (3 shapes here: one
+    capacity fits the 4 small images) and then two more refill thresholds;
+    a second batch of the same configuration takes the remembered shape
+    without timing; neither the shape nor the refill threshold changes the
+    image (an untuned batch renders the same bits)."""
     descs = scenes.config3(count=4, seed=9, min_side=128, max_side=256,
                            min_tris=1e4, max_tris=1e5)
     loaded = _load(device, descs)
-    b = V.Batch(device, [(cam, s, img) for s, cam, img, _, _ in loaded], spp=4,
-                bounces=5, autotune=True)
+    items = [(cam, s, img) for s, cam, img, _, _ in loaded]
+    b = V.Batch(device, items, spp=4, bounces=5, autotune=True)
     t = b.tuning()
-    assert t["candidates"] == 3
+    assert t["candidates"] == 5 and t["cached"] == 0
     assert t["pathsInFlight"] == 1 << 20
+    assert t["refill"] in (0, 32, 48)
     assert t["tunedMsPerSample"] > 0
     assert t["graphNodes"] > 10
     b.run()
     assert b.last_ms() > 0
-    assert all(np.isfinite(g).all() and g.reshape(-1, 4)[:, :3].mean() > 0
-               for g in _images(loaded))
+    tuned = _images(loaded)
+    assert all(np.isfinite(g).all() and g.reshape(-1, 4)[:, :3].mean() > 0 for g in tuned)
     b.close()
+    b2 = V.Batch(device, items, spp=4, bounces=5, autotune=True)
+    t2 = b2.tuning()
+    assert t2["cached"] == 1 and t2["candidates"] == 0
+    for k in ("pathsInFlight", "extendBlocks", "refill", "tunedMsPerSample"):
+        assert t2[k] == t[k], k
+    b2.close()
+    b3 = V.Batch(device, items, spp=4, bounces=5, autotune=False)
+    assert b3.tuning()["refill"] == 0
+    b3.run()
+    for g, want in zip(_images(loaded), tuned):
+        assert np.array_equal(g.view(np.uint32), want.view(np.uint32))
+    b3.close()
     _close(loaded)
 
 

@@ -85,7 +85,8 @@ class BatchTuning(C.Structure):
     """spg_BatchTuning (include/spgpu.h)."""
     _fields_ = [("pathsInFlight", C.c_uint32), ("extendBlocks", C.c_uint32),
                 ("candidates", C.c_uint32), ("tunedMsPerSample", C.c_float),
-                ("pixels", C.c_uint64), ("graphNodes", C.c_uint32)]
+                ("pixels", C.c_uint64), ("graphNodes", C.c_uint32),
+                ("refill", C.c_uint32), ("cached", C.c_uint32)]
 
 
 class GuideConfig(C.Structure):

@@ -7,6 +7,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "spg_internal.h"
@@ -35,6 +38,7 @@ struct spg_Batch
     void *guideBlock = nullptr;
     DevGuide dg{};
     uint32_t maxBlocks = 0;
+    uint32_t refill = 0;  // k_extend's refill threshold (0 = the default)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -124,6 +128,7 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
     p.seed = cfg.seed;
     p.tx = cfg.transcendentals;
     p.collect = cfg.collectMetrics;
+    p.refill = b->refill;
     p.scenes = b->dScenes;
     p.cams = b->dCams;
     p.images = b->dImages;
@@ -239,9 +244,44 @@ int timed(spg_Batch *b, const spg_TraceConfig &cfg, float *ms)
     return SPG_OK;
 }
 
+// What the autotuner chose for one configuration: later batches of the
+// same configuration on the same device reuse it without timing.
+struct TunedShape
+{
+    uint32_t capacity, maxBlocks, refill;
+    float ms;
+};
+
+// The configuration a tuned shape is kept for: the device, the items'
+// image sizes and scene sizes, and the trace settings that change the
+// amount of work per sample.
+std::string tuning_key(const spg_Batch *b, const spg_BatchItem *items)
+{
+    std::string k = std::to_string(b->dev->ordinal) + "|" + std::to_string(b->n);
+    for (uint32_t i = 0; i < b->n; ++i)
+    {
+        const spg_Scene *sc = items[i].scene;
+        size_t tris = 0;
+        for (const HostMesh &m : sc->meshes) tris += m.idx.size() / 3;
+        k += "|" + std::to_string(items[i].camera->width) + "x" +
+             std::to_string(items[i].camera->height) + ":" + std::to_string(tris) + ":" +
+             std::to_string(sc->objects.size());
+    }
+    const spg_TraceConfig &c = b->cfg;
+    k += "|" + std::to_string(c.samplesPerPixel) + "," + std::to_string(c.bounces) + "," +
+         std::to_string(c.estimator) + "," + std::to_string(c.rngMode) + "," +
+         std::to_string(c.transcendentals);
+    return k;
+}
+
+std::mutex gTunedLock;
+std::map<std::string, TunedShape> gTuned;
+
 // Candidates: wavefront capacity x resident traversal blocks, each timed on
-// one sample per pixel of the whole batch; the fastest is kept.
-int autotune(spg_Batch *b, uint32_t cus)
+// one sample per pixel of the whole batch, then the traversal's refill
+// threshold (idle lanes a wave waits for before taking new rays) at the
+// fastest of those; the fastest is kept, and remembered for the key.
+int autotune(spg_Batch *b, uint32_t cus, const std::string &key)
 {
     spg_TraceConfig one = b->cfg;
     one.samplesPerPixel = 1;
@@ -255,10 +295,11 @@ int autotune(spg_Batch *b, uint32_t cus)
     }
     const uint32_t blocks[] = {0u, cus * 16u, cus * 8u};
     float best = 0.0f, ms = 0.0f;
-    uint32_t bestCap = caps[0], bestBlocks = 0, tried = 0;
+    uint32_t bestCap = caps[0], bestBlocks = 0, bestRefill = 0, tried = 0;
     // warm-up: first launches pay for code loading and cold caches
     CHK(alloc_state(b, caps[0]));
     b->maxBlocks = 0;
+    b->refill = 0;
     CHK(timed(b, one, &ms));
     for (uint32_t c : caps)
     {
@@ -276,10 +317,24 @@ int autotune(spg_Batch *b, uint32_t cus)
             }
         }
     }
-    b->tuning.candidates = tried;
-    b->tuning.tunedMsPerSample = best;
     CHK(alloc_state(b, bestCap));
     b->maxBlocks = bestBlocks;
+    for (uint32_t rf : {32u, 48u})
+    {
+        b->refill = rf;
+        CHK(timed(b, one, &ms));
+        ++tried;
+        if (ms < best)
+        {
+            best = ms;
+            bestRefill = rf;
+        }
+    }
+    b->refill = bestRefill;
+    b->tuning.candidates = tried;
+    b->tuning.tunedMsPerSample = best;
+    std::lock_guard<std::mutex> lk(gTunedLock);
+    gTuned[key] = TunedShape{bestCap, bestBlocks, bestRefill, best};
     return SPG_OK;
 }
 
@@ -412,7 +467,40 @@ int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
     for (hipEvent_t &e : b->ev) step(hipEventCreate(&e), "hipEventCreate");
     int cus = 0;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev->ordinal);
-    if (!r && tune) r = autotune(b, (uint32_t)std::max(cus, 1));
+    const std::string key = tune ? tuning_key(b, items) : std::string();
+    bool cached = false;
+    if (!r && tune)
+    {
+        TunedShape shape{};
+        {
+            std::lock_guard<std::mutex> tl(gTunedLock);
+            auto it = gTuned.find(key);
+            if (it != gTuned.end())
+            {
+                shape = it->second;
+                cached = true;
+            }
+        }
+        if (cached)
+        {
+            // the shape tuned for this configuration before; one eager
+            // sample per pixel loads the kernels and sizes the persistent
+            // grids outside the capture
+            r = alloc_state(b, shape.capacity);
+            b->maxBlocks = shape.maxBlocks;
+            b->refill = shape.refill;
+            spg_TraceConfig one = c;
+            one.samplesPerPixel = 1;
+            one.collectMetrics = 0;
+            one.guideTrain = 0;
+            float ms = 0.0f;
+            if (!r) r = timed(b, one, &ms);
+            b->tuning.candidates = 0;
+            b->tuning.tunedMsPerSample = shape.ms;
+        }
+        else
+            r = autotune(b, (uint32_t)std::max(cus, 1), key);
+    }
     if (!r && !tune)
     {
         uint64_t want = slots * (c.rngMode == SPG_RNG_PCG ? c.samplesPerPixel : 1u);
@@ -455,6 +543,8 @@ int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
     }
     b->tuning.pathsInFlight = b->paths.capacity;
     b->tuning.extendBlocks = b->maxBlocks;
+    b->tuning.refill = b->refill;
+    b->tuning.cached = cached ? 1u : 0u;
     b->tuning.pixels = slots;
     b->tuning.graphNodes = kernels;
     if (b->guide) b->guide->pins++;

@@ -67,12 +67,13 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_guide_eval(
     float *out48, float *lobes40)
 {
     __shared__ _Float16 sW[kWLdsHalves];
-    __shared__ float scratch[kEvalWaves][kWaveScratchFloats];
+    __shared__ float sBias[kGRows];
+    __shared__ float scratch[kEvalWaves][kTileScratchFloats];
     stage_rows(sW, w16, kGRows);
+    for (int k = threadIdx.x; k < kGRows; k += blockDim.x) sBias[k] = bias[k];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float *scr = scratch[wave];
-    _Float16 *enc = (_Float16 *)scr;
     for (uint32_t tile = blockIdx.x * kEvalWaves + wave; tile * 64 < n;
          tile += gridDim.x * kEvalWaves)
     {
@@ -86,26 +87,16 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_guide_eval(
             nn = v3{x[3], x[4], x[5]};
             vv = v3{x[6], x[7], x[8]};
         }
-        encode_row(enc + lane * kLdsPitch, valid, pn, nn, vv);
-        wave_lds_sync();
-        h8 B[4][2];
-        load_input(enc, B);
-        wave_lds_sync();
-        f4 acc[4][4];
-        for (int l = 0; l < 4; ++l)
-        {
-            layer64(sW, bias, l, B, acc);
-            relu_to_b(acc, B);
-        }
-        f4 out[4][3];
-        layer_out(sW, bias, B, out);
-        store_outputs(scr, out);
-        wave_lds_sync();
+        Lobes L;
+        // the same per-wave path as the tracer's inference kernel
+        guide_forward_tile(sW, sBias, scr, valid, pn, nn, vv, L,
+                           [&](int mb, const float *o) {
+                               if (valid)
+                                   for (int k = 0; k < 16; ++k)
+                                       out48[(size_t)s * kGOut + 16 * mb + k] = o[k * 64];
+                           });
         if (valid)
         {
-            for (int k = 0; k < kGOut; ++k) out48[(size_t)s * kGOut + k] = scr[k * 64 + lane];
-            Lobes L;
-            outputs_to_lobes(scr + lane, L);
             float *d = lobes40 + (size_t)s * kGLobeFloats;
             for (int k = 0; k < kGLobes; ++k)
             {
@@ -116,7 +107,6 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_guide_eval(
                 d[18 + 3 * k] = L.mu[k][2];
             }
         }
-        wave_lds_sync();
     }
 }
 

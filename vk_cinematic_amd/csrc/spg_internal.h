@@ -190,6 +190,7 @@ struct TraceParams
     uint32_t bounces;
     float clampValue;
     uint32_t estimator, rngMode, seed, tx, collect;
+    uint32_t refill;             // k_extend's refill threshold, 0 = default
     DevGuide guide;              // SPG_EST_GUIDED only
     // batched launches (slotItem != nullptr): each slot belongs to an item
     // with its own scene, camera and image

@@ -30,6 +30,13 @@ constexpr int kWtLdsHalves = kGLayers * 64 * kLdsPitch;  // transposed weights
 // Per-wave scratch: FP32 outputs [48][64] (aliases the FP16 encodings
 // [64][kLdsPitch]).
 constexpr int kWaveScratchFloats = kGOut * 64;
+// Per-wave scratch of the inference kernels, which pass the encodings in
+// two halves of 32 features ([64][kEncPitch] halves; 72-byte rows keep a
+// fragment read's 16 rows on distinct bank pairs) and the outputs in tiles
+// of 16 rows ([16][64] floats).
+constexpr int kEncPitch = 36;
+constexpr int kTileScratchFloats =
+    64 * kEncPitch / 2 > 16 * 64 ? 64 * kEncPitch / 2 : 16 * 64;
 
 __device__ __forceinline__ f4 mfma16x16x32(h8 a, h8 b, f4 c)
 {
@@ -96,39 +103,52 @@ __device__ __forceinline__ void sincos_rev(float x, float *s, float *c)
 #endif
 }
 
-// Encoding of one hit into 64 features (see spg_guide.h), written as one
-// LDS row of the wave's encoding tile.  Invalid rays get zeros.  sin/cos(2^l
-// pi p) for p in [0, 1] is 2^(l-1) p revolutions (at most 64).
-__device__ __forceinline__ void encode_row(_Float16 *row, bool valid, v3 pn, v3 n,
-                                           v3 v)
+// Encoding of one hit into 64 features (see spg_guide.h), half kHalf of it
+// (features 32 kHalf .. 32 kHalf + 31) written to row[0..31].  Invalid rays
+// get zeros.  Feature 2k, 2k + 1 (k < 24) are sin, cos of 2^l pi p_a with
+// l = k / 3, a = k % 3: 2^(l-1) p_a revolutions (at most 64) for p in
+// [0, 1]; then the raw position, normal, view direction, 1, zeros.
+template <int kHalf>
+__device__ __forceinline__ void encode_half(_Float16 *row, bool valid, v3 pn, v3 n,
+                                            v3 v)
 {
-    float e[kGIn];
-    float p[3] = {pn.x, pn.y, pn.z};
-    for (int l = 0; l < 8; ++l)
-        for (int a = 0; a < 3; ++a)
-        {
-            float s, c;
-            sincos_rev(ldexpf(p[a], l - 1), &s, &c);
-            e[6 * l + 2 * a] = s;
-            e[6 * l + 2 * a + 1] = c;
-        }
-    e[48] = pn.x;
-    e[49] = pn.y;
-    e[50] = pn.z;
-    e[51] = n.x;
-    e[52] = n.y;
-    e[53] = n.z;
-    e[54] = v.x;
-    e[55] = v.y;
-    e[56] = v.z;
-    e[57] = 1.0f;
-    for (int k = 58; k < kGIn; ++k) e[k] = 0.0f;
-    for (int k = 0; k < kGIn; k += 4)
+    float e[32];
+    const float p[3] = {pn.x, pn.y, pn.z};
+    for (int k = 16 * kHalf; k < (kHalf ? 24 : 16); ++k)
+    {
+        float s, c;
+        sincos_rev(ldexpf(p[k % 3], k / 3 - 1), &s, &c);
+        e[2 * k - 32 * kHalf] = s;
+        e[2 * k + 1 - 32 * kHalf] = c;
+    }
+    if (kHalf)
+    {
+        e[16] = pn.x;
+        e[17] = pn.y;
+        e[18] = pn.z;
+        e[19] = n.x;
+        e[20] = n.y;
+        e[21] = n.z;
+        e[22] = v.x;
+        e[23] = v.y;
+        e[24] = v.z;
+        e[25] = 1.0f;
+        for (int k = 26; k < 32; ++k) e[k] = 0.0f;
+    }
+    for (int k = 0; k < 32; k += 4)
     {
         h4 q;
         for (int j = 0; j < 4; ++j) q[j] = valid ? (_Float16)e[k + j] : (_Float16)0.0f;
         *(h4 *)(row + k) = q;
     }
+}
+
+// All 64 features as one LDS row of the wave's encoding tile.
+__device__ __forceinline__ void encode_row(_Float16 *row, bool valid, v3 pn, v3 n,
+                                           v3 v)
+{
+    encode_half<0>(row, valid, pn, n, v);
+    encode_half<1>(row + 32, valid, pn, n, v);
 }
 
 // B operands of the input layer from the wave's encoding tile.
@@ -138,6 +158,16 @@ __device__ __forceinline__ void load_input(const _Float16 *enc, h8 (&B)[4][2])
     for (int rb = 0; rb < 4; ++rb)
         for (int c = 0; c < 2; ++c)
             B[rb][c] = lds_frag(enc + (rb * 16 + r) * kLdsPitch, 32 * c + 4 * g);
+}
+
+// K step c of the input layer's B operands from a tile of encoding halves
+// (rows of kEncPitch halves holding features 32 c .. 32 c + 31).
+__device__ __forceinline__ void load_input_half(const _Float16 *enc, h8 (&B)[4][2],
+                                                int c)
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    for (int rb = 0; rb < 4; ++rb)
+        B[rb][c] = lds_frag(enc + (rb * 16 + r) * kEncPitch, 4 * g);
 }
 
 // z = W_l h + b_l for 64 rows, D layout, l < 4.
@@ -208,6 +238,16 @@ __device__ __forceinline__ void store_outputs(float *scr, const f4 (&acc)[4][3])
                 scr[(mb * 16 + 4 * g + i) * 64 + rb * 16 + r] = acc[rb][mb][i];
 }
 
+// Output rows 16 mb .. 16 mb + 15 (D layout) -> wave scratch [16][64].
+__device__ __forceinline__ void store_output_tile(float *scr, const f4 (&acc)[4][3],
+                                                  int mb)
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    for (int rb = 0; rb < 4; ++rb)
+        for (int i = 0; i < 4; ++i)
+            scr[(4 * g + i) * 64 + rb * 16 + r] = acc[rb][mb][i];
+}
+
 // exp, 1/x and 1/sqrt(x) with the hardware's v_exp_f32 / v_rcp_f32 /
 // v_rsq_f32 (within a few units in the last place): the mixture is a
 // proposal, sampled and
@@ -238,8 +278,10 @@ __device__ __forceinline__ float fast_rsqrt(float x)
 #endif
 }
 
-// The mixture of one ray from its 48 outputs (o[k] at stride 64).
-__device__ __forceinline__ void outputs_to_lobes(const float *o, Lobes &L)
+// Mixture weights and concentrations of one ray from outputs 0..15 (o[k]
+// at stride 64): pi = softmax(o[0..7]), log kappa = kLogKappa0 +
+// kLogKappa1 sigmoid(o[8 + k]).
+__device__ __forceinline__ void lobes_weights(const float *o, Lobes &L)
 {
     float mx = o[0];
     for (int k = 1; k < kGLobes; ++k) mx = fmaxf(mx, o[k * 64]);
@@ -251,8 +293,28 @@ __device__ __forceinline__ void outputs_to_lobes(const float *o, Lobes &L)
         L.pi[k] *= inv;
         float sg = fast_rcp(1.0f + fast_exp(-o[(8 + k) * 64]));
         L.kappa[k] = fast_exp(kLogKappa0 + kLogKappa1 * sg);
-        float mx3 = o[(16 + 3 * k) * 64], my = o[(17 + 3 * k) * 64],
-              mz = o[(18 + 3 * k) * 64];
+    }
+}
+
+// Unnormalized mean directions from outputs first .. first + count - 1 (o[k]
+// at stride 64 is output first + k): output 16 + m is component m % 3 of
+// lobe m / 3 (m < 24); outputs 40..47 are unused.
+__device__ __forceinline__ void lobes_mean_raw(const float *o, int first, int count,
+                                               Lobes &L)
+{
+    for (int k = 0; k < count; ++k)
+    {
+        const int m = first + k - 16;
+        if (m >= 0 && m < 3 * kGLobes) L.mu[m / 3][m % 3] = o[k * 64];
+    }
+}
+
+// Mean directions normalized (a zero vector becomes +z).
+__device__ __forceinline__ void lobes_normalize(Lobes &L)
+{
+    for (int k = 0; k < kGLobes; ++k)
+    {
+        float mx3 = L.mu[k][0], my = L.mu[k][1], mz = L.mu[k][2];
         float len2 = mx3 * mx3 + my * my + mz * mz;
         if (len2 < 1e-38f)
         {
@@ -266,6 +328,57 @@ __device__ __forceinline__ void outputs_to_lobes(const float *o, Lobes &L)
         L.mu[k][1] = my * il;
         L.mu[k][2] = mz * il;
     }
+}
+
+// The mixture of one ray from its 48 outputs (o[k] at stride 64).
+__device__ __forceinline__ void outputs_to_lobes(const float *o, Lobes &L)
+{
+    lobes_weights(o, L);
+    lobes_mean_raw(o + 16 * 64, 16, 32, L);
+    lobes_normalize(L);
+}
+
+// One wave's 64 rays through the guide, for the inference kernels: the
+// encodings in two halves through the wave's scratch (kTileScratchFloats),
+// the MLP on MFMA (weights sW in LDS, biases bias), the outputs in three
+// tiles of 16 rows through the same scratch, decoded into the lane's
+// mixture.  raw(mb, o) sees each tile first (o[k * 64] = output 16 mb + k
+// of the lane's ray).  Every lane of the wave takes part.
+template <class Raw>
+__device__ __forceinline__ void guide_forward_tile(const _Float16 *sW, const float *bias,
+                                                   float *scr, bool valid, v3 pn, v3 n,
+                                                   v3 v, Lobes &L, Raw raw)
+{
+    const int lane = threadIdx.x & 63;
+    _Float16 *enc = (_Float16 *)scr;
+    h8 B[4][2];
+    encode_half<0>(enc + lane * kEncPitch, valid, pn, n, v);
+    wave_lds_sync();
+    load_input_half(enc, B, 0);
+    wave_lds_sync();
+    encode_half<1>(enc + lane * kEncPitch, valid, pn, n, v);
+    wave_lds_sync();
+    load_input_half(enc, B, 1);
+    wave_lds_sync();
+    f4 acc[4][4];
+    for (int l = 0; l < 4; ++l)
+    {
+        layer64(sW, bias, l, B, acc);
+        relu_to_b(acc, B);
+    }
+    f4 out[4][3];
+    layer_out(sW, bias, B, out);
+    for (int mb = 0; mb < 3; ++mb)
+    {
+        store_output_tile(scr, out, mb);
+        wave_lds_sync();
+        const float *o = scr + lane;
+        raw(mb, o);
+        if (mb == 0) lobes_weights(o, L);
+        else lobes_mean_raw(o, 16 * mb, 16, L);
+        wave_lds_sync();
+    }
+    lobes_normalize(L);
 }
 
 __device__ __forceinline__ void store_lobes(float *dst, uint32_t cap, uint32_t i,

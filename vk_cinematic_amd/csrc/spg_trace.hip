@@ -417,8 +417,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_mesh(
         if (kCount && s.h.prim != SPG_MISS) hits++;
     };
     SceneView sv{};
-    persistent_walk<kCount, false, kMeshRefill, kMeshShortStack>(sv, n, counter, spill,
-                                                                 begin, store, cnt, rays);
+    persistent_walk<kCount, false, kMeshShortStack>(sv, n, counter, spill, kMeshRefill,
+                                                    begin, store, cnt, rays);
     if (kCount)
     {
         wave_add(&stats[0], rays);
@@ -466,8 +466,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_scene(
         if (objOut) objOut[i] = h.inst;
         if (primOut) primOut[i] = h.prim;
     };
-    persistent_walk<false, true, kSceneRefill, kSceneShortStack>(sv, n, counter, spill,
-                                                                 begin, store, cnt, rays);
+    persistent_walk<false, true, kSceneShortStack>(sv, n, counter, spill, kSceneRefill,
+                                                   begin, store, cnt, rays);
 }
 
 // Logical block index spread so consecutive ranges of work stay on one XCD
@@ -594,8 +594,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
         p.paths.hinst[i] = s.out.inst;
         p.paths.hprim[i] = s.out.prim;
     };
-    persistent_walk<kCount, true, kSceneRefill, kSceneShortStack>(sv, n, counter, spill,
-                                                                  begin, store, cnt, rays);
+    const uint32_t refill = p.refill >= 1u && p.refill <= 64u ? p.refill : kSceneRefill;
+    persistent_walk<kCount, true, kSceneShortStack>(sv, n, counter, spill, refill, begin,
+                                                    store, cnt, rays);
     if (kCount)
     {
         wave_add(&p.paths.metrics[3], cnt.box);
@@ -1262,23 +1263,27 @@ __global__ __launch_bounds__(1024) void k_guide_emit(TraceParams p)
 
 // Guide inference for every queued ray that will scatter: hit surface ->
 // encoding -> MLP (FP16 MFMA, FP32 accumulation) -> vMF mixture, stored
-// per queue entry for k_shade.  Blocks of kInferWaves waves share the
-// weights in LDS; each wave evaluates 64 rays per step.
-constexpr int kInferWaves = 8;
+// per queue entry for k_shade.  One block of kInferWaves waves per CU
+// shares the weights and biases in LDS (41,344 + 1,216 bytes); each wave
+// evaluates 64 rays per step through 4,608 bytes of scratch, so the 16
+// waves (four per SIMD) fit in one CU's LDS.
+constexpr int kInferWaves = 16;
 template <bool kBatch>
 __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
                                                                   int q)
 {
     __shared__ _Float16 sW[kWLdsHalves];
-    __shared__ float waveOut[kInferWaves][kWaveScratchFloats];
+    __shared__ float sBias[kGRows];
+    __shared__ float waveOut[kInferWaves][kTileScratchFloats];
     stage_rows(sW, p.guide.w16, kGRows);
+    for (int k = threadIdx.x; k < kGRows; k += blockDim.x) sBias[k] = p.guide.bias[k];
     __syncthreads();
     const uint32_t n = p.paths.counters[q];
     const uint32_t cap = p.paths.capacity;
     const float *qo = p.paths.qo[q], *qd = p.paths.qd[q];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float *scr = waveOut[wave];
-    _Float16 *enc = (_Float16 *)scr;
+    unsigned long long evals = 0;
     for (uint32_t tile = xcd_block() * kInferWaves + wave; tile * 64 < n;
          tile += gridDim.x * kInferWaves)
     {
@@ -1310,33 +1315,14 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
                 }
             }
         }
-        const unsigned long long validMask = __ballot(valid);
-        if (lane == 0)
-            atomicAdd(&p.paths.metrics[7], (unsigned long long)__popcll(validMask));
+        evals += (unsigned long long)__popcll(__ballot(valid));  // wave-uniform
         if (p.guide.analytic) continue;  // wave-uniform
-        encode_row(enc + lane * kLdsPitch, valid, pn, N, V);
-        wave_lds_sync();
-        h8 B[4][2];
-        load_input(enc, B);
-        wave_lds_sync();
-        f4 acc[4][4];
-        for (int l = 0; l < 4; ++l)
-        {
-            layer64(sW, p.guide.bias, l, B, acc);
-            relu_to_b(acc, B);
-        }
-        f4 out[4][3];
-        layer_out(sW, p.guide.bias, B, out);
-        store_outputs(scr, out);
-        wave_lds_sync();
-        if (valid)
-        {
-            Lobes L;
-            outputs_to_lobes(scr + lane, L);
-            store_lobes(p.guide.lobes, cap, i, L);
-        }
-        wave_lds_sync();
+        Lobes L;
+        guide_forward_tile(sW, sBias, scr, valid, pn, N, V, L, [](int, const float *) {});
+        if (valid) store_lobes(p.guide.lobes, cap, i, L);
     }
+    // one counter update per wave (every lane reaches this point)
+    if (lane == 0 && evals) atomicAdd(&p.paths.metrics[7], evals);
 }
 
 template <bool kBatch>

@@ -469,16 +469,17 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
 }
 
 // Persistent driver: every wave walks rays [0, n) taken in chunks of
-// kRayChunk from *counter (zero on entry).  begin(ray, s) starts a ray and
-// returns false if it needs no walk (it is then stored at once);
+// kRayChunk from *counter (zero on entry), refilling its idle lanes once
+// `refill` (1..64, wave-uniform) of them wait.  begin(ray, s) starts a ray
+// and returns false if it needs no walk (it is then stored at once);
 // store(ray, s) writes the result.  Must be called by every lane of a
 // one-wave block.
-template <bool kCount, bool kTwoLevel, uint32_t kRefill, int kShort, class Begin,
-          class Store>
+template <bool kCount, bool kTwoLevel, int kShort, class Begin, class Store>
 __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
                                                 uint32_t *counter, int *spill,
-                                                Begin begin, Store store,
-                                                Counts &cnt, uint32_t &rays)
+                                                uint32_t refill, Begin begin,
+                                                Store store, Counts &cnt,
+                                                uint32_t &rays)
 {
     __shared__ int lds[kShort * 64];
     const uint32_t lane = threadIdx.x;
@@ -495,7 +496,7 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
         bool need = ray == SPG_MISS;
         unsigned long long needMask = __ballot(need);
         const uint32_t idle = (uint32_t)__popcll(needMask);
-        if ((idle >= kRefill || idle == 64u) && idle != 0u &&
+        if ((idle >= refill || idle == 64u) && idle != 0u &&
             !(exhausted && chunkNext >= chunkEnd))
         {
             if (chunkNext >= chunkEnd)
