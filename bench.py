@@ -948,6 +948,103 @@ def bench_copy(args, V, scenes):
                                        "equal": ok}}
 
 
+def bench_tiles(args, V, scenes):
+    """The drop-in path as the reference's worker pool drives it
+    (src/main.cpp:731-759, src/config.h:21): 16 host threads pull 64x64
+    tiles of the config5 image from a shared counter and call
+    spg_path_trace_tile on one shared context, each tile's pixels written
+    into one host image (host memory: the rate includes the device-to-host
+    copies).  One step = every tile once at 1 spp, 6 bounces, plain
+    unbiased tracer, double-evaluated transcendentals (the headline
+    configuration).  Also: the same frame as one spg_path_trace_tiles call
+    into host memory, and the device-resident call of the headline line."""
+    import threading
+
+    dev = V.Device(0)
+    desc = scenes.config5()
+    w, h = desc["camera"][0], desc["camera"][1]
+    s, cam = scenes.load_gpu(dev, desc)
+    tiles = V.ComputeTiles(w, h, 64, 64)
+    img = np.zeros(w * h * 4, np.float32)
+    threads = 16
+
+    def ctx_for(step):
+        return V.make_context(cam, s, spp=1, bounces=6, clamp=0.0,
+                              estimator=V.EST_UNBIASED, rng=V.RNG_PCG, seed=0x5EED,
+                              transcendentals=V.TX_DOUBLE, sample_offset=step)
+
+    def pool_frame(ctx):
+        lock = threading.Lock()
+        nxt = [0]
+        errors = []
+
+        def work():
+            try:
+                while True:
+                    with lock:
+                        i = nxt[0]
+                        nxt[0] += 1
+                    if i >= len(tiles):
+                        return
+                    V.sp_PathTraceTile(ctx, tiles[i], img)
+            except Exception as e:  # noqa: BLE001 - raised below
+                errors.append(e)
+
+        ts = [threading.Thread(target=work) for _ in range(threads)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        if errors:
+            raise errors[0]
+
+    for k in range(args.warmup):
+        pool_frame(ctx_for(k))
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        pool_frame(ctx_for(args.warmup + k))
+    wall = time.perf_counter() - t0
+    # the same frames as one batched host-memory call each, and device-resident
+    t1 = time.perf_counter()
+    for k in range(args.steps):
+        V.path_trace_tiles(ctx_for(args.warmup + k), tiles, img)
+    wall_batched = time.perf_counter() - t1
+    dimg = dev.alloc(w * h * 16)
+    V.path_trace_tiles_device(ctx_for(0), tiles, dimg)
+    dev.synchronize()
+    t2 = time.perf_counter()
+    for k in range(args.steps):
+        V.path_trace_tiles_device(ctx_for(args.warmup + k), tiles, dimg)
+    dev.synchronize()
+    wall_device = time.perf_counter() - t2
+    dimg.free()
+    samples = w * h
+    res = {
+        "metric": "Msamples/s through the drop-in tile call (16 host threads, "
+                  "64x64 tiles, host image)",
+        "value": samples * args.steps / wall / 1e6,
+        "unit": "Msamples/s",
+        "ms_per_step": wall * 1e3 / args.steps,
+        "dtype": "f32",
+        "config": {"workload": f"config5: {desc['triangles']} triangles, {w}x{h}, "
+                               f"{len(tiles)} tiles of 64x64 traced by {threads} "
+                               f"host threads calling spg_path_trace_tile on one "
+                               f"shared context into host memory, 1 spp per "
+                               f"frame, 6 bounces, plain unbiased tracer, "
+                               f"double-evaluated transcendentals",
+                   "tiles": len(tiles), "threads": threads},
+        "extra": {"tiles_per_s": len(tiles) * args.steps / wall,
+                  "batched_host_call_Msamples_per_s":
+                      samples * args.steps / wall_batched / 1e6,
+                  "device_resident_Msamples_per_s":
+                      samples * args.steps / wall_device / 1e6,
+                  "image_bytes_to_host_per_frame": 16 * samples},
+    }
+    s.close()
+    dev.close()
+    return res
+
+
 def bench_post(args, V, scenes):
     """The step after the path on a 1920x1080 RGBA32F image: progressive
     accumulation of a new image into the running mean, then the viewer's
@@ -1033,7 +1130,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="config5",
                     choices=["config5", "config4", "config3", "config1", "soup",
-                             "irradiance", "post", "copy"])
+                             "irradiance", "post", "copy", "tiles"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fast", action="store_true",
                     help="device float transcendentals instead of double-evaluated")
@@ -1085,7 +1182,8 @@ def main():
     else:
         res = {"config1": bench_config1, "config3": bench_config3,
                "soup": bench_soup, "irradiance": bench_irradiance,
-               "post": bench_post, "copy": bench_copy}[args.workload](
+               "post": bench_post, "copy": bench_copy,
+               "tiles": bench_tiles}[args.workload](
                    args, V, scenes)
     if args.workload == "config5" and not args.no_relmse:
         # the second half of the metric: wall-clock time to relMSE 1e-3,
