@@ -321,9 +321,11 @@ def bench_box(args, V, scenes, name):
     tx = V.TX_FAST if args.fast else V.TX_DOUBLE
     spp = args.spp or (1 if name == "config5" else 2)
 
+    est = V.EST_NEE if args.estimator == "nee" else V.EST_UNBIASED
+
     def ctx_for(step, collect=False):
         return V.make_context(cam, s, spp=spp, bounces=bounces, clamp=0.0,
-                              estimator=V.EST_UNBIASED, rng=V.RNG_PCG, seed=0x5EED,
+                              estimator=est, rng=V.RNG_PCG, seed=0x5EED,
                               transcendentals=tx, sample_offset=step * spp,
                               collect=collect)
 
@@ -370,7 +372,9 @@ def bench_box(args, V, scenes, name):
                                f"{len(desc['objects']) - 1} displaced icospheres of "
                                f"20480 in a closed box with a 1x1 opening), sky cube "
                                f"6x64x64 (sun 50), {w}x{h}, {spp} spp per step, "
-                               f"{bounces} bounces, plain unbiased tracer, per-pixel "
+                               f"{bounces} bounces, "
+                               f"{'NEE' if args.estimator == 'nee' else 'plain unbiased'} "
+                               f"tracer, per-pixel "
                                f"PCG, {'fast' if args.fast else 'double-evaluated'} "
                                f"transcendentals",
                    "image": [w, h], "spp_per_step": spp, "bounces": bounces,
@@ -1142,6 +1146,8 @@ def main():
     ap.add_argument("--spp", type=int, default=None,
                     help="config4/config5: samples per pixel per step of the plain "
                          "tracer (default 1 on config5, 2 on config4)")
+    ap.add_argument("--estimator", choices=["unbiased", "nee"], default="unbiased",
+                    help="config4/config5: the first tracer's estimator")
     ap.add_argument("--no-guided", action="store_true",
                     help="config4/config5: the plain tracer only (for counter runs)")
     ap.add_argument("--no-relmse", action="store_true",

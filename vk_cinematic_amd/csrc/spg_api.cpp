@@ -191,6 +191,7 @@ static void delete_device(spg_Device *d)
     hipStreamSynchronize(d->stream);
     free_paths(d);
     free_ptr(d->image);
+    if (d->hostStage) hipHostFree(d->hostStage);
     free_ptr(d->slotPixel);
     free_ptr(d->accum);
     free_ptr(d->persistBlock);
@@ -1071,12 +1072,6 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     std::lock_guard<std::mutex> g(d->lock);
     HIPCHK(hipSetDevice(d->ordinal));
     if (s->dirty) CHK(upload_tables(s));
-    if (!devImage)
-    {
-        CHK(ensure_buffer<float>(d->image, d->imageCount,
-                                 (size_t)c.width * c.height * 4));
-        devImage = (float *)d->image;
-    }
 
     // pixel slots in tile order (clamped to the image as
     // sp_PathTraceTile clamps, src/simd_path_tracer.cpp:186-189); kept from
@@ -1101,6 +1096,24 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
         d->slotsOnDevice = false;
     }
     const std::vector<uint32_t> &slots = d->slots;
+    if (!devImage)
+    {
+        // host pixels: the tiles' pixels gathered in slot order on the
+        // device, then one copy into pinned host memory and a scatter of
+        // the tile rows into the caller's image (a strided copy per tile
+        // into pageable memory costs a runtime round trip per row)
+        const size_t bytes = std::max<size_t>(slots.size(), 1) * 16;
+        CHK(ensure_buffer<float>(d->image, d->imageCount, bytes / 4));
+        devImage = (float *)d->image;
+        if (d->hostStageBytes < bytes)
+        {
+            if (d->hostStage) hipHostFree(d->hostStage);
+            d->hostStage = nullptr;
+            d->hostStageBytes = 0;
+            HIPCHK(hipHostMalloc(&d->hostStage, bytes, hipHostMallocDefault));
+            d->hostStageBytes = bytes;
+        }
+    }
     const uint32_t spp = cfg.samplesPerPixel;
     const bool pcg = cfg.rngMode == SPG_RNG_PCG;
     uint64_t want = (uint64_t)slots.size() * (pcg ? spp : 1);
@@ -1211,29 +1224,33 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
             }
             done += sp;
         }
-        HIPCHK(launch_write_pixels(d->stream, p, devImage));
+        if (hostImage) HIPCHK(launch_write_slots(d->stream, p, devImage + 4 * s0));
+        else HIPCHK(launch_write_pixels(d->stream, p, devImage));
     }
     hipEventRecord(d->ev[kEvents - 1], d->stream);
     unsigned long long m[16] = {0};
     HIPCHK(hipMemcpyAsync(m, d->paths.metrics, 128, hipMemcpyDeviceToHost,
                           d->stream));
+    if (hostImage && !slots.empty())
+        HIPCHK(hipMemcpyAsync(d->hostStage, devImage, slots.size() * 16,
+                              hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
     if (hostImage)
     {
+        // slot order is tile order, row by row within a tile, so each
+        // tile row is one contiguous run of the staging copy
+        const char *src = (const char *)d->hostStage;
         for (uint32_t k = 0; k < count; ++k)
         {
             uint32_t x1 = std::min(tiles[k].maxX, c.width);
             uint32_t y1 = std::min(tiles[k].maxY, c.height);
             if (x1 <= tiles[k].minX || y1 <= tiles[k].minY) continue;
-            size_t pitch = (size_t)c.width * 16;
-            size_t off = ((size_t)tiles[k].minY * c.width + tiles[k].minX) * 16;
-            HIPCHK(hipMemcpy2DAsync((char *)hostImage + off, pitch,
-                                    (const char *)devImage + off, pitch,
-                                    (size_t)(x1 - tiles[k].minX) * 16,
-                                    y1 - tiles[k].minY, hipMemcpyDeviceToHost,
-                                    d->stream));
+            const size_t run = (size_t)(x1 - tiles[k].minX) * 16;
+            for (uint32_t y = tiles[k].minY; y < y1; ++y, src += run)
+                memcpy((char *)hostImage + ((size_t)y * c.width + tiles[k].minX) * 16, src,
+                       run);
         }
     }
-    HIPCHK(hipStreamSynchronize(d->stream));
     auto t1 = std::chrono::steady_clock::now();
     for (int k = 0; k < 6; ++k) d->lastMs[k] = 0.0;
     for (int k = 0; k < tm.next; k += 2)

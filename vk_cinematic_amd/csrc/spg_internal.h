@@ -271,8 +271,10 @@ struct spg_Device
     spg::DevPaths paths{};
     size_t pathsBytes = 0;
     void *pathsBlock = nullptr;
-    void *image = nullptr;       // staging image for host-pixel calls
-    size_t imageCount = 0;
+    void *image = nullptr;       // host-pixel calls: the tiles' pixels in
+    size_t imageCount = 0;       // slot order (RGBA32F per slot)
+    void *hostStage = nullptr;   // pinned host copy of `image`
+    size_t hostStageBytes = 0;
     void *slotPixel = nullptr;   // pixel index per slot
     size_t slotPixelCount = 0;
     // the slot list of the last trace call (pixels of its tiles in tile
@@ -433,6 +435,8 @@ hipError_t launch_finalize(hipStream_t s, const TraceParams &p);
 hipError_t launch_guide_collect(hipStream_t s, const TraceParams &p);
 hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,
                                float *image);
+// the pass's pixels in slot order: out[slot] (single-scene launches)
+hipError_t launch_write_slots(hipStream_t s, const TraceParams &p, float *out);
 hipError_t launch_clear_counters(hipStream_t s, uint32_t *counters, int n);
 hipError_t launch_set_counter(hipStream_t s, uint32_t *counter, uint32_t v);
 hipError_t launch_clear_floats(hipStream_t s, float *p, size_t n);

@@ -40,6 +40,62 @@ __device__ __forceinline__ void wave_add(unsigned long long *dst,
     if (lane_id() == 0 && s) atomicAdd(dst, s);
 }
 
+// Queue positions for a 256-thread block: every wave's count of entries
+// for queue A (and B) is added with one atomic per block and queue, so a
+// launch makes one atomic per block-iteration instead of one per
+// wave-iteration (same-address atomics serialize in L2).  Returns the
+// wave's first position in each queue.  Called by every thread of the
+// block (block-uniform control flow); B may be null.
+__device__ __forceinline__ void block_queue_bases(uint32_t *qa, uint32_t na,
+                                                  uint32_t *qb, uint32_t nb,
+                                                  uint32_t &baseA, uint32_t &baseB)
+{
+    __shared__ uint32_t cnt[2][4], bases[2][4];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0)
+    {
+        cnt[0][w] = na;
+        cnt[1][w] = nb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2)
+    {
+        const int k = threadIdx.x;
+        uint32_t *q = k ? qb : qa;
+        const uint32_t c0 = cnt[k][0], c1 = cnt[k][1], c2 = cnt[k][2], c3 = cnt[k][3];
+        const uint32_t all = c0 + c1 + c2 + c3;
+        const uint32_t b = (q && all) ? atomicAdd(q, all) : 0u;
+        bases[k][0] = b;
+        bases[k][1] = b + c0;
+        bases[k][2] = b + c0 + c1;
+        bases[k][3] = b + c0 + c1 + c2;
+    }
+    __syncthreads();
+    baseA = bases[0][w];
+    baseB = bases[1][w];
+}
+
+// dst[0] += a, dst[1] += b summed over a 256-thread block, one atomic each.
+__device__ __forceinline__ void block_add2(unsigned long long *dst, unsigned long long a,
+                                           unsigned long long b)
+{
+    __shared__ unsigned long long part[2][4];
+    const unsigned long long sa = wave_sum(a), sb = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0)
+    {
+        part[0][w] = sa;
+        part[1][w] = sb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2)
+    {
+        const unsigned long long t = part[threadIdx.x][0] + part[threadIdx.x][1] +
+                                     part[threadIdx.x][2] + part[threadIdx.x][3];
+        if (t) atomicAdd(&dst[threadIdx.x], t);
+    }
+}
+
 // Surface at a hit (src/sp_scene.cpp:196-213 and :302-312): uv from the
 // barycentrics, smooth normal normalize(w n0 + u n1 + v n2) or the MT normal
 // cross(e1, e2), then world normal normalize(M n).
@@ -1005,18 +1061,15 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
             }
             store_rng(p, path, rng);
         }
-        // compaction of continuing rays into queue nq
-        unsigned long long mask = __ballot(push);
-        uint32_t total = (uint32_t)__popcll(mask);
-        uint32_t base = 0;
-        uint32_t lane = lane_id();
-        if (total)
-        {
-            int leader = __ffsll((long long)mask) - 1;
-            if ((int)lane == leader)
-                base = atomicAdd(&p.paths.counters[nq], total);
-            base = __shfl(base, leader);
-        }
+        // compaction of continuing rays into queue nq (and of shadow rays
+        // into queue 2): one atomic per block and queue
+        const uint32_t lane = lane_id();
+        const unsigned long long mask = __ballot(push);
+        const unsigned long long smask = kEst == SPG_EST_NEE ? __ballot(shadow) : 0ull;
+        uint32_t base, sbase;
+        block_queue_bases(&p.paths.counters[nq], (uint32_t)__popcll(mask),
+                          kEst == SPG_EST_NEE ? &p.paths.counters[2] : nullptr,
+                          (uint32_t)__popcll(smask), base, sbase);
         if (push)
         {
             uint32_t slotq = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
@@ -1030,16 +1083,6 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
         }
         if (kEst == SPG_EST_NEE)
         {
-            // shadow rays into queue 2
-            unsigned long long smask = __ballot(shadow);
-            uint32_t stotal = (uint32_t)__popcll(smask);
-            uint32_t sbase = 0;
-            if (stotal)
-            {
-                int leader = __ffsll((long long)smask) - 1;
-                if ((int)lane == leader) sbase = atomicAdd(&p.paths.counters[2], stotal);
-                sbase = __shfl(sbase, leader);
-            }
             if (shadow)
             {
                 uint32_t k = sbase + (uint32_t)__popcll(smask & ((1ull << lane) - 1ull));
@@ -1053,8 +1096,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
             }
         }
     }
-    wave_add(&p.paths.metrics[1], hits);
-    wave_add(&p.paths.metrics[2], misses);
+    block_add2(&p.paths.metrics[1], hits, misses);
 }
 
 // Shadow rays of next-event estimation (queue 2, traced by k_extend): one
@@ -1337,6 +1379,17 @@ __global__ void k_write_pixels(TraceParams p, float *image)
         float *dst = kBatch ? p.images[p.slotItem[slot]] : image;
         reinterpret_cast<float4 *>(dst)[pixel] = c;
     }
+}
+
+// Host-pixel calls: the chunk's pixels in slot order (tile order), so one
+// contiguous copy brings every tile to the host.
+__global__ void k_write_slots(TraceParams p, float *out)
+{
+    for (uint32_t slot = blockIdx.x * 256 + threadIdx.x; slot < p.slots;
+         slot += gridDim.x * 256)
+        reinterpret_cast<float4 *>(out)[slot] =
+            make_float4(p.accum[slot], p.accum[p.slots + slot],
+                        p.accum[2 * p.slots + slot], 1.0f);
 }
 
 __global__ void k_clear_u32(uint32_t *p, int n)
@@ -1643,6 +1696,13 @@ hipError_t launch_write_pixels(hipStream_t s, const TraceParams &p,
     else
         hipLaunchKernelGGL(k_write_pixels<false>, dim3(grid_for(p.slots, 256, 4096)),
                            dim3(256), 0, s, p, image);
+    return hipGetLastError();
+}
+
+hipError_t launch_write_slots(hipStream_t s, const TraceParams &p, float *out)
+{
+    hipLaunchKernelGGL(k_write_slots, dim3(grid_for(p.slots, 256, 4096)), dim3(256), 0, s,
+                       p, out);
     return hipGetLastError();
 }
 
