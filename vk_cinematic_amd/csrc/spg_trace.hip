@@ -473,8 +473,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_mesh(
         if (kCount && s.h.prim != SPG_MISS) hits++;
     };
     SceneView sv{};
-    persistent_walk<kCount, false, kMeshShortStack>(sv, n, counter, spill, kMeshRefill,
-                                                    begin, store, cnt, rays);
+    persistent_walk<kCount, false, kMeshShortStack, kMeshChunk>(
+        sv, n, counter, spill, kMeshRefill, begin, store, cnt, rays);
     if (kCount)
     {
         wave_add(&stats[0], rays);
@@ -522,8 +522,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_scene(
         if (objOut) objOut[i] = h.inst;
         if (primOut) primOut[i] = h.prim;
     };
-    persistent_walk<false, true, kSceneShortStack>(sv, n, counter, spill, kSceneRefill,
-                                                   begin, store, cnt, rays);
+    persistent_walk<false, true, kSceneShortStack, kSceneChunk>(
+        sv, n, counter, spill, kSceneRefill, begin, store, cnt, rays);
 }
 
 // Logical block index spread so consecutive ranges of work stay on one XCD
@@ -651,8 +651,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
         p.paths.hprim[i] = s.out.prim;
     };
     const uint32_t refill = p.refill >= 1u && p.refill <= 64u ? p.refill : kSceneRefill;
-    persistent_walk<kCount, true, kSceneShortStack>(sv, n, counter, spill, refill, begin,
-                                                    store, cnt, rays);
+    persistent_walk<kCount, true, kSceneShortStack, kSceneChunk>(
+        sv, n, counter, spill, refill, begin, store, cnt, rays);
     if (kCount)
     {
         wave_add(&p.paths.metrics[3], cnt.box);
@@ -1444,9 +1444,9 @@ size_t spill_ints()
            (size_t)(kStackDepth - kMinShortStack);
 }
 
-static uint32_t persist_blocks(uint32_t grid, uint64_t n)
+static uint32_t persist_blocks(uint32_t grid, uint64_t n, uint32_t chunk = kSceneChunk)
 {
-    uint64_t need = (n + kRayChunk - 1) / kRayChunk;
+    uint64_t need = (n + chunk - 1) / chunk;
     if (need < 1) need = 1;
     return need < grid ? (uint32_t)need : grid;
 }
@@ -1465,11 +1465,11 @@ hipError_t launch_closest_mesh(hipStream_t s, const DevMesh *mesh,
     static uint32_t gPlain = persistent_grid(k_closest_mesh<false>);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, ps.counter, 0u);
     if (stats)
-        hipLaunchKernelGGL(k_closest_mesh<true>, dim3(persist_blocks(gCount, n)),
+        hipLaunchKernelGGL(k_closest_mesh<true>, dim3(persist_blocks(gCount, n, kMeshChunk)),
                            dim3(kTraceBlock), 0, s, *mesh, ox, oy, oz, dx, dy, dz,
                            n, prim, t, u, v, stats, ps.counter, ps.spill);
     else
-        hipLaunchKernelGGL(k_closest_mesh<false>, dim3(persist_blocks(gPlain, n)),
+        hipLaunchKernelGGL(k_closest_mesh<false>, dim3(persist_blocks(gPlain, n, kMeshChunk)),
                            dim3(kTraceBlock), 0, s, *mesh, ox, oy, oz, dx, dy, dz,
                            n, prim, t, u, v, stats, ps.counter, ps.spill);
     return hipGetLastError();
@@ -1510,7 +1510,7 @@ hipError_t launch_closest_mesh_noalloc(hipStream_t s, const DevMesh *mesh,
     uint32_t *counter = (uint32_t *)scratchPtr;
     int *spill = (int *)((char *)scratchPtr + 256);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, counter, 0u);
-    hipLaunchKernelGGL(k_closest_mesh<false>, dim3(persist_blocks(g, n)),
+    hipLaunchKernelGGL(k_closest_mesh<false>, dim3(persist_blocks(g, n, kMeshChunk)),
                        dim3(kTraceBlock), 0, s, *mesh, ox, oy, oz, dx, dy, dz, n, prim,
                        t, u, v, nullptr, counter, spill);
     return hipGetLastError();

@@ -28,7 +28,12 @@
 
 namespace spg {
 
-constexpr uint32_t kRayChunk = 64;
+// Rays a wave takes from the shared counter with one atomic: the scene walk
+// of path-tracing rays is fastest at 64 (128: same, 256: -8% on the 4M-
+// triangle box scene), the single-mesh walk of incoherent rays at 128
+// (+23% over 64 on the 1M-triangle soup: with 64, the 65,536 same-address
+// atomics of a 2^22-ray launch serialize in L2; 256: -2% against 128).
+constexpr uint32_t kSceneChunk = 64, kMeshChunk = 128;
 // A wave refills its idle lanes once kRefill of them wait (or all do): each
 // refill stalls the wave on the new rays' loads and setup, so refilling one
 // lane per iteration adds a memory round trip to nearly every step, while
@@ -469,12 +474,13 @@ __device__ __forceinline__ void trav_begin_mesh(const DevMesh &M, Trav &s, v3 o,
 }
 
 // Persistent driver: every wave walks rays [0, n) taken in chunks of
-// kRayChunk from *counter (zero on entry), refilling its idle lanes once
+// kChunk from *counter (zero on entry), refilling its idle lanes once
 // `refill` (1..64, wave-uniform) of them wait.  begin(ray, s) starts a ray
 // and returns false if it needs no walk (it is then stored at once);
 // store(ray, s) writes the result.  Must be called by every lane of a
 // one-wave block.
-template <bool kCount, bool kTwoLevel, int kShort, class Begin, class Store>
+template <bool kCount, bool kTwoLevel, int kShort, uint32_t kChunk, class Begin,
+          class Store>
 __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
                                                 uint32_t *counter, int *spill,
                                                 uint32_t refill, Begin begin,
@@ -502,7 +508,7 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
             if (chunkNext >= chunkEnd)
             {
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(counter, kRayChunk);
+                if (lane == 0) base = atomicAdd(counter, kChunk);
                 // wave-uniform (a scalar register), so the loop's exit is
                 // uniform and the compiler keeps no per-lane copies of the
                 // lane state across it
@@ -515,8 +521,8 @@ __device__ __forceinline__ void persistent_walk(SceneView &sv, uint32_t n,
                 else
                 {
                     chunkNext = base;
-                    chunkEnd = base + kRayChunk < n ? base + kRayChunk : n;
-                    if (base + kRayChunk >= n) exhausted = true;
+                    chunkEnd = base + kChunk < n ? base + kChunk : n;
+                    if (base + kChunk >= n) exhausted = true;
                 }
             }
             uint32_t want = idle;
