@@ -819,7 +819,8 @@ def bench_relmse(args, V, scenes):
                          guide_kw=dict(seed=args.guide_seed, learning_rate=args.guide_lr,
                                        batch_size=args.guide_batch,
                                        steps_per_pass=args.guide_steps,
-                                       max_weight=args.guide_max_weight))
+                                       max_weight=args.guide_max_weight,
+                                       bsdf_fraction=args.guide_bsdf))
     print(json.dumps(res))
     dev.close()
 
@@ -1165,6 +1166,9 @@ def main():
     ap.add_argument("--guide-steps", type=int, default=4)
     ap.add_argument("--guide-max-weight", type=float, default=10.0)
     ap.add_argument("--guide-seed", type=int, default=1)
+    ap.add_argument("--guide-bsdf", type=float, default=0.5,
+                    help="probability of the BSDF sampler in the guided "
+                         "estimator's one-sample MIS")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "

@@ -143,6 +143,13 @@ typedef struct spo_Scene {
     uint32_t *emitAlias, *emitPair;
     int32_t *emitBase;
     uint32_t emitCount;
+    /* the spatial emitter choice (emitInstCount > 0; spo_scene_emitter_
+     * instances): 8 floats per emitter instance, emitter index per object,
+     * per-entry alias tables over each instance's triangles by area */
+    float *emitInst, *emitProbL;
+    int32_t *emitSlot;
+    uint32_t *emitAliasL;
+    uint32_t emitInstCount;
     float envSelect;     /* probability of sampling the environment */
 } spo_Scene;
 
@@ -226,6 +233,19 @@ int spo_scene_build_env(spo_Scene *s);
  * base: one entry per object); returns the entry count. */
 uint32_t spo_scene_emitter_table(const spo_Scene *s, float *pdfA, float *prob,
                                  uint32_t *alias, uint32_t *pair, int32_t *base);
+/* The spatial emitter choice's tables (next-event estimation; no reference
+ * counterpart), given spo_scene_emitter_table's base: per emitter instance
+ * 8 floats in rec (bounding-sphere centre xyz and radius of its world
+ * vertices, power = mean emission x world area, 1 / area, first table
+ * entry and triangle count as u32 bits); slot: per object its emitter index
+ * or -1; probL / aliasL (arrays of the table's entry count): per entry an
+ * alias table over its object's triangles by area, local indices.  Objects
+ * with zero total area get no record; more than 256 records give none.
+ * Returns the record count. */
+#define SPO_MAX_SPATIAL_EMITTERS 256
+uint32_t spo_scene_emitter_instances(const spo_Scene *s, const int32_t *base,
+                                     uint32_t entries, float *rec, int32_t *slot,
+                                     float *probL, uint32_t *aliasL);
 void spo_create_cubemap(const float *equirect, uint32_t ew, uint32_t eh,
                         uint32_t w, uint32_t h, float *out);
 void spo_create_irradiance_cubemap(const float *equirect, uint32_t ew, uint32_t eh,

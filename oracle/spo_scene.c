@@ -37,6 +37,7 @@ void spo_scene_destroy(spo_Scene *s)
     free(s->envAlias);
     free(s->emitPdfA); free(s->emitProb); free(s->emitAlias);
     free(s->emitPair); free(s->emitBase);
+    free(s->emitInst); free(s->emitProbL); free(s->emitSlot); free(s->emitAliasL);
     free(s);
 }
 
@@ -325,10 +326,85 @@ uint32_t spo_scene_emitter_table(const spo_Scene *s, float *pdfA, float *prob,
     return count;
 }
 
+/* Per emitter instance of the table (base >= 0): world vertices in
+ * double through the column-major model matrix (as the table's areas), the
+ * instance's bounding box and area sum in triangle order; an alias table
+ * over its triangles by area (restates build_emitter_instances of the
+ * product's spg_env_table.h). */
+uint32_t spo_scene_emitter_instances(const spo_Scene *s, const int32_t *base,
+                                     uint32_t entries, float *rec, int32_t *slot,
+                                     float *probL, uint32_t *aliasL)
+{
+    uint32_t ne = 0;
+    for (uint32_t g = 0; g < entries; ++g) { probL[g] = 1.0f; aliasL[g] = 0u; }
+    for (uint32_t i = 0; i < s->objectCount; ++i)
+    {
+        slot[i] = -1;
+        if (base[i] < 0) continue;
+        const spo_Object *o = &s->objects[i];
+        const spo_Mesh *me = &s->meshes[o->mesh];
+        const float *m = o->model.m;
+        const uint32_t nt = me->indexCount / 3;
+        double *area = (double *)malloc((nt ? nt : 1) * sizeof(double));
+        double total = 0.0, lo[3] = {INFINITY, INFINITY, INFINITY},
+               hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t k = 0; k < nt; ++k)
+        {
+            double w[3][3];
+            for (int v = 0; v < 3; ++v)
+            {
+                spo_v3 p = me->vertices[me->indices[3 * k + v]].p;
+                for (int r = 0; r < 3; ++r)
+                {
+                    w[v][r] = (double)m[r] * p.x + (double)m[4 + r] * p.y +
+                              (double)m[8 + r] * p.z + (double)m[12 + r];
+                    if (w[v][r] < lo[r]) lo[r] = w[v][r];
+                    if (w[v][r] > hi[r]) hi[r] = w[v][r];
+                }
+            }
+            double ux = w[1][0] - w[0][0], uy = w[1][1] - w[0][1], uz = w[1][2] - w[0][2];
+            double vx = w[2][0] - w[0][0], vy = w[2][1] - w[0][1], vz = w[2][2] - w[0][2];
+            double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+            double a = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            area[k] = isfinite(a) ? a : 0.0;
+            total += area[k];
+        }
+        if (!(total > 0.0) || !isfinite(total)) { free(area); continue; }
+        double *P = (double *)malloc((nt ? nt : 1) * sizeof(double));
+        for (uint32_t k = 0; k < nt; ++k) P[k] = area[k] / total;
+        alias_table(P, nt, probL + base[i], aliasL + base[i]);
+        free(P);
+        free(area);
+        if (ne < SPO_MAX_SPATIAL_EMITTERS)
+        {
+            double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+            uint32_t first = (uint32_t)base[i], count = nt;
+            float *r8 = rec + 8 * (size_t)ne;
+            r8[0] = (float)(0.5 * (lo[0] + hi[0]));
+            r8[1] = (float)(0.5 * (lo[1] + hi[1]));
+            r8[2] = (float)(0.5 * (lo[2] + hi[2]));
+            r8[3] = (float)(0.5 * sqrt(dx * dx + dy * dy + dz * dz));
+            r8[4] = (float)(object_emission(s, o) * total);
+            r8[5] = (float)(1.0 / total);
+            memcpy(&r8[6], &first, 4);
+            memcpy(&r8[7], &count, 4);
+        }
+        slot[i] = (int32_t)ne;
+        ++ne;
+    }
+    if (ne > SPO_MAX_SPATIAL_EMITTERS)
+    {
+        for (uint32_t i = 0; i < s->objectCount; ++i) slot[i] = -1;
+        ne = 0;
+    }
+    return ne;
+}
+
 int spo_scene_build_env(spo_Scene *s)
 {
     free(s->emitPdfA); free(s->emitProb); free(s->emitAlias);
     free(s->emitPair); free(s->emitBase);
+    free(s->emitInst); free(s->emitProbL); free(s->emitSlot); free(s->emitAliasL);
     s->emitPdfA = s->emitProb = NULL;
     s->emitAlias = s->emitPair = NULL;
     s->emitBase = NULL;
@@ -345,6 +421,15 @@ int spo_scene_build_env(spo_Scene *s)
         s->emitBase = (int32_t *)malloc(4 * (s->objectCount ? s->objectCount : 1));
         s->emitCount = spo_scene_emitter_table(s, s->emitPdfA, s->emitProb,
                                                s->emitAlias, s->emitPair, s->emitBase);
+        s->emitInst = (float *)malloc(4 * 8 * SPO_MAX_SPATIAL_EMITTERS);
+        s->emitSlot = (int32_t *)malloc(4 * (s->objectCount ? s->objectCount : 1));
+        s->emitProbL = (float *)malloc(4 * nn);
+        s->emitAliasL = (uint32_t *)malloc(4 * nn);
+        s->emitInstCount =
+            s->emitCount ? spo_scene_emitter_instances(s, s->emitBase, s->emitCount,
+                                                       s->emitInst, s->emitSlot,
+                                                       s->emitProbL, s->emitAliasL)
+                         : 0;
     }
     s->envSelect = 0.0f;
     free(s->envPdf); free(s->envProb); free(s->envAlias);

@@ -378,28 +378,101 @@ static void world_triangle(const spo_Scene *s, uint32_t i, uint32_t k, spo_v3 *A
     *C = m4point(o->model, m->vertices[m->indices[3 * k + 2]].p);
 }
 
-/* density per steradian of the emitter sampler reaching triangle k of
- * object i at distance t along unit direction d: pdfA t^2 / |cos| */
-static float emit_pdf(const spo_Scene *s, uint32_t i, uint32_t k, float t, spo_v3 d)
+/* The spatial emitter choice (emitInstCount > 0): emitter instance k seen
+ * from x with shading normal n weighs power / max(d^2, r^2), 0 when its
+ * bounding sphere lies wholly behind the tangent plane; sums in index
+ * order.  (The product's emit_weight, spg_trace.hip.) */
+static float emit_weight(const spo_Scene *s, uint32_t k, spo_v3 x, spo_v3 n)
+{
+    const float *a = s->emitInst + 8 * (size_t)k;
+    spo_v3 v = {a[0] - x.x, a[1] - x.y, a[2] - x.z};
+    float d2 = v3dot(v, v), r2 = a[3] * a[3];
+    return v3dot(n, v) > -a[3] ? a[4] / fmaxf(d2, r2) : 0.0f;
+}
+
+static float emit_weight_sum(const spo_Scene *s, spo_v3 x, spo_v3 n)
+{
+    float W = 0.0f;
+    for (uint32_t k = 0; k < s->emitInstCount; ++k) W += emit_weight(s, k, x, n);
+    return W;
+}
+
+/* density per unit area of the emitter sampler at x (normal n) on triangle
+ * k of object i: P(object | x, n) / area, or the global table's entry */
+static float emit_pdf_area(const spo_Scene *s, uint32_t i, uint32_t k, spo_v3 x, spo_v3 n)
 {
     int32_t b = s->emitBase[i];
     if (b < 0) return 0.0f;
+    if (s->emitInstCount == 0) return s->emitPdfA[b + k];
+    int32_t e = s->emitSlot[i];
+    if (e < 0) return 0.0f;
+    float W = emit_weight_sum(s, x, n);
+    if (!(W > 0.0f)) return 0.0f;
+    return emit_weight(s, (uint32_t)e, x, n) / W * s->emitInst[8 * (size_t)e + 5];
+}
+
+/* density per steradian of the emitter sampler at x (normal n) reaching
+ * triangle k of object i at distance t along unit direction d:
+ * pdfA t^2 / |cos| */
+static float emit_pdf(const spo_Scene *s, uint32_t i, uint32_t k, float t, spo_v3 d,
+                      spo_v3 x, spo_v3 nx)
+{
+    float pa = emit_pdf_area(s, i, k, x, nx);
+    if (!(pa > 0.0f)) return 0.0f;
     spo_v3 A, B, C;
     world_triangle(s, i, k, &A, &B, &C);
     spo_v3 n = v3cross(v3sub(B, A), v3sub(C, A));
     float c = fabsf(v3dot(n, d)) / sqrtf(v3dot(n, n));
-    return c > 0.0f ? s->emitPdfA[b + k] * (t * t) / c : 0.0f;
+    return c > 0.0f ? pa * (t * t) / c : 0.0f;
 }
 
-/* a point on an emissive triangle: entry by the alias table (r, e1),
- * uniform on the triangle (e2, e3); the unit direction from x and its
- * density per steradian */
+/* a point on an emissive triangle, uniform on the triangle (e2, e3); the
+ * unit direction from x and its density per steradian (0: nothing to
+ * sample).  With the spatial choice an emitter instance by its weight from
+ * (x, nx) -- the first whose running sum exceeds e4 x the total, else the
+ * last with a positive weight -- then a triangle of it by area (its alias
+ * table, r and e1); else an entry of the global table (r and e1). */
 static spo_v3 emit_direction(const spo_Scene *s, uint32_t r, float e1, float e2,
-                             float e3, spo_v3 x, float *pdf, uint32_t *inst,
-                             uint32_t *prim)
+                             float e3, float e4, spo_v3 x, spo_v3 nx, float *pdf,
+                             float *dist, uint32_t *inst, uint32_t *prim)
 {
-    uint32_t g = (uint32_t)(((uint64_t)r * s->emitCount) >> 32);
-    if (!(e1 < s->emitProb[g])) g = s->emitAlias[g];
+    uint32_t g;
+    float pa;
+    if (s->emitInstCount)
+    {
+        float W = emit_weight_sum(s, x, nx);
+        if (!(W > 0.0f))
+        {
+            *pdf = 0.0f;
+            *dist = 0.0f;
+            *inst = *prim = 0xFFFFFFFFu;
+            return v3(0.0f, 0.0f, 1.0f);
+        }
+        float target = e4 * W, acc = 0.0f, wp = 0.0f;
+        uint32_t pick = 0;
+        int found = 0;
+        for (uint32_t k = 0; k < s->emitInstCount; ++k)
+        {
+            float w = emit_weight(s, k, x, nx);
+            acc += w;
+            if (!found && w > 0.0f) { pick = k; wp = w; }
+            found = found || acc > target;
+        }
+        const float *b = s->emitInst + 8 * (size_t)pick;
+        uint32_t first, count;
+        memcpy(&first, &b[6], 4);
+        memcpy(&count, &b[7], 4);
+        uint32_t j = (uint32_t)(((uint64_t)r * count) >> 32);
+        if (!(e1 < s->emitProbL[first + j])) j = s->emitAliasL[first + j];
+        g = first + j;
+        pa = wp / W * b[5];
+    }
+    else
+    {
+        g = (uint32_t)(((uint64_t)r * s->emitCount) >> 32);
+        if (!(e1 < s->emitProb[g])) g = s->emitAlias[g];
+        pa = s->emitPdfA[g];
+    }
     *inst = s->emitPair[2 * g];
     *prim = s->emitPair[2 * g + 1];
     spo_v3 A, B, C;
@@ -412,7 +485,8 @@ static spo_v3 emit_direction(const spo_Scene *s, uint32_t r, float e1, float e2,
     spo_v3 w = v3norm(v);
     spo_v3 n = v3cross(v3sub(B, A), v3sub(C, A));
     float c = fabsf(v3dot(n, w)) / sqrtf(v3dot(n, n));
-    *pdf = c > 0.0f && dist2 > 0.0f ? s->emitPdfA[g] * dist2 / c : 0.0f;
+    *pdf = c > 0.0f && dist2 > 0.0f ? pa * dist2 / c : 0.0f;
+    *dist = sqrtf(dist2);
     return w;
 }
 
@@ -532,6 +606,7 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
     {
         const int nee = cfg->estimator == SPO_EST_NEE;
         float lastPdf = 0.0f;
+        spo_v3 lastN = v3(0.0f, 0.0f, 0.0f);  /* shading normal of the last scatter */
         spo_v3 beta = v3s(1.0f), L = v3(0, 0, 0);
         for (uint32_t bounce = 0; bounce < cfg->bounces; ++bounce)
         {
@@ -552,7 +627,7 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                     /* a BSDF-sampled ray that reached an emitter: its share
                      * against the emitter sampler (power heuristic) */
                     float pl = (1.0f - s->envSelect) *
-                               emit_pdf(s, h.object, h.prim, h.t, d);
+                               emit_pdf(s, h.object, h.prim, h.t, d, o, lastN);
                     float pb = lastPdf;
                     float pb2 = pb * pb;
                     float wm = pb2 / (pb2 + pl * pl);
@@ -578,12 +653,13 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                     float e0 = rng_u24(rng);
                     uint32_t r = rng_next(rng);
                     float e1 = rng_u24(rng), e2 = rng_u24(rng), e3 = rng_u24(rng);
+                    float e4 = rng_u24(rng);
                     int useEnv = e0 < s->envSelect;
                     if (useEnv ? s->envPdf != NULL : s->emitCount > 0)
                     {
                         spo_v3 x = v3add(v3add(o, v3mul(d, h.t)), v3mul(h.normal, 0.0001f));
                         spo_v3 wl, Le;
-                        float pl;
+                        float pl, dl = 0.0f;
                         uint32_t ti = 0xFFFFFFFFu, tp = 0;
                         if (useEnv)
                         {
@@ -601,13 +677,18 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                         else
                         {
                             float pg;
-                            wl = emit_direction(s, r, e1, e2, e3, x, &pg, &ti, &tp);
+                            wl = emit_direction(s, r, e1, e2, e3, e4, x, h.normal, &pg, &dl,
+                                                &ti, &tp);
                             pl = (1.0f - s->envSelect) * pg;
-                            float em[7];
-                            spo_v2 uv0 = {0, 0};
-                            spo_material_eval_internal(s, s->objects[ti].material, uv0,
-                                                       v3neg(wl), tx, em);
-                            Le = v3p(em + 3);
+                            Le = v3(0.0f, 0.0f, 0.0f);
+                            if (pg > 0.0f)  /* else nothing was sampled */
+                            {
+                                float em[7];
+                                spo_v2 uv0 = {0, 0};
+                                spo_material_eval_internal(s, s->objects[ti].material, uv0,
+                                                           v3neg(wl), tx, em);
+                                Le = v3p(em + 3);
+                            }
                         }
                         float cl = v3dot(h.normal, wl);
                         if (cl > 0.0f && pl > 0.0f)
@@ -623,7 +704,9 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                             v3put(so, x); v3put(sd, wl);
                             /* the sky is seen unless some object lies at a
                              * positive distance along the ray; an emitter
-                             * triangle must be the closest hit */
+                             * triangle must be the closest hit within 1.001 x
+                             * the light's distance (the product's shadow walk
+                             * starts with that bound as its best t) */
                             int visible;
                             if (ti == 0xFFFFFFFFu)
                             {
@@ -637,7 +720,8 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                                 if (spo_scene_intersect(s, so, sd, cfg->hit, tx, &sh,
                                                         metrics) < 0)
                                 { *fail = 1; return L; }
-                                visible = sh.t > 0.0f && sh.object == ti && sh.prim == tp;
+                                visible = sh.t > 0.0f && !(sh.t > dl * 1.001f) &&
+                                          sh.object == ti && sh.prim == tp;
                             }
                             metrics[2]++;
                             if (visible) L = v3add(L, C);
@@ -655,6 +739,7 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                     w = cosine / pdfc;
                 }
                 lastPdf = pdfc;
+                lastN = h.normal;
                 beta = v3had(beta, v3mul(X, w));
                 spo_v3 p = v3add(o, v3mul(d, h.t));
                 o = v3add(p, v3mul(h.normal, 0.0001f));

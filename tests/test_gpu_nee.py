@@ -113,3 +113,18 @@ def test_nee_error_paths(device):
     assert np.isfinite(img).all() and img.reshape(-1, 4)[:, :3].mean() > 0
     buf.free()
     s.close()
+
+
+def test_nee_spatial_emitter_choice_matches_oracle_bit_exact(device):
+    """Five emissive spheres of different size, height and power over a
+    floor (the spatial emitter choice: an emitter instance by power over
+    squared distance from the shading point, then a triangle by area):
+    bit-exact against the oracle in both RNG modes, with equal ray, hit and
+    miss counts."""
+    from test_nee import _lamps_scene
+    desc = _lamps_scene()
+    for rng, orng in [(V.RNG_PCG, SPO_RNG_PCG), (V.RNG_XORSHIFT_PIXEL, SPO_RNG_XORSHIFT_PIXEL)]:
+        img, m = gpu_render(device, desc, 8, 4, V.EST_NEE, rng, clamp=0.0)
+        px, met = oracle_render(desc, 8, 4, SPO_EST_NEE, orng, clamp=0.0)
+        assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
+        assert [m.values[k] for k in (1, 2, 3, 4)] == [int(met[k]) for k in (1, 2, 3, 4)]

@@ -576,6 +576,11 @@ static int upload_tables(spg_Scene *s)
     ds.emitAlias = ds.emitPair = nullptr;
     ds.emitBase = nullptr;
     ds.emitCount = 0;
+    ds.emitInst = nullptr;
+    ds.emitSlot = nullptr;
+    ds.emitProbL = nullptr;
+    ds.emitAliasL = nullptr;
+    ds.emitInstCount = 0;
     free_ptr(s->dEmit);
     {
         std::vector<EmitterInput> ein;
@@ -599,20 +604,39 @@ static int upload_tables(spg_Scene *s)
         std::vector<int32_t> base;
         if (build_emitter_table(ein, pdfA, prob, alias, pair, base))
         {
-            const size_t n = pdfA.size(), ni = base.size();
-            HIPCHK(hipMalloc(&s->dEmit, 20 * n + 4 * ni));
+            std::vector<float> rec, probL;
+            std::vector<int32_t> slot;
+            std::vector<uint32_t> aliasL;
+            build_emitter_instances(ein, base, rec, slot, probL, aliasL);
+            const size_t n = pdfA.size(), ni = base.size(), ne = rec.size() / 8;
+            // 16-byte records first, then the 4-byte arrays
+            const size_t oRec = 0, oPdf = 32 * ne, oProb = oPdf + 4 * n,
+                         oAlias = oProb + 4 * n, oPair = oAlias + 4 * n,
+                         oBase = oPair + 8 * n, oSlot = oBase + 4 * ni,
+                         oProbL = oSlot + 4 * ni, oAliasL = oProbL + 4 * n,
+                         bytes = oAliasL + 4 * n;
+            HIPCHK(hipMalloc(&s->dEmit, bytes));
             char *e = (char *)s->dEmit;
-            HIPCHK(hipMemcpy(e, pdfA.data(), 4 * n, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(e + 4 * n, prob.data(), 4 * n, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(e + 8 * n, alias.data(), 4 * n, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(e + 12 * n, pair.data(), 8 * n, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(e + 20 * n, base.data(), 4 * ni, hipMemcpyHostToDevice));
-            ds.emitPdfA = (const float *)e;
-            ds.emitProb = (const float *)(e + 4 * n);
-            ds.emitAlias = (const uint32_t *)(e + 8 * n);
-            ds.emitPair = (const uint32_t *)(e + 12 * n);
-            ds.emitBase = (const int32_t *)(e + 20 * n);
+            if (ne) HIPCHK(hipMemcpy(e + oRec, rec.data(), 32 * ne, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + oPdf, pdfA.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + oProb, prob.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + oAlias, alias.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + oPair, pair.data(), 8 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + oBase, base.data(), 4 * ni, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + oSlot, slot.data(), 4 * ni, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + oProbL, probL.data(), 4 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e + oAliasL, aliasL.data(), 4 * n, hipMemcpyHostToDevice));
+            ds.emitInst = (const float4 *)(e + oRec);
+            ds.emitPdfA = (const float *)(e + oPdf);
+            ds.emitProb = (const float *)(e + oProb);
+            ds.emitAlias = (const uint32_t *)(e + oAlias);
+            ds.emitPair = (const uint32_t *)(e + oPair);
+            ds.emitBase = (const int32_t *)(e + oBase);
+            ds.emitSlot = (const int32_t *)(e + oSlot);
+            ds.emitProbL = (const float *)(e + oProbL);
+            ds.emitAliasL = (const uint32_t *)(e + oAliasL);
             ds.emitCount = (uint32_t)n;
+            ds.emitInstCount = (uint32_t)ne;
         }
     }
     ds.envSelect = ds.envTex >= 0 ? (ds.emitCount ? 0.5f : 1.0f) : 0.0f;
@@ -957,6 +981,7 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
            oQo1 = take(12 * P), oQd0 = take(12 * P), oQd1 = take(12 * P),
            oQp0 = take(4 * P), oQp1 = take(4 * P), oQo2 = take(12 * P),
            oQd2 = take(12 * P), oQp2 = take(4 * P), oPdf = take(4 * P),
+           oLastN = take(12 * P),
            oNee = take(12 * P), oTgt = take(8 * P), oDist = take(4 * P),
            oCnt = take(64),
            oMet = take(128), oHt = take(4 * P), oHu = take(4 * P),
@@ -984,6 +1009,7 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
     p.qd[2] = (float *)(b + oQd2);
     p.qpath[2] = (uint32_t *)(b + oQp2);
     p.lastPdf = (float *)(b + oPdf);
+    p.lastN = (float *)(b + oLastN);
     p.nee = (float *)(b + oNee);
     p.neeTarget = (uint32_t *)(b + oTgt);
     p.neeDist = (float *)(b + oDist);
@@ -1370,6 +1396,7 @@ static int host_trace(const spg_Context *ctx, const spg_Tile *tiles,
                     std::chrono::steady_clock::now()};
     std::unique_lock<std::mutex> q(d->qlock);
     d->queue.push_back(&req);
+    d->qcv.notify_all();  // a leader gathering requests may be waiting
     while (!req.done)
     {
         if (d->leaderActive)
@@ -1378,6 +1405,15 @@ static int host_trace(const spg_Context *ctx, const spg_Tile *tiles,
             continue;
         }
         d->leaderActive = true;
+        // the callers of the last combined trace come back with their
+        // next tiles within microseconds: wait briefly for as many
+        // requests as it had, so a pool of callers keeps tracing its tiles
+        // together instead of one or two at a time
+        const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(500);
+        while (d->queue.size() < d->lastGroup &&
+               d->qcv.wait_until(q, until) != std::cv_status::timeout)
+        {
+        }
         std::vector<TileRequest *> group;
         TileRequest *first = d->queue.front();
         for (auto it = d->queue.begin(); it != d->queue.end();)
@@ -1390,6 +1426,7 @@ static int host_trace(const spg_Context *ctx, const spg_Tile *tiles,
             else
                 ++it;
         }
+        d->lastGroup = group.size();
         q.unlock();
         try
         {

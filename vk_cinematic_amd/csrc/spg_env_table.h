@@ -19,6 +19,9 @@
 
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
 
 #include <vector>
 
@@ -182,6 +185,94 @@ inline bool build_emitter_table(const std::vector<EmitterInput> &inst,
     }
     build_alias(P, prob, alias);
     return true;
+}
+
+// The spatial emitter choice of next-event estimation: per emitter
+// instance a bounding sphere and its power, so a shading point x can weigh
+// the instances by power / max(d^2, r^2) (0 for one wholly behind its
+// tangent plane) and then take a triangle of the chosen instance by area.
+// rec: 8 floats per emitter instance (centre xyz, radius, power = mean
+// emission x world area, 1 / area, first table entry and triangle count as
+// u32 bits); slot: per scene instance its emitter index or -1; probL /
+// aliasL: per table entry (the order of build_emitter_table) an alias table
+// over its instance's triangles by area, local indices.  Instances with
+// zero total area get no record; more than kMaxSpatialEmitters records
+// leave rec empty (the spatial choice off).  `base` is
+// build_emitter_table's.
+constexpr uint32_t kMaxSpatialEmitters = 256;
+inline void build_emitter_instances(const std::vector<EmitterInput> &inst,
+                                    const std::vector<int32_t> &base,
+                                    std::vector<float> &rec, std::vector<int32_t> &slot,
+                                    std::vector<float> &probL,
+                                    std::vector<uint32_t> &aliasL)
+{
+    rec.clear();
+    slot.assign(inst.size(), -1);
+    size_t entries = 0;
+    for (size_t i = 0; i < inst.size(); ++i)
+        if (base[i] >= 0) entries = std::max(entries, (size_t)base[i] + inst[i].triCount);
+    probL.assign(entries, 1.0f);
+    aliasL.assign(entries, 0u);
+    for (size_t i = 0; i < inst.size(); ++i)
+    {
+        if (base[i] < 0) continue;
+        const EmitterInput &e = inst[i];
+        const float *m = e.model;
+        std::vector<double> area(e.triCount);
+        double total = 0.0, lo[3] = {INFINITY, INFINITY, INFINITY},
+               hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t k = 0; k < e.triCount; ++k)
+        {
+            double w[3][3];
+            for (int v = 0; v < 3; ++v)
+            {
+                const float *p = e.verts + 8 * (size_t)e.idx[3 * k + v];
+                for (int r = 0; r < 3; ++r)
+                {
+                    w[v][r] = (double)m[r] * p[0] + (double)m[4 + r] * p[1] +
+                              (double)m[8 + r] * p[2] + (double)m[12 + r];
+                    lo[r] = std::min(lo[r], w[v][r]);
+                    hi[r] = std::max(hi[r], w[v][r]);
+                }
+            }
+            const double ux = w[1][0] - w[0][0], uy = w[1][1] - w[0][1],
+                         uz = w[1][2] - w[0][2];
+            const double vx = w[2][0] - w[0][0], vy = w[2][1] - w[0][1],
+                         vz = w[2][2] - w[0][2];
+            const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz,
+                         cz = ux * vy - uy * vx;
+            const double a = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            area[k] = std::isfinite(a) ? a : 0.0;
+            total += area[k];
+        }
+        if (!(total > 0.0) || !std::isfinite(total)) continue;
+        std::vector<double> P(e.triCount);
+        for (uint32_t k = 0; k < e.triCount; ++k) P[k] = area[k] / total;
+        std::vector<float> pr;
+        std::vector<uint32_t> al;
+        build_alias(P, pr, al);
+        for (uint32_t k = 0; k < e.triCount; ++k)
+        {
+            probL[(size_t)base[i] + k] = pr[k];
+            aliasL[(size_t)base[i] + k] = al[k];
+        }
+        const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        uint32_t first = (uint32_t)base[i], count = e.triCount;
+        float fb, cb;
+        memcpy(&fb, &first, 4);
+        memcpy(&cb, &count, 4);
+        slot[i] = (int32_t)(rec.size() / 8);
+        rec.insert(rec.end(), {(float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
+                               (float)(0.5 * (lo[2] + hi[2])),
+                               (float)(0.5 * sqrt(dx * dx + dy * dy + dz * dz)),
+                               (float)(e.lum * total), (float)(1.0 / total), fb, cb});
+    }
+    if (rec.size() / 8 > kMaxSpatialEmitters)
+    {
+        // too many for a per-point scan: the spatial choice is off
+        rec.clear();
+        slot.assign(inst.size(), -1);
+    }
 }
 
 }  // namespace spg

@@ -129,6 +129,15 @@ struct DevScene
     const uint32_t *emitPair;   // [2 * g]: instance, triangle
     const int32_t *emitBase;    // per instance: first entry or -1
     uint32_t emitCount;
+    // the spatial emitter choice (emitInstCount > 0): per emitter instance
+    // (centre, radius) and (power, 1 / area, first entry, triangle count);
+    // per instance its emitter index or -1; per entry an alias table over
+    // its instance's triangles by area (local indices)
+    const float4 *emitInst;
+    const int32_t *emitSlot;
+    const float *emitProbL;
+    const uint32_t *emitAliasL;
+    uint32_t emitInstCount;
     float envSelect;            // probability of sampling the environment
 };
 
@@ -157,6 +166,7 @@ struct DevPaths
     uint32_t *qpath[3];          // path index of each queued ray
     uint32_t *counters;          // [0..2] queue lengths
     float *lastPdf;              // density of each path's last sampled direction
+    float *lastN;                // [3][capacity] shading normal where it was sampled
     float *nee;                  // [3][capacity] pending shadow-ray contribution
     uint32_t *neeTarget;         // [2][capacity] instance, triangle the shadow
                                  // ray must reach (instance SPG_MISS: the sky)
@@ -264,6 +274,7 @@ struct spg_Device
     std::condition_variable qcv;
     std::deque<TileRequest *> queue;
     bool leaderActive = false;
+    size_t lastGroup = 0;        // requests the last combined trace took
     int liveScenes = 0;          // guarded by lock
     bool destroyRequested = false;
     hipStream_t stream = nullptr;

@@ -262,31 +262,107 @@ __device__ __forceinline__ void world_triangle(const DevScene &S, uint32_t inst,
     C = xform_point(I.model, v3{p2[0], p2[1], p2[2]});
 }
 
-// Density per steradian with which the emitter sampler reaches triangle
-// `prim` of instance `inst` at distance t along unit direction d:
-// pdfA * t^2 / |cos| at the light, 0 if the triangle is not in the table.
-__device__ __forceinline__ float emit_pdf(const DevScene &S, uint32_t inst, uint32_t prim,
-                                          float t, v3 d)
+// The spatial emitter choice (S.emitInstCount > 0): emitter instance k seen
+// from x with shading normal n weighs power / max(d^2, r^2) (d the distance
+// to its bounding sphere's centre, r the radius), 0 when the sphere lies
+// wholly behind the tangent plane; the weights are summed in index order.
+__device__ __forceinline__ float emit_weight(const DevScene &S, uint32_t k, v3 x, v3 n)
+{
+    const float4 a = S.emitInst[2 * k], b = S.emitInst[2 * k + 1];
+    const v3 v{a.x - x.x, a.y - x.y, a.z - x.z};
+    const float d2 = dot(v, v), r2 = a.w * a.w;
+    return dot(n, v) > -a.w ? b.x / fmaxf(d2, r2) : 0.0f;
+}
+
+__device__ __forceinline__ float emit_weight_sum(const DevScene &S, v3 x, v3 n)
+{
+    float W = 0.0f;
+    for (uint32_t k = 0; k < S.emitInstCount; ++k) W += emit_weight(S, k, x, n);
+    return W;
+}
+
+// Density per unit area of the emitter sampler on instance `inst`'s
+// triangle `prim`: the spatial choice's P(instance | x, n) / area, or the
+// global table's entry; 0 if not sampled.
+__device__ __forceinline__ float emit_pdf_area(const DevScene &S, uint32_t inst,
+                                               uint32_t prim, v3 x, v3 n)
 {
     const int32_t b = S.emitBase[inst];
     if (b < 0) return 0.0f;
-    v3 A, B, C;
-    world_triangle(S, inst, prim, A, B, C);
-    const v3 n = cross(B - A, C - A);
-    const float c = fabsf(dot(n, d)) / sqrtf(dot(n, n));
-    return c > 0.0f ? S.emitPdfA[b + prim] * (t * t) / c : 0.0f;
+    if (S.emitInstCount == 0) return S.emitPdfA[b + prim];
+    const int32_t k = S.emitSlot[inst];
+    if (k < 0) return 0.0f;
+    const float W = emit_weight_sum(S, x, n);
+    if (!(W > 0.0f)) return 0.0f;
+    return emit_weight(S, (uint32_t)k, x, n) / W * S.emitInst[2 * k + 1].y;
 }
 
-// A point on an emissive triangle drawn from the table (entry by the alias
-// table from r and e1, uniform on the triangle by e2, e3): the unit
-// direction from x, its density per steradian, the triangle.
+// Density per steradian with which the emitter sampler at x (shading normal
+// n) reaches triangle `prim` of instance `inst` at distance t along unit
+// direction d: pdfA * t^2 / |cos| at the light, 0 if it is not sampled.
+__device__ __forceinline__ float emit_pdf(const DevScene &S, uint32_t inst, uint32_t prim,
+                                          float t, v3 d, v3 x, v3 n)
+{
+    const float pa = emit_pdf_area(S, inst, prim, x, n);
+    if (!(pa > 0.0f)) return 0.0f;
+    v3 A, B, C;
+    world_triangle(S, inst, prim, A, B, C);
+    const v3 tn = cross(B - A, C - A);
+    const float c = fabsf(dot(tn, d)) / sqrtf(dot(tn, tn));
+    return c > 0.0f ? pa * (t * t) / c : 0.0f;
+}
+
+// A point on an emissive triangle (uniform on the triangle by e2, e3): the
+// unit direction from x, its density per steradian, the triangle.  With
+// the spatial choice an emitter instance is drawn by its weight from x and
+// n (the first whose running sum exceeds e4 x the total), then a triangle of
+// it by area (entry by its alias table from r and e1); else an entry of the
+// global table (r and e1).  pdf 0: nothing to sample.
 __device__ __forceinline__ v3 emit_direction(const DevScene &S, uint32_t r, float e1,
-                                             float e2, float e3, v3 x, float &pdf,
-                                             float &dist, uint32_t &inst,
+                                             float e2, float e3, float e4, v3 x, v3 nx,
+                                             float &pdf, float &dist, uint32_t &inst,
                                              uint32_t &prim)
 {
-    uint32_t g = (uint32_t)(((uint64_t)r * S.emitCount) >> 32);
-    if (!(e1 < S.emitProb[g])) g = S.emitAlias[g];
+    uint32_t g;
+    float pa;
+    if (S.emitInstCount)
+    {
+        const float W = emit_weight_sum(S, x, nx);
+        if (!(W > 0.0f))
+        {
+            pdf = 0.0f;
+            dist = 0.0f;
+            inst = prim = SPG_MISS;
+            return v3{0.0f, 0.0f, 1.0f};
+        }
+        const float target = e4 * W;
+        float acc = 0.0f, wp = 0.0f;
+        uint32_t pick = 0;
+        bool found = false;
+        for (uint32_t k = 0; k < S.emitInstCount; ++k)
+        {
+            const float w = emit_weight(S, k, x, nx);
+            acc += w;
+            if (!found && w > 0.0f)
+            {
+                pick = k;  // the last positive weight so far
+                wp = w;
+            }
+            found = found || acc > target;
+        }
+        const float4 b = S.emitInst[2 * pick + 1];
+        const uint32_t first = __float_as_uint(b.z), count = __float_as_uint(b.w);
+        uint32_t j = (uint32_t)(((uint64_t)r * count) >> 32);
+        if (!(e1 < S.emitProbL[first + j])) j = S.emitAliasL[first + j];
+        g = first + j;
+        pa = wp / W * b.y;
+    }
+    else
+    {
+        g = (uint32_t)(((uint64_t)r * S.emitCount) >> 32);
+        if (!(e1 < S.emitProb[g])) g = S.emitAlias[g];
+        pa = S.emitPdfA[g];
+    }
     inst = S.emitPair[2 * g];
     prim = S.emitPair[2 * g + 1];
     v3 A, B, C;
@@ -298,7 +374,7 @@ __device__ __forceinline__ v3 emit_direction(const DevScene &S, uint32_t r, floa
     const v3 w = normalize(v);
     const v3 n = cross(B - A, C - A);
     const float c = fabsf(dot(n, w)) / sqrtf(dot(n, n));
-    pdf = c > 0.0f && dist2 > 0.0f ? S.emitPdfA[g] * dist2 / c : 0.0f;
+    pdf = c > 0.0f && dist2 > 0.0f ? pa * dist2 / c : 0.0f;
     dist = sqrtf(dist2);
     return w;
 }
@@ -862,8 +938,10 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     {
                         // a BSDF-sampled ray that reached an emitter: its
                         // share against the emitter sampler (power heuristic)
+                        const v3 nPrev{p.paths.lastN[path], p.paths.lastN[cap + path],
+                                       p.paths.lastN[2 * cap + path]};
                         float pl = (1.0f - S.envSelect) *
-                                   emit_pdf(S, inst, p.paths.hprim[i], t, d);
+                                   emit_pdf(S, inst, p.paths.hprim[i], t, d, o, nPrev);
                         float pb = p.paths.lastPdf[path];
                         float pb2 = pb * pb;
                         float w = pb2 / (pb2 + pl * pl);
@@ -888,6 +966,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                         float e0 = rng.u24();
                         uint32_t r = rng.next();
                         float e1 = rng.u24(), e2 = rng.u24(), e3 = rng.u24();
+                        float e4 = rng.u24();
                         const bool useEnv = e0 < S.envSelect;
                         if (useEnv ? S.envTex >= 0 : S.emitCount > 0)
                         {
@@ -906,11 +985,17 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                             else
                             {
                                 float pg, dl;
-                                wl = emit_direction(S, r, e1, e2, e3, x, pg, dl, ti, tp);
+                                wl = emit_direction(S, r, e1, e2, e3, e4, x, N, pg, dl, ti,
+                                                    tp);
                                 p.paths.neeDist[path] = dl * 1.001f;
                                 pl = (1.0f - S.envSelect) * pg;
-                                const float *em = S.mats[S.inst[ti].material].emission;
-                                Le = v3{em[0], em[1], em[2]};
+                                Le = v3{0.0f, 0.0f, 0.0f};
+                                // pg = 0: nothing was sampled and ti is SPG_MISS
+                                if (pg > 0.0f)
+                                {
+                                    const float *em = S.mats[S.inst[ti].material].emission;
+                                    Le = v3{em[0], em[1], em[2]};
+                                }
                             }
                             float cl = dot(N, wl);
                             if (cl > 0.0f && pl > 0.0f)
@@ -939,7 +1024,13 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                                          : cosine * (1.0f / kPi);
                         float w = ps > 0.0f ? cosine / pdfc : kPi;
                         beta = had(beta, X * w);
-                        if (kEst == SPG_EST_NEE) p.paths.lastPdf[path] = pdfc;
+                        if (kEst == SPG_EST_NEE)
+                        {
+                            p.paths.lastPdf[path] = pdfc;
+                            p.paths.lastN[path] = N.x;
+                            p.paths.lastN[cap + path] = N.y;
+                            p.paths.lastN[2 * cap + path] = N.z;
+                        }
                         v3 P = o + d * t;
                         push = true;
                         no = P + N * 0.0001f;
