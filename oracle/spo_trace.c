@@ -379,15 +379,15 @@ static void world_triangle(const spo_Scene *s, uint32_t i, uint32_t k, spo_v3 *A
 }
 
 /* The spatial emitter choice (emitInstCount > 0): emitter instance k seen
- * from x with shading normal n weighs power / max(d^2, r^2), 0 when its
- * bounding sphere lies wholly behind the tangent plane; sums in index
+ * from x with shading normal n weighs power x (1 / max(d^2, r^2)), 0 when
+ * its bounding sphere lies wholly behind the tangent plane; sums in index
  * order.  (The product's emit_weight, spg_trace.hip.) */
 static float emit_weight(const spo_Scene *s, uint32_t k, spo_v3 x, spo_v3 n)
 {
     const float *a = s->emitInst + 8 * (size_t)k;
     spo_v3 v = {a[0] - x.x, a[1] - x.y, a[2] - x.z};
     float d2 = v3dot(v, v), r2 = a[3] * a[3];
-    return v3dot(n, v) > -a[3] ? a[4] / fmaxf(d2, r2) : 0.0f;
+    return v3dot(n, v) > -a[3] ? a[4] * (1.0f / fmaxf(d2, r2)) : 0.0f;
 }
 
 static float emit_weight_sum(const spo_Scene *s, spo_v3 x, spo_v3 n)

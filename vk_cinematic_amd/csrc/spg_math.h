@@ -58,7 +58,7 @@ __device__ __forceinline__ float rcp_cr(float x)
     return r;
 }
 #else
-inline float rcp_cr(float x) { return 1.0f / x; }
+__host__ __device__ inline float rcp_cr(float x) { return 1.0f / x; }
 #endif
 
 SPG_HD v3 mk(float x, float y, float z) { return v3{x, y, z}; }

@@ -263,15 +263,17 @@ __device__ __forceinline__ void world_triangle(const DevScene &S, uint32_t inst,
 }
 
 // The spatial emitter choice (S.emitInstCount > 0): emitter instance k seen
-// from x with shading normal n weighs power / max(d^2, r^2) (d the distance
-// to its bounding sphere's centre, r the radius), 0 when the sphere lies
-// wholly behind the tangent plane; the weights are summed in index order.
+// from x with shading normal n weighs power x 1 / max(d^2, r^2) (d the
+// distance to its bounding sphere's centre, r the radius; the reciprocal
+// correctly rounded), 0 when the sphere lies wholly behind the tangent
+// plane; the weights are summed in index order.
 __device__ __forceinline__ float emit_weight(const DevScene &S, uint32_t k, v3 x, v3 n)
 {
     const float4 a = S.emitInst[2 * k], b = S.emitInst[2 * k + 1];
     const v3 v{a.x - x.x, a.y - x.y, a.z - x.z};
     const float d2 = dot(v, v), r2 = a.w * a.w;
-    return dot(n, v) > -a.w ? b.x / fmaxf(d2, r2) : 0.0f;
+    const float w = b.x * rcp_cr(fmaxf(d2, r2));
+    return dot(n, v) > -a.w ? w : 0.0f;
 }
 
 __device__ __forceinline__ float emit_weight_sum(const DevScene &S, v3 x, v3 n)
