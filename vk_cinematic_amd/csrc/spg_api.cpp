@@ -1167,6 +1167,7 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     p.seed = cfg.seed;
     p.tx = cfg.transcendentals;
     p.collect = cfg.collectMetrics;
+    p.work = d->persist.counter;
     const bool train = guided && cfg.guideTrain;
     if (guided)
         CHK(guide_prepare_trace(cfg.guide, s, cap, cfg.bounces, train, p.guide));
@@ -1192,7 +1193,6 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
             p.sampleBase = cfg.sampleOffset + done;
             // accum and the paths are laid out with the slot count of this
             // chunk; accum rows are 3 x nslots
-            HIPCHK(launch_set_counter(d->stream, &p.paths.counters[0], nslots * sp));
             if (guided)
                 CHK(guide_begin_pass(cfg.guide, d->stream, p.guide,
                                      (p.sampleBase + 1u) * 0x9E3779B9u ^ cfg.seed ^
@@ -1208,7 +1208,6 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
                 tm.begin(kExtend);
                 HIPCHK(launch_extend(d->stream, p, q, d->persist));
                 tm.end();
-                HIPCHK(launch_set_counter(d->stream, &p.paths.counters[q ^ 1], 0));
                 if (guided)
                 {
                     tm.begin(kGuide);
@@ -1216,7 +1215,6 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
                     tm.end();
                     launches[kGuide]++;
                 }
-                if (nee) HIPCHK(launch_set_counter(d->stream, &p.paths.counters[2], 0));
                 tm.begin(kShade);
                 HIPCHK(launch_shade(d->stream, p, q));
                 tm.end();

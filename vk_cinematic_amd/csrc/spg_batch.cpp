@@ -134,6 +134,7 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
     p.images = b->dImages;
     Persist ps = b->dev->persist;
     ps.maxBlocks = b->maxBlocks;
+    p.work = ps.counter;
     const bool guided = cfg.estimator == SPG_EST_GUIDED;
     const bool nee = cfg.estimator == SPG_EST_NEE;
     const bool train = guided && cfg.guideTrain;
@@ -161,7 +162,6 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
             uint32_t sp = std::min(sppPass, spp - done);
             p.sppPass = sp;
             p.sampleBase = cfg.sampleOffset + done;
-            HIPCHK(launch_set_counter(st, &p.paths.counters[0], nslots * sp));
             if (guided)
             {
                 // the same pass salt as an eager trace of these slots
@@ -174,25 +174,19 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
             CHK(T0(kRaygen));
             HIPCHK(launch_raygen(st, p));
             CHK(T1());
-            k += 2;
+            ++k;
             int q = 0;
             for (uint32_t bb = 0; bb < cfg.bounces; ++bb)
             {
                 CHK(T0(kExtend));
                 HIPCHK(launch_extend(st, p, q, ps));
                 CHK(T1());
-                HIPCHK(launch_set_counter(st, &p.paths.counters[q ^ 1], 0));
-                k += 3;  // the extend launch includes its counter reset
+                ++k;
                 if (guided)
                 {
                     CHK(T0(kGuide));
                     HIPCHK(launch_guide_infer(st, p, q));
                     CHK(T1());
-                    ++k;
-                }
-                if (nee)
-                {
-                    HIPCHK(launch_set_counter(st, &p.paths.counters[2], 0));
                     ++k;
                 }
                 CHK(T0(kShade));
@@ -208,7 +202,7 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
                     CHK(T0(kShade));
                     HIPCHK(launch_resolve_shadow(st, p));
                     CHK(T1());
-                    k += 3;
+                    k += 2;
                 }
                 q ^= 1;
             }

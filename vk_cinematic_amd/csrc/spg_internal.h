@@ -201,6 +201,10 @@ struct TraceParams
     float clampValue;
     uint32_t estimator, rngMode, seed, tx, collect;
     uint32_t refill;             // k_extend's refill threshold, 0 = default
+    // the persistent closest-hit launch's work counter: zeroed by the
+    // kernel before each k_extend (k_raygen, k_shade, k_resolve_shadow),
+    // so the launch sequence needs no separate clearing launches
+    uint32_t *work;
     DevGuide guide;              // SPG_EST_GUIDED only
     // batched launches (slotItem != nullptr): each slot belongs to an item
     // with its own scene, camera and image

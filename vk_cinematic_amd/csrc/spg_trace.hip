@@ -621,6 +621,13 @@ template <bool kBatch>
 __global__ __launch_bounds__(256) void k_raygen(TraceParams p)
 {
     const uint32_t nPaths = p.slots * p.sppPass;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        // this pass's camera rays fill queue 0; the first k_extend starts
+        // its work counter at zero
+        p.paths.counters[0] = nPaths;
+        *p.work = 0u;
+    }
     for (uint32_t path = xcd_block() * 256 + threadIdx.x; path < nPaths;
          path += gridDim.x * 256)
     {
@@ -700,6 +707,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceParams p, int q,
     const uint32_t n = p.paths.counters[q];
     const uint32_t cap = p.paths.capacity;
     const float *qo = p.paths.qo[q], *qd = p.paths.qd[q];
+    if (!kShadow && blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        // the queues the following k_shade appends to start empty (their
+        // previous contents were consumed by the bounce before)
+        p.paths.counters[q ^ 1] = 0u;
+        p.paths.counters[2] = 0u;
+    }
     Counts cnt{0, 0, 0, 0};
     uint32_t rays = 0;
     SceneView sv = scene_view(p.scene);
@@ -840,6 +854,8 @@ template <bool kDouble, int kEst, bool kBatch>
 __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
 {
     const uint32_t n = p.paths.counters[q];
+    // k_extend has finished with the work counter: zero it for the next one
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p.work = 0u;
     const uint32_t cap = p.paths.capacity;
     const int nq = q ^ 1;
     uint32_t hits = 0, misses = 0;
@@ -1199,6 +1215,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
 __global__ __launch_bounds__(256) void k_resolve_shadow(TraceParams p)
 {
     const uint32_t n = p.paths.counters[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p.work = 0u;  // for the next k_extend
     const uint32_t cap = p.paths.capacity;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
     {
@@ -1655,7 +1672,7 @@ hipError_t launch_extend(hipStream_t s, const TraceParams &p, int q,
                           : (p.collect ? gCount : gPlain);
     if (ps.maxBlocks && g > ps.maxBlocks) g = ps.maxBlocks;
     dim3 grid(persist_blocks(g, n));
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, ps.counter, 0u);
+    // ps.counter was zeroed by the kernel before (TraceParams::work)
 #define SPG_EXT(C, B, S)                                                          \
     hipLaunchKernelGGL((k_extend<C, B, S>), grid, dim3(kTraceBlock), 0, s, p, q, \
                        ps.counter, ps.spill)
