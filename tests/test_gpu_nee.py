@@ -128,3 +128,19 @@ def test_nee_spatial_emitter_choice_matches_oracle_bit_exact(device):
         px, met = oracle_render(desc, 8, 4, SPO_EST_NEE, orng, clamp=0.0)
         assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
         assert [m.values[k] for k in (1, 2, 3, 4)] == [int(met[k]) for k in (1, 2, 3, 4)]
+
+
+@pytest.mark.parametrize("which", ["buried", "many"])
+def test_nee_emitter_choice_edges_match_oracle_bit_exact(device, which):
+    """Edges of the emitter choice, bit-exact against the oracle: lamps
+    below the floor with one above (at points on the upper lamp every
+    emitter's bounding sphere lies behind the tangent plane, total weight 0,
+    no emitter sample); 300 tiny lamps (more than the spatial choice scans:
+    the global table by power)."""
+    from test_nee import _buried_lamps_scene, _many_lamps_scene
+    desc = _buried_lamps_scene() if which == "buried" else _many_lamps_scene()
+    img, m = gpu_render(device, desc, 8, 4, V.EST_NEE, V.RNG_PCG, clamp=0.0)
+    px, met = oracle_render(desc, 8, 4, SPO_EST_NEE, SPO_RNG_PCG, clamp=0.0)
+    assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
+    assert [m.values[k] for k in (1, 2, 3, 4)] == [int(met[k]) for k in (1, 2, 3, 4)]
+    assert img.reshape(-1, 4)[:, :3].max() > 0

@@ -425,3 +425,59 @@ def test_spatial_emitter_nee_agrees_with_unbiased_and_cuts_error():
     en = np.mean([((x - ref) ** 2).mean() for x in n])
     assert en < eu / 3, (en, eu)
     O.spo_scene_destroy(s)
+
+
+def _buried_lamps_scene():
+    """The five-lamp floor with every lamp moved below the floor (y < 0):
+    from any point of the floor's upper side all emitter bounding spheres
+    lie behind the tangent plane, so the spatial choice has total weight 0
+    there and no emitter is sampled; one lamp stays above, seen from the
+    lamps' own surfaces."""
+    desc = _lamps_scene()
+    objs = []
+    for k, o in enumerate(desc["objects"]):
+        if k >= 2:  # object 0 is the floor, lamp 1 stays above it
+            o = (o[0], o[1], (o[2][0], -1.5, o[2][2]), o[3], o[4])
+        objs.append(o)
+    desc["objects"] = objs
+    return desc
+
+
+def _many_lamps_scene(n=300, seed=11):
+    """n tiny emissive icospheres (level 0, 20 triangles) over the floor:
+    more emitters than the spatial choice scans, so the global table is
+    used."""
+    r = np.random.default_rng(seed)
+    desc = _lamps_scene(n=0)
+    desc["meshes"] = [V.gen_icosphere(0) + (True,), V.gen_plane() + (True,)]
+    for k in range(8):  # eight emissions shared by the lamps
+        e = float(r.uniform(1.0, 20.0))
+        desc["materials"].append((1000 + k, (0.0, 0.0, 0.0), (e, e, e), 0.0,
+                                  0xFFFFFFFF, 0xFFFFFFFF))
+    for k in range(n):
+        desc["objects"].append((0, 1000 + k % 8, (float(r.uniform(-4, 4)),
+                                              float(r.uniform(0.3, 2.0)),
+                                              float(r.uniform(-4, 4))),
+                                (0.0, 0.0, 0.0, 1.0), (0.05, 0.05, 0.05)))
+    return desc
+
+
+def test_spatial_choice_is_off_above_256_emitters():
+    """300 emissive objects: no spatial records (the global table is used)
+    and every object's slot is -1."""
+    O = oracle()
+    desc = _many_lamps_scene()
+    s, _ = load_oracle(desc)
+    no = len(desc["objects"])
+    N = 300 * 20
+    pdfA, prob = np.zeros(N, np.float32), np.zeros(N, np.float32)
+    alias, pair = np.zeros(N, np.uint32), np.zeros(2 * N, np.uint32)
+    base = np.zeros(no, np.int32)
+    n = O.spo_scene_emitter_table(s, pdfA, prob, alias, pair, base)
+    assert n == N
+    rec = np.zeros(8 * 256, np.float32)
+    slot = np.zeros(no, np.int32)
+    probL, aliasL = np.zeros(N, np.float32), np.zeros(N, np.uint32)
+    assert O.spo_scene_emitter_instances(s, base, n, rec, slot, probL, aliasL) == 0
+    assert (slot == -1).all()
+    O.spo_scene_destroy(s)
