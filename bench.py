@@ -754,7 +754,8 @@ def time_to_relmse(V, dev, scenes, workload, ref_spp, ref_est, max_spp=8192,
                    train_passes=64, guide_kw=None, target=1e-3):
     """Wall-clock time to relMSE `target` of the plain tracer, the guided
     tracer (online training during its first `train_passes` one-sample
-    passes, then the frozen guide) and next-event estimation, each
+    passes, then the frozen guide), next-event estimation and the guided
+    tracer with next-event estimation (trained the same way), each
     accumulating every sample it renders, against a reference of `ref_spp`
     samples of `ref_est` rendered here."""
     desc = getattr(scenes, workload)()
@@ -793,18 +794,29 @@ def time_to_relmse(V, dev, scenes, workload, ref_spp, ref_est, max_spp=8192,
     guided = run("guided", guided_step)
     g.close()
     nee = run("nee", lambda a: a.add(64 if a.spp >= 64 else 16, V.EST_NEE, 0x7EED))
+    g = V.Guide(dev, **guide_kw)
+
+    def guided_nee_step(a):
+        if a.spp < train_passes:
+            a.add(1, V.EST_GUIDED_NEE, 0x8EED, guide=g, train=True)
+        else:
+            a.add(64, V.EST_GUIDED_NEE, 0x8EED, guide=g)
+
+    guided_nee = run("guided+nee", guided_nee_step)
+    g.close()
     s.close()
     res = {"workload": workload, "target": target, "wall_clock": True,
            "reference": {"spp": ref_spp,
                          "estimator": {V.EST_UNBIASED: "plain", V.EST_NEE: "nee"}[ref_est],
                          "own_relmse": floor, "render_s": ref_wall},
            "train_passes": train_passes, "guide": guide_kw}
-    for name, curve in (("plain", plain), ("guided", guided), ("nee", nee)):
+    for name, curve in (("plain", plain), ("guided", guided), ("nee", nee),
+                        ("guided_nee", guided_nee)):
         c = crossing(curve, target)
         res[name] = {"time_s": c[0] if c else None, "spp": c[1] if c else None,
                      "how": c[2] if c else None, "curve": curve}
     tp = res["plain"]["time_s"]
-    for name in ("guided", "nee"):
+    for name in ("guided", "nee", "guided_nee"):
         t = res[name]["time_s"]
         res[name]["speedup_vs_plain"] = tp / t if tp and t else None
     return res

@@ -54,13 +54,19 @@ enum spg_Estimator {
                              src/simd_path_tracer.cpp:107-175) */
     SPG_EST_UNBIASED = 1, /* same materials, cosine sampling with 1/pdf */
     SPG_EST_GUIDED = 2,   /* unbiased + learned guiding, one-sample MIS */
-    SPG_EST_NEE = 3       /* unbiased + next-event estimation toward the
+    SPG_EST_NEE = 3,      /* unbiased + next-event estimation toward the
                              background's cube-map emission (importance
                              sampled by texel) and toward emissive triangles
                              (by emission x area), one light sample and one
                              shadow ray per vertex, combined with BSDF
                              sampling by the power heuristic; no reference
                              counterpart */
+    SPG_EST_GUIDED_NEE = 4 /* the guided estimator plus the same
+                             next-event estimation: each light sample is
+                             weighted against the one-sample MIS density of
+                             BSDF and guide, and emission reached by a
+                             scattered ray against the light sampler (three
+                             techniques, power heuristic); needs a guide */
 };
 enum spg_Transcendentals {
     SPG_TX_DOUBLE = 1,    /* sin/cos/atan2/pow evaluated in double and
@@ -124,10 +130,11 @@ typedef struct {
     uint32_t transcendentals;  /* spg_Transcendentals */
     uint32_t sampleOffset;     /* first sample index (progressive passes) */
     uint32_t collectMetrics;   /* count AABB/triangle/mesh tests */
-    uint32_t guideTrain;       /* SPG_EST_GUIDED: collect training samples and
-                                  run the guide's training steps after each
-                                  pass */
-    spg_Guide *guide;          /* SPG_EST_GUIDED: the guide (spg_guide_create) */
+    uint32_t guideTrain;       /* SPG_EST_GUIDED(_NEE): collect training
+                                  samples and run the guide's training steps
+                                  after each pass */
+    spg_Guide *guide;          /* SPG_EST_GUIDED(_NEE): the guide
+                                  (spg_guide_create) */
 } spg_TraceConfig;
 
 /* sp_Context, src/simd_path_tracer.h:34-46 */

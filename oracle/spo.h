@@ -46,7 +46,8 @@ extern "C" {
 enum { SPO_RNG_XORSHIFT_TILE = 0, SPO_RNG_XORSHIFT_PIXEL = 1, SPO_RNG_PCG = 2 };
 enum { SPO_TX_LIBM = 0, SPO_TX_CR = 1 };
 enum { SPO_HIT_REFERENCE = 0, SPO_HIT_BRUTE = 1 };
-enum { SPO_EST_PARITY = 0, SPO_EST_UNBIASED = 1, SPO_EST_GUIDED = 2, SPO_EST_NEE = 3 };
+enum { SPO_EST_PARITY = 0, SPO_EST_UNBIASED = 1, SPO_EST_GUIDED = 2, SPO_EST_NEE = 3,
+       SPO_EST_GUIDED_NEE = 4 };
 
 #define SPO_MAX_METRICS 12
 #define SPO_NO_TEXTURE 0xFFFFFFFFu

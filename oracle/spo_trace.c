@@ -490,21 +490,130 @@ static spo_v3 emit_direction(const spo_Scene *s, uint32_t r, float e1, float e2,
     return w;
 }
 
+/* One light sample of next-event estimation at x (shading normal N, view V,
+ * material mat; spg_trace.hip nee_light_sample): the sky with probability
+ * envSelect, else an emitter (emit_direction); draws a 24-bit uniform, one
+ * 32-bit value, four 24-bit uniforms.  The power heuristic weighs the light
+ * density against the scattering density: the BSDF sampler's, or with
+ * `lobes` the one-sample MIS fb p_bsdf + (1 - fb) q of the guided
+ * estimator.  The sky is seen unless some object lies at a positive
+ * distance along the shadow ray; an emitter triangle must be the closest
+ * hit within 1.001 x the light's distance (the product's shadow walk starts
+ * with that bound as its best t).  Returns 1 with *C = beta o f cos Le w /
+ * p_light when the light is reached, 0 when not (or nothing was sampled),
+ * -1 if the scene query failed. */
+static int nee_sample(const spo_Scene *s, const spo_Config *cfg, spo_Rng *rng, spo_v3 x,
+                      spo_v3 N, spo_v3 V, const float *mat, spo_v3 beta,
+                      const float *lobes, float fb, uint64_t *metrics, spo_v3 *C)
+{
+    const int tx = cfg->tx;
+    float e0 = rng_u24(rng);
+    uint32_t r = rng_next(rng);
+    float e1 = rng_u24(rng), e2 = rng_u24(rng), e3 = rng_u24(rng);
+    float e4 = rng_u24(rng);
+    int useEnv = e0 < s->envSelect;
+    if (!(useEnv ? s->envPdf != NULL : s->emitCount > 0)) return 0;
+    spo_v3 wl, Le;
+    float pl, dl = 0.0f;
+    uint32_t ti = 0xFFFFFFFFu, tp = 0;
+    if (useEnv)
+    {
+        float wl3[3];
+        spo_env_direction(s->envProb, s->envAlias, s->envW, s->envH, r, e1, e2, e3, wl3);
+        wl = v3p(wl3);
+        pl = s->envSelect * spo_env_pdf(s->envPdf, s->envW, s->envH, wl3);
+        float bg[7];
+        spo_v2 uv0 = {0, 0};
+        spo_material_eval_internal(s, s->backgroundMaterialId, uv0, v3neg(wl), tx, bg);
+        Le = v3p(bg + 3);
+    }
+    else
+    {
+        float pg;
+        wl = emit_direction(s, r, e1, e2, e3, e4, x, N, &pg, &dl, &ti, &tp);
+        pl = (1.0f - s->envSelect) * pg;
+        Le = v3(0.0f, 0.0f, 0.0f);
+        if (pg > 0.0f)  /* else nothing was sampled */
+        {
+            float em[7];
+            spo_v2 uv0 = {0, 0};
+            spo_material_eval_internal(s, s->objects[ti].material, uv0, v3neg(wl), tx, em);
+            Le = v3p(em + 3);
+        }
+    }
+    float cl = v3dot(N, wl);
+    if (!(cl > 0.0f && pl > 0.0f)) return 0;
+    float rough = mat[6];
+    float ps = rough > 0.0f ? SPEC_PROB : 0.0f;
+    spo_v3 Xl = brdf_bracket(N, V, wl, mat, tx, 1);
+    float pcl = (1.0f - ps) * cl * (1.0f / SPO_PI);
+    float pb = ps > 0.0f ? pcl + ps * ggx_pdf(N, V, wl, rough) : pcl;
+    if (lobes) pb = fb * pb + (1.0f - fb) * mixture_pdf_f(lobes, wl, tx);
+    float pl2 = pl * pl;
+    float wmis = pl2 / (pl2 + pb * pb);
+    *C = v3had(v3had(beta, v3mul(Xl, (cl * wmis) / pl)), Le);
+    float so[3], sd[3];
+    v3put(so, x); v3put(sd, wl);
+    int visible;
+    if (ti == 0xFFFFFFFFu)
+    {
+        int occ = spo_scene_occluded(s, so, sd, cfg->hit, metrics);
+        if (occ < 0) return -1;
+        visible = !occ;
+    }
+    else
+    {
+        spo_SceneHit sh;
+        if (spo_scene_intersect(s, so, sd, cfg->hit, tx, &sh, metrics) < 0) return -1;
+        visible = sh.t > 0.0f && !(sh.t > dl * 1.001f) && sh.object == ti && sh.prim == tp;
+    }
+    metrics[2]++;
+    return visible;
+}
+
+/* Power-heuristic share of emission reached by a scattered ray (density pb
+ * at the previous vertex, there shading normal lastN) against the light
+ * sampler: toward object i's triangle k at distance t along d from o, or
+ * (i = ~0) the sky along d (spg_trace.hip nee_hit_weight). */
+static float nee_hit_weight(const spo_Scene *s, uint32_t i, uint32_t k, float t, spo_v3 d,
+                            spo_v3 o, spo_v3 lastN, float pb)
+{
+    float pl;
+    if (i == 0xFFFFFFFFu)
+    {
+        float dd[3];
+        v3put(dd, d);
+        pl = s->envPdf ? s->envSelect * spo_env_pdf(s->envPdf, s->envW, s->envH, dd) : 0.0f;
+    }
+    else
+        pl = (1.0f - s->envSelect) * emit_pdf(s, i, k, t, d, o, lastN);
+    float pb2 = pb * pb;
+    return pb2 / (pb2 + pl * pl);
+}
+
 /* The guided estimator with the verification mixture (the product's
  * SPG_EST_GUIDED under SPG_GUIDE_ANALYTIC_MIXTURE): at each hit that may
  * scatter, three 24-bit uniforms and one-sample MIS of the BSDF lobe and
  * the mixture (guided_scatter); vertex records (E, X, cos/pdf); the
  * radiance by the reverse recurrence L_j = E_j + (X_j o L_j+1) cos_j/pdf_j,
  * and, for the training sink, each scattering vertex's weight
- * Y(X_j o L_j+1 cos_j/pdf_j) (luminance 0.2126, 0.7152, 0.0722). */
+ * Y(X_j o L_j+1 cos_j/pdf_j) (luminance 0.2126, 0.7152, 0.0722).  With
+ * SPO_EST_GUIDED_NEE (the product's SPG_EST_GUIDED_NEE) emission reached by
+ * a scattered ray is weighted against the light sampler (nee_hit_weight at
+ * the previous vertex with its MIS density), and each scattering vertex
+ * draws one light sample after its scatter uniforms (nee_sample against the
+ * one-sample MIS density), whose reached contribution joins E_j. */
 static spo_v3 trace_guided(const spo_Scene *s, const spo_Config *cfg, spo_v3 o,
                            spo_v3 d, spo_Rng *rng, uint64_t *metrics, int *fail,
                            uint32_t pixel, uint32_t sample)
 {
     const int tx = cfg->tx;
     const float fb = cfg->bsdfFraction;
+    const int nee = cfg->estimator == SPO_EST_GUIDED_NEE;
     spo_v3 E[16], X[16], P[16], Nv[16], Vv[16], Wv[16];
     float cv[16];
+    float lastPdf = 0.0f;
+    spo_v3 lastN = v3(0.0f, 0.0f, 0.0f);
     uint32_t vc = 0;
     uint32_t bounces = cfg->bounces < 16 ? cfg->bounces : 16;
     for (uint32_t bounce = 0; bounce < bounces; ++bounce)
@@ -525,6 +634,9 @@ static spo_v3 trace_guided(const spo_Scene *s, const spo_Config *cfg, spo_v3 o,
             float mat[7];
             spo_material_eval_internal(s, h.materialId, h.uv, V, tx, mat);
             E[vc] = v3p(mat + 3);
+            if (nee && vc > 0 && s->emitCount && s->emitBase[h.object] >= 0)
+                E[vc] = v3mul(E[vc],
+                              nee_hit_weight(s, h.object, h.prim, h.t, d, o, lastN, lastPdf));
             if (vc + 1 < cfg->bounces)
             {
                 float u0 = rng_u24(rng), u1 = rng_u24(rng), u2 = rng_u24(rng);
@@ -535,6 +647,18 @@ static spo_v3 trace_guided(const spo_Scene *s, const spo_Config *cfg, spo_v3 o,
                 float pdf;
                 spo_v3 wi = guided_scatter(L, h.normal, V, mat[6], fb, u0, u1, u2, tx, &pdf);
                 float cosv = v3dot(h.normal, wi);
+                if (nee)
+                {
+                    /* one light sample against the one-sample MIS density
+                     * of BSDF and mixture; reached, it adds to this
+                     * vertex's emission term */
+                    spo_v3 x = v3add(Ph, v3mul(h.normal, 0.0001f));
+                    spo_v3 C;
+                    int vis = nee_sample(s, cfg, rng, x, h.normal, V, mat, v3s(1.0f), L, fb,
+                                         metrics, &C);
+                    if (vis < 0) { *fail = 1; return v3(0, 0, 0); }
+                    if (vis) E[vc] = v3add(E[vc], C);
+                }
                 if (cosv > 0.0f && pdf > 0.0f)
                 {
                     X[vc] = brdf_bracket(h.normal, V, wi, mat, tx, 1);
@@ -542,6 +666,8 @@ static spo_v3 trace_guided(const spo_Scene *s, const spo_Config *cfg, spo_v3 o,
                     P[vc] = Ph; Nv[vc] = h.normal; Vv[vc] = V; Wv[vc] = wi;
                     o = v3add(Ph, v3mul(h.normal, 0.0001f));
                     d = wi;
+                    lastPdf = pdf;
+                    lastN = h.normal;
                     push = 1;
                 }
             }
@@ -553,6 +679,9 @@ static spo_v3 trace_guided(const spo_Scene *s, const spo_Config *cfg, spo_v3 o,
             spo_v2 uv0 = {0, 0};
             spo_material_eval_internal(s, s->backgroundMaterialId, uv0, V, tx, mat);
             E[vc] = v3p(mat + 3);
+            if (nee && vc > 0)
+                E[vc] = v3mul(E[vc], nee_hit_weight(s, 0xFFFFFFFFu, 0, h.t, d, o, lastN,
+                                                    lastPdf));
         }
         vc++;
         if (!push) break;
@@ -600,7 +729,7 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
     spo_film_position(c, px, py, filmP);
     spo_v3 o = c->position;
     spo_v3 d = v3norm(v3sub(v3p(filmP), c->position));
-    if (cfg->estimator == SPO_EST_GUIDED)
+    if (cfg->estimator == SPO_EST_GUIDED || cfg->estimator == SPO_EST_GUIDED_NEE)
         return trace_guided(s, cfg, o, d, rng, metrics, fail, y * c->width + x, sample);
     if (cfg->estimator == SPO_EST_UNBIASED || cfg->estimator == SPO_EST_NEE)
     {
@@ -626,11 +755,7 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                 {
                     /* a BSDF-sampled ray that reached an emitter: its share
                      * against the emitter sampler (power heuristic) */
-                    float pl = (1.0f - s->envSelect) *
-                               emit_pdf(s, h.object, h.prim, h.t, d, o, lastN);
-                    float pb = lastPdf;
-                    float pb2 = pb * pb;
-                    float wm = pb2 / (pb2 + pl * pl);
+                    float wm = nee_hit_weight(s, h.object, h.prim, h.t, d, o, lastN, lastPdf);
                     L = v3add(L, v3had(beta, v3mul(v3p(mat + 3), wm)));
                 }
                 else
@@ -646,87 +771,14 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                 float cosine = v3dot(h.normal, wi);
                 if (nee && bounce + 1 < cfg->bounces)
                 {
-                    /* next-event estimation: the sky (probability
-                     * envSelect) or an emissive triangle, power-heuristic
-                     * weight against the BSDF sampler's density, a shadow
-                     * ray that must reach the light */
-                    float e0 = rng_u24(rng);
-                    uint32_t r = rng_next(rng);
-                    float e1 = rng_u24(rng), e2 = rng_u24(rng), e3 = rng_u24(rng);
-                    float e4 = rng_u24(rng);
-                    int useEnv = e0 < s->envSelect;
-                    if (useEnv ? s->envPdf != NULL : s->emitCount > 0)
-                    {
-                        spo_v3 x = v3add(v3add(o, v3mul(d, h.t)), v3mul(h.normal, 0.0001f));
-                        spo_v3 wl, Le;
-                        float pl, dl = 0.0f;
-                        uint32_t ti = 0xFFFFFFFFu, tp = 0;
-                        if (useEnv)
-                        {
-                            float wl3[3];
-                            spo_env_direction(s->envProb, s->envAlias, s->envW, s->envH,
-                                              r, e1, e2, e3, wl3);
-                            wl = v3p(wl3);
-                            pl = s->envSelect * spo_env_pdf(s->envPdf, s->envW, s->envH, wl3);
-                            float bg[7];
-                            spo_v2 uv0 = {0, 0};
-                            spo_material_eval_internal(s, s->backgroundMaterialId, uv0,
-                                                       v3neg(wl), tx, bg);
-                            Le = v3p(bg + 3);
-                        }
-                        else
-                        {
-                            float pg;
-                            wl = emit_direction(s, r, e1, e2, e3, e4, x, h.normal, &pg, &dl,
-                                                &ti, &tp);
-                            pl = (1.0f - s->envSelect) * pg;
-                            Le = v3(0.0f, 0.0f, 0.0f);
-                            if (pg > 0.0f)  /* else nothing was sampled */
-                            {
-                                float em[7];
-                                spo_v2 uv0 = {0, 0};
-                                spo_material_eval_internal(s, s->objects[ti].material, uv0,
-                                                           v3neg(wl), tx, em);
-                                Le = v3p(em + 3);
-                            }
-                        }
-                        float cl = v3dot(h.normal, wl);
-                        if (cl > 0.0f && pl > 0.0f)
-                        {
-                            spo_v3 Xl = brdf_bracket(h.normal, V, wl, mat, tx, 1);
-                            float pcl = (1.0f - ps) * cl * (1.0f / SPO_PI);
-                            float pb = ps > 0.0f ? pcl + ps * ggx_pdf(h.normal, V, wl, rough)
-                                                 : pcl;
-                            float pl2 = pl * pl;
-                            float wmis = pl2 / (pl2 + pb * pb);
-                            spo_v3 C = v3had(v3had(beta, v3mul(Xl, (cl * wmis) / pl)), Le);
-                            float so[3], sd[3];
-                            v3put(so, x); v3put(sd, wl);
-                            /* the sky is seen unless some object lies at a
-                             * positive distance along the ray; an emitter
-                             * triangle must be the closest hit within 1.001 x
-                             * the light's distance (the product's shadow walk
-                             * starts with that bound as its best t) */
-                            int visible;
-                            if (ti == 0xFFFFFFFFu)
-                            {
-                                int occ = spo_scene_occluded(s, so, sd, cfg->hit, metrics);
-                                if (occ < 0) { *fail = 1; return L; }
-                                visible = !occ;
-                            }
-                            else
-                            {
-                                spo_SceneHit sh;
-                                if (spo_scene_intersect(s, so, sd, cfg->hit, tx, &sh,
-                                                        metrics) < 0)
-                                { *fail = 1; return L; }
-                                visible = sh.t > 0.0f && !(sh.t > dl * 1.001f) &&
-                                          sh.object == ti && sh.prim == tp;
-                            }
-                            metrics[2]++;
-                            if (visible) L = v3add(L, C);
-                        }
-                    }
+                    /* next-event estimation against the BSDF sampler's
+                     * density */
+                    spo_v3 x = v3add(v3add(o, v3mul(d, h.t)), v3mul(h.normal, 0.0001f));
+                    spo_v3 C;
+                    int vis = nee_sample(s, cfg, rng, x, h.normal, V, mat, beta, NULL, 0.0f,
+                                         metrics, &C);
+                    if (vis < 0) { *fail = 1; return L; }
+                    if (vis) L = v3add(L, C);
                 }
                 if (!(cosine > 0.0f)) break;
                 spo_v3 X = brdf_bracket(h.normal, V, wi, mat, tx, 1);
@@ -755,14 +807,7 @@ static spo_v3 trace_sample(const spo_Scene *s, const spo_Camera *c,
                 if (nee && bounce > 0)
                 {
                     /* the BSDF-sampled direction's share (power heuristic) */
-                    float dd[3];
-                    v3put(dd, d);
-                    float pe = s->envPdf ? s->envSelect *
-                                               spo_env_pdf(s->envPdf, s->envW, s->envH, dd)
-                                         : 0.0f;
-                    float pb = lastPdf;
-                    float pb2 = pb * pb;
-                    float wm = pb2 / (pb2 + pe * pe);
+                    float wm = nee_hit_weight(s, 0xFFFFFFFFu, 0, h.t, d, o, lastN, lastPdf);
                     L = v3add(L, v3had(beta, v3mul(v3p(mat + 3), wm)));
                 }
                 else

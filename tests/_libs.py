@@ -32,7 +32,7 @@ u32ref = C.POINTER(C.c_uint32)
 SPO_RNG_XORSHIFT_TILE, SPO_RNG_XORSHIFT_PIXEL, SPO_RNG_PCG = 0, 1, 2
 SPO_TX_LIBM, SPO_TX_CR = 0, 1
 SPO_HIT_REFERENCE, SPO_HIT_BRUTE = 0, 1
-SPO_EST_PARITY, SPO_EST_UNBIASED, SPO_EST_GUIDED, SPO_EST_NEE = 0, 1, 2, 3
+SPO_EST_PARITY, SPO_EST_UNBIASED, SPO_EST_GUIDED, SPO_EST_NEE, SPO_EST_GUIDED_NEE = 0, 1, 2, 3, 4
 NO_TEXTURE = 0xFFFFFFFF
 
 

@@ -31,11 +31,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _oracle_guided(desc, spp, bounces, orng, seed, hit=SPO_HIT_BRUTE, sah=False,
-                   tile=None, sink=0):
+                   tile=None, sink=0, est=SPO_EST_GUIDED):
     O = oracle()
     s, cam = load_oracle(desc, sah=sah)
     w, h = desc["camera"][0], desc["camera"][1]
-    cfg = SpoConfig(rng=orng, tx=SPO_TX_CR, hit=hit, estimator=SPO_EST_GUIDED,
+    cfg = SpoConfig(rng=orng, tx=SPO_TX_CR, hit=hit, estimator=est,
                     samplesPerPixel=spp, bounces=bounces, radianceClamp=0.0,
                     jitterYFirst=1, seed=seed, bsdfFraction=0.5)
     px = np.zeros(w * h * 4, np.float32)

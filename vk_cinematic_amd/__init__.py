@@ -24,7 +24,7 @@ SPG_OK = 0
 SPG_NO_TEXTURE = 0xFFFFFFFF
 SPG_MISS = 0xFFFFFFFF
 RNG_XORSHIFT_PIXEL, RNG_PCG = 1, 2
-EST_PARITY, EST_UNBIASED, EST_GUIDED, EST_NEE = 0, 1, 2, 3
+EST_PARITY, EST_UNBIASED, EST_GUIDED, EST_NEE, EST_GUIDED_NEE = 0, 1, 2, 3, 4
 TX_DOUBLE, TX_FAST = 1, 2
 METRIC_ELAPSED_NS, METRIC_PATHS, METRIC_RAYS, METRIC_HITS, METRIC_MISSES = 0, 1, 2, 3, 4
 METRIC_AABB_TESTS, METRIC_MESH_TESTS = 10, 11

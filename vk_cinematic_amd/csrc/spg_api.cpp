@@ -1082,10 +1082,12 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     if (cfg.rngMode != SPG_RNG_XORSHIFT_PIXEL && cfg.rngMode != SPG_RNG_PCG)
         return fail(SPG_E_INVALID, "trace: unknown rngMode");
     if (cfg.estimator != SPG_EST_PARITY && cfg.estimator != SPG_EST_UNBIASED &&
-        cfg.estimator != SPG_EST_GUIDED && cfg.estimator != SPG_EST_NEE)
+        cfg.estimator != SPG_EST_GUIDED && cfg.estimator != SPG_EST_NEE &&
+        cfg.estimator != SPG_EST_GUIDED_NEE)
         return fail(SPG_E_INVALID, "trace: unknown estimator");
-    const bool guided = cfg.estimator == SPG_EST_GUIDED;
-    const bool nee = cfg.estimator == SPG_EST_NEE;
+    const bool guided =
+        cfg.estimator == SPG_EST_GUIDED || cfg.estimator == SPG_EST_GUIDED_NEE;
+    const bool nee = cfg.estimator == SPG_EST_NEE || cfg.estimator == SPG_EST_GUIDED_NEE;
     if (guided && (!cfg.guide || cfg.guide->dev != d))
         return fail(SPG_E_INVALID, "trace: the guided estimator needs a guide on the scene's device");
     if (cfg.transcendentals != SPG_TX_DOUBLE && cfg.transcendentals != SPG_TX_FAST)
@@ -1308,7 +1310,9 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
 static bool combinable(const TileRequest &a, const TileRequest &b)
 {
     const spg_TraceConfig &ca = a.ctx->config, &cb = b.ctx->config;
-    if (ca.estimator == SPG_EST_GUIDED && (ca.guideTrain || cb.guideTrain)) return false;
+    if ((ca.estimator == SPG_EST_GUIDED || ca.estimator == SPG_EST_GUIDED_NEE) &&
+        (ca.guideTrain || cb.guideTrain))
+        return false;
     return a.ctx->scene == b.ctx->scene && a.hostPixels == b.hostPixels &&
            memcmp(a.ctx->camera, b.ctx->camera, sizeof(spg_Camera)) == 0 &&
            memcmp(&ca, &cb, sizeof(spg_TraceConfig)) == 0;

@@ -135,8 +135,9 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
     Persist ps = b->dev->persist;
     ps.maxBlocks = b->maxBlocks;
     p.work = ps.counter;
-    const bool guided = cfg.estimator == SPG_EST_GUIDED;
-    const bool nee = cfg.estimator == SPG_EST_NEE;
+    const bool guided =
+        cfg.estimator == SPG_EST_GUIDED || cfg.estimator == SPG_EST_GUIDED_NEE;
+    const bool nee = cfg.estimator == SPG_EST_NEE || cfg.estimator == SPG_EST_GUIDED_NEE;
     const bool train = guided && cfg.guideTrain;
     if (guided)
     {
@@ -357,9 +358,10 @@ int spg_batch_create(spg_Device *dev, const spg_BatchItem *items, uint32_t n,
     if (c.samplesPerPixel == 0 || c.bounces == 0 || c.bounces > kMaxBounces)
         return fail(SPG_E_INVALID, "spg_batch_create: samplesPerPixel > 0, bounces 1..16");
     if (c.estimator != SPG_EST_PARITY && c.estimator != SPG_EST_UNBIASED &&
-        c.estimator != SPG_EST_NEE && c.estimator != SPG_EST_GUIDED)
+        c.estimator != SPG_EST_NEE && c.estimator != SPG_EST_GUIDED &&
+        c.estimator != SPG_EST_GUIDED_NEE)
         return fail(SPG_E_INVALID, "spg_batch_create: unknown estimator");
-    const bool guided = c.estimator == SPG_EST_GUIDED;
+    const bool guided = c.estimator == SPG_EST_GUIDED || c.estimator == SPG_EST_GUIDED_NEE;
     if (guided && (!c.guide || c.guide->dev != dev))
         return fail(SPG_E_INVALID,
                     "spg_batch_create: the guided estimator needs a guide on the device");

@@ -847,6 +847,91 @@ __device__ __forceinline__ void record_scatter(const DevGuide &G, v3 pn, v3 N, v
     for (int k = 60; k < kGRecordFloats; ++k) r[k] = 0.0f;
 }
 
+// One light sample of next-event estimation at x (shading normal N, view V,
+// material m): the sky with probability envSelect, else an emitter
+// (emit_direction).  Draw order: a 24-bit uniform choosing sky or emitter,
+// one 32-bit draw for the table entry, four 24-bit uniforms.  With a usable
+// direction the contribution beta o (f cos Le w / p_light) -- w the power
+// heuristic against the scattering density scatterPdf(wl, cos) -- waits in
+// the path's NEE slot with its target (SPG_MISS: the sky) and the function
+// returns true with the shadow ray (so, sd).
+template <bool kDouble, class ScatterPdf>
+__device__ __forceinline__ bool nee_light_sample(const TraceParams &p, const DevScene &S,
+                                                 uint32_t path, Rng &rng, v3 x, v3 N,
+                                                 v3 V, const MatOut &m, v3 beta,
+                                                 ScatterPdf scatterPdf, v3 &so, v3 &sd)
+{
+    const uint32_t cap = p.paths.capacity;
+    float e0 = rng.u24();
+    uint32_t r = rng.next();
+    float e1 = rng.u24(), e2 = rng.u24(), e3 = rng.u24();
+    float e4 = rng.u24();
+    const bool useEnv = e0 < S.envSelect;
+    if (!(useEnv ? S.envTex >= 0 : S.emitCount > 0)) return false;
+    v3 wl, Le;
+    float pl;
+    uint32_t ti = SPG_MISS, tp = 0;
+    if (useEnv)
+    {
+        wl = env_direction(S, r, e1, e2, e3);
+        pl = S.envSelect * env_pdf(S, wl);
+        v2 uv0{0.0f, 0.0f};
+        Le = eval_material<kDouble>(S, S.background, uv0, -wl).emission;
+    }
+    else
+    {
+        float pg, dl;
+        wl = emit_direction(S, r, e1, e2, e3, e4, x, N, pg, dl, ti, tp);
+        p.paths.neeDist[path] = dl * 1.001f;
+        pl = (1.0f - S.envSelect) * pg;
+        Le = v3{0.0f, 0.0f, 0.0f};
+        // pg = 0: nothing was sampled and ti is SPG_MISS
+        if (pg > 0.0f)
+        {
+            const float *em = S.mats[S.inst[ti].material].emission;
+            Le = v3{em[0], em[1], em[2]};
+        }
+    }
+    float cl = dot(N, wl);
+    if (!(cl > 0.0f && pl > 0.0f)) return false;
+    v3 Xl = brdf_bracket<kDouble>(N, V, wl, m, true);
+    float pb = scatterPdf(wl, cl);
+    float pl2 = pl * pl;
+    float wmis = pl2 / (pl2 + pb * pb);
+    v3 C = had(had(beta, Xl * ((cl * wmis) / pl)), Le);
+    p.paths.nee[path] = C.x;
+    p.paths.nee[cap + path] = C.y;
+    p.paths.nee[2 * cap + path] = C.z;
+    p.paths.neeTarget[path] = ti;
+    p.paths.neeTarget[cap + path] = tp;
+    so = x;
+    sd = wl;
+    return true;
+}
+
+// Power-heuristic share of emission reached by a scattered ray (density pb
+// at the previous vertex) against the light sampler: toward the emitter
+// triangle (inst, prim) at distance t along d from o, or (inst SPG_MISS)
+// the sky along d.
+__device__ __forceinline__ float nee_hit_weight(const TraceParams &p, const DevScene &S,
+                                                uint32_t path, uint32_t inst, uint32_t prim,
+                                                float t, v3 d, v3 o)
+{
+    const uint32_t cap = p.paths.capacity;
+    float pl;
+    if (inst == SPG_MISS)
+        pl = S.envSelect * env_pdf(S, d);
+    else
+    {
+        const v3 nPrev{p.paths.lastN[path], p.paths.lastN[cap + path],
+                       p.paths.lastN[2 * cap + path]};
+        pl = (1.0f - S.envSelect) * emit_pdf(S, inst, prim, t, d, o, nPrev);
+    }
+    float pb = p.paths.lastPdf[path];
+    float pb2 = pb * pb;
+    return pb2 / (pb2 + pl * pl);
+}
+
 // One bounce of every queued path: surface, material, scatter, vertex record
 // (parity) or throughput update (unbiased), append continuing rays to queue
 // q^1 with a ballot + one atomic per wave.
@@ -956,13 +1041,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     {
                         // a BSDF-sampled ray that reached an emitter: its
                         // share against the emitter sampler (power heuristic)
-                        const v3 nPrev{p.paths.lastN[path], p.paths.lastN[cap + path],
-                                       p.paths.lastN[2 * cap + path]};
-                        float pl = (1.0f - S.envSelect) *
-                                   emit_pdf(S, inst, p.paths.hprim[i], t, d, o, nPrev);
-                        float pb = p.paths.lastPdf[path];
-                        float pb2 = pb * pb;
-                        float w = pb2 / (pb2 + pl * pl);
+                        float w = nee_hit_weight(p, S, path, inst, p.paths.hprim[i], t, d, o);
                         L = L + had(beta, m.emission * w);
                     }
                     else
@@ -976,63 +1055,16 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     float cosine = dot(N, wi);
                     if (kEst == SPG_EST_NEE && vc + 1 < p.bounces)
                     {
-                        // next-event estimation: the sky (probability
-                        // envSelect) or an emissive triangle, weighted by
-                        // the power heuristic against the BSDF sampler's
+                        // next-event estimation against the BSDF sampler's
                         // density; the contribution waits for the shadow
                         // ray's answer
-                        float e0 = rng.u24();
-                        uint32_t r = rng.next();
-                        float e1 = rng.u24(), e2 = rng.u24(), e3 = rng.u24();
-                        float e4 = rng.u24();
-                        const bool useEnv = e0 < S.envSelect;
-                        if (useEnv ? S.envTex >= 0 : S.emitCount > 0)
-                        {
-                            const v3 x = (o + d * t) + N * 0.0001f;
-                            v3 wl, Le;
-                            float pl;
-                            uint32_t ti = SPG_MISS, tp = 0;
-                            if (useEnv)
-                            {
-                                wl = env_direction(S, r, e1, e2, e3);
-                                pl = S.envSelect * env_pdf(S, wl);
-                                v2 uv0{0.0f, 0.0f};
-                                Le = eval_material<kDouble>(S, S.background, uv0, -wl)
-                                         .emission;
-                            }
-                            else
-                            {
-                                float pg, dl;
-                                wl = emit_direction(S, r, e1, e2, e3, e4, x, N, pg, dl, ti,
-                                                    tp);
-                                p.paths.neeDist[path] = dl * 1.001f;
-                                pl = (1.0f - S.envSelect) * pg;
-                                Le = v3{0.0f, 0.0f, 0.0f};
-                                // pg = 0: nothing was sampled and ti is SPG_MISS
-                                if (pg > 0.0f)
-                                {
-                                    const float *em = S.mats[S.inst[ti].material].emission;
-                                    Le = v3{em[0], em[1], em[2]};
-                                }
-                            }
-                            float cl = dot(N, wl);
-                            if (cl > 0.0f && pl > 0.0f)
-                            {
-                                v3 Xl = brdf_bracket<kDouble>(N, V, wl, m, true);
-                                float pb = bsdf_pdf(N, V, wl, cl, m.roughness, ps);
-                                float pl2 = pl * pl;
-                                float wmis = pl2 / (pl2 + pb * pb);
-                                v3 C = had(had(beta, Xl * ((cl * wmis) / pl)), Le);
-                                p.paths.nee[path] = C.x;
-                                p.paths.nee[cap + path] = C.y;
-                                p.paths.nee[2 * cap + path] = C.z;
-                                p.paths.neeTarget[path] = ti;
-                                p.paths.neeTarget[cap + path] = tp;
-                                shadow = true;
-                                so = x;
-                                sd = wl;
-                            }
-                        }
+                        const v3 x = (o + d * t) + N * 0.0001f;
+                        shadow = nee_light_sample<kDouble>(
+                            p, S, path, rng, x, N, V, m, beta,
+                            [&](v3 wl, float cl) {
+                                return bsdf_pdf(N, V, wl, cl, m.roughness, ps);
+                            },
+                            so, sd);
                     }
                     if (cosine > 0.0f && vc + 1 < p.bounces)
                     {
@@ -1065,10 +1097,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     {
                         // the BSDF-sampled direction's share of the
                         // environment (power heuristic)
-                        float pe = S.envSelect * env_pdf(S, d);
-                        float pb = p.paths.lastPdf[path];
-                        float pb2 = pb * pb;
-                        float w = pb2 / (pb2 + pe * pe);
+                        float w = nee_hit_weight(p, S, path, SPG_MISS, 0, t, d, o);
                         L = L + had(beta, m.emission * w);
                     }
                     else
@@ -1088,6 +1117,13 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                 // (probability fb) and the guide's vMF mixture, pdf
                 // fb cos/pi + (1 - fb) q(wi); the vertex record is
                 // (E, X, cos/pdf) for the reverse recurrence of k_finalize.
+                // With next-event estimation (SPG_EST_GUIDED_NEE) emission
+                // reached by a scattered ray carries its power-heuristic
+                // share against the light sampler, and each scattering
+                // vertex takes one light sample weighted against that
+                // scattering density; k_resolve_shadow adds it to the
+                // vertex's E once its shadow ray reaches the light.
+                constexpr bool kNee = kEst == SPG_EST_GUIDED_NEE;
                 v3 E{0.0f, 0.0f, 0.0f}, X{0.0f, 0.0f, 0.0f};
                 float c = 0.0f;
                 if (hit)
@@ -1100,6 +1136,8 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     MatOut m = eval_material<kDouble>(
                         S, S.inst[inst].material, uv, V);
                     E = m.emission;
+                    if (kNee && vc > 0 && S.emitCount > 0 && S.emitBase[inst] >= 0)
+                        E = E * nee_hit_weight(p, S, path, inst, p.paths.hprim[i], t, d, o);
                     if (vc + 1 < p.bounces)
                     {
                         const LobeRef Lg{p.guide.lobes, cap, i};
@@ -1121,6 +1159,18 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                         if (p.guide.rec)
                             record_scatter(p.guide, guide_pn(p.guide, o + d * t), N, V,
                                            m.roughness, u0, u1, u2, Lg, wi, pdf, o + d * t);
+                        if (kNee)
+                        {
+                            const v3 x = (o + d * t) + N * 0.0001f;
+                            const v3 one{1.0f, 1.0f, 1.0f};
+                            shadow = nee_light_sample<kDouble>(
+                                p, S, path, rng, x, N, V, m, one,
+                                [&](v3 wl, float cl) {
+                                    return fb * bsdf_pdf(N, V, wl, cl, m.roughness, ps) +
+                                           (1.0f - fb) * mixture_pdf<kDouble>(Lg, wl);
+                                },
+                                so, sd);
+                        }
                         if (cosv > 0.0f && pdf > 0.0f)
                         {
                             X = brdf_bracket<kDouble>(N, V, wi, m, true);
@@ -1129,6 +1179,13 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                             push = true;
                             no = P + N * 0.0001f;
                             nd = wi;
+                            if (kNee)
+                            {
+                                p.paths.lastPdf[path] = pdf;
+                                p.paths.lastN[path] = N.x;
+                                p.paths.lastN[cap + path] = N.y;
+                                p.paths.lastN[2 * cap + path] = N.z;
+                            }
                             if (p.guide.train)
                             {
                                 v3 pn = guide_pn(p.guide, P);
@@ -1157,6 +1214,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
                     MatOut m = eval_material<kDouble>(S, S.background,
                                                       uv0, V);
                     E = m.emission;
+                    if (kNee && vc > 0) E = E * nee_hit_weight(p, S, path, SPG_MISS, 0, t, d, o);
                 }
                 float *vt = p.paths.vtx + (size_t)vc * 7 * cap + path;
                 vt[0] = E.x;
@@ -1174,10 +1232,11 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
         // into queue 2): one atomic per block and queue
         const uint32_t lane = lane_id();
         const unsigned long long mask = __ballot(push);
-        const unsigned long long smask = kEst == SPG_EST_NEE ? __ballot(shadow) : 0ull;
+        constexpr bool kShadow = kEst == SPG_EST_NEE || kEst == SPG_EST_GUIDED_NEE;
+        const unsigned long long smask = kShadow ? __ballot(shadow) : 0ull;
         uint32_t base, sbase;
         block_queue_bases(&p.paths.counters[nq], (uint32_t)__popcll(mask),
-                          kEst == SPG_EST_NEE ? &p.paths.counters[2] : nullptr,
+                          kShadow ? &p.paths.counters[2] : nullptr,
                           (uint32_t)__popcll(smask), base, sbase);
         if (push)
         {
@@ -1190,7 +1249,7 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
             p.paths.qd[nq][2 * cap + slotq] = nd.z;
             p.paths.qpath[nq][slotq] = path;
         }
-        if (kEst == SPG_EST_NEE)
+        if (kShadow)
         {
             if (shadow)
             {
@@ -1211,7 +1270,8 @@ __global__ __launch_bounds__(256) void k_shade(TraceParams p, int q)
 // Shadow rays of next-event estimation (queue 2, traced by k_extend): one
 // that reaches its light (the sky: no hit; an emissive triangle: that
 // triangle is the closest hit) adds its pending contribution to the path's
-// radiance.
+// radiance (SPG_EST_NEE) or to the emission term of the vertex that took
+// the sample (SPG_EST_GUIDED_NEE).
 __global__ __launch_bounds__(256) void k_resolve_shadow(TraceParams p)
 {
     const uint32_t n = p.paths.counters[2];
@@ -1227,10 +1287,14 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(TraceParams p)
                                  ? !hitAny
                                  : hitAny && p.paths.hinst[i] == ti && p.paths.hprim[i] == tp;
         if (!visible) continue;
-        p.paths.radiance[path] = p.paths.radiance[path] + p.paths.nee[path];
-        p.paths.radiance[cap + path] = p.paths.radiance[cap + path] + p.paths.nee[cap + path];
-        p.paths.radiance[2 * cap + path] =
-            p.paths.radiance[2 * cap + path] + p.paths.nee[2 * cap + path];
+        // the unbiased estimator's radiance, or the guided one's record of
+        // the vertex that took the light sample (the last one written)
+        float *acc = p.paths.radiance + path;
+        if (p.estimator == SPG_EST_GUIDED_NEE)
+            acc = p.paths.vtx + (size_t)(p.paths.vcount[path] - 1u) * 7 * cap + path;
+        acc[0] = acc[0] + p.paths.nee[path];
+        acc[cap] = acc[cap] + p.paths.nee[cap + path];
+        acc[2 * cap] = acc[2 * cap] + p.paths.nee[2 * cap + path];
     }
 }
 
@@ -1724,6 +1788,10 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
             if (dbl) SPG_SHADE_B(true, SPG_EST_NEE);
             else SPG_SHADE_B(false, SPG_EST_NEE);
             break;
+        case SPG_EST_GUIDED_NEE:
+            if (dbl) SPG_SHADE_B(true, SPG_EST_GUIDED_NEE);
+            else SPG_SHADE_B(false, SPG_EST_GUIDED_NEE);
+            break;
         default:
             if (dbl) SPG_SHADE_B(true, SPG_EST_GUIDED);
             else SPG_SHADE_B(false, SPG_EST_GUIDED);
@@ -1744,6 +1812,10 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
     case SPG_EST_NEE:
         if (dbl) SPG_SHADE(true, SPG_EST_NEE);
         else SPG_SHADE(false, SPG_EST_NEE);
+        break;
+    case SPG_EST_GUIDED_NEE:
+        if (dbl) SPG_SHADE(true, SPG_EST_GUIDED_NEE);
+        else SPG_SHADE(false, SPG_EST_GUIDED_NEE);
         break;
     default:
         if (dbl) SPG_SHADE(true, SPG_EST_GUIDED);
