@@ -368,6 +368,18 @@ static void rng_for_pixel(spo_Rng *rng, const spo_Config *cfg, uint32_t width,
 }
 
 /* world-space vertices of triangle k of object i */
+/* Sign of the determinant of a column-major 4x4's upper 3x3 in double,
+ * col0 . (col1 x col2) (the product's model_orient, spg_api.cpp): +1 keeps
+ * a mesh's winding in world space, -1 mirrors it, 0 degenerate. */
+static int model_orient(const float *m)
+{
+    const double x = (double)m[5] * m[10] - (double)m[6] * m[9];
+    const double y = (double)m[6] * m[8] - (double)m[4] * m[10];
+    const double z = (double)m[4] * m[9] - (double)m[5] * m[8];
+    const double det = (double)m[0] * x + (double)m[1] * y + (double)m[2] * z;
+    return det > 0.0 ? 1 : det < 0.0 ? -1 : 0;
+}
+
 static void world_triangle(const spo_Scene *s, uint32_t i, uint32_t k, spo_v3 *A,
                            spo_v3 *B, spo_v3 *C)
 {
@@ -484,8 +496,16 @@ static spo_v3 emit_direction(const spo_Scene *s, uint32_t r, float e1, float e2,
     float dist2 = v3dot(v, v);
     spo_v3 w = v3norm(v);
     spo_v3 n = v3cross(v3sub(B, A), v3sub(C, A));
-    float c = fabsf(v3dot(n, w)) / sqrtf(v3dot(n, n));
-    *pdf = c > 0.0f && dist2 > 0.0f ? pa * dist2 / c : 0.0f;
+    float fw = v3dot(n, w);
+    float c = fabsf(fw) / sqrtf(v3dot(n, n));
+    /* a triangle clearly facing away from x (|cos| > 1e-3 at the light) can
+     * never be the closest hit -- the intersection test accepts front faces
+     * only (dot(d, n) < 0 in the mesh's frame; a mirroring instance
+     * transform flips the world winding) -- so its sample casts no shadow
+     * ray: pdf 0 */
+    int orient = model_orient(s->objects[*inst].model.m);
+    int away = orient != 0 && (orient > 0 ? fw > 0.0f : fw < 0.0f) && c > 1e-3f;
+    *pdf = c > 0.0f && dist2 > 0.0f && !away ? pa * dist2 / c : 0.0f;
     *dist = sqrtf(dist2);
     return w;
 }

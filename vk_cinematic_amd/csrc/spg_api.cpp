@@ -91,6 +91,17 @@ static v3 m4point(const M4 &a, v3 p)
                a.m[12 + i] * 1.0f;
     return v3{r[0], r[1], r[2]};
 }
+// Sign of the determinant of a column-major 4x4's upper 3x3, evaluated in
+// double as col0 . (col1 x col2); 0 when it is 0 or not finite.
+static int32_t model_orient(const float *m)
+{
+    const double x = (double)m[5] * m[10] - (double)m[6] * m[9];
+    const double y = (double)m[6] * m[8] - (double)m[4] * m[10];
+    const double z = (double)m[4] * m[9] - (double)m[5] * m[8];
+    const double det = (double)m[0] * x + (double)m[1] * y + (double)m[2] * z;
+    return det > 0.0 ? 1 : det < 0.0 ? -1 : 0;
+}
+
 static m34 rows34(const float *m)
 {
     m34 r;
@@ -531,6 +542,7 @@ static int upload_tables(spg_Scene *s)
         d.materialId = o.materialId;
         d.nodes = (const float4 *)s->meshes[o.mesh].dNodes;
         d.nodeCount = s->meshes[o.mesh].nodeCount;
+        d.orient = model_orient(o.model);
         inst.push_back(d);
     }
     CHK(upload(&s->dTexs, texs.data(), texs.size()));

@@ -80,7 +80,8 @@ struct DevInstance
     uint32_t mesh;
     int32_t material;   // dense material slot, -1 = not registered
     uint32_t materialId;
-    uint32_t pad;
+    int32_t orient;     // sign of det of M's 3x3 (0: degenerate): +1 keeps the
+                        // mesh's winding in world space, -1 mirrors it
 };
 static_assert(sizeof(DevInstance) == 128, "DevInstance layout");
 static_assert(offsetof(DevInstance, cullScale) == 96, "DevInstance layout");

@@ -375,8 +375,16 @@ __device__ __forceinline__ v3 emit_direction(const DevScene &S, uint32_t r, floa
     const float dist2 = dot(v, v);
     const v3 w = normalize(v);
     const v3 n = cross(B - A, C - A);
-    const float c = fabsf(dot(n, w)) / sqrtf(dot(n, n));
-    pdf = c > 0.0f && dist2 > 0.0f ? pa * dist2 / c : 0.0f;
+    const float fw = dot(n, w);
+    const float c = fabsf(fw) / sqrtf(dot(n, n));
+    // the closest-hit test accepts a triangle only when the ray meets its
+    // front (Moller-Trumbore's dot(d, n) < 0 in the mesh's frame): one
+    // clearly facing away (|cos| > 1e-3 at the light) is never reached, so
+    // its sample casts no shadow ray (pdf 0, no contribution, as the ray's
+    // answer would have been)
+    const int32_t orient = S.inst[inst].orient;
+    const bool away = orient != 0 && (orient > 0 ? fw > 0.0f : fw < 0.0f) && c > 1e-3f;
+    pdf = c > 0.0f && dist2 > 0.0f && !away ? pa * dist2 / c : 0.0f;
     dist = sqrtf(dist2);
     return w;
 }
