@@ -8,6 +8,7 @@
  * (src/bvh.cpp:203-311); the per-box slab test (src/simd.h:198-271) and the
  * Moller-Trumbore test (src/ray_intersection.cpp:156-190) are the
  * reference's. */
+#include <math.h>
 #include <pthread.h>
 #include <stdatomic.h>
 #include <stdlib.h>
@@ -252,6 +253,32 @@ static uint32_t f2u(float f)
  * first) and their count.  Hits are ordered by a key of the entry distance
  * with its two low bits replaced by the child slot (ties and near-ties go
  * to the lower slot); the sign bit is cleared so -0 and +0 agree. */
+static int finite3f(float x, float y, float z)
+{
+    return isfinite(x) && isfinite(y) && isfinite(z);
+}
+
+/* The GPU's slab distances of child c: a plane at p + q s is crossed at
+ * fma(q, s inv, (p - o) inv) (s inv exact, s a power of two), near and far
+ * picked per axis by the sign of inv; entry clamped at 0. */
+static int wide_slab_fold(const uint8_t *N, int c, const float *a, const float *b,
+                          spo_v3 inv, float *entry)
+{
+    const float iv[3] = {inv.x, inv.y, inv.z};
+    float nr[3], fr[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        float tl = fmaf((float)N[32 + 4 * k + c], a[k], b[k]);
+        float th = fmaf((float)N[32 + 4 * (3 + k) + c], a[k], b[k]);
+        nr[k] = iv[k] > 0.0f ? tl : th;
+        fr[k] = iv[k] > 0.0f ? th : tl;
+    }
+    float e = fmaxf(fmaxf(0.0f, nr[0]), fmaxf(nr[1], nr[2]));
+    float x = fminf(fminf(fr[0], fr[1]), fr[2]);
+    *entry = e;
+    return e <= x;
+}
+
 static int wide_visit(const uint8_t *N, spo_v3 o, spo_v3 inv, float tc,
                       int32_t *refs, uint64_t *box)
 {
@@ -259,14 +286,33 @@ static int wide_visit(const uint8_t *N, spo_v3 o, spo_v3 inv, float tc,
     memcpy(ref, N + 16, 16);
     uint32_t key[4];
     int m = 0;
+    float p[3], a[3], b[3];
+    memcpy(p, N, 12);
+    const float iv[3] = {inv.x, inv.y, inv.z}, ov[3] = {o.x, o.y, o.z};
+    for (int k = 0; k < 3; ++k)
+    {
+        a[k] = wide_scale(N[12 + k]) * iv[k];
+        b[k] = (p[k] - ov[k]) * iv[k];
+    }
+    /* finite ray and per-axis factors: the GPU's folded form; otherwise
+     * the decoded box and the reference's compare-and-select test */
+    const int fold = finite3f(o.x, o.y, o.z) && finite3f(inv.x, inv.y, inv.z) &&
+                     finite3f(a[0], a[1], a[2]) && finite3f(b[0], b[1], b[2]);
     for (int c = 0; c < 4; ++c)
     {
         key[c] = 0xFFFFFFFFu;
         if (ref[c] == EMPTY_CHILD) continue;
         ++*box;
         float lo[3], hi[3], e = 0.0f;
-        wide_box(N, c, lo, hi);
-        if (slab(lo, hi, o, inv, &e) && e <= tc)
+        int hit;
+        if (fold)
+            hit = wide_slab_fold(N, c, a, b, inv, &e);
+        else
+        {
+            wide_box(N, c, lo, hi);
+            hit = slab(lo, hi, o, inv, &e);
+        }
+        if (hit && e <= tc)
         {
             key[c] = (f2u(e) & 0x7FFFFFFCu) | (uint32_t)c;
             ++m;

@@ -6,7 +6,7 @@
 // bit-identical between the host build, the gfx950 build and the reference's
 // own x86 build.  Min/Max are compare-and-select like the reference's
 // (src/math_utils.h:9-19) -- not v_min/v_max, whose NaN and signed-zero
-// handling differ -- except in slab_hit_mm, which is used only where no
+// handling differ -- except in slab_hit_t, which is used only where no
 // operand can be NaN and the two agree.
 #pragma once
 
@@ -142,48 +142,26 @@ SPG_HD bool slab_hit(float lox, float loy, float loz, float hix, float hiy,
     return entry <= exit_;
 }
 
-// The same test with IEEE min/max (v_min3/v_max3 on the GPU instead of a
-// compare and a select per operation).  Equal to slab_hit whenever no slab
-// distance is NaN: min/max and compare-and-select differ only on a NaN
-// operand (and on the sign of a zero, which no comparison below sees).  A
-// distance is NaN only from 0 * inf or a non-finite origin, so callers use
-// this form when every component of o and invD is finite (boxes are
-// finite: non-finite vertices and transforms are rejected at input).
-SPG_HD bool slab_hit_mm(float lox, float loy, float loz, float hix, float hiy,
-                        float hiz, v3 o, v3 invD, float *entryOut)
-{
-    float t0x = (lox - o.x) * invD.x, t1x = (hix - o.x) * invD.x;
-    float t0y = (loy - o.y) * invD.y, t1y = (hiy - o.y) * invD.y;
-    float t0z = (loz - o.z) * invD.z, t1z = (hiz - o.z) * invD.z;
-    float entry = fmaxf(fmaxf(0.0f, fminf(t0x, t1x)),
-                        fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
-    float exit_ = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    *entryOut = entry;
-    return entry <= exit_;
-}
-
-// slab_hit_mm with each axis's planes given as near and far: the caller
-// passes lo as near where invD > 0 and hi where invD < 0.  With a finite
-// origin and a finite, nonzero inverse direction, (lo - o) * invD and
-// (hi - o) * invD are ordered like lo and hi (monotone rounding of the
-// difference and of the product by a positive factor; reversed by a
-// negative one), so the near distances are slab_hit_mm's per-axis minima
-// and the far ones its maxima: entry, exit and the result are the same.
-SPG_HD bool slab_hit_nf(float nx, float ny, float nz, float fx, float fy,
-                        float fz, v3 o, v3 invD, float *entryOut)
-{
-    float tnx = (nx - o.x) * invD.x, tfx = (fx - o.x) * invD.x;
-    float tny = (ny - o.y) * invD.y, tfy = (fy - o.y) * invD.y;
-    float tnz = (nz - o.z) * invD.z, tfz = (fz - o.z) * invD.z;
-    float entry = fmaxf(fmaxf(0.0f, tnx), fmaxf(tny, tnz));
-    float exit_ = fminf(fminf(tfx, tfy), tfz);
-    *entryOut = entry;
-    return entry <= exit_;
-}
-
 SPG_HD bool finite3(v3 a)
 {
     return isfinite(a.x) && isfinite(a.y) && isfinite(a.z);
+}
+
+// The slab test on plane distances already formed (the four-wide node
+// visit's t = q (s invD) + (p - o) invD for a plane at p + q s): near (nx,
+// ny, nz) and far (fx, fy, fz) per axis, entry clamped at 0, with IEEE
+// min/max (v_max3 / v_min3 on the GPU instead of a compare and a select per
+// operation).  Callers use it only when the distances cannot be NaN (finite
+// origin, inverse direction and per-axis factors); there min/max and the
+// reference's compare-and-select agree but for the sign of a zero, which no
+// comparison sees.
+SPG_HD bool slab_hit_t(float nx, float ny, float nz, float fx, float fy, float fz,
+                       float *entryOut)
+{
+    float entry = fmaxf(fmaxf(0.0f, nx), fmaxf(ny, nz));
+    float exit_ = fminf(fminf(fx, fy), fz);
+    *entryOut = entry;
+    return entry <= exit_;
 }
 
 // ---- Moller-Trumbore, src/ray_intersection.cpp:156-190, with e1 = b - a

@@ -140,7 +140,7 @@ struct Trav
     uint32_t inst;
     uint32_t triNext, triEnd;  // pending triangles of the current leaf
     uint32_t mode;             // the next step's work (kMode*)
-    bool mmW, mmL;             // world / mesh-space ray is finite: slab_hit_mm
+    bool mmW, mmL;             // world / mesh-space ray is finite
     bool anyHit;               // occlusion query: stop at the first instance
                                // hit with a positive world t
 };
@@ -259,11 +259,20 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
                        : 0xFFFFFFFFu;
         };
         uint32_t key[4];
+        // a plane at p + q s (q the 8-bit field) is crossed at
+        // t = (p + q s - o) inv = q (s inv) + (p - o) inv: per node and axis
+        // a = s inv (exact: s is a power of two) and b = (p - o) inv, then
+        // one FMA per plane
+        const float ax = sx * inv.x, ay = sy * inv.y, az = sz * inv.z;
+        const float bx = (A.x - o.x) * inv.x, by = (A.y - o.y) * inv.y,
+                    bz = (A.z - o.z) * inv.z;
+        const bool fold = (blas ? s.mmL : s.mmW) && finite3(v3{ax, ay, az}) &&
+                          finite3(v3{bx, by, bz});
         {
-            // finite origin and inverse direction: per axis the near plane
-            // is lo where inv > 0 (then (lo - o) * inv <= (hi - o) * inv by
-            // monotone rounding) and hi where inv < 0, so choosing the
-            // fields once per node gives slab_hit_mm's values
+            // per axis the near plane is the lo field where inv > 0 (a
+            // positive a makes t increase with q) and the hi field where
+            // inv < 0, so choosing the fields once per node orders each
+            // axis's two distances
             const bool px = inv.x > 0.0f, py = inv.y > 0.0f, pz = inv.z > 0.0f;
             const uint32_t nx = px ? q0 : q3, fx = px ? q3 : q0;
             const uint32_t ny = py ? q1 : q4, fy = py ? q4 : q1;
@@ -272,17 +281,17 @@ __device__ __forceinline__ bool trav_step(const SceneView &sv, Trav &s,
             for (int k = 0; k < 4; ++k)
             {
                 float e;
-                bool h = slab_hit_nf(dq(nx, k, sx, A.x), dq(ny, k, sy, A.y),
-                                     dq(nz, k, sz, A.z), dq(fx, k, sx, A.x),
-                                     dq(fy, k, sy, A.y), dq(fz, k, sz, A.z), o, inv,
-                                     &e);
+                bool h = slab_hit_t(dq(nx, k, ax, bx), dq(ny, k, ay, by), dq(nz, k, az, bz),
+                                    dq(fx, k, ax, bx), dq(fy, k, ay, by), dq(fz, k, az, bz),
+                                    &e);
                 key[k] = keyOf(k, h, e);
             }
         }
-        if (!(blas ? s.mmL : s.mmW))
+        if (!fold)
         {
-            // a non-finite origin or inverse direction (an axis-parallel
-            // ray): the reference's compare-and-select form
+            // a non-finite origin, inverse direction or per-axis factor (an
+            // axis-parallel ray, or s inv overflowing): the reference's
+            // compare-and-select form on the decoded box
 #pragma unroll
             for (int k = 0; k < 4; ++k)
             {
