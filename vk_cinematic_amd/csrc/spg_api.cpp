@@ -288,6 +288,7 @@ static void free_scene_device(spg_Scene *s)
         m.dTris = nullptr;
         free_ptr(m.dVerts);
         free_ptr(m.dIdx);
+        free_ptr(m.dShade);
     }
     free_ptr(s->dTlasNodes);
     free_ptr(s->dInst);
@@ -487,6 +488,29 @@ static int build_mesh(HostMesh &m, const spg_BuildOpts &opt)
     m.nodeCount = (uint32_t)nn;
     CHK(upload(&m.dVerts, m.verts.data(), m.verts.size()));
     CHK(upload(&m.dIdx, m.idx.data(), m.idx.size()));
+    // shading records by triangle index, one 64-byte fetch at a hit instead
+    // of the index and three vertex reads: (n0.xyz, n1.x) (n1.yz, n2.xy)
+    // (n2.z, uv0, uv1.x) (uv1.y, uv2, 0); a flat mesh stores
+    // cross(b - a, c - a) (the same FP32 operations as the kernel's) as n0
+    std::vector<float4> shade(4 * (size_t)tc);
+    for (uint32_t k = 0; k < tc; ++k)
+    {
+        const float *p0 = &m.verts[8 * (size_t)m.idx[3 * k]];
+        const float *p1 = &m.verts[8 * (size_t)m.idx[3 * k + 1]];
+        const float *p2 = &m.verts[8 * (size_t)m.idx[3 * k + 2]];
+        v3 n0{p0[3], p0[4], p0[5]}, n1{p1[3], p1[4], p1[5]}, n2{p2[3], p2[4], p2[5]};
+        if (!m.smooth)
+        {
+            v3 a{p0[0], p0[1], p0[2]}, b{p1[0], p1[1], p1[2]}, c{p2[0], p2[1], p2[2]};
+            n0 = cross(b - a, c - a);
+            n1 = n2 = v3{0.0f, 0.0f, 0.0f};
+        }
+        shade[4 * (size_t)k] = make_float4(n0.x, n0.y, n0.z, n1.x);
+        shade[4 * (size_t)k + 1] = make_float4(n1.y, n1.z, n2.x, n2.y);
+        shade[4 * (size_t)k + 2] = make_float4(n2.z, p0[6], p0[7], p1[6]);
+        shade[4 * (size_t)k + 3] = make_float4(p1[7], p2[6], p2[7], 0.0f);
+    }
+    CHK(upload(&m.dShade, shade.data(), shade.size()));
     m.built = true;
     return SPG_OK;
 }
@@ -677,6 +701,7 @@ int spg_scene_build(spg_Scene *s, const spg_BuildOpts *opts)
         d.tris = (const float4 *)m.dTris;
         d.verts = (const float *)m.dVerts;
         d.idx = (const uint32_t *)m.dIdx;
+        d.shade = (const float4 *)m.dShade;
         d.smooth = m.smooth;
         d.triCount = (uint32_t)(m.idx.size() / 3);
         dm.push_back(d);
@@ -1528,6 +1553,7 @@ static DevMesh dev_mesh(const HostMesh &m)
     d.tris = (const float4 *)m.dTris;
     d.verts = (const float *)m.dVerts;
     d.idx = (const uint32_t *)m.dIdx;
+    d.shade = (const float4 *)m.dShade;
     d.smooth = m.smooth;
     d.triCount = (uint32_t)(m.idx.size() / 3);
     return d;

@@ -63,6 +63,9 @@ struct DevMesh
                             // right after the nodes in the same allocation
     const float *verts;     // VertexPNT, 8 floats
     const uint32_t *idx;    // 3 per triangle
+    const float4 *shade;    // per triangle (by index), 4 float4: its three
+                            // vertex normals (smooth; flat: cross(b-a, c-a))
+                            // and uvs, what shading reads at a hit
     uint32_t smooth;
     uint32_t triCount;
 };
@@ -356,7 +359,8 @@ struct HostMesh
     // device copies
     // one allocation of 64-byte records: nodes, then triangles in leaf
     // order (dTris points into it)
-    void *dNodes = nullptr, *dTris = nullptr, *dVerts = nullptr, *dIdx = nullptr;
+    void *dNodes = nullptr, *dTris = nullptr, *dVerts = nullptr, *dIdx = nullptr,
+         *dShade = nullptr;
     uint32_t nodeCount = 0;
     float boundsLo[3], boundsHi[3];
 };

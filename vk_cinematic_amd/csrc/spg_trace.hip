@@ -105,25 +105,21 @@ __device__ __forceinline__ void surface_at(const DevScene &S, uint32_t inst,
 {
     const DevInstance &I = S.inst[inst];
     const DevMesh &M = S.meshes[I.mesh];
-    uint32_t i0 = M.idx[3 * prim], i1 = M.idx[3 * prim + 1],
-             i2 = M.idx[3 * prim + 2];
-    const float *p0 = M.verts + 8 * i0, *p1 = M.verts + 8 * i1,
-                *p2 = M.verts + 8 * i2;
+    // the triangle's shading record: vertex normals (or the flat normal)
+    // and uvs in one 64-byte fetch
+    const float4 *r = M.shade + 4 * (size_t)prim;
+    const float4 a = r[0], b = r[1], c = r[2], d = r[3];
     float w = 1.0f - u - v;
-    uv.x = (p0[6] * w + p1[6] * u) + p2[6] * v;
-    uv.y = (p0[7] * w + p1[7] * u) + p2[7] * v;
+    uv.x = (c.y * w + c.w * u) + d.y * v;
+    uv.y = (c.z * w + d.x * u) + d.z * v;
     v3 ln;
     if (M.smooth)
     {
-        v3 n0{p0[3], p0[4], p0[5]}, n1{p1[3], p1[4], p1[5]},
-            n2{p2[3], p2[4], p2[5]};
+        v3 n0{a.x, a.y, a.z}, n1{a.w, b.x, b.y}, n2{b.z, b.w, c.x};
         ln = normalize((n0 * w + n1 * u) + n2 * v);
     }
     else
-    {
-        v3 a{p0[0], p0[1], p0[2]}, b{p1[0], p1[1], p1[2]}, c{p2[0], p2[1], p2[2]};
-        ln = cross(b - a, c - a);
-    }
+        ln = v3{a.x, a.y, a.z};
     worldN = normalize(xform_vector(I.model, ln));
 }
 
