@@ -1247,7 +1247,9 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
                 tm.begin(kExtend);
                 HIPCHK(launch_extend(d->stream, p, q, d->persist));
                 tm.end();
-                if (guided)
+                // at the last bounce no hit scatters (a path at bounce b
+                // holds b vertices), so there is nothing to infer
+                if (guided && b + 1 < cfg.bounces)
                 {
                     tm.begin(kGuide);
                     HIPCHK(launch_guide_infer(d->stream, p, q));

@@ -183,7 +183,7 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
                 HIPCHK(launch_extend(st, p, q, ps));
                 CHK(T1());
                 ++k;
-                if (guided)
+                if (guided && bb + 1 < cfg.bounces)  // the last bounce scatters nothing
                 {
                     CHK(T0(kGuide));
                     HIPCHK(launch_guide_infer(st, p, q));

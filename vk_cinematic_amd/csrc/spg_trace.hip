@@ -1535,8 +1535,9 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(TraceParams p,
                 }
             }
         }
-        evals += (unsigned long long)__popcll(__ballot(valid));  // wave-uniform
-        if (p.guide.analytic) continue;  // wave-uniform
+        const unsigned long long vmask = __ballot(valid);  // wave-uniform
+        evals += (unsigned long long)__popcll(vmask);
+        if (p.guide.analytic || vmask == 0ull) continue;  // wave-uniform
         Lobes L;
         guide_forward_tile(sW, sBias, scr, valid, pn, N, V, L, [](int, const float *) {});
         if (valid) store_lobes(p.guide.lobes, cap, i, L);
