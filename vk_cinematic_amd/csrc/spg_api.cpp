@@ -1018,7 +1018,7 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
            oQo1 = take(12 * P), oQd0 = take(12 * P), oQd1 = take(12 * P),
            oQp0 = take(4 * P), oQp1 = take(4 * P), oQo2 = take(12 * P),
            oQd2 = take(12 * P), oQp2 = take(4 * P), oPdf = take(4 * P),
-           oLastN = take(12 * P),
+           oLastN = take(12 * P), oLastW = take(4 * P),
            oNee = take(12 * P), oTgt = take(8 * P), oDist = take(4 * P),
            oCnt = take(64),
            oMet = take(128), oHt = take(4 * P), oHu = take(4 * P),
@@ -1047,6 +1047,7 @@ int alloc_path_state(DevPaths &p, void *&block, size_t &bytes, uint32_t capacity
     p.qpath[2] = (uint32_t *)(b + oQp2);
     p.lastPdf = (float *)(b + oPdf);
     p.lastN = (float *)(b + oLastN);
+    p.lastW = (float *)(b + oLastW);
     p.nee = (float *)(b + oNee);
     p.neeTarget = (uint32_t *)(b + oTgt);
     p.neeDist = (float *)(b + oDist);

@@ -171,6 +171,7 @@ struct DevPaths
     uint32_t *counters;          // [0..2] queue lengths
     float *lastPdf;              // density of each path's last sampled direction
     float *lastN;                // [3][capacity] shading normal where it was sampled
+    float *lastW;                // the spatial emitter choice's weight sum there
     float *nee;                  // [3][capacity] pending shadow-ray contribution
     uint32_t *neeTarget;         // [2][capacity] instance, triangle the shadow
                                  // ray must reach (instance SPG_MISS: the sky)

@@ -272,36 +272,67 @@ __device__ __forceinline__ float emit_weight(const DevScene &S, uint32_t k, v3 x
     return dot(n, v) > -a.w ? w : 0.0f;
 }
 
-__device__ __forceinline__ float emit_weight_sum(const DevScene &S, v3 x, v3 n)
+// One scan of the spatial choice's weights from x (shading normal n): their
+// sum W in index order, the running sum at the end of each of 8 equal
+// groups of emitters (the last ones may be short or empty) and the last
+// positive weight.  The emitter drawn is the first whose running sum
+// exceeds e4 W; emit_direction finds its group from the group sums and
+// rescans only that group from the sum before it, which repeats the full
+// scan's additions and so its result.
+constexpr int kEmitGroups = 8;
+struct EmitScan
 {
-    float W = 0.0f;
-    for (uint32_t k = 0; k < S.emitInstCount; ++k) W += emit_weight(S, k, x, n);
-    return W;
+    float W;
+    float G[kEmitGroups];
+    uint32_t lastPos;
+    float lastPosW;
+};
+
+__device__ __forceinline__ void emit_scan(const DevScene &S, v3 x, v3 n, EmitScan &e)
+{
+    const uint32_t cnt = S.emitInstCount, gs = (cnt + kEmitGroups - 1) / kEmitGroups;
+    e.W = 0.0f;
+    e.lastPos = 0;
+    e.lastPosW = 0.0f;
+#pragma unroll
+    for (int g = 0; g < kEmitGroups; ++g)
+    {
+        const uint32_t k1 = min(cnt, (uint32_t)(g + 1) * gs);
+        for (uint32_t k = (uint32_t)g * gs; k < k1; ++k)
+        {
+            const float w = emit_weight(S, k, x, n);
+            e.W += w;
+            e.lastPos = w > 0.0f ? k : e.lastPos;
+            e.lastPosW = w > 0.0f ? w : e.lastPosW;
+        }
+        e.G[g] = e.W;
+    }
 }
 
 // Density per unit area of the emitter sampler on instance `inst`'s
-// triangle `prim`: the spatial choice's P(instance | x, n) / area, or the
-// global table's entry; 0 if not sampled.
+// triangle `prim`: the spatial choice's P(instance | x, n) / area (W the
+// weight sum at x, kept from the light sample drawn there), or the global
+// table's entry; 0 if not sampled.
 __device__ __forceinline__ float emit_pdf_area(const DevScene &S, uint32_t inst,
-                                               uint32_t prim, v3 x, v3 n)
+                                               uint32_t prim, v3 x, v3 n, float W)
 {
     const int32_t b = S.emitBase[inst];
     if (b < 0) return 0.0f;
     if (S.emitInstCount == 0) return S.emitPdfA[b + prim];
     const int32_t k = S.emitSlot[inst];
     if (k < 0) return 0.0f;
-    const float W = emit_weight_sum(S, x, n);
     if (!(W > 0.0f)) return 0.0f;
     return emit_weight(S, (uint32_t)k, x, n) / W * S.emitInst[2 * k + 1].y;
 }
 
 // Density per steradian with which the emitter sampler at x (shading normal
-// n) reaches triangle `prim` of instance `inst` at distance t along unit
-// direction d: pdfA * t^2 / |cos| at the light, 0 if it is not sampled.
+// n, weight sum W) reaches triangle `prim` of instance `inst` at distance t
+// along unit direction d: pdfA * t^2 / |cos| at the light, 0 if it is not
+// sampled.
 __device__ __forceinline__ float emit_pdf(const DevScene &S, uint32_t inst, uint32_t prim,
-                                          float t, v3 d, v3 x, v3 n)
+                                          float t, v3 d, v3 x, v3 n, float W)
 {
-    const float pa = emit_pdf_area(S, inst, prim, x, n);
+    const float pa = emit_pdf_area(S, inst, prim, x, n, W);
     if (!(pa > 0.0f)) return 0.0f;
     v3 A, B, C;
     world_triangle(S, inst, prim, A, B, C);
@@ -312,20 +343,20 @@ __device__ __forceinline__ float emit_pdf(const DevScene &S, uint32_t inst, uint
 
 // A point on an emissive triangle (uniform on the triangle by e2, e3): the
 // unit direction from x, its density per steradian, the triangle.  With
-// the spatial choice an emitter instance is drawn by its weight from x and
-// n (the first whose running sum exceeds e4 x the total), then a triangle of
-// it by area (entry by its alias table from r and e1); else an entry of the
-// global table (r and e1).  pdf 0: nothing to sample.
-__device__ __forceinline__ v3 emit_direction(const DevScene &S, uint32_t r, float e1,
-                                             float e2, float e3, float e4, v3 x, v3 nx,
-                                             float &pdf, float &dist, uint32_t &inst,
-                                             uint32_t &prim)
+// the spatial choice (the scan `e` from x and nx) an emitter instance is
+// drawn by its weight (the first whose running sum exceeds e4 x the total),
+// then a triangle of it by area (entry by its alias table from r and e1);
+// else an entry of the global table (r and e1).  pdf 0: nothing to sample.
+__device__ __forceinline__ v3 emit_direction(const DevScene &S, const EmitScan &e,
+                                             uint32_t r, float e1, float e2, float e3,
+                                             float e4, v3 x, v3 nx, float &pdf,
+                                             float &dist, uint32_t &inst, uint32_t &prim)
 {
     uint32_t g;
     float pa;
     if (S.emitInstCount)
     {
-        const float W = emit_weight_sum(S, x, nx);
+        const float W = e.W;
         if (!(W > 0.0f))
         {
             pdf = 0.0f;
@@ -334,19 +365,35 @@ __device__ __forceinline__ v3 emit_direction(const DevScene &S, uint32_t r, floa
             return v3{0.0f, 0.0f, 1.0f};
         }
         const float target = e4 * W;
-        float acc = 0.0f, wp = 0.0f;
-        uint32_t pick = 0;
-        bool found = false;
-        for (uint32_t k = 0; k < S.emitInstCount; ++k)
+        // the first group whose running sum exceeds the target and the sum
+        // before it; none (the target rounds to W): the last positive weight
+        int gsel = kEmitGroups;
+        float base = 0.0f;
+#pragma unroll
+        for (int j = kEmitGroups - 1; j >= 0; --j)
         {
-            const float w = emit_weight(S, k, x, nx);
-            acc += w;
-            if (!found && w > 0.0f)
+            const bool in = e.G[j] > target;
+            gsel = in ? j : gsel;
+            base = in ? (j > 0 ? e.G[j - 1] : 0.0f) : base;
+        }
+        uint32_t pick = e.lastPos;
+        float wp = e.lastPosW;
+        if (gsel < kEmitGroups)
+        {
+            // the crossing weight is positive (the running sum grows only
+            // by positive weights), so it is the pick
+            const uint32_t cnt = S.emitInstCount, gs = (cnt + kEmitGroups - 1) / kEmitGroups;
+            const uint32_t k0 = (uint32_t)gsel * gs, k1 = min(cnt, k0 + gs);
+            float acc = base;
+            bool found = false;
+            for (uint32_t k = k0; k < k1 && !found; ++k)
             {
-                pick = k;  // the last positive weight so far
-                wp = w;
+                const float w = emit_weight(S, k, x, nx);
+                acc += w;
+                found = acc > target;
+                pick = found ? k : pick;
+                wp = found ? w : wp;
             }
-            found = found || acc > target;
         }
         const float4 b = S.emitInst[2 * pick + 1];
         const uint32_t first = __float_as_uint(b.z), count = __float_as_uint(b.w);
@@ -870,6 +917,16 @@ __device__ __forceinline__ bool nee_light_sample(const TraceParams &p, const Dev
     uint32_t r = rng.next();
     float e1 = rng.u24(), e2 = rng.u24(), e3 = rng.u24();
     float e4 = rng.u24();
+    // the spatial choice's weights from x, whichever light is drawn: their
+    // sum is kept for the MIS weight of an emitter that the scattered ray
+    // may reach next (the same x and N there give the same sum)
+    EmitScan scan;
+    scan.W = 0.0f;
+    if (S.emitInstCount)
+    {
+        emit_scan(S, x, N, scan);
+        p.paths.lastW[path] = scan.W;
+    }
     const bool useEnv = e0 < S.envSelect;
     if (!(useEnv ? S.envTex >= 0 : S.emitCount > 0)) return false;
     v3 wl, Le;
@@ -885,7 +942,7 @@ __device__ __forceinline__ bool nee_light_sample(const TraceParams &p, const Dev
     else
     {
         float pg, dl;
-        wl = emit_direction(S, r, e1, e2, e3, e4, x, N, pg, dl, ti, tp);
+        wl = emit_direction(S, scan, r, e1, e2, e3, e4, x, N, pg, dl, ti, tp);
         p.paths.neeDist[path] = dl * 1.001f;
         pl = (1.0f - S.envSelect) * pg;
         Le = v3{0.0f, 0.0f, 0.0f};
@@ -916,7 +973,9 @@ __device__ __forceinline__ bool nee_light_sample(const TraceParams &p, const Dev
 // Power-heuristic share of emission reached by a scattered ray (density pb
 // at the previous vertex) against the light sampler: toward the emitter
 // triangle (inst, prim) at distance t along d from o, or (inst SPG_MISS)
-// the sky along d.
+// the sky along d.  The ray left the previous vertex from the point where
+// that vertex's light sample was drawn (o), with the shading normal and
+// the spatial choice's weight sum stored there.
 __device__ __forceinline__ float nee_hit_weight(const TraceParams &p, const DevScene &S,
                                                 uint32_t path, uint32_t inst, uint32_t prim,
                                                 float t, v3 d, v3 o)
@@ -929,7 +988,8 @@ __device__ __forceinline__ float nee_hit_weight(const TraceParams &p, const DevS
     {
         const v3 nPrev{p.paths.lastN[path], p.paths.lastN[cap + path],
                        p.paths.lastN[2 * cap + path]};
-        pl = (1.0f - S.envSelect) * emit_pdf(S, inst, prim, t, d, o, nPrev);
+        pl = (1.0f - S.envSelect) *
+             emit_pdf(S, inst, prim, t, d, o, nPrev, p.paths.lastW[path]);
     }
     float pb = p.paths.lastPdf[path];
     float pb2 = pb * pb;
