@@ -803,6 +803,7 @@ def time_to_relmse(V, dev, scenes, workload, ref_spp, ref_est, max_spp=8192,
             a.add(64, V.EST_GUIDED_NEE, 0x8EED, guide=g)
 
     guided_nee = run("guided+nee", guided_nee_step)
+    learned_split = g.light_split()
     g.close()
     s.close()
     res = {"workload": workload, "target": target, "wall_clock": True,
@@ -815,6 +816,7 @@ def time_to_relmse(V, dev, scenes, workload, ref_spp, ref_est, max_spp=8192,
         c = crossing(curve, target)
         res[name] = {"time_s": c[0] if c else None, "spp": c[1] if c else None,
                      "how": c[2] if c else None, "curve": curve}
+    res["guided_nee"]["light_split"] = learned_split
     tp = res["plain"]["time_s"]
     for name in ("guided", "nee", "guided_nee"):
         t = res[name]["time_s"]

@@ -472,17 +472,27 @@ int spg_guide_debug(spg_Guide *g, uint32_t flags, uint32_t recordCap);
  * min(*count, cap). */
 int spg_guide_debug_records(spg_Guide *g, float *records, uint32_t cap,
                             uint32_t *count);
-/* Guide checkpoint (no reference counterpart; the survey's optimizer state
- * that must be serializable): the configuration, the FP32 master
- * parameters, both Adam moments and the step count, in a little-endian file
- * ("SPGGUIDE", version 1).  Training steps are deterministic (fixed-order
- * reductions), so a guide loaded from a checkpoint continues training
- * bit-identically to the guide that wrote it. */
+/* Guide checkpoint (no reference counterpart; the optimizer state that
+ * must be serializable): the configuration, the FP32 master parameters,
+ * both Adam moments, the step count and the learned light split's sums, in
+ * a little-endian file ("SPGGUIDE", version 2).  Training steps are
+ * deterministic (fixed-order reductions, integer light sums), so a guide
+ * loaded from a checkpoint continues training bit-identically to the guide
+ * that wrote it. */
 int spg_guide_save(spg_Guide *g, const char *path);
-/* Restores a checkpoint into g; its configuration replaces g's.
- * SPG_E_INVALID for a file that is not a version-1 checkpoint of this
- * network. */
+/* Restores a checkpoint into g; its configuration replaces g's.  Version-1
+ * files (no light sums) load with the light split unlearned.
+ * SPG_E_INVALID for a file that is not a checkpoint of this network. */
 int spg_guide_load(spg_Guide *g, const char *path);
+/* Learned light split (no reference counterpart): SPG_EST_GUIDED_NEE
+ * draws a light sample from the sky with this probability (else from the
+ * emitters) when the scene has both.  Training passes learn it as the
+ * sky's share of the luminance their reached light samples delivered
+ * (clamped to [0.05, 0.95]); before any, the scene's default split (0.5)
+ * is used and *skyFraction reads -1.  spg_guide_set_light_split fixes it
+ * (0.05..0.95) or, with -1, forgets it. */
+int spg_guide_light_split(spg_Guide *g, float *skyFraction);
+int spg_guide_set_light_split(spg_Guide *g, float skyFraction);
 
 /* ---- after the path ---- */
 /* Progressive accumulation across trace calls (the reference's open TODO,

@@ -230,6 +230,8 @@ void spo_env_direction(const float *prob, const uint32_t *alias, uint32_t w,
  * triangles of instances whose material has a constant emission; sets the
  * environment's selection probability (0.5 with both, else 1 or 0). */
 int spo_scene_build_env(spo_Scene *s);
+/* Overrides the sky share of light samples (scenes with sky and emitters). */
+void spo_scene_set_light_split(spo_Scene *s, float skyFraction);
 /* The emitter table alone (arrays of the scene's emissive triangle count,
  * base: one entry per object); returns the entry count. */
 uint32_t spo_scene_emitter_table(const spo_Scene *s, float *pdfA, float *prob,

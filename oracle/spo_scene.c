@@ -456,6 +456,14 @@ int spo_scene_build_env(spo_Scene *s)
     return 1;
 }
 
+/* The sky share of the light samples when the scene has both kinds (the
+ * product's learned light split, spg_guide_set_light_split, fixed for a
+ * comparison); no effect otherwise.  Call after spo_scene_build_env. */
+void spo_scene_set_light_split(spo_Scene *s, float skyFraction)
+{
+    if (s->envPdf && s->emitCount) s->envSelect = skyFraction;
+}
+
 /* sp_EvaluateMaterial, src/sp_material_system.cpp:59-105, plus the
  * cube-map emission texture this build adds (sampled by direction).  A
  * missing material yields emission (1,0,1) and zero albedo/roughness

@@ -114,6 +114,7 @@ def oracle():
     _sig(L, "spo_scene_build_env", C.c_int, vp)
     _sig(L, "spo_sincos_cr_array", None, f32p, C.c_uint32, f32p, f32p)
     _sig(L, "spo_scene_emitter_table", C.c_uint32, vp, f32p, f32p, u32p, u32p, i32p)
+    _sig(L, "spo_scene_set_light_split", None, vp, C.c_float)
     _sig(L, "spo_scene_emitter_instances", C.c_uint32, vp, i32p, C.c_uint32, f32p, i32p,
          f32p, u32p)
     _sig(L, "spo_create_cubemap", None, f32p, C.c_uint32, C.c_uint32, C.c_uint32,

@@ -59,16 +59,21 @@ def test_small_scenes_bit_exact(device, rng, orng, scene):
     assert np.isfinite(img).all() and img.reshape(-1, 4)[:, :3].max() > 0
 
 
-def test_config5_tiles_bit_exact(device):
+@pytest.mark.parametrize("split", [None, 0.2])
+def test_config5_tiles_bit_exact(device, split):
     """Tiles of the full-size config 5 (4M triangles; the sky through the
-    opening and 54 emissive spheres), 2 spp, 6 bounces: bit for bit against
-    the oracle over the same SAH hierarchy, equal ray, hit and miss
-    counts."""
+    opening and 54 emissive spheres), 2 spp, 6 bounces, with the default
+    sky/emitter split of the light samples and with a fixed one (what the
+    learned split sets): bit for bit against the oracle over the same SAH
+    hierarchy, equal ray, hit and miss counts."""
     desc = scenes.config5()
     w, h = desc["camera"][0], desc["camera"][1]
     s, cam = scenes.load_gpu(device, desc)
     g = V.Guide(device, seed=1)
     g.debug(analytic=True)
+    if split is not None:
+        g.set_light_split(split)
+        assert g.light_split() == np.float32(split)
     ctx = V.make_context(cam, s, spp=2, bounces=6, clamp=0.0,
                          estimator=V.EST_GUIDED_NEE, rng=V.RNG_PCG, seed=0x5EED, guide=g,
                          collect=True)
@@ -76,7 +81,8 @@ def test_config5_tiles_bit_exact(device):
         img = np.zeros(w * h * 4, np.float32)
         m = V.sp_PathTraceTile(ctx, V.Tile(*t), img)
         px, met, _ = _oracle_guided(desc, 2, 6, SPO_RNG_PCG, 0x5EED, hit=SPO_HIT_REFERENCE,
-                                    sah=True, tile=t, est=SPO_EST_GUIDED_NEE)
+                                    sah=True, tile=t, est=SPO_EST_GUIDED_NEE,
+                                    light_split=split)
         a = img.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
         b = px.reshape(h, w, 4)[t[1]:t[3], t[0]:t[2]]
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), t
@@ -173,4 +179,40 @@ def test_config5_trained_mean_equals_plain(device):
     assert abs(gm.mean() - pm.mean()) < 5 * se + 2e-3 * pm.mean()
     g.close()
     dimg.free()
+    s.close()
+
+
+def test_light_split_is_learned_and_reproducible(device, tmp_path):
+    """Training passes on config 5 (480x270) learn the sky share of the
+    light samples from the luminance they delivered: none before training,
+    within [0.05, 0.95] after, below the default 0.5 (most sky samples end
+    on the box's walls; the 54 emissive spheres deliver most of the direct
+    light), the same bits from two trainings with the same seed, and kept
+    through a checkpoint."""
+    desc = scenes.config5(480, 270)
+    s, cam = scenes.load_gpu(device, desc)
+    tiles = V.ComputeTiles(480, 270, 64, 64)
+    img = np.zeros(480 * 270 * 4, np.float32)
+    splits = []
+    for rep in range(2):
+        g = V.Guide(device, seed=2, batch_size=4096, steps_per_pass=2)
+        assert g.light_split() == -1.0
+        for k in range(6):
+            V.path_trace_tiles(V.make_context(cam, s, spp=1, bounces=6, clamp=0.0,
+                                              estimator=V.EST_GUIDED_NEE, rng=V.RNG_PCG,
+                                              seed=3, sample_offset=k, guide=g, train=True,
+                                              transcendentals=V.TX_FAST), tiles, img)
+        splits.append(g.light_split())
+        if rep == 1:
+            g.save(tmp_path / "g.ckpt")
+            g2 = V.Guide(device, seed=9)
+            g2.load(tmp_path / "g.ckpt")
+            assert g2.light_split() == splits[-1]
+            g2.set_light_split(-1)
+            assert g2.light_split() == -1.0
+            g2.close()
+        g.close()
+    print(f"\nlearned sky share {splits[0]:.4f}")
+    assert splits[0] == splits[1]
+    assert 0.05 <= splits[0] < 0.5
     s.close()

@@ -31,9 +31,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _oracle_guided(desc, spp, bounces, orng, seed, hit=SPO_HIT_BRUTE, sah=False,
-                   tile=None, sink=0, est=SPO_EST_GUIDED):
+                   tile=None, sink=0, est=SPO_EST_GUIDED, light_split=None):
     O = oracle()
     s, cam = load_oracle(desc, sah=sah)
+    if light_split is not None:
+        O.spo_scene_set_light_split(s, light_split)
     w, h = desc["camera"][0], desc["camera"][1]
     cfg = SpoConfig(rng=orng, tx=SPO_TX_CR, hit=hit, estimator=est,
                     samplesPerPixel=spp, bounces=bounces, radianceClamp=0.0,

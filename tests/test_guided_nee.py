@@ -5,7 +5,8 @@ sampler of SPG_EST_NEE -- combined by the power heuristic.  Checked against
 a closed form (the inside of an emitting sphere, where every vertex sees the
 light and a dropped or doubled MIS weight moves the mean), against the plain
 unbiased estimator's mean, and for the error reduction NEE brings over the
-guided estimator alone (CPU only)."""
+guided estimator alone; any sky/emitter split of the light samples (the
+product learns one) leaves the mean unchanged (CPU only)."""
 import ctypes as C
 import math
 
@@ -13,8 +14,9 @@ import numpy as np
 import pytest
 
 import vk_cinematic_amd as V
-from _libs import (SPO_EST_GUIDED, SPO_EST_GUIDED_NEE, SPO_EST_UNBIASED, SPO_HIT_BRUTE,
-                   SPO_RNG_PCG, SPO_TX_CR, SpoConfig, oracle)
+from vk_cinematic_amd import scenes
+from _libs import (SPO_EST_GUIDED, SPO_EST_GUIDED_NEE, SPO_EST_NEE, SPO_EST_UNBIASED,
+                   SPO_HIT_BRUTE, SPO_RNG_PCG, SPO_TX_CR, SpoConfig, oracle)
 from _scenes import load_oracle
 from test_nee import _lamps_scene
 
@@ -95,4 +97,35 @@ def test_agrees_with_unbiased_and_cuts_the_guided_error():
     assert n > 100
     w = sink[:n, 12]
     assert np.all(w > 0) and np.all(np.isfinite(w))
+    O.spo_scene_destroy(s)
+
+
+def _sky_and_lamp_scene(w=20, h=16):
+    """Config 1 (sphere on a ground quad under the sky cube with its sun)
+    plus one emissive sphere: light samples split between sky and
+    emitter."""
+    desc = scenes.config1(w, h, env="cube", env_size=16)
+    desc["objects"] = list(desc["objects"]) + [
+        (0, 777, (1.6, 0.6, 0.8), (0.0, 0.0, 0.0, 1.0), (0.3, 0.3, 0.3))]
+    desc["materials"] = list(desc["materials"]) + [
+        (777, (0.0, 0.0, 0.0), (12.0, 9.0, 6.0), 0.0, V.SPG_NO_TEXTURE, V.SPG_NO_TEXTURE)]
+    return desc
+
+
+@pytest.mark.parametrize("est", [SPO_EST_GUIDED_NEE, SPO_EST_NEE])
+def test_any_light_split_is_unbiased(est):
+    """The sky share of the light samples (the product's learned light
+    split) changes only the variance: with the sky chosen for 0.1 or 0.8 of
+    the light samples the image mean equals the plain estimator's within 4
+    standard errors."""
+    O = oracle()
+    desc = _sky_and_lamp_scene()
+    s, cam = load_oracle(desc)
+    u = [_render(O, s, cam, 20, 16, _cfg(32, 3, 300 + k, SPO_EST_UNBIASED))[0].mean()
+         for k in range(6)]
+    for split in (0.1, 0.8):
+        O.spo_scene_set_light_split(s, split)
+        g = [_render(O, s, cam, 20, 16, _cfg(32, 3, 400 + k, est))[0].mean() for k in range(6)]
+        se = math.sqrt(np.var(u) / 6 + np.var(g) / 6)
+        assert abs(np.mean(u) - np.mean(g)) < 4 * se + 1e-4, (split, np.mean(u), np.mean(g))
     O.spo_scene_destroy(s)

@@ -216,6 +216,8 @@ _SIGS = {
     "spg_guide_debug_records": (_int, [_vp, _vp, _u32, C.POINTER(_u32)]),
     "spg_guide_save": (_int, [_vp, C.c_char_p]),
     "spg_guide_load": (_int, [_vp, C.c_char_p]),
+    "spg_guide_light_split": (_int, [_vp, C.POINTER(C.c_float)]),
+    "spg_guide_set_light_split": (_int, [_vp, C.c_float]),
 }
 
 _lib = None
@@ -698,6 +700,16 @@ class Guide:
     def save(self, path):
         """Checkpoint: configuration, parameters, Adam moments and steps."""
         check(lib().spg_guide_save(self.handle, os.fsencode(path)))
+
+    def light_split(self):
+        """The learned sky share of guided + NEE light samples (-1 before
+        any training pass delivered light)."""
+        f = C.c_float()
+        check(lib().spg_guide_light_split(self.handle, C.byref(f)))
+        return f.value
+
+    def set_light_split(self, sky_fraction):
+        check(lib().spg_guide_set_light_split(self.handle, float(sky_fraction)))
 
     def load(self, path):
         """Restores a checkpoint written by save (its configuration too)."""

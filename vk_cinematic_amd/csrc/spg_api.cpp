@@ -1211,6 +1211,15 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     const bool train = guided && cfg.guideTrain;
     if (guided)
         CHK(guide_prepare_trace(cfg.guide, s, cap, cfg.bounces, train, p.guide));
+    if (cfg.estimator == SPG_EST_GUIDED_NEE)
+    {
+        // the guided tracer with next-event estimation learns the split of
+        // its light samples between the sky and the emitters: each kind's
+        // share of the light its training passes' samples delivered
+        p.learnLight = train ? 1u : 0u;
+        if (p.scene.envTex >= 0 && p.scene.emitCount > 0)
+            p.scene.envSelect = guide_light_split(cfg.guide, p.scene.envSelect);
+    }
 
     LaunchTimer tm{d};
     uint32_t launches[6] = {0};
@@ -1301,6 +1310,11 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
         HIPCHK(hipMemcpyAsync(d->hostStage, devImage, slots.size() * 16,
                               hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
+    if (p.learnLight)
+    {
+        cfg.guide->lightStat[0] += m[8];
+        cfg.guide->lightStat[1] += m[9];
+    }
     if (hostImage)
     {
         // slot order is tile order, row by row within a tile, so each

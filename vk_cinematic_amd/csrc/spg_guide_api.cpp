@@ -206,6 +206,13 @@ static int alloc_path_buffers(void *&block, uint32_t capacity, uint32_t bounces,
     return SPG_OK;
 }
 
+float guide_light_split(const spg_Guide *g, float fallback)
+{
+    const double sky = (double)g->lightStat[0], em = (double)g->lightStat[1];
+    if (!(sky + em > 0.0)) return fallback;
+    return (float)std::min(0.95, std::max(0.05, sky / (sky + em)));
+}
+
 // ownBlock != nullptr: the per-path buffers are allocated there for the
 // caller (a batch, whose captured graph must not see them move) instead of
 // in the guide's own block.
@@ -638,7 +645,7 @@ int spg_guide_samples(spg_Guide *g, float *rows13, uint32_t cap, uint32_t *count
 // Checkpoint layout: magic, version, parameter count, Adam steps, the
 // configuration, then params, m, v (FP32, kGParams each).
 static const char kGuideMagic[8] = {'S', 'P', 'G', 'G', 'U', 'I', 'D', 'E'};
-constexpr uint32_t kGuideVersion = 1;
+constexpr uint32_t kGuideVersion = 2;  // 2: + the learned light split's sums
 
 int spg_guide_save(spg_Guide *g, const char *path)
 {
@@ -663,7 +670,8 @@ int spg_guide_save(spg_Guide *g, const char *path)
     const uint32_t head[3] = {kGuideVersion, (uint32_t)kGParams, steps};
     bool ok = fwrite(kGuideMagic, 8, 1, f) == 1 && fwrite(head, sizeof head, 1, f) == 1 &&
               fwrite(&cfg, sizeof cfg, 1, f) == 1 &&
-              fwrite(st.data(), sizeof(float), st.size(), f) == st.size();
+              fwrite(st.data(), sizeof(float), st.size(), f) == st.size() &&
+              fwrite(g->lightStat, sizeof g->lightStat, 1, f) == 1;
     ok = (fclose(f) == 0) && ok;
     return ok ? SPG_OK : fail(SPG_E_INVALID, std::string("spg_guide_save: write failed: ") + path);
 }
@@ -677,13 +685,16 @@ int spg_guide_load(spg_Guide *g, const char *path)
     uint32_t head[3];
     spg_GuideConfig cfg;
     std::vector<float> st((size_t)3 * kGParams);
+    uint64_t light[2] = {0, 0};
     bool ok = fread(magic, 8, 1, f) == 1 && fread(head, sizeof head, 1, f) == 1 &&
               fread(&cfg, sizeof cfg, 1, f) == 1 &&
-              memcmp(magic, kGuideMagic, 8) == 0 && head[0] == kGuideVersion &&
-              head[1] == (uint32_t)kGParams &&
+              memcmp(magic, kGuideMagic, 8) == 0 &&
+              (head[0] == 1 || head[0] == kGuideVersion) && head[1] == (uint32_t)kGParams &&
               fread(st.data(), sizeof(float), st.size(), f) == st.size();
+    // version 2 adds the learned light split's sums after the moments
+    if (ok && head[0] == kGuideVersion) ok = fread(light, sizeof light, 1, f) == 1;
     char extra;
-    ok = ok && fread(&extra, 1, 1, f) == 0;  // nothing after the moments
+    ok = ok && fread(&extra, 1, 1, f) == 0;  // nothing after them
     fclose(f);
     if (!ok) return fail(SPG_E_INVALID, std::string("spg_guide_load: not a guide checkpoint: ") + path);
     if (cfg.batchSize == 0 || cfg.batchSize % 64 || cfg.batchSize > (1u << 20) ||
@@ -701,6 +712,35 @@ int spg_guide_load(spg_Guide *g, const char *path)
     HIPCHK(launch_guide_convert(d->stream, make_args(g, 0, 64, false, 0.0f)));
     HIPCHK(hipStreamSynchronize(d->stream));
     g->cfg = cfg;
+    g->lightStat[0] = light[0];
+    g->lightStat[1] = light[1];
+    return SPG_OK;
+}
+
+int spg_guide_light_split(spg_Guide *g, float *skyFraction)
+{
+    if (!g || !skyFraction) return fail(SPG_E_INVALID, "spg_guide_light_split: NULL");
+    std::lock_guard<std::mutex> lk(g->dev->lock);
+    *skyFraction = guide_light_split(g, -1.0f);
+    return SPG_OK;
+}
+
+int spg_guide_set_light_split(spg_Guide *g, float skyFraction)
+{
+    if (!g) return fail(SPG_E_INVALID, "spg_guide_set_light_split: NULL");
+    if (!(skyFraction >= 0.05f && skyFraction <= 0.95f) && skyFraction != -1.0f)
+        return fail(SPG_E_INVALID,
+                    "spg_guide_set_light_split: sky fraction in [0.05, 0.95], or -1");
+    std::lock_guard<std::mutex> lk(g->dev->lock);
+    if (skyFraction == -1.0f)
+    {
+        g->lightStat[0] = g->lightStat[1] = 0;
+        return SPG_OK;
+    }
+    // sums in the learned form whose split is skyFraction (in double, then
+    // rounded to float as the tracer rounds it)
+    g->lightStat[0] = (uint64_t)llround((double)skyFraction * 4294967296.0);
+    g->lightStat[1] = (uint64_t)4294967296ull - g->lightStat[0];
     return SPG_OK;
 }
 

@@ -206,11 +206,15 @@ struct TraceParams
     float clampValue;
     uint32_t estimator, rngMode, seed, tx, collect;
     uint32_t refill;             // k_extend's refill threshold, 0 = default
+    // learned light split: k_resolve_shadow adds the luminance of the
+    // reached light samples, sky and emitters apart, into metrics[8], [9]
+    // (fixed point, 2^-16 units)
+    uint32_t learnLight;
     // the persistent closest-hit launch's work counter: zeroed by the
     // kernel before each k_extend (k_raygen, k_shade, k_resolve_shadow),
     // so the launch sequence needs no separate clearing launches
     uint32_t *work;
-    DevGuide guide;              // SPG_EST_GUIDED only
+    DevGuide guide;              // SPG_EST_GUIDED(_NEE) only
     // batched launches (slotItem != nullptr): each slot belongs to an item
     // with its own scene, camera and image
     const uint32_t *slotItem;
@@ -342,6 +346,9 @@ struct spg_Guide
     uint32_t *gcnt = nullptr, *scanBase = nullptr;  // training-sample scan
     uint32_t pathCap = 0, pathBounces = 0;
     uint32_t pins = 0;                // live batches capturing this guide's buffers
+    // learned light split of SPG_EST_GUIDED_NEE: luminance of the reached
+    // light samples of its training passes, sky and emitters (2^-16 units)
+    uint64_t lightStat[2] = {0, 0};
     // verification hooks (spg_guide_debug)
     uint32_t analytic = 0;
     void *recBlock = nullptr;         // count (256 B), then the records
@@ -415,6 +422,10 @@ constexpr uint32_t kMaxPathsInFlight = 1u << 21;
 constexpr uint32_t kMaxBounces = 16;
 
 // guide hooks of the tracer (spg_guide_api.cpp)
+// The learned sky share of SPG_EST_GUIDED_NEE's light samples: the sky's
+// part of the light samples' delivered luminance, clamped to [0.05, 0.95];
+// `fallback` before any was delivered.
+float guide_light_split(const spg_Guide *g, float fallback);
 int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
                         uint32_t bounces, bool train, DevGuide &out,
                         void **ownBlock = nullptr);

@@ -1341,6 +1341,7 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(TraceParams p)
     const uint32_t n = p.paths.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) *p.work = 0u;  // for the next k_extend
     const uint32_t cap = p.paths.capacity;
+    unsigned long long lum[2] = {0ull, 0ull};  // learned light split: sky, emitters
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
     {
         const uint32_t path = p.paths.qpath[2][i];
@@ -1351,14 +1352,40 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(TraceParams p)
                                  ? !hitAny
                                  : hitAny && p.paths.hinst[i] == ti && p.paths.hprim[i] == tp;
         if (!visible) continue;
+        const v3 C{p.paths.nee[path], p.paths.nee[cap + path], p.paths.nee[2 * cap + path]};
         // the unbiased estimator's radiance, or the guided one's record of
         // the vertex that took the light sample (the last one written)
         float *acc = p.paths.radiance + path;
         if (p.estimator == SPG_EST_GUIDED_NEE)
             acc = p.paths.vtx + (size_t)(p.paths.vcount[path] - 1u) * 7 * cap + path;
-        acc[0] = acc[0] + p.paths.nee[path];
-        acc[cap] = acc[cap] + p.paths.nee[cap + path];
-        acc[2 * cap] = acc[2 * cap] + p.paths.nee[2 * cap + path];
+        acc[0] = acc[0] + C.x;
+        acc[cap] = acc[cap] + C.y;
+        acc[2 * cap] = acc[2 * cap] + C.z;
+        if (p.learnLight)
+        {
+            // luminance in 2^-16 units, capped at 2^24 per sample; integer
+            // sums do not depend on the order of the additions
+            const float y = 0.2126f * C.x + 0.7152f * C.y + 0.0722f * C.z;
+            const float f = fminf(fmaxf(y, 0.0f) * 65536.0f, 1.6777216e12f);
+            lum[ti == SPG_MISS ? 0 : 1] += (unsigned long long)f;
+        }
+    }
+    if (p.learnLight)  // uniform: every thread of the block reaches this
+    {
+        __shared__ unsigned long long sLum[2][4];
+        for (int k = 0; k < 2; ++k)
+        {
+            unsigned long long v = lum[k];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if ((threadIdx.x & 63) == 0) sLum[k][threadIdx.x >> 6] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2)
+        {
+            const unsigned long long v = sLum[threadIdx.x][0] + sLum[threadIdx.x][1] +
+                                         sLum[threadIdx.x][2] + sLum[threadIdx.x][3];
+            if (v) atomicAdd(&p.paths.metrics[8 + threadIdx.x], v);
+        }
     }
 }
 
@@ -1894,7 +1921,9 @@ hipError_t launch_shade(hipStream_t s, const TraceParams &p, int q)
 
 hipError_t launch_resolve_shadow(hipStream_t s, const TraceParams &p)
 {
-    uint32_t g = grid_for((uint64_t)p.slots * p.sppPass, 256, 4096);
+    // at most 1024 blocks: with the learned light split each block adds its
+    // sums with one atomic per light kind
+    uint32_t g = grid_for((uint64_t)p.slots * p.sppPass, 256, 1024);
     hipLaunchKernelGGL(k_resolve_shadow, dim3(g), dim3(256), 0, s, p);
     return hipGetLastError();
 }
