@@ -216,3 +216,40 @@ def test_light_split_is_learned_and_reproducible(device, tmp_path):
     assert splits[0] == splits[1]
     assert 0.05 <= splits[0] < 0.5
     s.close()
+
+
+def test_batch_training_replays_equal_eager_training(device):
+    """Online training in the captured graph with light samples: config 1
+    with an emissive sphere (sky and emitter light), each replay one pass
+    with its sample collection, Adam steps and light sums.  After five
+    replays the guide's parameters and learned light split equal those of
+    five eager training traces bit for bit (the split learned after the
+    first pass is the one the later passes draw with), and so do the last
+    images."""
+    from test_guided_nee import _sky_and_lamp_scene
+    desc = _sky_and_lamp_scene(96, 64)
+    s, cam = scenes.load_gpu(device, desc)
+    w, h = 96, 64
+    kw = dict(spp=1, bounces=4, clamp=0.0, estimator=V.EST_GUIDED_NEE, rng=V.RNG_PCG,
+              seed=0x99, transcendentals=V.TX_FAST)
+    ga = V.Guide(device, seed=5, batch_size=2048, steps_per_pass=2)
+    gb = V.Guide(device, seed=5, batch_size=2048, steps_per_pass=2)
+    img = device.alloc(w * h * 16)
+    b = V.Batch(device, [(cam, s, img)], autotune=False, guide=ga, train=True, **kw)
+    eager = device.alloc(w * h * 16)
+    for _ in range(5):
+        b.run()
+        V.path_trace_tiles_device(V.make_context(cam, s, guide=gb, train=True, **kw),
+                                  [V.Tile(0, 0, w, h)], eager)
+    assert ga.stats()["steps"] == gb.stats()["steps"] == 10
+    assert np.array_equal(ga.params().view(np.uint32), gb.params().view(np.uint32))
+    assert ga.light_split() == gb.light_split() and 0.05 <= ga.light_split() <= 0.95
+    a = img.download(np.float32, w * h * 4)
+    e = eager.download(np.float32, w * h * 4)
+    assert np.array_equal(a.view(np.uint32), e.view(np.uint32))
+    b.close()
+    for x in (img, eager):
+        x.free()
+    ga.close()
+    gb.close()
+    s.close()

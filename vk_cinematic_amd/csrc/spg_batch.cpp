@@ -5,6 +5,7 @@
 // each ray through its own scene's hierarchy, shading reads its scene's
 // materials, and the pixel writer scatters into the item's image.
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -39,6 +40,10 @@ struct spg_Batch
     DevGuide dg{};
     uint32_t maxBlocks = 0;
     uint32_t refill = 0;  // k_extend's refill threshold (0 = the default)
+    // guided + NEE: the replays learn the light split (training) and draw
+    // with it; the split the items' device scenes hold now (-1: default)
+    bool learnLight = false;
+    float splitOnDevice = -1.0f;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -144,6 +149,8 @@ int record(spg_Batch *b, hipStream_t st, const spg_TraceConfig &cfg,
         p.guide = b->dg;
         p.guide.train = train ? 1u : 0u;
     }
+    p.learnLight = cfg.estimator == SPG_EST_GUIDED_NEE && train ? 1u : 0u;
+    b->learnLight = p.learnLight != 0;
     uint32_t k = 0;
     HIPCHK(launch_clear_counters(st, (uint32_t *)b->paths.metrics, 32));
     ++k;
@@ -560,6 +567,27 @@ int spg_batch_run(spg_Batch *b, spg_Metrics *metrics)
             return fail(SPG_E_STATE, "spg_batch_run: a scene changed after the batch was made");
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = d->stream;
+    if (b->cfg.estimator == SPG_EST_GUIDED_NEE && b->guide)
+    {
+        // the learned light split, written into the items' device scenes
+        // (every item traces the same scene) when it changed; the graph's
+        // kernels read it from there
+        const DevScene &S0 = b->scenes[0]->dev_scene;
+        if (S0.envTex >= 0 && S0.emitCount > 0)
+        {
+            const float split = guide_light_split(b->guide, S0.envSelect);
+            if (split != b->splitOnDevice)
+            {
+                char *base = (char *)const_cast<DevScene *>(b->dScenes);
+                for (uint32_t k = 0; k < b->n; ++k)
+                    HIPCHK(hipMemcpyAsync(base + k * sizeof(DevScene) +
+                                              offsetof(DevScene, envSelect),
+                                          &split, sizeof(float), hipMemcpyHostToDevice, st));
+                HIPCHK(hipStreamSynchronize(st));  // `split` is on this stack
+                b->splitOnDevice = split;
+            }
+        }
+    }
     HIPCHK(hipEventRecord(b->ev[0], st));
     HIPCHK(hipGraphLaunch(b->exec, st));
     HIPCHK(hipEventRecord(b->ev[1], st));
@@ -571,6 +599,11 @@ int spg_batch_run(spg_Batch *b, spg_Metrics *metrics)
     HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     b->lastMs = ms;
     for (int k = 0; k < 8; ++k) d->lastCounters[k] = m[k];
+    if (b->learnLight)
+    {
+        b->guide->lightStat[0] += m[8];
+        b->guide->lightStat[1] += m[9];
+    }
     if (metrics)
     {
         metrics->values[SPG_METRIC_ELAPSED_NS] += (uint64_t)
