@@ -66,7 +66,9 @@ enum spg_Estimator {
                              weighted against the one-sample MIS density of
                              BSDF and guide, and emission reached by a
                              scattered ray against the light sampler (three
-                             techniques, power heuristic); needs a guide */
+                             techniques, power heuristic); training passes
+                             also learn the sky/emitter split of the light
+                             samples (spg_guide_light_split); needs a guide */
 };
 enum spg_Transcendentals {
     SPG_TX_DOUBLE = 1,    /* sin/cos/atan2/pow evaluated in double and
