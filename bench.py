@@ -754,8 +754,10 @@ def time_to_relmse(V, dev, scenes, workload, ref_spp, ref_est, max_spp=8192,
                    train_passes=64, guide_kw=None, target=1e-3):
     """Wall-clock time to relMSE `target` of the plain tracer, the guided
     tracer (online training during its first `train_passes` one-sample
-    passes, then the frozen guide), next-event estimation and the guided
-    tracer with next-event estimation (trained the same way), each
+    passes, then the frozen guide), next-event estimation (learning its
+    sky / emitter split from its own calls) and the guided tracer with
+    next-event estimation (trained the same way, the split learned by its
+    training passes), each
     accumulating every sample it renders, against a reference of `ref_spp`
     samples of `ref_est` rendered here."""
     desc = getattr(scenes, workload)()
@@ -793,7 +795,10 @@ def time_to_relmse(V, dev, scenes, workload, ref_spp, ref_est, max_spp=8192,
 
     guided = run("guided", guided_step)
     g.close()
+    s.learn_light_split(True)  # NEE learns its sky / emitter split from its own calls
     nee = run("nee", lambda a: a.add(64 if a.spp >= 64 else 16, V.EST_NEE, 0x7EED))
+    nee_split = s.light_split()
+    s.learn_light_split(False)
     g = V.Guide(dev, **guide_kw)
 
     def guided_nee_step(a):
@@ -817,6 +822,7 @@ def time_to_relmse(V, dev, scenes, workload, ref_spp, ref_est, max_spp=8192,
         res[name] = {"time_s": c[0] if c else None, "spp": c[1] if c else None,
                      "how": c[2] if c else None, "curve": curve}
     res["guided_nee"]["light_split"] = learned_split
+    res["nee"]["light_split"] = nee_split
     tp = res["plain"]["time_s"]
     for name in ("guided", "nee", "guided_nee"):
         t = res[name]["time_s"]

@@ -189,6 +189,14 @@ int spg_scene_add_object(spg_Scene *scene, uint32_t meshId,
                          uint32_t *objectId);
 /* sp_BuildSceneBroadphase + the per-mesh midphase builds; uploads to HBM. */
 int spg_scene_build(spg_Scene *scene, const spg_BuildOpts *opts);
+/* Learned light split of SPG_EST_NEE (no reference counterpart): with
+ * enable != 0 each NEE call on this scene adds the luminance its reached
+ * light samples delivered, sky and emitters apart, and later calls draw
+ * the sky with the learned share (clamped to [0.05, 0.95]) instead of 0.5
+ * when the scene has both; enable = 0 stops and forgets it (the default).
+ * spg_scene_light_split reads it (-1 before any).  Batches draw with 0.5. */
+int spg_scene_learn_light_split(spg_Scene *scene, uint32_t enable);
+int spg_scene_light_split(spg_Scene *scene, float *skyFraction);
 /* Host-only SAH build of a triangle list (tris9 = a,b,c per triangle), no
  * device involved: nodes16 receives the 64-byte nodes (layout as
  * spg_mesh_bvh_export), order the leaf primitive order. */

@@ -144,3 +144,50 @@ def test_nee_emitter_choice_edges_match_oracle_bit_exact(device, which):
     assert np.array_equal(img.view(np.uint32), px.view(np.uint32))
     assert [m.values[k] for k in (1, 2, 3, 4)] == [int(met[k]) for k in (1, 2, 3, 4)]
     assert img.reshape(-1, 4)[:, :3].max() > 0
+
+
+def test_nee_learned_light_split(device):
+    """A scene told to learn its light split: NEE calls on config 5
+    (480x270) add the luminance their reached light samples delivered and
+    later calls draw with the learned sky share -- none before the first
+    call, within [0.05, 0.95] and below the default 0.5 after (the sky is
+    mostly hidden by the box), the same bits on a second scene given the
+    same calls, forgotten when learning stops.  Images stay finite and
+    their mean matches the default split's within the Monte Carlo error."""
+    desc = scenes.config5(480, 270)
+    tiles = V.ComputeTiles(480, 270, 64, 64)
+    splits, means = [], []
+    for rep in range(2):
+        s, cam = scenes.load_gpu(device, desc)
+        s.learn_light_split(True)
+        assert s.light_split() == -1.0
+        img = np.zeros(480 * 270 * 4, np.float32)
+        m = []
+        for k in range(4):
+            V.path_trace_tiles(V.make_context(cam, s, spp=4, bounces=6, clamp=0.0,
+                                              estimator=V.EST_NEE, rng=V.RNG_PCG,
+                                              seed=11, sample_offset=4 * k,
+                                              transcendentals=V.TX_FAST), tiles, img)
+            assert np.isfinite(img).all()
+            m.append(img.reshape(-1, 4)[:, :3].mean())
+        splits.append(s.light_split())
+        means.append(m)
+        s.learn_light_split(False)
+        assert s.light_split() == -1.0
+        s.close()
+    assert splits[0] == splits[1] and means[0] == means[1]
+    assert 0.05 <= splits[0] < 0.5
+    # the default split's mean over the same samples, for comparison
+    s, cam = scenes.load_gpu(device, desc)
+    img = np.zeros(480 * 270 * 4, np.float32)
+    d = []
+    for k in range(4):
+        V.path_trace_tiles(V.make_context(cam, s, spp=4, bounces=6, clamp=0.0,
+                                          estimator=V.EST_NEE, rng=V.RNG_PCG, seed=11,
+                                          sample_offset=4 * k, transcendentals=V.TX_FAST),
+                           tiles, img)
+        d.append(img.reshape(-1, 4)[:, :3].mean())
+    s.close()
+    learned, default = np.array(means[0][1:]), np.array(d[1:])
+    se = np.sqrt(learned.var(ddof=1) / 3 + default.var(ddof=1) / 3)
+    assert abs(learned.mean() - default.mean()) < 5 * se + 0.02 * default.mean()

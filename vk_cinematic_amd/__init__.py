@@ -217,6 +217,8 @@ _SIGS = {
     "spg_guide_save": (_int, [_vp, C.c_char_p]),
     "spg_guide_load": (_int, [_vp, C.c_char_p]),
     "spg_guide_light_split": (_int, [_vp, C.POINTER(C.c_float)]),
+    "spg_scene_learn_light_split": (_int, [_vp, C.c_uint32]),
+    "spg_scene_light_split": (_int, [_vp, C.POINTER(C.c_float)]),
     "spg_guide_set_light_split": (_int, [_vp, C.c_float]),
 }
 
@@ -419,6 +421,16 @@ class Scene:
     def build(self, max_leaf=4, bins=16, threads=0):
         opts = BuildOpts(max_leaf, bins, threads)
         check(lib().spg_scene_build(self.handle, C.byref(opts)))
+
+    def learn_light_split(self, enable=True):
+        """NEE calls on this scene learn the sky share of their light
+        samples (enable=False stops and forgets it)."""
+        check(lib().spg_scene_learn_light_split(self.handle, int(bool(enable))))
+
+    def light_split(self):
+        f = C.c_float()
+        check(lib().spg_scene_light_split(self.handle, C.byref(f)))
+        return f.value
 
     def bvh_info(self, mesh):
         n, l, d, c = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()

@@ -680,6 +680,23 @@ static int upload_tables(spg_Scene *s)
     return SPG_OK;
 }
 
+int spg_scene_learn_light_split(spg_Scene *s, uint32_t enable)
+{
+    if (!s) return fail(SPG_E_INVALID, "spg_scene_learn_light_split: NULL");
+    std::lock_guard<std::mutex> g(s->dev->lock);
+    s->learnLight = enable != 0;
+    if (!enable) s->lightStat[0] = s->lightStat[1] = 0;
+    return SPG_OK;
+}
+
+int spg_scene_light_split(spg_Scene *s, float *skyFraction)
+{
+    if (!s || !skyFraction) return fail(SPG_E_INVALID, "spg_scene_light_split: NULL");
+    std::lock_guard<std::mutex> g(s->dev->lock);
+    *skyFraction = light_split(s->lightStat, -1.0f);
+    return SPG_OK;
+}
+
 int spg_scene_build(spg_Scene *s, const spg_BuildOpts *opts)
 {
     if (!s) return fail(SPG_E_INVALID, "spg_scene_build: NULL scene");
@@ -1220,6 +1237,13 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
         if (p.scene.envTex >= 0 && p.scene.emitCount > 0)
             p.scene.envSelect = guide_light_split(cfg.guide, p.scene.envSelect);
     }
+    else if (cfg.estimator == SPG_EST_NEE && s->learnLight)
+    {
+        // the same for the NEE tracer, learned by the scene from its calls
+        p.learnLight = 1u;
+        if (p.scene.envTex >= 0 && p.scene.emitCount > 0)
+            p.scene.envSelect = light_split(s->lightStat, p.scene.envSelect);
+    }
 
     LaunchTimer tm{d};
     uint32_t launches[6] = {0};
@@ -1312,8 +1336,10 @@ static int trace(const spg_Context *ctx, const spg_Tile *tiles, uint32_t count,
     HIPCHK(hipStreamSynchronize(d->stream));
     if (p.learnLight)
     {
-        cfg.guide->lightStat[0] += m[8];
-        cfg.guide->lightStat[1] += m[9];
+        uint64_t *stat = cfg.estimator == SPG_EST_GUIDED_NEE ? cfg.guide->lightStat
+                                                             : s->lightStat;
+        stat[0] += m[8];
+        stat[1] += m[9];
     }
     if (hostImage)
     {

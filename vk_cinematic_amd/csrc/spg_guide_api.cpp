@@ -206,11 +206,16 @@ static int alloc_path_buffers(void *&block, uint32_t capacity, uint32_t bounces,
     return SPG_OK;
 }
 
-float guide_light_split(const spg_Guide *g, float fallback)
+float light_split(const uint64_t stat[2], float fallback)
 {
-    const double sky = (double)g->lightStat[0], em = (double)g->lightStat[1];
+    const double sky = (double)stat[0], em = (double)stat[1];
     if (!(sky + em > 0.0)) return fallback;
     return (float)std::min(0.95, std::max(0.05, sky / (sky + em)));
+}
+
+float guide_light_split(const spg_Guide *g, float fallback)
+{
+    return light_split(g->lightStat, fallback);
 }
 
 // ownBlock != nullptr: the per-path buffers are allocated there for the

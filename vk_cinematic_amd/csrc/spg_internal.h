@@ -399,6 +399,11 @@ struct spg_Scene
     uint32_t backgroundId = 0;
     bool built = false;
     bool dirty = true;
+    // learned light split of SPG_EST_NEE (spg_scene_learn_light_split): the
+    // luminance its calls' reached light samples delivered, sky and
+    // emitters (2^-16 units)
+    bool learnLight = false;
+    uint64_t lightStat[2] = {0, 0};
     spg::BvhBuildResult tlas;
     uint32_t tlasPending = 0;       // stack bound of the wide instance tree
     // device-side tables
@@ -422,9 +427,11 @@ constexpr uint32_t kMaxPathsInFlight = 1u << 21;
 constexpr uint32_t kMaxBounces = 16;
 
 // guide hooks of the tracer (spg_guide_api.cpp)
-// The learned sky share of SPG_EST_GUIDED_NEE's light samples: the sky's
-// part of the light samples' delivered luminance, clamped to [0.05, 0.95];
-// `fallback` before any was delivered.
+// A learned sky share of light samples from the delivered luminance sums
+// (sky, emitters): the sky's part, clamped to [0.05, 0.95]; `fallback`
+// before any was delivered.  guide_light_split: SPG_EST_GUIDED_NEE's, kept
+// by the guide.
+float light_split(const uint64_t stat[2], float fallback);
 float guide_light_split(const spg_Guide *g, float fallback);
 int guide_prepare_trace(spg_Guide *g, const spg_Scene *s, uint32_t capacity,
                         uint32_t bounces, bool train, DevGuide &out,
