@@ -205,9 +205,10 @@ float spo_random_bilateral(uint32_t *state)
 }
 
 /* Per-pixel xorshift32 seed (no reference counterpart: the reference seeds
- * every tile with one constant, src/main.cpp:738-739).  A 32-bit integer
- * finalizer of base + golden-ratio * index; zero is remapped since xorshift32
- * stays at zero forever. */
+ * every tile with one constant, src/main.cpp:738-739).  The published
+ * 32-bit integer finalizer "lowbias32" (C. Wellons, hash-prospector) of
+ * base + golden-ratio * index; zero is remapped since xorshift32 stays at
+ * zero forever. */
 uint32_t spo_pixel_seed(uint32_t base, uint32_t pixelIndex)
 {
     uint32_t x = base + pixelIndex * 0x9E3779B9u;

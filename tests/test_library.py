@@ -84,6 +84,61 @@ def test_pixel_seed_matches_oracle():
             assert V.pixel_seed(base, idx) != 0
 
 
+def _xorshift_right_inverse(y, s):
+    """x with x ^ (x >> s) == y on 32 bits (fixed point of x = y ^ (x >> s))."""
+    x = y
+    for _ in range(32 // s + 1):
+        x = y ^ (x >> s)
+    return x
+
+
+def test_pixel_seed_is_the_published_finalizer_of_a_golden_ratio_sequence():
+    """The per-pixel seed is C. Wellons' "lowbias32" integer finalizer
+    (hash-prospector: xorshift 16, multiply 0x7FEB352D, xorshift 15,
+    multiply 0x846CA68B, xorshift 16) of base + floor(2^32 / phi) * index,
+    with 0 remapped because a xorshift32 state of 0 never leaves it.  Pinned
+    without restating the forward steps: running the inverse (xorshift
+    inverses, modular inverses of the multipliers) on the seed recovers the
+    golden-ratio sequence, the one input that the finalizer sends to 0 gets
+    the remap, and one flipped input bit flips every output bit with
+    probability 1/2 (avalanche, within sampling error)."""
+    import math
+    import random
+    O = oracle()
+    M = 1 << 32
+    golden = int((math.sqrt(5) - 1) / 2 * M)
+    ia, ib = pow(0x7FEB352D, -1, M), pow(0x846CA68B, -1, M)
+
+    def inverse(y):
+        x = _xorshift_right_inverse(y, 16)
+        x = _xorshift_right_inverse(x * ib % M, 15)
+        return _xorshift_right_inverse(x * ia % M, 16)
+
+    r = random.Random(5)
+    cases = [(r.getrandbits(32), r.getrandbits(32)) for _ in range(3000)]
+    cases += [(b, i) for b in (0, 1, 0xFFFFFFFF) for i in (1, 2, 1920 * 1080 - 1)]
+    for base, idx in cases:
+        y = O.spo_pixel_seed(base, idx)
+        assert V.pixel_seed(base, idx) == y
+        assert inverse(y) == (base + golden * idx) % M, (base, idx)
+    # the finalizer fixes 0: the index whose sequence value is 0 gets the remap
+    gi = pow(golden, -1, M)
+    for base in (0, 5, 0xDEADBEEF):
+        idx = (-base * gi) % M
+        assert O.spo_pixel_seed(base, idx) == V.pixel_seed(base, idx) == 0x6D2B79F5
+    n = 2048
+    flips = np.zeros((32, 32))
+    for _ in range(n):
+        x = r.getrandbits(32)
+        y = O.spo_pixel_seed(x, 0)
+        for i in range(32):
+            d = y ^ O.spo_pixel_seed(x ^ (1 << i), 0)
+            flips[i] += [(d >> j) & 1 for j in range(32)]
+    p = flips / n
+    # sampling error of one probability: 0.5 / sqrt(2048) = 0.011
+    assert np.abs(p - 0.5).max() < 0.06 and np.abs(p - 0.5).mean() < 0.015
+
+
 def test_sky_generators():
     sky = V.gen_sky_cubemap(16, sun_radius=0.2).reshape(6, 16, 16, 4)
     assert np.all(sky[..., 3] == 1) and np.isclose(sky[..., :3].max(), 50.0)
